@@ -1,0 +1,152 @@
+"""ORACLE — test infrastructure only.  ctypes binding of oracle/_build/liboracle.so."""
+from __future__ import annotations
+
+import ctypes as C
+import os
+import subprocess
+
+import numpy as np
+
+HERE = os.path.dirname(os.path.abspath(__file__))
+_LIB = None
+
+
+def lib_path() -> str:
+    return os.path.join(HERE, "_build", "liboracle.so")
+
+
+def build() -> str:
+    """Compile the oracle with make (g++ on its own sources)."""
+    subprocess.run(["make", "-C", HERE, "-s"], check=True)
+    return lib_path()
+
+
+def _lib():
+    global _LIB
+    if _LIB is None:
+        p = lib_path()
+        if not os.path.exists(p):
+            build()
+        L = C.CDLL(p)
+        P = C.c_void_p
+        u32p = np.ctypeslib.ndpointer(np.uint32, flags="C")
+        i32p = np.ctypeslib.ndpointer(np.int32, flags="C")
+        L.or_create.restype = P
+        L.or_create.argtypes = [C.c_uint32, C.c_uint32, i32p, i32p, C.c_int]
+        L.or_destroy.argtypes = [P]
+        L.or_set_threads.argtypes = [C.c_int]
+        L.or_load.restype = C.c_int
+        L.or_load.argtypes = [P, C.c_uint32, C.c_uint32, i32p, C.c_int32, C.c_int32,
+                              C.c_uint32, C.c_uint32, C.c_uint32]
+        L.or_step.argtypes = [P, C.c_uint32, C.c_void_p]
+        L.or_digest.argtypes = [P, u32p]
+        L.or_chain.argtypes = [P, u32p]
+        L.or_read_state.argtypes = [P, i32p]
+        L.or_read_derived.argtypes = [P, i32p]
+        L.or_read_bond.argtypes = [P, u32p]
+        L.or_write_state_derived.argtypes = [P, i32p, C.c_void_p]
+        L.or_debug_counters.argtypes = [P, u32p]
+        L.or_substep_index.restype = C.c_uint32
+        L.or_substep_index.argtypes = [P]
+        L.or_apply_impulse.restype = C.c_uint32
+        L.or_apply_impulse.argtypes = [P, C.c_int32, C.c_int32, C.c_int32, C.c_int32,
+                                       C.c_int32, C.c_uint32]
+        L.or_spawn.restype = C.c_uint32
+        L.or_spawn.argtypes = [P, C.c_uint32, i32p, i32p]
+        _LIB = L
+    return _LIB
+
+
+def set_threads(n: int) -> None:
+    _lib().or_set_threads(int(n))
+
+
+class OracleWorld:
+    """One 2D world on the CPU oracle, with the reference engine's host API."""
+
+    def __init__(self, image, luts, mats, iters: int = 4, max_particles=None, max_cells=None):
+        L = _lib()
+        self.max_particles = int(max_particles or image.n)
+        self.max_cells = int(max_cells or image.cellTotal)
+        self._h = L.or_create(self.max_particles, self.max_cells,
+                              np.ascontiguousarray(luts, np.int32),
+                              np.ascontiguousarray(mats, np.int32).reshape(-1), int(iters))
+        self.load(image)
+
+    def load(self, image):
+        rc = _lib().or_load(self._h, image.n, image.nFluid,
+                            np.ascontiguousarray(image.state, np.int32).reshape(-1),
+                            image.domW, image.domH, image.cellShift, image.gridW, image.gridH)
+        if rc != 0:
+            raise ValueError("oracle: scene exceeds capacity")
+        self.n, self.nFluid = image.n, image.nFluid
+
+    def close(self):
+        if self._h:
+            _lib().or_destroy(self._h)
+            self._h = None
+
+    __del__ = close
+
+    def step(self, count: int, stream: bool = False):
+        """Advance `count` substeps; with stream=True return the per-substep
+        8-word digests as uint32 [count, 8]."""
+        if stream:
+            out = np.zeros((count, 8), np.uint32)
+            _lib().or_step(self._h, count, out.ctypes.data)
+            return out
+        _lib().or_step(self._h, count, None)
+        return None
+
+    def digest(self) -> np.ndarray:
+        a = np.zeros(8, np.uint32)
+        _lib().or_digest(self._h, a)
+        return a
+
+    def chain(self) -> np.ndarray:
+        a = np.zeros(8, np.uint32)
+        _lib().or_chain(self._h, a)
+        return a
+
+    def read_state(self) -> np.ndarray:
+        a = np.zeros(self.n * 8, np.int32)
+        _lib().or_read_state(self._h, a)
+        return a.reshape(-1, 8)
+
+    def read_derived(self) -> np.ndarray:
+        a = np.zeros(self.n * 16, np.int32)
+        _lib().or_read_derived(self._h, a)
+        return a.reshape(-1, 16)
+
+    def read_bond(self) -> np.ndarray:
+        a = np.zeros(self.n * 48, np.uint32)
+        _lib().or_read_bond(self._h, a)
+        return a.reshape(-1, 48)
+
+    def write_state_derived(self, state, derived=None):
+        st = np.ascontiguousarray(state, np.int32).reshape(-1)
+        dv = None if derived is None else np.ascontiguousarray(derived, np.int32).reshape(-1)
+        _lib().or_write_state_derived(self._h, st, None if dv is None else dv.ctypes.data)
+
+    def debug_counters(self) -> np.ndarray:
+        a = np.zeros(16, np.uint32)
+        _lib().or_debug_counters(self._h, a)
+        return a
+
+    @property
+    def substep(self) -> int:
+        return int(_lib().or_substep_index(self._h))
+
+    def apply_impulse(self, cx, cy, r, ix, iy, limit=3000) -> int:
+        return int(_lib().or_apply_impulse(self._h, cx, cy, r, ix, iy, limit))
+
+    def spawn(self, fluid8, boundary8) -> int:
+        f = np.ascontiguousarray(fluid8, np.int32).reshape(-1)
+        b = np.ascontiguousarray(boundary8, np.int32).reshape(-1)
+        if b.size == 0:
+            b = np.zeros(1, np.int32)
+        n = int(_lib().or_spawn(self._h, f.size // 8, f, b))
+        if n != self.n:
+            self.nFluid += f.size // 8
+            self.n = n
+        return n
