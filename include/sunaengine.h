@@ -1,0 +1,142 @@
+/* sunaengine.h — C ABI of the MI355X-native deterministic PBF substep.
+ *
+ * The drop-in boundary for the reference engine's hot path: the JS `Engine`
+ * object of src/engine.js (createEngine / loadScene / step / digest / chainHex /
+ * readState / readDerived / applyImpulse / spawnFluid / debugCounters /
+ * destroy) and the replay/verify loops built on it.  Plain pointers and
+ * sizes only.  Every call is synchronous with respect to the caller except
+ * sn_step*, which enqueue device work; every read call waits for it.
+ *
+ * Error convention: functions return 0 on success and a negative SN_E* code
+ * on failure; sn_last_error() then holds a message starting with "[aether]"
+ * for contract violations, mirroring the reference's thrown Error text.
+ *
+ * A "world" is one or more independent scenes resident in HBM and advanced
+ * together (one scene reproduces the reference engine exactly; many scenes
+ * are the batched mode).  Each scene's H_set / H_slot / chain depend only on
+ * that scene's own particles, so a scene's results are identical alone and
+ * inside any batch.
+ */
+#ifndef SUNAENGINE_H
+#define SUNAENGINE_H
+
+#include <stddef.h>
+#include <stdint.h>
+
+#ifdef __cplusplus
+extern "C" {
+#endif
+
+#define SN_OK 0
+#define SN_E_ARG -1         /* invalid argument / contract violation (reference: assert throw) */
+#define SN_E_CAPACITY -2    /* exceeds maxParticles / maxCells / maxScenes */
+#define SN_E_DEVICE -3      /* HIP runtime error */
+#define SN_E_UNSUPPORTED -4 /* a scene reached a material path this build does not run */
+#define SN_E_STATE -5       /* call out of order (e.g. step before load) */
+
+typedef struct sn_engine sn_engine;
+
+/* createEngine options (src/engine.js:2786-2820). */
+typedef struct sn_opts {
+  uint32_t max_particles; /* total particle slots over all scenes of a world */
+  uint32_t max_cells;     /* total grid cells over all scenes */
+  uint32_t max_scenes;    /* scenes per world (1 for the reference engine) */
+  uint32_t iters;         /* density iterations per substep, 1..16 (ITERS = 4) */
+  uint32_t stream_cap;    /* per-scene H ring capacity in substeps (0 = 4096) */
+  uint32_t device;        /* HIP device ordinal */
+} sn_opts;
+
+/* One scene as loadScene writes it (src/engine.js:3888-3937): n records of 8
+ * int32 words {pos.x, pos.y, vel.x, vel.y, matId, flags, temp, fuel}, already
+ * validated and built by the host layer, fluid slots first. */
+typedef struct sn_scene {
+  uint32_t n;
+  uint32_t n_fluid;
+  const int32_t* state8; /* n * 8 words */
+  int32_t dom_w;
+  int32_t dom_h;
+  uint32_t cell_shift;
+  const int32_t* st_acc; /* optional n * 2 words of carried surface-tension
+                            acceleration (Derived.stAcc); NULL = zero */
+} sn_scene;
+
+/* createEngine: allocate every device buffer; luts = the 16010-word frozen
+ * kernel-table image, mats = 64 x 48 material words (src/engine.js:2991-3376). */
+int sn_create(const sn_opts* opts, const int32_t* luts, const int32_t* mats, sn_engine** out);
+void sn_destroy(sn_engine* e);
+const char* sn_last_error(const sn_engine* e);
+
+/* loadScene (src/engine.js:3704-3951): replaces the world with one scene. */
+int sn_load_scene(sn_engine* e, const sn_scene* scene);
+/* Batched form: replaces the world with `count` independent scenes. */
+int sn_load_scenes(sn_engine* e, const sn_scene* scenes, uint32_t count);
+
+/* step(count) (src/engine.js:3963-3999): enqueue exactly `count` substeps of
+ * every scene of the world.  Each substep's 8-word digest of every scene is
+ * appended to that scene's H ring. */
+int sn_step(sn_engine* e, uint32_t count);
+/* Wait for enqueued work; returns SN_E_UNSUPPORTED if a scene left the
+ * supported material paths (the run is then void and must be reloaded). */
+int sn_sync(sn_engine* e);
+
+/* digest() (src/engine.js:4453-4466) of scene `scene`: acc[0..3] = H_set,
+ * acc[4..7] = H_slot, computed from the current state. */
+int sn_read_hash(sn_engine* e, uint32_t scene, uint32_t acc_out[8]);
+/* The per-substep digests appended since the last sn_reset_stream, oldest
+ * first: out holds max_substeps * 8 words, *n_out receives the count copied. */
+int sn_read_hash_stream(sn_engine* e, uint32_t scene, uint32_t* out, size_t max_substeps,
+                        size_t* n_out);
+int sn_reset_stream(sn_engine* e);
+/* chainHex() words (src/engine.js:4406). */
+int sn_read_chain(sn_engine* e, uint32_t scene, uint32_t chain_out[8]);
+/* readState() (src/engine.js:4286-4289): n * 8 words in slot order. */
+int sn_read_state(sn_engine* e, uint32_t scene, int32_t* out, size_t words);
+/* readDerived() (src/engine.js:4290): n * 16 words in slot order. */
+int sn_read_derived(sn_engine* e, uint32_t scene, int32_t* out, size_t words);
+/* debugCounters() (src/engine.js:4293): 16 words. */
+int sn_debug_counters(sn_engine* e, uint32_t scene, uint32_t out[16]);
+/* Substeps encoded since load for `scene` (the reference's `_substep`). */
+int sn_substep_count(sn_engine* e, uint32_t scene, uint32_t* out);
+
+/* applyImpulse(cx, cy, r, ix, iy, limit) (src/engine.js:6077-6098) on the
+ * exact current state of `scene`: fluid slots in ascending order whose
+ * centre is within r (IEEE double test), at most `limit`, get (ix, iy) added
+ * to their velocity, clamped to +-VMAX.  *hits receives the count. */
+int sn_apply_impulse(sn_engine* e, uint32_t scene, int32_t cx, int32_t cy, int32_t r, int32_t ix,
+                     int32_t iy, uint32_t limit, uint32_t* hits);
+
+/* spawnFluid(list, {noWeld}) (src/engine.js:5901-6020) on a one-scene world:
+ * k new fluid records (8 words each) are appended after the fluid block and
+ * the boundary block (given again as nb records) moves up by k slots.
+ * *n_out receives the new n, unchanged when capacity would be exceeded. */
+int sn_spawn_fluid(sn_engine* e, const int32_t* fluid8, uint32_t k, const int32_t* boundary8,
+                   uint32_t nb, uint32_t* n_out);
+
+/* Device-resident input schedule for batched / served runs: `count` stir
+ * events {scene, substep, cx, cy, r, ix, iy, limit}, sorted by (scene,
+ * substep, arming order).  Each is applied as applyImpulse immediately before
+ * its scene's substep `substep` (the push timing of src/demo_boot.js:244-272). */
+typedef struct sn_event {
+  uint32_t scene;
+  uint32_t substep;
+  int32_t cx, cy, r, ix, iy;
+  uint32_t limit;
+} sn_event;
+int sn_set_events(sn_engine* e, const sn_event* events, uint32_t count);
+
+/* Scene activity mask for continuous batching: inactive scenes keep their
+ * state and are not advanced by sn_step. */
+int sn_set_active(sn_engine* e, const uint8_t* active, uint32_t count);
+
+/* Run `count` substeps one at a time with HIP events around each kernel
+ * class on the engine's stream; out (device ms summed over the substeps):
+ * [0] whole substep, [1] predict+count, [2] cell scan, [3] scatter,
+ * [4] canonical gather, [5] neighbour build + iteration-0 density,
+ * [6] density iterations 1.. + colour passes, [7] xsph+normals+conduct,
+ * [8] surface tension + thermal + hash, [9] chain fold. */
+int sn_profile_step(sn_engine* e, uint32_t count, float out[10]);
+
+#ifdef __cplusplus
+}
+#endif
+#endif /* SUNAENGINE_H */

@@ -1,0 +1,157 @@
+// Q16.16 integer core for gfx950 (and the host, for CPU tests of this header).
+//
+// Bit-exact restatement of the reference's fixed-point API (src/fixed.wgsl):
+// fp_mul_s / i64_shr_rne_sat (:146-170), fp_add_sat / fp_sub_sat / fp_neg_sat /
+// fp_abs (:175-200), fp_shr_rne (:205-213), fp_divshift (:285-309), isqrt64
+// (:318-332).  The reference builds 64-bit products from 16-bit halves; on
+// CDNA4 the exact product is one v_mul_lo_u32 + v_mul_hi_i32 pair, and the
+// round-half-to-even shift is folded into one biased 64-bit shift.
+//
+// Debug events (src/fixed.wgsl:62-77) are counted into a per-thread `Dbg`
+// held in registers and flushed once per kernel; the counters are u32 sums,
+// so the totals do not depend on execution order.
+#pragma once
+#include <stdint.h>
+
+#if defined(__HIPCC__) || defined(__HIP__)
+#include <hip/hip_runtime.h>
+#define SN_HD __host__ __device__ __forceinline__
+#else
+#define SN_HD inline
+#endif
+
+namespace sn {
+
+constexpr int32_t I32_MAX = 2147483647;
+constexpr int32_t I32_MIN = -2147483647 - 1;
+
+struct Dbg {
+  uint32_t mul = 0;   // dbg[0] multiply/shift saturation
+  uint32_t div = 0;   // dbg[1] divide saturation / out-of-contract shift
+  uint32_t div0 = 0;  // dbg[2] divide by zero
+  uint32_t add = 0;   // dbg[5] add/sub/neg saturation
+};
+
+SN_HD int32_t wadd(int32_t a, int32_t b) { return (int32_t)((uint32_t)a + (uint32_t)b); }
+SN_HD int32_t wsub(int32_t a, int32_t b) { return (int32_t)((uint32_t)a - (uint32_t)b); }
+SN_HD int32_t wmul(int32_t a, int32_t b) { return (int32_t)((uint32_t)a * (uint32_t)b); }
+SN_HD int32_t clampi(int32_t x, int32_t lo, int32_t hi) { return x < lo ? lo : (x > hi ? hi : x); }
+SN_HD int32_t maxi(int32_t a, int32_t b) { return a > b ? a : b; }
+SN_HD int32_t mini(int32_t a, int32_t b) { return a < b ? a : b; }
+
+// RNE(v / 2^s) saturated to i32, s in 1..31.  floor-shift of v biased by
+// (2^(s-1) - 1 + lsb(floor(v / 2^s))) is exactly round-half-to-even.
+SN_HD int32_t shr_rne_sat64(int64_t v, unsigned s, Dbg& g) {
+  int64_t odd = (v >> s) & 1;
+  int64_t q = (v + ((int64_t)1 << (s - 1)) - 1 + odd) >> s;
+  if ((int64_t)(int32_t)q != q) {
+    g.mul++;
+    return v < 0 ? I32_MIN : I32_MAX;
+  }
+  return (int32_t)q;
+}
+
+SN_HD int32_t fp_mul_s(int32_t a, int32_t b, unsigned s, Dbg& g) {
+  return shr_rne_sat64((int64_t)a * (int64_t)b, s, g);
+}
+
+SN_HD int32_t fp_add_sat(int32_t a, int32_t b, Dbg& g) {
+  int32_t s = wadd(a, b);
+  if (((a ^ s) & (b ^ s)) < 0) { g.add++; return a < 0 ? I32_MIN : I32_MAX; }
+  return s;
+}
+SN_HD int32_t fp_sub_sat(int32_t a, int32_t b, Dbg& g) {
+  int32_t s = wsub(a, b);
+  if (((a ^ b) & (a ^ s)) < 0) { g.add++; return a < 0 ? I32_MIN : I32_MAX; }
+  return s;
+}
+SN_HD int32_t fp_neg_sat(int32_t a, Dbg& g) {
+  if (a == I32_MIN) { g.add++; return I32_MAX; }
+  return -a;
+}
+SN_HD int32_t fp_abs(int32_t a, Dbg& g) { return a < 0 ? fp_neg_sat(a, g) : a; }
+
+// RNE arithmetic shift of an i32 (|result| <= 2^30, cannot overflow).
+SN_HD int32_t fp_shr_rne(int32_t x, unsigned s) {
+  int32_t odd = (x >> s) & 1;
+  return (int32_t)(((int64_t)x + ((int64_t)1 << (s - 1)) - 1 + odd) >> s);
+}
+
+SN_HD int clz32(uint32_t x) {
+#if defined(__HIP_DEVICE_COMPILE__)
+  return x == 0 ? 32 : __clz((int)x);
+#else
+  return x == 0 ? 32 : __builtin_clz(x);
+#endif
+}
+
+// floor(2^30 / dn) for dn in [2^15, 2^16): float estimate, then an exact
+// integer remainder correction (the estimate is within 1 of the quotient).
+SN_HD int32_t recip30(int32_t dn) {
+#if defined(__HIP_DEVICE_COMPILE__)
+  int32_t q = (int32_t)(1073741824.0f * __builtin_amdgcn_rcpf((float)dn));
+#else
+  int32_t q = (int32_t)(1073741824.0f / (float)dn);
+#endif
+  int32_t r = 1073741824 - q * dn;
+  if (r < 0) { q -= 1; r += dn; }
+  if (r < 0) { q -= 1; r += dn; }
+  if (r >= dn) { q += 1; r -= dn; }
+  if (r >= dn) { q += 1; }
+  return q;
+}
+
+// fp_divshift (src/fixed.wgsl:285-309).
+SN_HD int32_t fp_divshift(int32_t num, int32_t den, unsigned k, Dbg& g) {
+  if (den == 0) {
+    g.div0++;
+    if (num == 0) return 0;
+    return num > 0 ? I32_MAX : I32_MIN;
+  }
+  int32_t d = den, nn = num;
+  if (d < 0) { g.div++; d = fp_abs(d, g); nn = fp_neg_sat(nn, g); }
+  int e = 16 - clz32((uint32_t)d);
+  int32_t dn = e >= 0 ? (d >> e) : (d << (-e));
+  int32_t Rn = recip30(dn);
+  int s = 30 + e - (int)k;
+  if (s > 31) { g.div++; return 0; }
+  if (s < 1) {
+    g.div++;
+    if (nn == 0) return 0;
+    return nn > 0 ? I32_MAX : I32_MIN;
+  }
+  return shr_rne_sat64((int64_t)nn * (int64_t)Rn, (unsigned)s, g);
+}
+
+// isqrt64 (src/fixed.wgsl:318-332) for a 32-bit input: floor(sqrt(n)).
+SN_HD uint32_t isqrt32(uint32_t n) {
+  uint32_t rem = 0, root = 0;
+  for (int i = 15; i >= 0; --i) {
+    rem = (rem << 2) | ((n >> (2 * i)) & 3u);
+    uint32_t trial = (root << 2) + 1u;
+    root <<= 1;
+    if (rem >= trial) { rem -= trial; root |= 1u; }
+  }
+  return root;
+}
+
+// ---- MurmurHash3 pieces of the state hash (src/hash/statehash.wgsl:54-76) --
+SN_HD uint32_t rotl32(uint32_t x, uint32_t r) { return (x << (r & 31u)) | (x >> ((32u - r) & 31u)); }
+SN_HD uint32_t mm_step(uint32_t h, uint32_t k) {
+  k *= 0xcc9e2d51u;
+  k = rotl32(k, 15);
+  k *= 0x1b873593u;
+  h ^= k;
+  h = rotl32(h, 13);
+  return h * 5u + 0xe6546b64u;
+}
+SN_HD uint32_t fmix32(uint32_t h) {
+  h ^= h >> 16;
+  h *= 0x85ebca6bu;
+  h ^= h >> 13;
+  h *= 0xc2b2ae35u;
+  h ^= h >> 16;
+  return h;
+}
+
+}  // namespace sn

@@ -1,0 +1,1740 @@
+// MI355X-native deterministic PBF substep (2D) — kernels and C ABI.
+//
+// The reference (src/sim.wgsl, driven by src/engine.js:4003-4155) runs ~60
+// dispatches per substep over particles in SLOT order and rebuilds a
+// canonical neighbour order each substep.  This build keeps the particles
+// permanently in CELL-SORTED order instead: every substep gathers the
+// persistent state into the new (cell, slot-id) order, and all per-particle
+// and pair passes run on that layout, so a particle's neighbours are its
+// neighbours in memory.  The slot id travels with the particle and is what
+// H_slot hashes, so results are independent of the memory order.
+//
+// Why this is bit-exact: every pair accumulator of the 2D solver is a plain
+// wrapping i32 sum of per-term-clamped values (order-free), and the only
+// order-sensitive step — which candidates survive the MAXNBR truncation in
+// buildNbr (src/sim.wgsl:1190-1254) — walks candidates in exactly the
+// reference's canonical order: stencil rows dy = -1..1, cells dx = -1..1,
+// ascending slot id within a cell.  In sorted space that is three contiguous
+// ranges, one per stencil row.
+//
+// Per substep (k density iterations), kernels in order:
+//   predict_count  pred, cell, arrival rank           (src/sim.wgsl:958-1016)
+//   scan x3        exclusive prefix of cell counts      (:1025-1106)
+//   scatter        bucket = arrival order               (:1117-1124)
+//   canon_gather   ascending-id rank within the cell, gather state (:1155-1168)
+//   nbr_solveA     buildNbr fused with iteration 0's solveA (:1190-1318)
+//   colour x4      solveB + applyDp of one colour, one wave per cell group,
+//                  race-free because same-colour neighbours share a cell
+//                  (:1337-1474); the last iteration also finalizes (:2245-2265)
+//   solveA         iterations 1..k-1 (:1270-1318), then colour x4 again
+//   xnc            xsph + normals + conduct (:2300-2582)
+//   st_thermal     surfaceTension + thermal + formation flags + state hash
+//                  (:2409-2707, :3039-3097, src/hash/statehash.wgsl:83-159)
+//   hash_finish    chain fold (statehash.wgsl:164-170), H ring append
+//
+// Material paths: frictionless, bondless rows (no friction/bond/seal/rigid
+// terms — those passes are bit-inert for them, src/sim.wgsl:1698-1702,
+// 2062, 2747, 2830).  A transition into any other row sets the scene's
+// fault word and sn_sync reports SN_E_UNSUPPORTED.
+#include <hip/hip_runtime.h>
+
+#include <algorithm>
+#include <cstdio>
+#include <cstring>
+#include <string>
+#include <vector>
+
+#include "../../include/sunaengine.h"
+#include "fixed.hpp"
+
+using namespace sn;
+
+namespace {
+
+// ---- constants (src/sim.wgsl:55-171) --------------------------------------
+constexpr int32_t H = 163840;
+constexpr int32_t RHO0 = 1048576;
+constexpr int32_t TWO_RHO0 = 2097152;
+constexpr int32_t HS = 10240;
+constexpr int32_t H2S = 104857600;
+constexpr unsigned ACC = 8;
+constexpr uint32_t MAXNBR = 48;
+constexpr uint32_t TN = 3202;
+constexpr int32_t LAM_MAX = 4194304;
+constexpr int32_t S_TERM = 16777216;
+constexpr int32_t DP_TERM = 2097152;
+constexpr int32_t XS_TERM = 2097152;
+constexpr int32_t ST_TERM = 4194304;
+constexpr int32_t RINV_MAX = 262144;
+constexpr int32_t WALL = 32768;
+constexpr int32_t C_MAX = 58000;
+constexpr int32_t RHO_FLOOR_RINV = 32768;
+constexpr int32_t RHO_FLOOR_K = 65536;
+constexpr int32_t AMBIENT = 1310720;
+constexpr int32_t TEMP_MIN = -33554432, TEMP_MAX = 536870912;
+constexpr int32_t DT_CLAMP = 16777216;
+constexpr int32_t HT_TERM = 16777216;
+constexpr int32_t CORR_TERM = 16777216;
+constexpr int32_t CORR_NEVER = 2147483647;
+constexpr uint32_t FLAG_BURNING = 1, FLAG_DEAD = 2, FLAG_ANCHOR = 4, FLAG_RIGID = 16,
+                   FLAG_CONGEAL = 32, FLAG_BURST = 64, FLAG_SEAL = 128;
+constexpr uint32_t WELD_TAG_MASK = 0x000fff00u;
+constexpr uint32_t NBR_SEAL_BIT = 0x80000000u, NBR_EXT_SEAL_BIT = 0x40000000u,
+                   NBR_CONGEAL_BIT = 0x20000000u;
+constexpr uint32_t NBR_BND = 0x80000000u;  // neighbour entry: j is a boundary slot
+constexpr uint32_t MAT_WORDS = 48, MAT_MAX = 64;
+constexpr uint32_t MAT_PHOTO_ROW = 15;
+constexpr uint32_t BOND_PERIOD = 4;
+constexpr uint32_t INACTIVE = 0xffffffffu;
+// per-particle info bits, rebuilt every substep by canon_gather
+constexpr uint32_t INF_DEAD = 1, INF_FLUID = 2, INF_SEAL = 4, INF_CONGEAL = 8;
+// scene fault bits
+constexpr uint32_t FAULT_MATERIAL = 1;
+
+enum MatW {
+  M_EPS = 0, M_DPMAX, M_XSPH, M_PHANTOM, M_GCOH, M_GCUR, M_ADH, M_GRAV,
+  M_COND, M_COOL, M_HEATCAP, M_MELTPT, M_MELTTO, M_FREEZEPT, M_FREEZETO,
+  M_BOILPT, M_BOILTO, M_CONDPT, M_CONDTO, M_IGNITE, M_FUEL0, M_BURNRATE,
+  M_BURNHEAT, M_BURNTO, M_SPLITTO, M_SPLITPCT, M_SPAWNT, M_TINT, M_CORRODE,
+  M_CORRPT, M_CORRTO, M_FRIC, M_BONDK, M_BONDBREAK, M_BONDREFORM, M_BONDFORMV,
+  M_BONDFORMP, M_BONDMU, M_BONDGROUP, M_RAMPLO, M_RAMPHI, M_RAMPINV,
+  M_RAMPSIGN, M_SOLIDMODE, M_TEMPTARGET, M_TEMPDRIVE, M_BURSTP, M_SEALMODE
+};
+
+#define HIPCHECK(x)                                                              \
+  do {                                                                           \
+    hipError_t err_ = (x);                                                       \
+    if (err_ != hipSuccess) {                                                    \
+      throw DevError(std::string(#x) + ": " + hipGetErrorString(err_));          \
+    }                                                                            \
+  } while (0)
+
+struct DevError {
+  std::string msg;
+  explicit DevError(std::string m) : msg(std::move(m)) {}
+};
+
+// ---- per-scene constants and dynamic state ---------------------------------
+struct SceneP {
+  uint32_t n, nFluid, gridW, gridH;
+  uint32_t cellShift, cellBase, cellTotal, partBase;
+  int32_t domW, domH, vmax, posShift;   // posShift = partBase - (particles of earlier scenes)
+  uint32_t weldHold, cap, cellCap, pad;
+};
+struct SceneD {
+  uint32_t substep;    // substeps encoded since load (`_substep`, src/engine.js:3811)
+  uint32_t bondDirty;  // formation runs next substep regardless of the period
+  uint32_t fault;
+  uint32_t streamPos;  // entries written to this scene's H ring since reset
+  uint32_t acc[8];
+  uint32_t chain[8];
+  uint32_t dbg[16];
+  uint32_t evCursor;   // next unapplied stir event of this scene
+  uint32_t evCount;    // events of this scene in the schedule
+  uint32_t evBase;     // first event of this scene in the schedule
+  uint32_t fired;      // substep of the event group being applied (bookkeeping)
+};
+
+struct Event {
+  uint32_t scene, substep;
+  int32_t cx, cy, r, ix, iy;
+  uint32_t limit;
+};
+
+// Device pointers of one world, passed by value to every kernel.
+struct Dev {
+  // persistent state, current cell-sorted order
+  int4* ps;        // pos.x, pos.y, vel.x, vel.y
+  int4* pt;        // matId, flags, temp, fuel
+  uint32_t* id;    // slot index within the scene
+  int2* st;        // stAcc (Derived.stAcc, consumed by the next predict)
+  // this substep's input snapshot, new order
+  int4* psY;
+  int4* ptY;
+  uint32_t* idY;
+  int2* stY;
+  const uint32_t* sidOf;  // sorted position -> scene, INACTIVE if unused
+  // temporaries
+  int2* pred0;     // pred in the old order (predict)
+  uint32_t* cell0; // global cell in the old order
+  uint32_t* rank0; // arrival rank in the cell
+  uint32_t* bucket;
+  uint32_t* bucketId;
+  uint32_t* cellCount;
+  uint32_t* cellStart;
+  uint32_t* blockSums;
+  int2* pred;
+  uint32_t* cellY;
+  uint32_t* info;
+  int32_t* rho;
+  int32_t* lam;
+  int2* dp;
+  int2* vtmp;
+  int32_t* rinv;
+  int2* nrm;
+  int32_t* heat;
+  int32_t* corr;
+  uint32_t* nbr;
+  uint32_t* nbrN;
+  const int4* lut4;       // {W, G, C, S} per LUT bin
+  const int32_t* lutA;    // adhesion table
+  const int32_t* mats;
+  const SceneP* sp;
+  SceneD* sd;
+  uint32_t* hring;        // [S][ringCap][8]
+  uint32_t* hacc;         // scratch accumulators for digest-only [S][8]
+  const Event* events;
+  uint32_t* hitFlag;      // slot-ordered hit flags for stir events
+  uint32_t* hitScan;
+  uint32_t N, C, S, ringCap;
+  uint32_t nEvents;
+};
+
+__device__ __forceinline__ const int32_t* MAT(const Dev& D, uint32_t m) { return D.mats + m * MAT_WORDS; }
+
+// ---- shared helpers (src/sim.wgsl:477-594) -------------------------------
+struct CC { int32_t x, y; };
+__device__ __forceinline__ CC cell_coord(const SceneP& P, int2 p) {
+  int32_t cx = p.x >> P.cellShift;   // originX = originY = 0
+  int32_t cy = p.y >> P.cellShift;
+  return CC{clampi(cx, 0, (int32_t)P.gridW - 1), clampi(cy, 0, (int32_t)P.gridH - 1)};
+}
+__device__ __forceinline__ int32_t lut_index(int2 d, Dbg& g) {
+  int32_t ax = wadd(fp_abs(d.x, g), 8) >> 4;
+  int32_t ay = wadd(fp_abs(d.y, g), 8) >> 4;
+  if (ax > HS || ay > HS) return -1;
+  int32_t r2 = ax * ax + ay * ay;
+  if (r2 >= H2S) return -1;
+  return r2 >> 15;
+}
+__device__ __forceinline__ int2 pred_delta(int2 pi, int2 pj, Dbg& g) {
+  return make_int2(fp_sub_sat(pi.x, pj.x, g), fp_sub_sat(pi.y, pj.y, g));
+}
+__device__ __forceinline__ int32_t grad_q16(int32_t gt, int32_t d, Dbg& g) { return fp_mul_s(gt, d, 26, g); }
+__device__ __forceinline__ int2 wall_clamp(const SceneP& P, int2 p) {
+  return make_int2(clampi(p.x, WALL, P.domW - WALL), clampi(p.y, WALL, P.domH - WALL));
+}
+__device__ __forceinline__ int32_t mat_mix(int32_t a, int32_t b) { return wadd(a, b) >> 1; }
+__device__ __forceinline__ int32_t sym_k(int32_t rhoi, int32_t rhoj, Dbg& g) {
+  int32_t den = maxi(RHO_FLOOR_K, fp_add_sat(rhoi, rhoj, g));
+  return clampi(fp_divshift(TWO_RHO0, den, 16, g), 0, RINV_MAX);
+}
+__device__ __forceinline__ uint32_t id_hash(uint32_t i) {
+  uint32_t x = i + 0x9e3779b9u;
+  x = (x ^ (x >> 16)) * 0x21f0aaadu;
+  x = (x ^ (x >> 15)) * 0x735a2d97u;
+  return x ^ (x >> 15);
+}
+__device__ __forceinline__ uint32_t pick_product(uint32_t primary, uint32_t splitTo, uint32_t pct, uint32_t i) {
+  if (pct == 0u) return primary;
+  if ((id_hash(i) & 255u) < pct) return splitTo;
+  return primary;
+}
+__device__ __forceinline__ bool seal_class(const int32_t* m) {
+  return m[M_BONDK] != 0 && m[M_BONDREFORM] == 0 && m[M_SOLIDMODE] != 0 && m[M_PHANTOM] == 0;
+}
+__device__ __forceinline__ bool fuse_row(const int32_t* m, uint32_t id) {
+  return m[M_BONDK] != 0 && m[M_BONDREFORM] == 0 && m[M_BONDGROUP] >= 0 && id != MAT_PHOTO_ROW;
+}
+// Rows this build advances: every term they take part in is one of the
+// kernels above (frictionless, bondless, not a seal endpoint).
+__device__ __forceinline__ bool supported_row(const int32_t* m) {
+  return m[M_FRIC] == 0 && m[M_BONDK] == 0;
+}
+
+// Fold the per-thread debug counters into the scene's dbg words: one atomic
+// per nonzero counter per wave when the wave is uniform in scene.
+__device__ __forceinline__ void flush_dbg(const Dev& D, uint32_t s, const Dbg& g, uint32_t trunc) {
+  uint32_t v[5] = {g.mul, g.div, g.div0, g.add, trunc};
+  const uint32_t slot[5] = {0, 1, 2, 5, 6};
+#pragma unroll
+  for (int k = 0; k < 5; ++k) {
+    uint32_t x = v[k];
+    if (__any(x != 0)) {
+      if (s != INACTIVE && x) atomicAdd(&D.sd[s].dbg[slot[k]], x);
+    }
+  }
+}
+
+// ===================================================================== predict
+// predict + gridCount (src/sim.wgsl:958-1016) over the old order; the cell
+// counter's atomic return value is the particle's arrival rank in its cell,
+// which replaces the reference's second atomic pass in `scatter`.
+__global__ __launch_bounds__(256) void k_predict_count(Dev D) {
+  uint32_t p = blockIdx.x * 256u + threadIdx.x;
+  if (p >= D.N) return;
+  uint32_t s = D.sidOf[p];
+  if (s == INACTIVE) return;
+  const SceneP& P = D.sp[s];
+  Dbg g;
+  int4 a = D.ps[p];
+  int4 b = D.pt[p];
+  uint32_t id = D.id[p];
+  int2 pos = make_int2(a.x, a.y);
+  int2 pr = pos;
+  if (id < P.nFluid) {
+    const int32_t* M = MAT(D, (uint32_t)b.x);
+    uint32_t fl = (uint32_t)b.y;
+    if (M[M_PHANTOM] == 0 && (fl & FLAG_ANCHOR) == 0u) {
+      int2 st = D.st[p];
+      int32_t vmaxN = fp_neg_sat(P.vmax, g);
+      int32_t vx = fp_add_sat(fp_add_sat(a.z, st.x, g), 0, g);
+      int32_t vy = fp_add_sat(fp_add_sat(fp_add_sat(a.w, st.y, g), M[M_GRAV], g), 0, g);
+      vx = clampi(vx, vmaxN, P.vmax);
+      vy = clampi(vy, vmaxN, P.vmax);
+      pr = wall_clamp(P, make_int2(fp_add_sat(pos.x, vx, g), fp_add_sat(pos.y, vy, g)));
+    }
+  }
+  CC c = cell_coord(P, pr);
+  uint32_t cell = P.cellBase + (uint32_t)c.y * P.gridW + (uint32_t)c.x;
+  D.pred0[p] = pr;
+  D.cell0[p] = cell;
+  D.rank0[p] = atomicAdd(&D.cellCount[cell], 1u);
+  flush_dbg(D, s, g, 0);
+}
+
+// ======================================================================= scan
+// Exclusive prefix of cellCount into cellStart: 4096 cells per block,
+// block totals scanned by one block, then a downsweep that also clears the
+// counts for the next substep.  Integer adds in a fixed tree: exact.
+constexpr uint32_t SCAN_PER_THREAD = 16, SCAN_BLOCK = 256 * SCAN_PER_THREAD;
+
+__device__ __forceinline__ uint32_t block_excl_scan(uint32_t v, uint32_t* sh, uint32_t& total) {
+  // 256 threads, Hillis-Steele in LDS
+  uint32_t t = threadIdx.x;
+  sh[t] = v;
+  __syncthreads();
+  for (uint32_t off = 1; off < 256; off <<= 1) {
+    uint32_t add = t >= off ? sh[t - off] : 0u;
+    __syncthreads();
+    sh[t] += add;
+    __syncthreads();
+  }
+  total = sh[255];
+  uint32_t incl = sh[t];
+  __syncthreads();
+  return incl - v;
+}
+
+__global__ __launch_bounds__(256) void k_scan_reduce(const uint32_t* in, uint32_t n, uint32_t* blockSums) {
+  __shared__ uint32_t sh[256];
+  uint32_t base = blockIdx.x * SCAN_BLOCK + threadIdx.x * SCAN_PER_THREAD;
+  uint32_t s = 0;
+#pragma unroll
+  for (uint32_t k = 0; k < SCAN_PER_THREAD; ++k) s += (base + k < n) ? in[base + k] : 0u;
+  uint32_t total;
+  block_excl_scan(s, sh, total);
+  if (threadIdx.x == 0) blockSums[blockIdx.x] = total;
+}
+
+__global__ __launch_bounds__(256) void k_scan_top(uint32_t* blockSums, uint32_t nb) {
+  __shared__ uint32_t sh[256];
+  uint32_t base = threadIdx.x * SCAN_PER_THREAD;
+  uint32_t v[SCAN_PER_THREAD];
+  uint32_t s = 0;
+#pragma unroll
+  for (uint32_t k = 0; k < SCAN_PER_THREAD; ++k) {
+    v[k] = (base + k < nb) ? blockSums[base + k] : 0u;
+    s += v[k];
+  }
+  uint32_t total;
+  uint32_t run = block_excl_scan(s, sh, total);
+#pragma unroll
+  for (uint32_t k = 0; k < SCAN_PER_THREAD; ++k) {
+    if (base + k < nb) blockSums[base + k] = run;
+    run += v[k];
+  }
+}
+
+__global__ __launch_bounds__(256) void k_scan_down(uint32_t* inout_count, uint32_t n, const uint32_t* blockSums,
+                                                   uint32_t* out, bool clear) {
+  __shared__ uint32_t sh[256];
+  uint32_t base = blockIdx.x * SCAN_BLOCK + threadIdx.x * SCAN_PER_THREAD;
+  uint32_t v[SCAN_PER_THREAD];
+  uint32_t s = 0;
+#pragma unroll
+  for (uint32_t k = 0; k < SCAN_PER_THREAD; ++k) {
+    v[k] = (base + k < n) ? inout_count[base + k] : 0u;
+    s += v[k];
+  }
+  uint32_t total;
+  uint32_t run = block_excl_scan(s, sh, total) + blockSums[blockIdx.x];
+#pragma unroll
+  for (uint32_t k = 0; k < SCAN_PER_THREAD; ++k) {
+    if (base + k < n) {
+      out[base + k] = run;
+      if (clear) inout_count[base + k] = 0u;
+    }
+    run += v[k];
+  }
+  // the running sum after the last element of the last block is the total
+  if (blockIdx.x == gridDim.x - 1 && threadIdx.x == 255) out[n] = run;
+}
+
+// ==================================================================== scatter
+__global__ __launch_bounds__(256) void k_scatter(Dev D) {
+  uint32_t p = blockIdx.x * 256u + threadIdx.x;
+  if (p >= D.N) return;
+  uint32_t s = D.sidOf[p];
+  if (s == INACTIVE) return;
+  int32_t shift = D.sp[s].posShift;
+  uint32_t c = D.cell0[p];
+  uint32_t q = (uint32_t)((int32_t)D.cellStart[c] + shift) + D.rank0[p];
+  D.bucket[q] = p;
+  D.bucketId[q] = D.id[p];
+}
+
+// =============================================================== canon_gather
+// canonicalize (src/sim.wgsl:1155-1168): rank = #{ids in my cell below mine};
+// the particle's new position is cellStart + rank.  The whole persistent
+// record moves there, together with pred and the cell.
+__global__ __launch_bounds__(256) void k_canon_gather(Dev D) {
+  uint32_t t = blockIdx.x * 256u + threadIdx.x;
+  if (t >= D.N) return;
+  uint32_t s = D.sidOf[t];
+  if (s == INACTIVE) return;
+  const SceneP& P = D.sp[s];
+  uint32_t me = D.bucket[t];
+  uint32_t myid = D.bucketId[t];
+  uint32_t c = D.cell0[me];
+  uint32_t lo = (uint32_t)((int32_t)D.cellStart[c] + P.posShift);
+  uint32_t hi = (uint32_t)((int32_t)D.cellStart[c + 1] + P.posShift);
+  uint32_t rank = 0;
+  for (uint32_t u = lo; u < hi; ++u) rank += (D.bucketId[u] < myid) ? 1u : 0u;
+  uint32_t q = lo + rank;
+  int4 pt = D.pt[me];
+  D.psY[q] = D.ps[me];
+  D.ptY[q] = pt;
+  D.idY[q] = myid;
+  D.stY[q] = D.st[me];
+  D.pred[q] = D.pred0[me];
+  D.cellY[q] = c;
+  uint32_t fl = (uint32_t)pt.y;
+  bool fluid = myid < P.nFluid;
+  D.info[q] = ((fl & FLAG_DEAD) ? INF_DEAD : 0u) | (fluid ? INF_FLUID : 0u) |
+              ((fluid && (fl & FLAG_SEAL)) ? INF_SEAL : 0u) |
+              ((fluid && (fl & FLAG_CONGEAL)) ? INF_CONGEAL : 0u);
+}
+
+// ===================================================== buildNbr + solveA(it 0)
+// Accumulate one solveA term (src/sim.wgsl:1281-1301).
+struct SolveAAcc {
+  int32_t rho, S, sgx, sgy;
+};
+__device__ __forceinline__ void solveA_term(SolveAAcc& a, int2 d, int4 L, Dbg& g) {
+  a.rho = wadd(a.rho, L.x);
+  int32_t gx = grad_q16(L.y, d.x, g), gy = grad_q16(L.y, d.y, g);
+  a.sgx = wadd(a.sgx, gx);
+  a.sgy = wadd(a.sgy, gy);
+  a.S = wadd(wadd(a.S, clampi(fp_mul_s(gx, gx, 11, g), 0, S_TERM)), clampi(fp_mul_s(gy, gy, 11, g), 0, S_TERM));
+}
+__device__ __forceinline__ void solveA_finish(const Dev& D, uint32_t p, SolveAAcc& a, int32_t eps, Dbg& g) {
+  a.S = wadd(wadd(a.S, clampi(fp_mul_s(a.sgx, a.sgx, 11, g), 0, S_TERM)),
+             clampi(fp_mul_s(a.sgy, a.sgy, 11, g), 0, S_TERM));
+  int32_t Cq = clampi(fp_shr_rne(wsub(a.rho, RHO0), 4), 0, C_MAX);
+  D.rho[p] = a.rho;
+  D.lam[p] = clampi(fp_neg_sat(fp_divshift(Cq, wadd(a.S, eps), 25, g), g), -LAM_MAX, LAM_MAX);
+}
+
+__global__ __launch_bounds__(256) void k_nbr_solveA(Dev D) {
+  uint32_t p = blockIdx.x * 256u + threadIdx.x;
+  uint32_t s = p < D.N ? D.sidOf[p] : INACTIVE;
+  Dbg g;
+  uint32_t dropped = 0;
+  if (s != INACTIVE) {
+    const SceneP& P = D.sp[s];
+    uint32_t inf = D.info[p];
+    bool fluid = (inf & INF_FLUID) != 0u;
+    SolveAAcc acc{D.lut4[0].x, 0, 0, 0};
+    uint32_t cnt = 0;
+    if (inf & INF_DEAD) {
+      D.nbrN[p] = 0u;
+    } else {
+      int2 pi = D.pred[p];
+      uint32_t lc = D.cellY[p] - P.cellBase;
+      int32_t cx = (int32_t)(lc % P.gridW), cy = (int32_t)(lc / P.gridW);
+      int32_t xlo = maxi(cx - 1, 0), xhi = mini(cx + 1, (int32_t)P.gridW - 1);
+      bool sealSeen = false, congealSeen = false;
+      uint32_t* out = D.nbr + (size_t)p * MAXNBR;
+      for (int32_t dy = -1; dy <= 1; ++dy) {
+        int32_t gy = cy + dy;
+        if (gy < 0 || gy >= (int32_t)P.gridH) continue;
+        uint32_t rowc = P.cellBase + (uint32_t)gy * P.gridW;
+        uint32_t lo = (uint32_t)((int32_t)D.cellStart[rowc + (uint32_t)xlo] + P.posShift);
+        uint32_t hi = (uint32_t)((int32_t)D.cellStart[rowc + (uint32_t)xhi + 1u] + P.posShift);
+        for (uint32_t q = lo; q < hi; ++q) {
+          if (q == p) continue;
+          uint32_t iq = D.info[q];
+          if (iq & INF_DEAD) continue;
+          int2 d = pred_delta(pi, D.pred[q], g);
+          int32_t idx = lut_index(d, g);
+          if (idx < 0) continue;
+          if (iq & INF_SEAL) sealSeen = true;
+          if (iq & INF_CONGEAL) congealSeen = true;
+          if (cnt >= MAXNBR) { dropped++; continue; }
+          out[cnt] = q | ((iq & INF_FLUID) ? 0u : NBR_BND);
+          cnt++;
+          if (fluid) solveA_term(acc, d, D.lut4[idx], g);
+        }
+      }
+      D.nbrN[p] = cnt | (sealSeen ? NBR_SEAL_BIT : 0u) | (congealSeen ? NBR_CONGEAL_BIT : 0u);
+    }
+    if (fluid) solveA_finish(D, p, acc, MAT(D, (uint32_t)D.ptY[p].x)[M_EPS], g);
+  }
+  flush_dbg(D, s, g, dropped);
+}
+
+// ================================================================ solveA (k>0)
+__global__ __launch_bounds__(256) void k_solveA(Dev D) {
+  uint32_t p = blockIdx.x * 256u + threadIdx.x;
+  uint32_t s = p < D.N ? D.sidOf[p] : INACTIVE;
+  Dbg g;
+  if (s != INACTIVE && (D.info[p] & INF_FLUID)) {
+    int2 pi = D.pred[p];
+    uint32_t cnt = D.nbrN[p] & 0x1fffffffu;
+    const uint32_t* nb = D.nbr + (size_t)p * MAXNBR;
+    SolveAAcc acc{D.lut4[0].x, 0, 0, 0};
+    for (uint32_t k = 0; k < cnt; ++k) {
+      uint32_t q = nb[k] & ~NBR_BND;
+      int2 d = pred_delta(pi, D.pred[q], g);
+      int32_t idx = lut_index(d, g);
+      if (idx < 0) continue;
+      solveA_term(acc, d, D.lut4[idx], g);
+    }
+    solveA_finish(D, p, acc, MAT(D, (uint32_t)D.ptY[p].x)[M_EPS], g);
+  }
+  flush_dbg(D, s, g, 0);
+}
+
+// ================================================================ colour pass
+// solveB_at + applyDp_at for one colour (src/sim.wgsl:1337-1474).  One wave
+// per group of up to 8 same-row cells of that colour.  Same-colour
+// neighbours always share a cell (cells of one colour are >= 2 cells apart
+// and the stencil is 3x3), and no other wave reads this group's positions,
+// so computing every dp of the group before writing any pred reproduces the
+// reference's two-dispatch deferral exactly.  On the last iteration the
+// finalize of these particles (src/sim.wgsl:2245-2265) follows in place.
+__device__ __forceinline__ int2 solveB_at(const Dev& D, uint32_t p, Dbg& g) {
+  int2 pi = D.pred[p];
+  int32_t li = D.lam[p];
+  uint32_t cnt = D.nbrN[p] & 0x1fffffffu;
+  const uint32_t* nb = D.nbr + (size_t)p * MAXNBR;
+  int32_t dpx = 0, dpy = 0;
+  for (uint32_t k = 0; k < cnt; ++k) {
+    uint32_t e = nb[k];
+    uint32_t q = e & ~NBR_BND;
+    int2 d = pred_delta(pi, D.pred[q], g);
+    int32_t idx = lut_index(d, g);
+    if (idx < 0) continue;
+    int32_t lj = (e & NBR_BND) ? li : D.lam[q];
+    int4 L = D.lut4[idx];
+    int32_t co = fp_add_sat(fp_add_sat(li, lj, g), L.w, g);
+    dpx = wadd(dpx, clampi(fp_mul_s(co, grad_q16(L.y, d.x, g), 20 - ACC, g), -DP_TERM, DP_TERM));
+    dpy = wadd(dpy, clampi(fp_mul_s(co, grad_q16(L.y, d.y, g), 20 - ACC, g), -DP_TERM, DP_TERM));
+  }
+  return make_int2(dpx, dpy);
+}
+
+__global__ __launch_bounds__(256) void k_colour(Dev D, const uint2* work, uint32_t nwork, uint32_t last) {
+  uint32_t lane = threadIdx.x & 63u;
+  uint32_t w = blockIdx.x * 4u + (threadIdx.x >> 6);
+  if (w >= nwork) return;               // whole wave exits together
+  uint2 item = work[w];                 // {first global cell, ncell | scene << 8}
+  uint32_t ncell = item.y & 0xffu;
+  uint32_t s = item.y >> 8;
+  const SceneP& P = D.sp[s];
+  Dbg g;
+  // lanes < ncell own one cell of the group: its sorted range
+  uint32_t myLo = 0, myCnt = 0;
+  if (lane < ncell) {
+    uint32_t c = item.x + 2u * lane;
+    myLo = (uint32_t)((int32_t)D.cellStart[c] + P.posShift);
+    myCnt = (uint32_t)((int32_t)D.cellStart[c + 1] + P.posShift) - myLo;
+  }
+  // exclusive prefix over the group's cells (ncell <= 8)
+  uint32_t T = 0, myEx = 0;
+  for (uint32_t k = 0; k < ncell; ++k) {
+    uint32_t ck = __shfl(myCnt, (int)k);
+    if (k < lane) myEx += ck;
+    T += ck;
+  }
+  // pass 1: dp of every particle of the group
+  for (uint32_t f0 = 0; f0 < T; f0 += 64u) {
+    uint32_t f = f0 + lane;
+    uint32_t p = 0xffffffffu;
+    for (uint32_t k = 0; k < ncell; ++k) {
+      uint32_t ek = __shfl(myEx, (int)k), ck = __shfl(myCnt, (int)k), lk = __shfl(myLo, (int)k);
+      if (f >= ek && f < ek + ck) p = lk + (f - ek);
+    }
+    if (p != 0xffffffffu && (D.info[p] & INF_FLUID)) D.dp[p] = solveB_at(D, p, g);
+  }
+  // pass 2: apply (and finalize on the last iteration)
+  for (uint32_t f0 = 0; f0 < T; f0 += 64u) {
+    uint32_t f = f0 + lane;
+    uint32_t p = 0xffffffffu;
+    for (uint32_t k = 0; k < ncell; ++k) {
+      uint32_t ek = __shfl(myEx, (int)k), ck = __shfl(myCnt, (int)k), lk = __shfl(myLo, (int)k);
+      if (f >= ek && f < ek + ck) p = lk + (f - ek);
+    }
+    if (p == 0xffffffffu || !(D.info[p] & INF_FLUID)) continue;
+    int4 b = D.ptY[p];
+    uint32_t fl = (uint32_t)b.y;
+    int2 pr = D.pred[p];
+    if ((fl & (FLAG_RIGID | FLAG_ANCHOR)) == 0u) {
+      const int32_t* M = MAT(D, (uint32_t)b.x);
+      int32_t dm = M[M_DPMAX], dmN = fp_neg_sat(dm, g);
+      int2 dp = D.dp[p];
+      int32_t ddx = clampi(fp_shr_rne(dp.x, ACC), dmN, dm);
+      int32_t ddy = clampi(fp_shr_rne(dp.y, ACC), dmN, dm);
+      pr = wall_clamp(P, make_int2(fp_add_sat(pr.x, ddx, g), fp_add_sat(pr.y, ddy, g)));
+      D.pred[p] = pr;
+    }
+    if (last) {
+      int4 a = D.psY[p];
+      int32_t vmaxN = fp_neg_sat(P.vmax, g);
+      int32_t vx = clampi(fp_sub_sat(pr.x, a.x, g), vmaxN, P.vmax);
+      int32_t vy = clampi(fp_sub_sat(pr.y, a.y, g), vmaxN, P.vmax);
+      D.vtmp[p] = make_int2(vx, vy);
+      D.rinv[p] = clampi(fp_divshift(RHO0, maxi(D.rho[p], RHO_FLOOR_RINV), 16, g), 0, RINV_MAX);
+    }
+  }
+  flush_dbg(D, s, g, 0);
+}
+
+// ========================================================= xsph+normals+conduct
+// src/sim.wgsl:2300-2384 and :2528-2582 share one gather over fluid
+// neighbours; each keeps its own accumulators.  Writes the final position and
+// velocity of the substep (boundary slots: position kept, velocity zeroed).
+__global__ __launch_bounds__(256) void k_xnc(Dev D) {
+  uint32_t p = blockIdx.x * 256u + threadIdx.x;
+  uint32_t s = p < D.N ? D.sidOf[p] : INACTIVE;
+  Dbg g;
+  if (s != INACTIVE) {
+    uint32_t inf = D.info[p];
+    if (!(inf & INF_FLUID)) {
+      int4 a = D.psY[p];
+      D.ps[p] = make_int4(a.x, a.y, 0, 0);
+    } else {
+      int4 b = D.ptY[p];
+      uint32_t mi = (uint32_t)b.x;
+      const int32_t* Mi = MAT(D, mi);
+      uint32_t fl = (uint32_t)b.y;
+      bool doX = (fl & (FLAG_RIGID | FLAG_ANCHOR)) == 0u;
+      int32_t ci = Mi[M_XSPH], ki = Mi[M_COND], Ti = b.z, corri = Mi[M_CORRODE];
+      int2 pi = D.pred[p], vi = D.vtmp[p];
+      int32_t rhoi = D.rho[p];
+      uint32_t cnt = D.nbrN[p] & 0x1fffffffu;
+      const uint32_t* nb = D.nbr + (size_t)p * MAXNBR;
+      int32_t ax = 0, ay = 0, sx = 0, sy = 0, q = 0, c = 0;
+      for (uint32_t k = 0; k < cnt; ++k) {
+        uint32_t e = nb[k];
+        if (e & NBR_BND) continue;
+        int2 d = pred_delta(pi, D.pred[e], g);
+        int32_t idx = lut_index(d, g);
+        if (idx < 0) continue;
+        int4 L = D.lut4[idx];
+        int4 bj = D.ptY[e];
+        const int32_t* Mj = MAT(D, (uint32_t)bj.x);
+        if (doX) {
+          int32_t K = sym_k(rhoi, D.rho[e], g);
+          int32_t w = fp_mul_s(L.x, K, 16, g);
+          int32_t cij = mat_mix(ci, Mj[M_XSPH]);
+          int2 vj = D.vtmp[e];
+          ax = wadd(ax, clampi(fp_mul_s(fp_mul_s(fp_sub_sat(vj.x, vi.x, g), w, 20 - ACC, g), cij, 16, g), -XS_TERM, XS_TERM));
+          ay = wadd(ay, clampi(fp_mul_s(fp_mul_s(fp_sub_sat(vj.y, vi.y, g), w, 20 - ACC, g), cij, 16, g), -XS_TERM, XS_TERM));
+        }
+        int32_t rj = D.rinv[e];
+        sx = wadd(sx, fp_mul_s(grad_q16(L.y, d.x, g), rj, 16, g));
+        sy = wadd(sy, fp_mul_s(grad_q16(L.y, d.y, g), rj, 16, g));
+        int32_t cij = mat_mix(ki, Mj[M_COND]);
+        int32_t wk = fp_mul_s(L.x, cij, 16, g);
+        int32_t dT = clampi(fp_sub_sat(bj.z, Ti, g), -DT_CLAMP, DT_CLAMP);
+        q = wadd(q, clampi(fp_mul_s(dT, wk, 20 - ACC, g), -HT_TERM, HT_TERM));
+        if ((uint32_t)bj.x != mi) {
+          int32_t attack = Mj[M_CORRODE];
+          int32_t spend = (Mj[M_CORRPT] != CORR_NEVER) ? corri : 0;
+          int32_t str = wadd(attack, spend);
+          if (str != 0) c = wadd(c, clampi(fp_mul_s(L.x, str, 20 - ACC, g), 0, CORR_TERM));
+        }
+      }
+      int32_t vx = vi.x, vy = vi.y;
+      if (doX) {
+        vx = fp_add_sat(vx, fp_shr_rne(ax, ACC), g);
+        vy = fp_add_sat(vy, fp_shr_rne(ay, ACC), g);
+      }
+      D.ps[p] = make_int4(pi.x, pi.y, vx, vy);
+      D.nrm[p] = make_int2(fp_mul_s(sx, H, 16, g), fp_mul_s(sy, H, 16, g));
+      D.heat[p] = q;
+      D.corr[p] = c;
+    }
+  }
+  flush_dbg(D, s, g, 0);
+}
+
+// ==================================================== surfaceTension + thermal
+// src/sim.wgsl:2409-2494 (stAcc for the next substep), :2612-2707 (thermal,
+// elementwise), bondForm2's flag clear on formation substeps (:3086-3096;
+// bond rows stay empty for bondless rows), then the particle's state hash
+// (src/hash/statehash.wgsl:83-159) reduced per scene.
+__global__ __launch_bounds__(256) void k_st_thermal_hash(Dev D) {
+  uint32_t p = blockIdx.x * 256u + threadIdx.x;
+  uint32_t s = p < D.N ? D.sidOf[p] : INACTIVE;
+  Dbg g;
+  uint32_t words[8];
+  uint32_t myid = 0;
+  if (s != INACTIVE) {
+    const SceneP& P = D.sp[s];
+    const SceneD& SD = D.sd[s];
+    bool formation = ((SD.substep % BOND_PERIOD) == 0u) || SD.bondDirty;
+    uint32_t inf = D.info[p];
+    int4 b = D.ptY[p];
+    myid = D.idY[p];
+    int4 a = D.ps[p];
+    int2 stOut = D.stY[p];
+    if (inf & INF_FLUID) {
+      uint32_t mat0 = (uint32_t)b.x;
+      const int32_t* M = MAT(D, mat0);
+      // ---- surfaceTension
+      int2 pi = D.pred[p], ni = D.nrm[p];
+      int32_t rhoi = D.rho[p];
+      uint32_t cnt = D.nbrN[p] & 0x1fffffffu;
+      const uint32_t* nb = D.nbr + (size_t)p * MAXNBR;
+      int32_t ax = 0, ay = 0;
+      for (uint32_t k = 0; k < cnt; ++k) {
+        uint32_t e = nb[k];
+        uint32_t q = e & ~NBR_BND;
+        int2 d = pred_delta(pi, D.pred[q], g);
+        int32_t idx = lut_index(d, g);
+        if (idx < 0) continue;
+        if (e & NBR_BND) {
+          int32_t at = D.lutA[idx];
+          ax = wsub(ax, clampi(fp_mul_s(fp_mul_s(at, d.x, 22, g), M[M_ADH], 16 - ACC, g), -ST_TERM, ST_TERM));
+          ay = wsub(ay, clampi(fp_mul_s(fp_mul_s(at, d.y, 22, g), M[M_ADH], 16 - ACC, g), -ST_TERM, ST_TERM));
+          continue;
+        }
+        int32_t K = sym_k(rhoi, D.rho[q], g);
+        int32_t ct = D.lut4[idx].z;
+        const int32_t* Mj = MAT(D, (uint32_t)D.ptY[q].x);
+        int32_t gCoh = mat_mix(M[M_GCOH], Mj[M_GCOH]);
+        int32_t gCur = mat_mix(M[M_GCUR], Mj[M_GCUR]);
+        int32_t fx = fp_neg_sat(fp_mul_s(fp_mul_s(ct, d.x, 22, g), gCoh, 16, g), g);
+        int32_t fy = fp_neg_sat(fp_mul_s(fp_mul_s(ct, d.y, 22, g), gCoh, 16, g), g);
+        int2 nj = D.nrm[q];
+        int32_t cx = fp_mul_s(fp_sub_sat(ni.x, nj.x, g), gCur, 16, g);
+        int32_t cy = fp_mul_s(fp_sub_sat(ni.y, nj.y, g), gCur, 16, g);
+        fx = fp_add_sat(fx, fp_neg_sat(cx, g), g);
+        fy = fp_add_sat(fy, fp_neg_sat(cy, g), g);
+        ax = wadd(ax, clampi(fp_mul_s(fx, K, 16 - ACC, g), -ST_TERM, ST_TERM));
+        ay = wadd(ay, clampi(fp_mul_s(fy, K, 16 - ACC, g), -ST_TERM, ST_TERM));
+      }
+      stOut = make_int2(fp_shr_rne(ax, ACC), fp_shr_rne(ay, ACC));
+      // ---- thermal
+      int32_t T = fp_add_sat(b.z, fp_shr_rne(D.heat[p], ACC), g);
+      T = fp_add_sat(T, fp_mul_s(fp_sub_sat(AMBIENT, T, g), M[M_COOL], 16, g), g);
+      int32_t fuel = b.w;
+      uint32_t fl0 = (uint32_t)b.y;
+      bool burning = (fl0 & FLAG_BURNING) != 0u;
+      if (!burning && fuel > 0 && T >= M[M_IGNITE]) burning = true;
+      if (burning) {
+        T = fp_add_sat(T, M[M_BURNHEAT], g);
+        fuel = maxi(0, fp_sub_sat(fuel, M[M_BURNRATE], g));
+      }
+      T = clampi(T, TEMP_MIN, TEMP_MAX);
+      T = fp_add_sat(T, fp_mul_s(fp_sub_sat(M[M_TEMPTARGET], T, g), M[M_TEMPDRIVE], 16, g), g);
+      int32_t dose = fp_shr_rne(D.corr[p], ACC);
+      uint32_t mat = mat0;
+      bool froze = false;
+      uint32_t splitTo = (uint32_t)M[M_SPLITTO], pct = (uint32_t)M[M_SPLITPCT];
+      if (burning && fuel <= 0) mat = pick_product((uint32_t)M[M_BURNTO], splitTo, pct, myid);
+      else if (dose >= M[M_CORRPT]) mat = pick_product((uint32_t)M[M_CORRTO], splitTo, pct, myid);
+      else if (T >= M[M_BOILPT]) mat = pick_product((uint32_t)M[M_BOILTO], splitTo, pct, myid);
+      else if (T >= M[M_MELTPT]) mat = pick_product((uint32_t)M[M_MELTTO], splitTo, pct, myid);
+      else if (T <= M[M_CONDPT]) mat = pick_product((uint32_t)M[M_CONDTO], splitTo, pct, myid);
+      else if (T <= M[M_FREEZEPT]) {
+        mat = pick_product((uint32_t)M[M_FREEZETO], splitTo, pct, myid);
+        froze = true;
+      }
+      const int32_t* Mn = MAT(D, mat);
+      if (mat != mat0) {
+        fuel = Mn[M_FUEL0];
+        burning = false;
+      }
+      uint32_t fl = fl0 & ~(FLAG_BURNING | FLAG_DEAD | FLAG_SEAL);
+      if (mat != mat0) fl &= ~(FLAG_ANCHOR | FLAG_CONGEAL | FLAG_BURST);
+      if (froze && mat != mat0 && fuse_row(Mn, mat)) fl |= FLAG_CONGEAL;
+      if (burning) fl |= FLAG_BURNING;
+      if (Mn[M_PHANTOM] != 0) fl |= FLAG_DEAD;
+      if (seal_class(Mn)) fl |= FLAG_SEAL;
+      if (!supported_row(Mn)) atomicOr(&D.sd[s].fault, FAULT_MATERIAL);
+      // ---- bondForm2's flag update on formation substeps
+      if (formation) {
+        fl &= ~FLAG_CONGEAL;
+        if ((fl & FLAG_RIGID) == 0u && (fl & WELD_TAG_MASK) != P.weldHold) fl &= ~WELD_TAG_MASK;
+      }
+      b = make_int4((int32_t)mat, (int32_t)fl, T, fuel);
+    }
+    D.pt[p] = b;
+    D.id[p] = myid;
+    D.st[p] = stOut;
+    words[0] = (uint32_t)a.x; words[1] = (uint32_t)a.y; words[2] = (uint32_t)a.z; words[3] = (uint32_t)a.w;
+    words[4] = (uint32_t)b.x; words[5] = (uint32_t)b.y; words[6] = (uint32_t)b.z; words[7] = (uint32_t)b.w;
+  }
+  flush_dbg(D, s, g, 0);
+  // ---- state hash: 4 salted lanes over the 8 words, H_set and H_slot terms
+  const uint32_t SEED[4] = {0x9e3779b1u, 0x85ebca77u, 0xc2b2ae3du, 0x27d4eb2fu};
+  const uint32_t SALT[4] = {0x00000000u, 0x51633e2du, 0x9c8f2a17u, 0xd0e5b943u};
+  uint32_t hv[8];
+  if (s != INACTIVE) {
+#pragma unroll
+    for (int l = 0; l < 4; ++l) {
+      uint32_t h = SEED[l];
+#pragma unroll
+      for (int w = 0; w < 8; ++w) h = mm_step(h, words[w] ^ SALT[l]);
+      hv[l] = fmix32(h ^ 8u);
+      hv[4 + l] = fmix32(mm_step(h, myid) ^ 9u);
+    }
+  } else {
+#pragma unroll
+    for (int l = 0; l < 8; ++l) hv[l] = 0u;
+  }
+  // wave-level sum when the wave is one scene, else per-lane atomics
+  uint32_t s0 = __shfl(s, 0);
+  bool uniform = __all(s == s0);
+  if (uniform) {
+    if (s0 == INACTIVE) return;
+#pragma unroll
+    for (int l = 0; l < 8; ++l) {
+      uint32_t v = hv[l];
+      for (int off = 32; off >= 1; off >>= 1) v += __shfl_xor(v, off);
+      hv[l] = v;
+    }
+    if ((threadIdx.x & 63u) == 0u) {
+#pragma unroll
+      for (int l = 0; l < 8; ++l) atomicAdd(&D.sd[s0].acc[l], hv[l]);
+    }
+  } else if (s != INACTIVE) {
+#pragma unroll
+    for (int l = 0; l < 8; ++l) atomicAdd(&D.sd[s].acc[l], hv[l]);
+  }
+}
+
+// Digest of the current state without advancing (digest(), src/engine.js:4453).
+__global__ __launch_bounds__(256) void k_hash_state(Dev D, uint32_t s) {
+  const SceneP& P = D.sp[s];
+  uint32_t t = blockIdx.x * 256u + threadIdx.x;
+  if (t >= P.n) return;
+  uint32_t p = P.partBase + t;
+  const uint32_t SEED[4] = {0x9e3779b1u, 0x85ebca77u, 0xc2b2ae3du, 0x27d4eb2fu};
+  const uint32_t SALT[4] = {0x00000000u, 0x51633e2du, 0x9c8f2a17u, 0xd0e5b943u};
+  int4 a = D.ps[p], b = D.pt[p];
+  uint32_t myid = D.id[p];
+  uint32_t words[8] = {(uint32_t)a.x, (uint32_t)a.y, (uint32_t)a.z, (uint32_t)a.w,
+                       (uint32_t)b.x, (uint32_t)b.y, (uint32_t)b.z, (uint32_t)b.w};
+  for (int l = 0; l < 4; ++l) {
+    uint32_t h = SEED[l];
+    for (int w = 0; w < 8; ++w) h = mm_step(h, words[w] ^ SALT[l]);
+    atomicAdd(&D.hacc[s * 8 + l], fmix32(h ^ 8u));
+    atomicAdd(&D.hacc[s * 8 + 4 + l], fmix32(mm_step(h, myid) ^ 9u));
+  }
+}
+
+// Per scene: fold the chain, append this substep's digest to the H ring,
+// clear the accumulators, advance the substep counter.
+__global__ void k_hash_finish(Dev D, const uint8_t* active) {
+  uint32_t s = blockIdx.x * 64u + threadIdx.x;
+  if (s >= D.S || !active[s]) return;
+  SceneD& SD = D.sd[s];
+  uint32_t a[8];
+  for (int l = 0; l < 8; ++l) a[l] = SD.acc[l];
+  for (int j = 0; j < 8; ++j) SD.chain[j] = mm_step(SD.chain[j], a[j] ^ rotl32(a[(j + 1) & 7], 7));
+  uint32_t slot = SD.streamPos % D.ringCap;
+  uint32_t* r = D.hring + ((size_t)s * D.ringCap + slot) * 8;
+  for (int l = 0; l < 8; ++l) { r[l] = a[l]; SD.acc[l] = 0u; }
+  SD.streamPos++;
+  bool formation = ((SD.substep % BOND_PERIOD) == 0u) || SD.bondDirty;
+  if (formation) SD.bondDirty = 0u;
+  SD.substep++;
+}
+
+// ============================================================ stir events
+// applyImpulse (src/engine.js:6077-6098) for scenes whose next scheduled event
+// is due at their current substep.  k_impulse_mark flags hits in slot order;
+// an exclusive scan over the flags ranks them; k_impulse_apply adds the
+// impulse to the first `limit` hits of each scene.
+__global__ __launch_bounds__(256) void k_impulse_mark(Dev D, uint32_t group) {
+  uint32_t p = blockIdx.x * 256u + threadIdx.x;
+  if (p >= D.N) return;
+  uint32_t s = D.sidOf[p];
+  if (s == INACTIVE) return;
+  const SceneP& P = D.sp[s];
+  const SceneD& SD = D.sd[s];
+  uint32_t id = D.id[p];
+  uint32_t slot = P.partBase + id;
+  uint32_t hit = 0;
+  uint32_t e = SD.evCursor + group;
+  if (id < P.nFluid && e < SD.evBase + SD.evCount) {
+    const Event& E = D.events[e];
+    if (E.substep == SD.substep) {
+      int4 a = D.ps[p];
+      double dx = (double)a.x - (double)E.cx;
+      double dy = (double)a.y - (double)E.cy;
+      // IEEE double, no contraction (built with -ffp-contract=off), as JS
+      double q = dx * dx;
+      q = q + dy * dy;
+      double r2 = (double)E.r * (double)E.r;
+      hit = (q > r2) ? 0u : 1u;
+    }
+  }
+  if (id < P.cap) D.hitFlag[slot] = hit;
+}
+
+__global__ __launch_bounds__(256) void k_impulse_apply(Dev D, uint32_t group) {
+  uint32_t p = blockIdx.x * 256u + threadIdx.x;
+  if (p >= D.N) return;
+  uint32_t s = D.sidOf[p];
+  if (s == INACTIVE) return;
+  const SceneP& P = D.sp[s];
+  const SceneD& SD = D.sd[s];
+  uint32_t id = D.id[p];
+  uint32_t e = SD.evCursor + group;
+  if (id >= P.nFluid || e >= SD.evBase + SD.evCount) return;
+  const Event& E = D.events[e];
+  if (E.substep != SD.substep) return;
+  uint32_t slot = P.partBase + id;
+  if (!D.hitFlag[slot]) return;
+  uint32_t rank = D.hitScan[slot] - D.hitScan[P.partBase];
+  if (rank >= E.limit) return;
+  int4 a = D.ps[p];
+  int64_t vx = (int64_t)a.z + E.ix, vy = (int64_t)a.w + E.iy;
+  vx = vx < -65536 ? -65536 : (vx > 65536 ? 65536 : vx);
+  vy = vy < -65536 ? -65536 : (vy > 65536 ? 65536 : vy);
+  a.z = (int32_t)vx;
+  a.w = (int32_t)vy;
+  D.ps[p] = a;
+}
+
+// Advance each scene's event cursor past the group just applied.
+__global__ void k_impulse_advance(Dev D, const uint8_t* active, uint32_t groups) {
+  uint32_t s = blockIdx.x * 64u + threadIdx.x;
+  if (s >= D.S || !active[s]) return;
+  SceneD& SD = D.sd[s];
+  uint32_t end = SD.evBase + SD.evCount;
+  uint32_t k = 0;
+  while (k < groups && SD.evCursor < end && D.events[SD.evCursor].substep == SD.substep) {
+    SD.evCursor++;
+    k++;
+  }
+  // events scheduled for a substep already passed are dropped in order
+  while (SD.evCursor < end && D.events[SD.evCursor].substep < SD.substep) SD.evCursor++;
+}
+
+// Scatter the sorted state back to slot order for read-back.
+__global__ __launch_bounds__(256) void k_to_slots(Dev D, uint32_t s, int32_t* st8, int32_t* dv16) {
+  const SceneP& P = D.sp[s];
+  uint32_t t = blockIdx.x * 256u + threadIdx.x;
+  if (t >= P.n) return;
+  uint32_t p = P.partBase + t;
+  uint32_t i = D.id[p];
+  int4 a = D.ps[p], b = D.pt[p];
+  int32_t* o = st8 + (size_t)i * 8;
+  o[0] = a.x; o[1] = a.y; o[2] = a.z; o[3] = a.w;
+  o[4] = b.x; o[5] = b.y; o[6] = b.z; o[7] = b.w;
+  if (dv16) {
+    int32_t* d = dv16 + (size_t)i * 16;
+    int2 st = D.st[p];
+    for (int k = 0; k < 16; ++k) d[k] = 0;
+    d[0] = a.x; d[1] = a.y;     // pred == pos after finalize (boundary: always)
+    d[8] = st.x; d[9] = st.y;
+    if (i < P.nFluid && D.sd[s].substep > 0) {
+      int2 vt = D.vtmp[p], nr = D.nrm[p];
+      d[4] = vt.x; d[5] = vt.y;
+      d[6] = nr.x; d[7] = nr.y;
+      d[10] = D.rho[p]; d[11] = D.lam[p]; d[12] = D.rinv[p];
+      d[13] = D.heat[p]; d[14] = D.corr[p];
+    }
+  }
+}
+
+inline uint32_t div_up(uint32_t a, uint32_t b) { return (a + b - 1) / b; }
+
+}  // namespace
+
+// ========================================================================= host
+struct sn_engine {
+  sn_opts opts{};
+  int device = 0;
+  hipStream_t stream = nullptr;
+  std::string err;
+  Dev D{};
+  // host mirrors
+  std::vector<SceneP> sp;
+  std::vector<SceneD> sdInit;
+  std::vector<uint8_t> active;
+  uint8_t* dActive = nullptr;
+  std::vector<uint2> cwork[4];
+  uint2* dWork[4] = {nullptr, nullptr, nullptr, nullptr};
+  uint32_t nWork[4] = {0, 0, 0, 0};
+  uint32_t S = 0, N = 0, C = 0;
+  uint32_t maxEventsPerSubstep = 0;
+  uint32_t colourG = 2;
+  Event* dEvents = nullptr;
+  uint32_t eventCap = 0;
+  std::vector<void*> allocs;
+  hipGraphExec_t graph = nullptr;
+  uint32_t graphSubsteps = 0;
+  bool loaded = false;
+  hipEvent_t ev[10];
+  // last sn_profile_step timings
+};
+
+namespace {
+
+template <class T>
+T* dalloc(sn_engine* e, size_t count) {
+  void* p = nullptr;
+  HIPCHECK(hipMalloc(&p, std::max<size_t>(count, 1) * sizeof(T)));
+  e->allocs.push_back(p);
+  return static_cast<T*>(p);
+}
+
+int fail(sn_engine* e, int code, const std::string& msg) {
+  if (e) e->err = msg;
+  return code;
+}
+
+void drop_graph(sn_engine* e) {
+  if (e->graph) {
+    hipGraphExecDestroy(e->graph);
+    e->graph = nullptr;
+    e->graphSubsteps = 0;
+  }
+}
+
+void enqueue_scan(sn_engine* e, uint32_t* count, uint32_t n, uint32_t* out, bool clear) {
+  uint32_t nb = div_up(std::max(n, 1u), SCAN_BLOCK);
+  hipLaunchKernelGGL(k_scan_reduce, dim3(nb), dim3(256), 0, e->stream, count, n, e->D.blockSums);
+  hipLaunchKernelGGL(k_scan_top, dim3(1), dim3(256), 0, e->stream, e->D.blockSums, nb);
+  hipLaunchKernelGGL(k_scan_down, dim3(nb), dim3(256), 0, e->stream, count, n, e->D.blockSums, out, clear);
+}
+
+// One substep of every active scene, enqueued on the engine stream.
+void enqueue_substep(sn_engine* e, hipEvent_t* marks) {
+  Dev& D = e->D;
+  const uint32_t gN = div_up(std::max(D.N, 1u), 256);
+  hipStream_t st = e->stream;
+  // scheduled stir events due at this substep (applied in arming order)
+  for (uint32_t gi = 0; gi < e->maxEventsPerSubstep; ++gi) {
+    hipLaunchKernelGGL(k_impulse_mark, dim3(gN), dim3(256), 0, st, D, gi);
+    enqueue_scan(e, D.hitFlag, D.N, D.hitScan, false);
+    hipLaunchKernelGGL(k_impulse_apply, dim3(gN), dim3(256), 0, st, D, gi);
+  }
+  if (e->maxEventsPerSubstep)
+    hipLaunchKernelGGL(k_impulse_advance, dim3(div_up(e->S, 64)), dim3(64), 0, st, D, e->dActive,
+                       e->maxEventsPerSubstep);
+  // marks (profiling only): 0 predict | 1 scan | 2 scatter | 3 canon_gather |
+  // 4 nbr_solveA | 5 solveA(k>0) + colour passes | 6 xnc | 7 st_thermal_hash |
+  // 8 hash_finish | 9 end
+  auto mark = [&](int k) {
+    if (marks) hipEventRecord(marks[k], st);
+  };
+  mark(0);
+  hipLaunchKernelGGL(k_predict_count, dim3(gN), dim3(256), 0, st, D);
+  mark(1);
+  enqueue_scan(e, D.cellCount, D.C, D.cellStart, true);
+  mark(2);
+  hipLaunchKernelGGL(k_scatter, dim3(gN), dim3(256), 0, st, D);
+  mark(3);
+  hipLaunchKernelGGL(k_canon_gather, dim3(gN), dim3(256), 0, st, D);
+  mark(4);
+  hipLaunchKernelGGL(k_nbr_solveA, dim3(gN), dim3(256), 0, st, D);
+  mark(5);
+  const uint32_t iters = e->opts.iters;
+  for (uint32_t it = 0; it < iters; ++it) {
+    if (it > 0) hipLaunchKernelGGL(k_solveA, dim3(gN), dim3(256), 0, st, D);
+    for (uint32_t c = 0; c < 4; ++c) {
+      if (e->nWork[c] == 0) continue;
+      hipLaunchKernelGGL(k_colour, dim3(div_up(e->nWork[c], 4)), dim3(256), 0, st, D, e->dWork[c],
+                         e->nWork[c], (uint32_t)(it + 1 == iters));
+    }
+  }
+  mark(6);
+  hipLaunchKernelGGL(k_xnc, dim3(gN), dim3(256), 0, st, D);
+  mark(7);
+  hipLaunchKernelGGL(k_st_thermal_hash, dim3(gN), dim3(256), 0, st, D);
+  mark(8);
+  hipLaunchKernelGGL(k_hash_finish, dim3(div_up(e->S, 64)), dim3(64), 0, st, D, e->dActive);
+  mark(9);
+}
+
+// Colour work: every active scene's cells of colour c, grouped by up to G
+// consecutive same-colour cells of one row.
+void build_colour_work(sn_engine* e, uint32_t G) {
+  for (uint32_t c = 0; c < 4; ++c) {
+    auto& w = e->cwork[c];
+    w.clear();
+    uint32_t a = c & 1u, b = c >> 1;
+    for (uint32_t s = 0; s < e->S; ++s) {
+      if (!e->active[s] || e->sp[s].n == 0) continue;
+      const SceneP& P = e->sp[s];
+      for (uint32_t cy = b; cy < P.gridH; cy += 2) {
+        for (uint32_t cx = a; cx < P.gridW; cx += 2 * G) {
+          uint32_t ncell = std::min(G, (P.gridW - cx + 1) / 2);
+          w.push_back(make_uint2(P.cellBase + cy * P.gridW + cx, ncell | (s << 8)));
+        }
+      }
+    }
+    e->nWork[c] = (uint32_t)w.size();
+  }
+}
+
+void upload_layout(sn_engine* e) {
+  // sidOf: sorted position -> scene for the live range of each active scene
+  std::vector<uint32_t> sid(e->N, INACTIVE);
+  uint32_t before = 0;
+  for (uint32_t s = 0; s < e->S; ++s) {
+    SceneP& P = e->sp[s];
+    P.posShift = (int32_t)P.partBase - (int32_t)before;
+    if (e->active[s]) {
+      for (uint32_t k = 0; k < P.n; ++k) sid[P.partBase + k] = s;
+      before += P.n;
+    }
+  }
+  // inactive scenes keep no cells: their counts stay zero
+  HIPCHECK(hipMemcpyAsync((void*)e->D.sidOf, sid.data(), sizeof(uint32_t) * e->N, hipMemcpyHostToDevice, e->stream));
+  HIPCHECK(hipMemcpyAsync((void*)e->D.sp, e->sp.data(), sizeof(SceneP) * e->S, hipMemcpyHostToDevice, e->stream));
+  HIPCHECK(hipMemcpyAsync(e->dActive, e->active.data(), e->S, hipMemcpyHostToDevice, e->stream));
+  // the cell-count total of all active scenes is what the scan writes at C
+  build_colour_work(e, e->colourG);
+  for (uint32_t c = 0; c < 4; ++c) {
+    if (e->nWork[c])
+      HIPCHECK(hipMemcpyAsync(e->dWork[c], e->cwork[c].data(), sizeof(uint2) * e->nWork[c], hipMemcpyHostToDevice,
+                              e->stream));
+  }
+  HIPCHECK(hipStreamSynchronize(e->stream));
+}
+
+int check_fault(sn_engine* e) {
+  std::vector<SceneD> sd(e->S);
+  HIPCHECK(hipMemcpyAsync(sd.data(), e->D.sd, sizeof(SceneD) * e->S, hipMemcpyDeviceToHost, e->stream));
+  HIPCHECK(hipStreamSynchronize(e->stream));
+  for (uint32_t s = 0; s < e->S; ++s) {
+    if (sd[s].fault & FAULT_MATERIAL) {
+      return fail(e, SN_E_UNSUPPORTED,
+                  "[aether] scene " + std::to_string(s) +
+                      " reached a material with friction, bonds or containment, which this build does not run");
+    }
+  }
+  return SN_OK;
+}
+
+}  // namespace
+
+// =================================================================== C ABI
+extern "C" {
+
+int sn_create(const sn_opts* opts, const int32_t* luts, const int32_t* mats, sn_engine** out) {
+  if (!opts || !luts || !mats || !out) return SN_E_ARG;
+  *out = nullptr;
+  sn_engine* e = new sn_engine();
+  e->opts = *opts;
+  if (e->opts.max_scenes == 0) e->opts.max_scenes = 1;
+  if (e->opts.iters == 0) e->opts.iters = 4;
+  if (e->opts.stream_cap == 0) e->opts.stream_cap = 4096;
+  if (e->opts.iters > 16) {
+    int rc = fail(e, SN_E_ARG, "[aether] iters must be an integer in 1..16");
+    *out = e;
+    return rc;
+  }
+  try {
+    HIPCHECK(hipSetDevice((int)e->opts.device));
+    HIPCHECK(hipStreamCreateWithFlags(&e->stream, hipStreamNonBlocking));
+    for (auto& ev : e->ev) HIPCHECK(hipEventCreate(&ev));
+    const uint32_t NC = e->opts.max_particles, CC = e->opts.max_cells, SC = e->opts.max_scenes;
+    Dev& D = e->D;
+    D.ps = dalloc<int4>(e, NC);
+    D.pt = dalloc<int4>(e, NC);
+    D.id = dalloc<uint32_t>(e, NC);
+    D.st = dalloc<int2>(e, NC);
+    D.psY = dalloc<int4>(e, NC);
+    D.ptY = dalloc<int4>(e, NC);
+    D.idY = dalloc<uint32_t>(e, NC);
+    D.stY = dalloc<int2>(e, NC);
+    D.sidOf = dalloc<uint32_t>(e, NC);
+    D.pred0 = dalloc<int2>(e, NC);
+    D.cell0 = dalloc<uint32_t>(e, NC);
+    D.rank0 = dalloc<uint32_t>(e, NC);
+    D.bucket = dalloc<uint32_t>(e, NC);
+    D.bucketId = dalloc<uint32_t>(e, NC);
+    D.cellCount = dalloc<uint32_t>(e, CC);
+    D.cellStart = dalloc<uint32_t>(e, (size_t)CC + 1);
+    D.blockSums = dalloc<uint32_t>(e, div_up(std::max(CC, NC), SCAN_BLOCK) + 1);
+    D.pred = dalloc<int2>(e, NC);
+    D.cellY = dalloc<uint32_t>(e, NC);
+    D.info = dalloc<uint32_t>(e, NC);
+    D.rho = dalloc<int32_t>(e, NC);
+    D.lam = dalloc<int32_t>(e, NC);
+    D.dp = dalloc<int2>(e, NC);
+    D.vtmp = dalloc<int2>(e, NC);
+    D.rinv = dalloc<int32_t>(e, NC);
+    D.nrm = dalloc<int2>(e, NC);
+    D.heat = dalloc<int32_t>(e, NC);
+    D.corr = dalloc<int32_t>(e, NC);
+    D.nbr = dalloc<uint32_t>(e, (size_t)NC * MAXNBR);
+    D.nbrN = dalloc<uint32_t>(e, NC);
+    D.hitFlag = dalloc<uint32_t>(e, NC);
+    D.hitScan = dalloc<uint32_t>(e, (size_t)NC + 1);
+    int4* lut4 = dalloc<int4>(e, TN);
+    int32_t* lutA = dalloc<int32_t>(e, TN);
+    std::vector<int4> h4(TN);
+    for (uint32_t i = 0; i < TN; ++i) h4[i] = make_int4(luts[i], luts[TN + i], luts[2 * TN + i], luts[4 * TN + i]);
+    HIPCHECK(hipMemcpy(lut4, h4.data(), sizeof(int4) * TN, hipMemcpyHostToDevice));
+    HIPCHECK(hipMemcpy(lutA, luts + 3 * TN, sizeof(int32_t) * TN, hipMemcpyHostToDevice));
+    D.lut4 = lut4;
+    D.lutA = lutA;
+    int32_t* dm = dalloc<int32_t>(e, MAT_MAX * MAT_WORDS);
+    HIPCHECK(hipMemcpy(dm, mats, sizeof(int32_t) * MAT_MAX * MAT_WORDS, hipMemcpyHostToDevice));
+    D.mats = dm;
+    D.sp = dalloc<SceneP>(e, SC);
+    D.sd = dalloc<SceneD>(e, SC);
+    D.hacc = dalloc<uint32_t>(e, (size_t)SC * 8);
+    D.hring = dalloc<uint32_t>(e, (size_t)SC * e->opts.stream_cap * 8);
+    D.ringCap = e->opts.stream_cap;
+    D.events = dalloc<Event>(e, 1);
+    e->dActive = dalloc<uint8_t>(e, SC);
+    // colour work tables: at most one item per cell
+    for (uint32_t c = 0; c < 4; ++c) e->dWork[c] = dalloc<uint2>(e, CC / 2 + SC * 64 + 64);
+    HIPCHECK(hipMemset(D.cellCount, 0, sizeof(uint32_t) * CC));
+  } catch (const DevError& d) {
+    *out = e;
+    return fail(e, SN_E_DEVICE, d.msg);
+  }
+  *out = e;
+  return SN_OK;
+}
+
+void sn_destroy(sn_engine* e) {
+  if (!e) return;
+  hipSetDevice((int)e->opts.device);
+  if (e->stream) hipStreamSynchronize(e->stream);
+  drop_graph(e);
+  for (void* p : e->allocs) hipFree(p);
+  if (e->dEvents) hipFree(e->dEvents);
+  for (auto& ev : e->ev)
+    if (ev) hipEventDestroy(ev);
+  if (e->stream) hipStreamDestroy(e->stream);
+  delete e;
+}
+
+const char* sn_last_error(const sn_engine* e) { return e ? e->err.c_str() : "null engine"; }
+
+int sn_load_scenes(sn_engine* e, const sn_scene* scenes, uint32_t count) {
+  if (!e || (!scenes && count)) return SN_E_ARG;
+  if (count == 0) return fail(e, SN_E_ARG, "[aether] a world needs at least one scene");
+  if (count > e->opts.max_scenes)
+    return fail(e, SN_E_CAPACITY, "[aether] " + std::to_string(count) + " scenes exceed maxScenes " +
+                                      std::to_string(e->opts.max_scenes));
+  uint64_t totN = 0, totC = 0;
+  std::vector<SceneP> sp(count);
+  for (uint32_t s = 0; s < count; ++s) {
+    const sn_scene& sc = scenes[s];
+    if (sc.n_fluid > sc.n) return fail(e, SN_E_ARG, "[aether] scene.nFluid out of range");
+    if (sc.n && !sc.state8) return fail(e, SN_E_ARG, "[aether] scene state missing");
+    if (sc.cell_shift < 1 || sc.cell_shift > 27)
+      return fail(e, SN_E_ARG, "[aether] scene.cellShift outside the 1..27 contract");
+    if ((int64_t(1) << sc.cell_shift) < H)
+      return fail(e, SN_E_ARG, "[aether] cells smaller than the kernel radius");
+    if (sc.dom_w <= 2 * WALL || sc.dom_h <= 2 * WALL)
+      return fail(e, SN_E_ARG, "[aether] domain smaller than the wall band");
+    uint64_t cs = uint64_t(1) << sc.cell_shift;
+    uint32_t gw = (uint32_t)std::max<uint64_t>(1, ((uint64_t)sc.dom_w + cs - 1) / cs);
+    uint32_t gh = (uint32_t)std::max<uint64_t>(1, ((uint64_t)sc.dom_h + cs - 1) / cs);
+    if ((uint64_t)gw * gh > 1048576u)
+      return fail(e, SN_E_ARG, "[aether] cellTotal exceeds the 2-level scan limit 1048576");
+    for (uint32_t i = 0; i < sc.n; ++i) {
+      uint32_t m = (uint32_t)sc.state8[(size_t)i * 8 + 4];
+      if (m >= MAT_MAX) return fail(e, SN_E_ARG, "[aether] matId outside the material table");
+    }
+    SceneP& P = sp[s];
+    std::memset(&P, 0, sizeof(P));
+    P.n = sc.n;
+    P.nFluid = sc.n_fluid;
+    P.gridW = gw;
+    P.gridH = gh;
+    P.cellShift = sc.cell_shift;
+    P.cellTotal = gw * gh;
+    P.cellBase = (uint32_t)totC;
+    P.partBase = (uint32_t)totN;
+    P.cap = sc.n;
+    P.cellCap = gw * gh;
+    P.domW = sc.dom_w;
+    P.domH = sc.dom_h;
+    P.vmax = 65536;
+    P.weldHold = 0;
+    totN += sc.n;
+    totC += (uint64_t)gw * gh;
+  }
+  if (totN > e->opts.max_particles)
+    return fail(e, SN_E_CAPACITY, "[aether] scene.n = " + std::to_string(totN) + " exceeds maxParticles = " +
+                                      std::to_string(e->opts.max_particles));
+  if (totC > e->opts.max_cells)
+    return fail(e, SN_E_CAPACITY, "[aether] cellTotal " + std::to_string(totC) + " exceeds allocated maxCells " +
+                                      std::to_string(e->opts.max_cells));
+  try {
+    HIPCHECK(hipSetDevice((int)e->opts.device));
+    HIPCHECK(hipStreamSynchronize(e->stream));
+    drop_graph(e);
+    // material check: every row present must be one this build runs
+    std::vector<int32_t> mats(MAT_MAX * MAT_WORDS);
+    HIPCHECK(hipMemcpy(mats.data(), e->D.mats, sizeof(int32_t) * mats.size(), hipMemcpyDeviceToHost));
+    for (uint32_t s = 0; s < count; ++s)
+      for (uint32_t i = 0; i < scenes[s].n_fluid; ++i) {
+        uint32_t m = (uint32_t)scenes[s].state8[(size_t)i * 8 + 4];
+        const int32_t* M = &mats[m * MAT_WORDS];
+        if (M[M_FRIC] != 0 || M[M_BONDK] != 0)
+          return fail(e, SN_E_UNSUPPORTED,
+                      "[aether] material row " + std::to_string(m) +
+                          " has friction or bonds; this build runs frictionless, bondless rows");
+      }
+    e->S = count;
+    e->N = (uint32_t)totN;
+    e->C = (uint32_t)totC;
+    e->sp = sp;
+    e->active.assign(count, 1);
+    Dev& D = e->D;
+    D.N = e->N;
+    D.C = e->C;
+    D.S = e->S;
+    // state in slot order (the first substep sorts it)
+    std::vector<int4> ps(e->N), pt(e->N);
+    std::vector<uint32_t> id(e->N);
+    std::vector<int2> st(e->N, make_int2(0, 0));
+    for (uint32_t s = 0; s < count; ++s) {
+      const sn_scene& sc = scenes[s];
+      for (uint32_t i = 0; i < sc.n; ++i) {
+        const int32_t* w = sc.state8 + (size_t)i * 8;
+        uint32_t p = sp[s].partBase + i;
+        ps[p] = make_int4(w[0], w[1], w[2], w[3]);
+        pt[p] = make_int4(w[4], w[5], w[6], w[7]);
+        id[p] = i;
+        if (sc.st_acc) st[p] = make_int2(sc.st_acc[2 * i], sc.st_acc[2 * i + 1]);
+      }
+    }
+    if (e->N) {
+      HIPCHECK(hipMemcpyAsync(D.ps, ps.data(), sizeof(int4) * e->N, hipMemcpyHostToDevice, e->stream));
+      HIPCHECK(hipMemcpyAsync(D.pt, pt.data(), sizeof(int4) * e->N, hipMemcpyHostToDevice, e->stream));
+      HIPCHECK(hipMemcpyAsync(D.id, id.data(), sizeof(uint32_t) * e->N, hipMemcpyHostToDevice, e->stream));
+      HIPCHECK(hipMemcpyAsync(D.st, st.data(), sizeof(int2) * e->N, hipMemcpyHostToDevice, e->stream));
+    }
+    std::vector<SceneD> sd(count);
+    static const uint32_t CHAIN_INIT[8] = {0x243f6a88u, 0x85a308d3u, 0x13198a2eu, 0x03707344u,
+                                           0xa4093822u, 0x299f31d0u, 0x082efa98u, 0xec4e6c89u};
+    for (auto& d : sd) {
+      std::memset(&d, 0, sizeof(d));
+      d.bondDirty = 1;
+      for (int j = 0; j < 8; ++j) d.chain[j] = CHAIN_INIT[j];
+    }
+    HIPCHECK(hipMemcpyAsync(D.sd, sd.data(), sizeof(SceneD) * count, hipMemcpyHostToDevice, e->stream));
+    HIPCHECK(hipMemsetAsync(D.cellCount, 0, sizeof(uint32_t) * std::max(e->C, 1u), e->stream));
+    e->maxEventsPerSubstep = 0;
+    // colour group size: about one wave of particles per group at the
+    // initial occupancy of occupied cells
+    {
+      uint64_t occupied = 0, parts = 0;
+      for (uint32_t s = 0; s < count; ++s) {
+        const sn_scene& sc = scenes[s];
+        std::vector<uint8_t> seen(sp[s].cellTotal, 0);
+        for (uint32_t i = 0; i < sc.n; ++i) {
+          int32_t x = sc.state8[(size_t)i * 8], y = sc.state8[(size_t)i * 8 + 1];
+          int32_t cx = std::min(std::max(x >> sp[s].cellShift, 0), (int32_t)sp[s].gridW - 1);
+          int32_t cy = std::min(std::max(y >> sp[s].cellShift, 0), (int32_t)sp[s].gridH - 1);
+          uint8_t& v = seen[(size_t)cy * sp[s].gridW + cx];
+          occupied += v ? 0 : 1;
+          v = 1;
+        }
+        parts += sc.n;
+      }
+      double avg = occupied ? (double)parts / (double)occupied : 64.0;
+      uint32_t G = (uint32_t)(64.0 / std::max(avg, 1.0) + 0.5);
+      e->colourG = std::min(8u, std::max(1u, G));
+    }
+    upload_layout(e);
+    e->loaded = true;
+  } catch (const DevError& d) {
+    return fail(e, SN_E_DEVICE, d.msg);
+  }
+  return SN_OK;
+}
+
+int sn_load_scene(sn_engine* e, const sn_scene* scene) { return sn_load_scenes(e, scene, 1); }
+
+int sn_step(sn_engine* e, uint32_t count) {
+  if (!e) return SN_E_ARG;
+  if (!e->loaded) return fail(e, SN_E_STATE, "[aether] step() called before loadScene()");
+  if (count == 0) return SN_OK;
+  try {
+    HIPCHECK(hipSetDevice((int)e->opts.device));
+    const uint32_t G = 8;
+    if (!e->graph) {
+      hipGraph_t g;
+      HIPCHECK(hipStreamBeginCapture(e->stream, hipStreamCaptureModeThreadLocal));
+      for (uint32_t k = 0; k < G; ++k) enqueue_substep(e, nullptr);
+      HIPCHECK(hipStreamEndCapture(e->stream, &g));
+      HIPCHECK(hipGraphInstantiate(&e->graph, g, nullptr, nullptr, 0));
+      HIPCHECK(hipGraphDestroy(g));
+      e->graphSubsteps = G;
+    }
+    uint32_t done = 0;
+    while (count - done >= e->graphSubsteps) {
+      HIPCHECK(hipGraphLaunch(e->graph, e->stream));
+      done += e->graphSubsteps;
+    }
+    for (; done < count; ++done) enqueue_substep(e, nullptr);
+    HIPCHECK(hipGetLastError());
+  } catch (const DevError& d) {
+    return fail(e, SN_E_DEVICE, d.msg);
+  }
+  return SN_OK;
+}
+
+int sn_sync(sn_engine* e) {
+  if (!e) return SN_E_ARG;
+  try {
+    HIPCHECK(hipSetDevice((int)e->opts.device));
+    HIPCHECK(hipStreamSynchronize(e->stream));
+    if (!e->loaded) return SN_OK;
+    return check_fault(e);
+  } catch (const DevError& d) {
+    return fail(e, SN_E_DEVICE, d.msg);
+  }
+}
+
+int sn_profile_step(sn_engine* e, uint32_t count, float out[10]) {
+  if (!e || !out) return SN_E_ARG;
+  if (!e->loaded) return fail(e, SN_E_STATE, "[aether] step() called before loadScene()");
+  try {
+    HIPCHECK(hipSetDevice((int)e->opts.device));
+    float acc[10] = {0, 0, 0, 0, 0, 0, 0, 0, 0, 0};
+    for (uint32_t k = 0; k < count; ++k) {
+      enqueue_substep(e, e->ev);
+      HIPCHECK(hipEventSynchronize(e->ev[9]));
+      for (int m = 0; m < 9; ++m) {
+        float t;
+        HIPCHECK(hipEventElapsedTime(&t, e->ev[m], e->ev[m + 1]));
+        acc[1 + m] += t;
+        acc[0] += t;
+      }
+    }
+    for (int k = 0; k < 10; ++k) out[k] = acc[k];
+  } catch (const DevError& d) {
+    return fail(e, SN_E_DEVICE, d.msg);
+  }
+  return check_fault(e);
+}
+
+static int read_sd(sn_engine* e, uint32_t scene, SceneD& out) {
+  if (!e->loaded) return fail(e, SN_E_STATE, "[aether] no scene loaded");
+  if (scene >= e->S) return fail(e, SN_E_ARG, "[aether] scene index out of range");
+  HIPCHECK(hipSetDevice((int)e->opts.device));
+  HIPCHECK(hipStreamSynchronize(e->stream));
+  HIPCHECK(hipMemcpy(&out, e->D.sd + scene, sizeof(SceneD), hipMemcpyDeviceToHost));
+  if (out.fault & FAULT_MATERIAL) return check_fault(e);
+  return SN_OK;
+}
+
+int sn_read_hash(sn_engine* e, uint32_t scene, uint32_t acc_out[8]) {
+  if (!e || !acc_out) return SN_E_ARG;
+  try {
+    SceneD sd;
+    int rc = read_sd(e, scene, sd);
+    if (rc) return rc;
+    Dev& D = e->D;
+    HIPCHECK(hipMemsetAsync(D.hacc, 0, sizeof(uint32_t) * 8 * e->S, e->stream));
+    hipLaunchKernelGGL(k_hash_state, dim3(div_up(std::max(e->sp[scene].n, 1u), 256)), dim3(256), 0, e->stream, D,
+                       scene);
+    HIPCHECK(hipMemcpyAsync(acc_out, D.hacc + 8 * scene, 32, hipMemcpyDeviceToHost, e->stream));
+    HIPCHECK(hipStreamSynchronize(e->stream));
+  } catch (const DevError& d) {
+    return fail(e, SN_E_DEVICE, d.msg);
+  }
+  return SN_OK;
+}
+
+int sn_read_hash_stream(sn_engine* e, uint32_t scene, uint32_t* out, size_t max_substeps, size_t* n_out) {
+  if (!e || (!out && max_substeps) || !n_out) return SN_E_ARG;
+  try {
+    SceneD sd;
+    int rc = read_sd(e, scene, sd);
+    if (rc) return rc;
+    size_t avail = sd.streamPos;
+    if (avail > e->D.ringCap)
+      return fail(e, SN_E_CAPACITY, "[aether] H ring overflowed: read the stream at least every " +
+                                        std::to_string(e->D.ringCap) + " substeps");
+    size_t n = std::min(avail, max_substeps);
+    if (n)
+      HIPCHECK(hipMemcpy(out, e->D.hring + (size_t)scene * e->D.ringCap * 8, n * 32, hipMemcpyDeviceToHost));
+    *n_out = n;
+  } catch (const DevError& d) {
+    return fail(e, SN_E_DEVICE, d.msg);
+  }
+  return SN_OK;
+}
+
+int sn_reset_stream(sn_engine* e) {
+  if (!e) return SN_E_ARG;
+  if (!e->loaded) return fail(e, SN_E_STATE, "[aether] no scene loaded");
+  try {
+    HIPCHECK(hipSetDevice((int)e->opts.device));
+    std::vector<SceneD> sd(e->S);
+    HIPCHECK(hipStreamSynchronize(e->stream));
+    HIPCHECK(hipMemcpy(sd.data(), e->D.sd, sizeof(SceneD) * e->S, hipMemcpyDeviceToHost));
+    for (auto& d : sd) d.streamPos = 0;
+    HIPCHECK(hipMemcpy(e->D.sd, sd.data(), sizeof(SceneD) * e->S, hipMemcpyHostToDevice));
+  } catch (const DevError& d) {
+    return fail(e, SN_E_DEVICE, d.msg);
+  }
+  return SN_OK;
+}
+
+int sn_read_chain(sn_engine* e, uint32_t scene, uint32_t chain_out[8]) {
+  if (!e || !chain_out) return SN_E_ARG;
+  try {
+    SceneD sd;
+    int rc = read_sd(e, scene, sd);
+    if (rc) return rc;
+    for (int j = 0; j < 8; ++j) chain_out[j] = sd.chain[j];
+  } catch (const DevError& d) {
+    return fail(e, SN_E_DEVICE, d.msg);
+  }
+  return SN_OK;
+}
+
+int sn_debug_counters(sn_engine* e, uint32_t scene, uint32_t out[16]) {
+  if (!e || !out) return SN_E_ARG;
+  try {
+    SceneD sd;
+    int rc = read_sd(e, scene, sd);
+    if (rc) return rc;
+    for (int j = 0; j < 16; ++j) out[j] = sd.dbg[j];
+  } catch (const DevError& d) {
+    return fail(e, SN_E_DEVICE, d.msg);
+  }
+  return SN_OK;
+}
+
+int sn_substep_count(sn_engine* e, uint32_t scene, uint32_t* out) {
+  if (!e || !out) return SN_E_ARG;
+  try {
+    SceneD sd;
+    int rc = read_sd(e, scene, sd);
+    if (rc) return rc;
+    *out = sd.substep;
+  } catch (const DevError& d) {
+    return fail(e, SN_E_DEVICE, d.msg);
+  }
+  return SN_OK;
+}
+
+static int read_slots(sn_engine* e, uint32_t scene, int32_t* st8, int32_t* dv16) {
+  const SceneP& P = e->sp[scene];
+  int32_t* dst = nullptr;
+  int32_t* ddv = nullptr;
+  HIPCHECK(hipMalloc((void**)&dst, std::max<size_t>(P.n, 1) * 32));
+  if (dv16) HIPCHECK(hipMalloc((void**)&ddv, std::max<size_t>(P.n, 1) * 64));
+  hipLaunchKernelGGL(k_to_slots, dim3(div_up(std::max(P.n, 1u), 256)), dim3(256), 0, e->stream, e->D, scene,
+                     dst, ddv);
+  if (st8) HIPCHECK(hipMemcpyAsync(st8, dst, (size_t)P.n * 32, hipMemcpyDeviceToHost, e->stream));
+  if (dv16) HIPCHECK(hipMemcpyAsync(dv16, ddv, (size_t)P.n * 64, hipMemcpyDeviceToHost, e->stream));
+  HIPCHECK(hipStreamSynchronize(e->stream));
+  hipFree(dst);
+  if (ddv) hipFree(ddv);
+  return SN_OK;
+}
+
+int sn_read_state(sn_engine* e, uint32_t scene, int32_t* out, size_t words) {
+  if (!e || !out) return SN_E_ARG;
+  try {
+    SceneD sd;
+    int rc = read_sd(e, scene, sd);
+    if (rc) return rc;
+    if (words < (size_t)e->sp[scene].n * 8) return fail(e, SN_E_ARG, "[aether] readState buffer too small");
+    return read_slots(e, scene, out, nullptr);
+  } catch (const DevError& d) {
+    return fail(e, SN_E_DEVICE, d.msg);
+  }
+}
+
+int sn_read_derived(sn_engine* e, uint32_t scene, int32_t* out, size_t words) {
+  if (!e || !out) return SN_E_ARG;
+  try {
+    SceneD sd;
+    int rc = read_sd(e, scene, sd);
+    if (rc) return rc;
+    if (words < (size_t)e->sp[scene].n * 16) return fail(e, SN_E_ARG, "[aether] readDerived buffer too small");
+    return read_slots(e, scene, nullptr, out);
+  } catch (const DevError& d) {
+    return fail(e, SN_E_DEVICE, d.msg);
+  }
+}
+
+int sn_apply_impulse(sn_engine* e, uint32_t scene, int32_t cx, int32_t cy, int32_t r, int32_t ix, int32_t iy,
+                     uint32_t limit, uint32_t* hits) {
+  if (!e || !hits) return SN_E_ARG;
+  try {
+    SceneD sd;
+    int rc = read_sd(e, scene, sd);
+    if (rc) return rc;
+    // a one-event schedule at this scene's current substep, applied now
+    Event ev{scene, sd.substep, cx, cy, r, ix, iy, limit};
+    Event* dev = nullptr;
+    HIPCHECK(hipMalloc((void**)&dev, sizeof(Event)));
+    HIPCHECK(hipMemcpy(dev, &ev, sizeof(Event), hipMemcpyHostToDevice));
+    Dev D = e->D;
+    D.events = dev;
+    // temporarily point this scene's cursor at the single event
+    std::vector<SceneD> all(e->S);
+    HIPCHECK(hipMemcpy(all.data(), e->D.sd, sizeof(SceneD) * e->S, hipMemcpyDeviceToHost));
+    std::vector<SceneD> tmp = all;
+    for (uint32_t s = 0; s < e->S; ++s) {
+      tmp[s].evBase = 0;
+      tmp[s].evCursor = 0;
+      tmp[s].evCount = (s == scene) ? 1u : 0u;
+    }
+    HIPCHECK(hipMemcpy(e->D.sd, tmp.data(), sizeof(SceneD) * e->S, hipMemcpyHostToDevice));
+    uint32_t gN = div_up(std::max(D.N, 1u), 256);
+    hipLaunchKernelGGL(k_impulse_mark, dim3(gN), dim3(256), 0, e->stream, D, 0u);
+    enqueue_scan(e, D.hitFlag, D.N, D.hitScan, false);
+    hipLaunchKernelGGL(k_impulse_apply, dim3(gN), dim3(256), 0, e->stream, D, 0u);
+    HIPCHECK(hipStreamSynchronize(e->stream));
+    const SceneP& P = e->sp[scene];
+    uint32_t total = 0;
+    if (P.n) {
+      uint32_t lastScan = 0, lastFlag = 0, base = 0;
+      HIPCHECK(hipMemcpy(&base, D.hitScan + P.partBase, 4, hipMemcpyDeviceToHost));
+      HIPCHECK(hipMemcpy(&lastScan, D.hitScan + P.partBase + P.n - 1, 4, hipMemcpyDeviceToHost));
+      HIPCHECK(hipMemcpy(&lastFlag, D.hitFlag + P.partBase + P.n - 1, 4, hipMemcpyDeviceToHost));
+      total = lastScan + lastFlag - base;
+    }
+    *hits = std::min(total, limit);
+    HIPCHECK(hipMemcpy(e->D.sd, all.data(), sizeof(SceneD) * e->S, hipMemcpyHostToDevice));
+    hipFree(dev);
+  } catch (const DevError& d) {
+    return fail(e, SN_E_DEVICE, d.msg);
+  }
+  return SN_OK;
+}
+
+int sn_set_events(sn_engine* e, const sn_event* events, uint32_t count) {
+  if (!e || (!events && count)) return SN_E_ARG;
+  if (!e->loaded) return fail(e, SN_E_STATE, "[aether] no scene loaded");
+  try {
+    HIPCHECK(hipSetDevice((int)e->opts.device));
+    HIPCHECK(hipStreamSynchronize(e->stream));
+    std::vector<SceneD> sd(e->S);
+    HIPCHECK(hipMemcpy(sd.data(), e->D.sd, sizeof(SceneD) * e->S, hipMemcpyDeviceToHost));
+    std::vector<Event> ev(count);
+    std::vector<uint32_t> perSub;
+    uint32_t maxGroup = 0;
+    for (uint32_t k = 0; k < count; ++k) {
+      const sn_event& x = events[k];
+      if (x.scene >= e->S) return fail(e, SN_E_ARG, "[aether] event scene out of range");
+      if (k && (x.scene < events[k - 1].scene ||
+                (x.scene == events[k - 1].scene && x.substep < events[k - 1].substep)))
+        return fail(e, SN_E_ARG, "[aether] events must be sorted by (scene, substep)");
+      ev[k] = Event{x.scene, x.substep, x.cx, x.cy, x.r, x.ix, x.iy, x.limit};
+    }
+    for (auto& d : sd) { d.evBase = 0; d.evCount = 0; d.evCursor = 0; }
+    for (uint32_t k = 0; k < count;) {
+      uint32_t s = ev[k].scene, j = k;
+      while (j < count && ev[j].scene == s) ++j;
+      sd[s].evBase = k;
+      sd[s].evCount = j - k;
+      sd[s].evCursor = k;
+      // skip events already in the past of this scene
+      while (sd[s].evCursor < j && ev[sd[s].evCursor].substep < sd[s].substep) sd[s].evCursor++;
+      for (uint32_t a = k; a < j;) {
+        uint32_t b = a;
+        while (b < j && ev[b].substep == ev[a].substep) ++b;
+        maxGroup = std::max(maxGroup, b - a);
+        a = b;
+      }
+      k = j;
+    }
+    if (count > e->eventCap) {
+      if (e->dEvents) hipFree(e->dEvents);
+      e->dEvents = nullptr;
+      HIPCHECK(hipMalloc((void**)&e->dEvents, sizeof(Event) * count));
+      e->eventCap = count;
+    }
+    if (count) HIPCHECK(hipMemcpy(e->dEvents, ev.data(), sizeof(Event) * count, hipMemcpyHostToDevice));
+    e->D.events = e->dEvents;
+    e->D.nEvents = count;
+    HIPCHECK(hipMemcpy(e->D.sd, sd.data(), sizeof(SceneD) * e->S, hipMemcpyHostToDevice));
+    e->maxEventsPerSubstep = maxGroup;
+    drop_graph(e);
+  } catch (const DevError& d) {
+    return fail(e, SN_E_DEVICE, d.msg);
+  }
+  return SN_OK;
+}
+
+int sn_set_active(sn_engine* e, const uint8_t* active, uint32_t count) {
+  if (!e || !active) return SN_E_ARG;
+  if (!e->loaded) return fail(e, SN_E_STATE, "[aether] no scene loaded");
+  if (count != e->S) return fail(e, SN_E_ARG, "[aether] activity mask length != scene count");
+  try {
+    HIPCHECK(hipSetDevice((int)e->opts.device));
+    HIPCHECK(hipStreamSynchronize(e->stream));
+    // Deactivating a scene freezes it in its current order; the sorted layout
+    // of each scene is self-contained, so the others are unaffected.
+    for (uint32_t s = 0; s < count; ++s) e->active[s] = active[s] ? 1 : 0;
+    upload_layout(e);
+    drop_graph(e);
+  } catch (const DevError& d) {
+    return fail(e, SN_E_DEVICE, d.msg);
+  }
+  return SN_OK;
+}
+
+int sn_spawn_fluid(sn_engine* e, const int32_t* fluid8, uint32_t k, const int32_t* boundary8, uint32_t nb,
+                   uint32_t* n_out) {
+  if (!e || !n_out || (k && !fluid8) || (nb && !boundary8)) return SN_E_ARG;
+  if (!e->loaded) return fail(e, SN_E_STATE, "[aether] no scene loaded");
+  if (e->S != 1) return fail(e, SN_E_ARG, "[aether] spawnFluid needs a one-scene world");
+  const SceneP P = e->sp[0];
+  *n_out = P.n;
+  if (k == 0) return SN_OK;
+  if (nb != P.n - P.nFluid) return fail(e, SN_E_ARG, "[aether] spawnFluid: boundary block size mismatch");
+  uint32_t newN = P.n + k;
+  if (newN > e->opts.max_particles) return SN_OK;  // refused deterministically, n unchanged
+  try {
+    SceneD sd;
+    int rc = read_sd(e, 0, sd);
+    if (rc) return rc;
+    std::vector<int32_t> st((size_t)P.n * 8), dv((size_t)P.n * 16);
+    rc = read_slots(e, 0, st.data(), dv.data());
+    if (rc) return rc;
+    std::vector<int32_t> nst((size_t)newN * 8), acc((size_t)newN * 2, 0);
+    std::memcpy(nst.data(), st.data(), (size_t)P.nFluid * 32);
+    std::memcpy(nst.data() + (size_t)P.nFluid * 8, fluid8, (size_t)k * 32);
+    if (nb) std::memcpy(nst.data() + (size_t)(P.nFluid + k) * 8, boundary8, (size_t)nb * 32);
+    for (uint32_t i = 0; i < P.nFluid; ++i) {
+      acc[2 * i] = dv[(size_t)i * 16 + 8];
+      acc[2 * i + 1] = dv[(size_t)i * 16 + 9];
+    }
+    sn_scene sc{newN, P.nFluid + k, nst.data(), P.domW, P.domH, P.cellShift, acc.data()};
+    rc = sn_load_scenes(e, &sc, 1);
+    if (rc) return rc;
+    // carry the engine-side counters across the reload; formation is dirty
+    SceneD nd = sd;
+    nd.bondDirty = 1;
+    nd.evBase = nd.evCount = nd.evCursor = 0;
+    for (int j = 0; j < 8; ++j) nd.acc[j] = 0;
+    HIPCHECK(hipMemcpy(e->D.sd, &nd, sizeof(SceneD), hipMemcpyHostToDevice));
+    *n_out = newN;
+  } catch (const DevError& d) {
+    return fail(e, SN_E_DEVICE, d.msg);
+  }
+  return SN_OK;
+}
+
+}  // extern "C"

@@ -1,0 +1,241 @@
+"""The engine object: the reference's createEngine()/Engine API over the C ABI.
+
+Mirrors ``createEngine`` / ``Engine`` of ``src/engine.js`` (``:2786-3518``,
+``:3704-4480``, ``:5901-6098``): same method names in snake_case, same
+argument meaning, same refusal rules (``SceneError`` for what loadScene
+asserts, ``SnError`` for engine-side failures).  All compute runs in the HIP
+library; this module only validates, packs arrays and reads results.
+"""
+from __future__ import annotations
+
+import ctypes as C
+from typing import Iterable, List, Optional, Sequence
+
+import numpy as np
+
+from . import materials as mt
+from . import native
+from . import scenes as sc
+from . import tables
+
+ITERS = 4
+MAXNBR = 48
+PARTICLE_WORDS = 8
+DERIVED_WORDS = 16
+CHAIN_INIT = [0x243F6A88, 0x85A308D3, 0x13198A2E, 0x03707344,
+              0xA4093822, 0x299F31D0, 0x082EFA98, 0xEC4E6C89]
+
+
+def hex8(words) -> str:
+    return "".join("%08x" % (int(w) & 0xFFFFFFFF) for w in words)
+
+
+def digest_dict(acc) -> dict:
+    acc = np.asarray(acc, dtype=np.uint32)
+    return {"acc": acc, "hex": hex8(acc), "setHex": hex8(acc[:4]), "slotHex": hex8(acc[4:])}
+
+
+class Engine:
+    """One world of one or more independent 2D scenes resident on one GPU."""
+
+    def __init__(self, max_particles: int = 20000, max_cells: int = 65536, iters: int = ITERS,
+                 max_scenes: int = 1, stream_cap: int = 4096, device: int = 0, mats=None, luts=None):
+        if not (isinstance(iters, int) and 1 <= iters <= 16):
+            raise sc.SceneError(f"iters = {iters} must be an integer in 1..16 (shipped default {ITERS})")
+        self.mats_rows = mt.MATS if mats is None else mats
+        self._mats_img = mt.mats_image(self.mats_rows)
+        self._luts = tables.load_luts() if luts is None else np.ascontiguousarray(luts, np.int32)
+        self.max_particles, self.max_cells, self.iters = max_particles, max_cells, iters
+        self.max_scenes = max_scenes
+        L = native.lib()
+        o = native.sn_opts(max_particles, max_cells, max_scenes, iters, stream_cap, device)
+        h = C.c_void_p()
+        rc = L.sn_create(C.byref(o), self._luts.ctypes.data, self._mats_img.ctypes.data, C.byref(h))
+        self._h = h
+        if rc != native.SN_OK:
+            msg = L.sn_last_error(h).decode() if h else f"sn_create failed ({rc})"
+            if h:
+                L.sn_destroy(h)
+            self._h = None
+            raise native.SnError(rc, msg)
+        self.scenes: List[sc.Scene] = []
+        self.images: List[sc.LoadImage] = []
+
+    # ------------------------------------------------------------ lifecycle
+    def destroy(self):
+        if getattr(self, "_h", None):
+            native.lib().sn_destroy(self._h)
+            self._h = None
+
+    __del__ = destroy
+
+    def _check(self, rc):
+        native.check(self._h, rc)
+
+    # ----------------------------------------------------------------- load
+    def load_scene(self, scene: sc.Scene, st_acc: Optional[np.ndarray] = None):
+        """loadScene (src/engine.js:3704): validate, build the state words, upload."""
+        return self.load_scenes([scene], None if st_acc is None else [st_acc])
+
+    def load_scenes(self, scenes: Sequence[sc.Scene], st_accs=None):
+        imgs = [sc.load_image(s, self.mats_rows, max_particles=self.max_particles,
+                              max_cells=self.max_cells) for s in scenes]
+        return self._upload(list(scenes), imgs, st_accs)
+
+    def _upload(self, scenes, imgs, st_accs=None):
+        arr = (native.sn_scene * len(imgs))()
+        keep = []
+        for k, im in enumerate(imgs):
+            st = np.ascontiguousarray(im.state, np.int32)
+            keep.append(st)
+            acc_ptr = None
+            if st_accs is not None and st_accs[k] is not None:
+                a = np.ascontiguousarray(st_accs[k], np.int32).reshape(-1)
+                keep.append(a)
+                acc_ptr = a.ctypes.data
+            arr[k] = native.sn_scene(im.n, im.nFluid, st.ctypes.data, im.domW, im.domH,
+                                     im.cellShift, acc_ptr)
+        self._check(native.lib().sn_load_scenes(self._h, arr, len(imgs)))
+        self.scenes, self.images = scenes, imgs
+        return self
+
+    @property
+    def n(self) -> int:
+        return self.images[0].n
+
+    @property
+    def n_fluid(self) -> int:
+        return self.images[0].nFluid
+
+    # ----------------------------------------------------------------- step
+    def step(self, count: int = 1) -> int:
+        """step(count) (src/engine.js:3963): exactly `count` substeps, enqueued."""
+        if not isinstance(count, (int, np.integer)) or count < 0:
+            raise sc.SceneError("step(count) needs a non-negative integer")
+        if not self.images:
+            raise sc.SceneError("step() called before loadScene()")
+        self._check(native.lib().sn_step(self._h, int(count)))
+        return self.substep_count()
+
+    def sync(self):
+        self._check(native.lib().sn_sync(self._h))
+
+    def check_errors(self):
+        self.sync()
+
+    def substep_count(self, scene: int = 0) -> int:
+        v = C.c_uint32()
+        self._check(native.lib().sn_substep_count(self._h, scene, C.byref(v)))
+        return int(v.value)
+
+    # ------------------------------------------------------------- readback
+    def digest(self, scene: int = 0) -> dict:
+        """digest() (src/engine.js:4453): H_set ‖ H_slot of the current state."""
+        a = np.zeros(8, np.uint32)
+        self._check(native.lib().sn_read_hash(self._h, scene, a.ctypes.data))
+        return digest_dict(a)
+
+    def chain(self, scene: int = 0) -> np.ndarray:
+        a = np.zeros(8, np.uint32)
+        self._check(native.lib().sn_read_chain(self._h, scene, a.ctypes.data))
+        return a
+
+    def chain_hex(self, scene: int = 0) -> str:
+        return hex8(self.chain(scene))
+
+    def hash_stream(self, scene: int = 0) -> np.ndarray:
+        """Per-substep 8-word digests since the last reset_stream(), uint32 [k, 8]."""
+        cap = self.substep_count(scene) + 1
+        out = np.zeros((max(cap, 1), 8), np.uint32)
+        n = C.c_size_t()
+        self._check(native.lib().sn_read_hash_stream(self._h, scene, out.ctypes.data, cap,
+                                                     C.byref(n)))
+        return out[: n.value]
+
+    def reset_stream(self):
+        self._check(native.lib().sn_reset_stream(self._h))
+
+    def read_state(self, scene: int = 0) -> np.ndarray:
+        n = self.images[scene].n if scene < len(self.images) else 0
+        out = np.zeros((max(n, 1), 8), np.int32)
+        self._check(native.lib().sn_read_state(self._h, scene, out.ctypes.data, out.size))
+        return out[:n]
+
+    def read_derived(self, scene: int = 0) -> np.ndarray:
+        n = self.images[scene].n if scene < len(self.images) else 0
+        out = np.zeros((max(n, 1), 16), np.int32)
+        self._check(native.lib().sn_read_derived(self._h, scene, out.ctypes.data, out.size))
+        return out[:n]
+
+    def debug_counters(self, scene: int = 0) -> np.ndarray:
+        a = np.zeros(16, np.uint32)
+        self._check(native.lib().sn_debug_counters(self._h, scene, a.ctypes.data))
+        return a
+
+    def assert_spec_valid_regime(self, where: str = "run", scene: int = 0) -> bool:
+        """assertSpecValidRegime (src/engine.js:4427-4450)."""
+        names = ["mul/shift saturation", "divide saturation or bad shift", "divide by zero",
+                 "sqrt of negative", "rsqrt of non-positive", "add/sub/neg saturation",
+                 "neighbour-list truncation", "cell/scan capacity overflow", "bond row saturation"]
+        dbg = self.debug_counters(scene)
+        hit = [f"{names[i] if i < len(names) else f'RESERVED dbg[{i}]'} (dbg[{i}]={int(v)})"
+               for i, v in enumerate(dbg) if v]
+        if hit:
+            raise sc.SceneError(f"[aether] {where}: simulation left the spec-valid regime — "
+                                + "; ".join(hit))
+        return True
+
+    # --------------------------------------------------------------- inputs
+    def apply_impulse(self, cx, cy, radius, ix, iy, limit: int = 3000, scene: int = 0) -> int:
+        """applyImpulse (src/engine.js:6077) on the exact current state."""
+        hits = C.c_uint32()
+        self._check(native.lib().sn_apply_impulse(self._h, scene, int(cx), int(cy), int(radius),
+                                                  int(ix), int(iy), int(limit), C.byref(hits)))
+        return int(hits.value)
+
+    def spawn_fluid(self, particles: Iterable[dict], no_weld: bool = True) -> int:
+        """spawnFluid(list, {noWeld}) (src/engine.js:5901); returns the new n."""
+        plist = list(particles)
+        if not plist:
+            return self.n
+        scene = self.scenes[0]
+        fl, bd = sc.spawn_records(scene, plist, self.mats_rows)
+        n_out = C.c_uint32()
+        fl = np.ascontiguousarray(fl, np.int32)
+        bd = np.ascontiguousarray(bd, np.int32)
+        self._check(native.lib().sn_spawn_fluid(self._h, fl.ctypes.data, len(plist),
+                                                bd.ctypes.data if bd.size else None,
+                                                bd.shape[0], C.byref(n_out)))
+        new_n = int(n_out.value)
+        if new_n != self.images[0].n:
+            im = self.images[0]
+            im.n, im.nFluid = new_n, im.nFluid + len(plist)
+        return new_n
+
+    def set_events(self, events: Sequence[tuple]):
+        """Device-resident stir schedule: (scene, substep, cx, cy, r, ix, iy[, limit])."""
+        evs = sorted(((e[0], e[1], k, e) for k, e in enumerate(events)))
+        arr = (native.sn_event * max(len(evs), 1))()
+        for j, (_, _, _, e) in enumerate(evs):
+            lim = e[7] if len(e) > 7 else 3000
+            arr[j] = native.sn_event(e[0], e[1], e[2], e[3], e[4], e[5], e[6], lim)
+        self._check(native.lib().sn_set_events(self._h, arr, len(evs)))
+
+    def set_active(self, mask: Sequence[bool]):
+        m = np.ascontiguousarray(np.asarray(mask, dtype=np.uint8))
+        self._check(native.lib().sn_set_active(self._h, m.ctypes.data, m.size))
+
+    PROFILE_KEYS = ("total", "predict", "scan", "scatter", "canon_gather", "nbr_solveA",
+                    "solve", "xnc", "st_thermal_hash", "hash_finish")
+
+    def profile_step(self, count: int = 1) -> dict:
+        """Device ms per kernel class, summed over `count` substeps."""
+        out = np.zeros(10, np.float32)
+        self._check(native.lib().sn_profile_step(self._h, int(count), out.ctypes.data))
+        return {k: float(v) for k, v in zip(self.PROFILE_KEYS, out)}
+
+
+def create_engine(**opts) -> Engine:
+    """createEngine(opts) with the reference's option names where they exist."""
+    m = {"maxParticles": "max_particles", "maxCells": "max_cells", "maxScenes": "max_scenes"}
+    return Engine(**{m.get(k, k): v for k, v in opts.items()})

@@ -1,0 +1,105 @@
+"""ctypes binding of the C ABI in include/sunaengine.h.
+
+The shared library is the product: there is no CPU fallback.  If
+libsunaengine_amd.so is missing or cannot be loaded, every entry point
+raises NativeMissing with the reason.
+"""
+from __future__ import annotations
+
+import ctypes as C
+import os
+
+HERE = os.path.dirname(os.path.abspath(__file__))
+LIB_PATH = os.path.join(HERE, "libsunaengine_amd.so")
+
+# error codes (include/sunaengine.h)
+SN_OK, SN_E_ARG, SN_E_CAPACITY, SN_E_DEVICE, SN_E_UNSUPPORTED, SN_E_STATE = 0, -1, -2, -3, -4, -5
+
+#: every symbol the header declares; tests check the library exports them all
+EXPORTS = [
+    "sn_create", "sn_destroy", "sn_last_error", "sn_load_scene", "sn_load_scenes", "sn_step",
+    "sn_sync", "sn_read_hash", "sn_read_hash_stream", "sn_reset_stream", "sn_read_chain",
+    "sn_read_state", "sn_read_derived", "sn_debug_counters", "sn_substep_count",
+    "sn_apply_impulse", "sn_spawn_fluid", "sn_set_events", "sn_set_active", "sn_profile_step",
+]
+
+
+class NativeMissing(RuntimeError):
+    pass
+
+
+class SnError(RuntimeError):
+    """A failed sn_* call; `code` is the SN_E_* value, the message is sn_last_error()."""
+
+    def __init__(self, code: int, msg: str):
+        super().__init__(msg)
+        self.code = code
+
+
+class sn_opts(C.Structure):
+    _fields_ = [("max_particles", C.c_uint32), ("max_cells", C.c_uint32),
+                ("max_scenes", C.c_uint32), ("iters", C.c_uint32),
+                ("stream_cap", C.c_uint32), ("device", C.c_uint32)]
+
+
+class sn_scene(C.Structure):
+    _fields_ = [("n", C.c_uint32), ("n_fluid", C.c_uint32), ("state8", C.c_void_p),
+                ("dom_w", C.c_int32), ("dom_h", C.c_int32), ("cell_shift", C.c_uint32),
+                ("st_acc", C.c_void_p)]
+
+
+class sn_event(C.Structure):
+    _fields_ = [("scene", C.c_uint32), ("substep", C.c_uint32), ("cx", C.c_int32),
+                ("cy", C.c_int32), ("r", C.c_int32), ("ix", C.c_int32), ("iy", C.c_int32),
+                ("limit", C.c_uint32)]
+
+
+_LIB = None
+
+
+def lib():
+    """Load the library once; raise NativeMissing if it is not built."""
+    global _LIB
+    if _LIB is not None:
+        return _LIB
+    if not os.path.exists(LIB_PATH):
+        raise NativeMissing(f"{LIB_PATH} is not built; run `python -m sunaengine_amd.build`")
+    try:
+        L = C.CDLL(LIB_PATH)
+    except OSError as e:
+        raise NativeMissing(f"cannot load {LIB_PATH}: {e}") from e
+    P, u32, i32, sz = C.c_void_p, C.c_uint32, C.c_int32, C.c_size_t
+    sig = {
+        "sn_create": (C.c_int, [C.POINTER(sn_opts), P, P, C.POINTER(P)]),
+        "sn_destroy": (None, [P]),
+        "sn_last_error": (C.c_char_p, [P]),
+        "sn_load_scene": (C.c_int, [P, C.POINTER(sn_scene)]),
+        "sn_load_scenes": (C.c_int, [P, P, u32]),
+        "sn_step": (C.c_int, [P, u32]),
+        "sn_sync": (C.c_int, [P]),
+        "sn_read_hash": (C.c_int, [P, u32, P]),
+        "sn_read_hash_stream": (C.c_int, [P, u32, P, sz, C.POINTER(sz)]),
+        "sn_reset_stream": (C.c_int, [P]),
+        "sn_read_chain": (C.c_int, [P, u32, P]),
+        "sn_read_state": (C.c_int, [P, u32, P, sz]),
+        "sn_read_derived": (C.c_int, [P, u32, P, sz]),
+        "sn_debug_counters": (C.c_int, [P, u32, P]),
+        "sn_substep_count": (C.c_int, [P, u32, C.POINTER(u32)]),
+        "sn_apply_impulse": (C.c_int, [P, u32, i32, i32, i32, i32, i32, u32, C.POINTER(u32)]),
+        "sn_spawn_fluid": (C.c_int, [P, P, u32, P, u32, C.POINTER(u32)]),
+        "sn_set_events": (C.c_int, [P, P, u32]),
+        "sn_set_active": (C.c_int, [P, P, u32]),
+        "sn_profile_step": (C.c_int, [P, u32, P]),
+    }
+    for name, (res, args) in sig.items():
+        f = getattr(L, name)
+        f.restype = res
+        f.argtypes = args
+    _LIB = L
+    return L
+
+
+def check(handle, rc: int) -> None:
+    if rc != SN_OK:
+        msg = lib().sn_last_error(handle)
+        raise SnError(rc, (msg or b"").decode() or f"sn error {rc}")
