@@ -150,3 +150,85 @@ class OracleWorld:
             self.nFluid += f.size // 8
             self.n = n
         return n
+
+
+def _lib3():
+    L = _lib()
+    if not hasattr(L, "_or3_ready"):
+        P = C.c_void_p
+        u32p = np.ctypeslib.ndpointer(np.uint32, flags="C")
+        i32p = np.ctypeslib.ndpointer(np.int32, flags="C")
+        L.or3_create.restype = P
+        L.or3_create.argtypes = [C.c_uint32, C.c_uint32, i32p, C.c_uint32, C.c_int]
+        L.or3_destroy.argtypes = [P]
+        L.or3_load.restype = C.c_int
+        L.or3_load.argtypes = [P, C.c_uint32, i32p, C.c_uint32, C.c_uint32, C.c_uint32]
+        L.or3_set_inputs.argtypes = [P, C.c_int32, C.c_int32, C.c_uint32, C.c_void_p, C.c_int32]
+        L.or3_step.argtypes = [P, C.c_uint32, C.c_void_p]
+        L.or3_digest.argtypes = [P, u32p]
+        L.or3_chain.argtypes = [P, u32p]
+        L.or3_read.argtypes = [P, i32p]
+        L.or3_debug_counters.argtypes = [P, u32p]
+        L.or3_fnv_digest.argtypes = [P, u32p]
+        L._or3_ready = True
+    return L
+
+
+class OracleWorld3:
+    """One 3D world on the CPU oracle (3d/sim3d.wgsl restated)."""
+
+    def __init__(self, scene3, luts3, iters: int = 3, max_particles=None, max_cells=None):
+        L = _lib3()
+        self.n = scene3.n
+        cells = scene3.gx * scene3.gy * scene3.gz
+        lut = np.ascontiguousarray(luts3, np.int32)
+        self._h = L.or3_create(int(max_particles or self.n), int(max_cells or cells), lut, lut.size,
+                               int(iters))
+        rc = L.or3_load(self._h, self.n, np.ascontiguousarray(scene3.parts, np.int32).reshape(-1),
+                        scene3.gx, scene3.gy, scene3.gz)
+        if rc:
+            raise ValueError("oracle3: scene exceeds capacity")
+
+    def close(self):
+        if self._h:
+            _lib3().or3_destroy(self._h)
+            self._h = None
+
+    __del__ = close
+
+    def set_inputs(self, stir_amp=0, stir_phase=0, ball_on=0, ball=(0, 0, 0), ball_reach=0):
+        b = (C.c_int32 * 3)(*ball)
+        _lib3().or3_set_inputs(self._h, stir_amp, stir_phase, ball_on, C.cast(b, C.c_void_p), ball_reach)
+
+    def step(self, count: int, stream: bool = False):
+        if stream:
+            out = np.zeros((count, 8), np.uint32)
+            _lib3().or3_step(self._h, count, out.ctypes.data)
+            return out
+        _lib3().or3_step(self._h, count, None)
+        return None
+
+    def digest(self):
+        a = np.zeros(8, np.uint32)
+        _lib3().or3_digest(self._h, a)
+        return a
+
+    def chain(self):
+        a = np.zeros(8, np.uint32)
+        _lib3().or3_chain(self._h, a)
+        return a
+
+    def read(self):
+        a = np.zeros(self.n * 16, np.int32)
+        _lib3().or3_read(self._h, a)
+        return a.reshape(-1, 16)
+
+    def debug_counters(self):
+        a = np.zeros(16, np.uint32)
+        _lib3().or3_debug_counters(self._h, a)
+        return a
+
+    def fnv_digest(self):
+        a = np.zeros(8, np.uint32)
+        _lib3().or3_fnv_digest(self._h, a)
+        return a

@@ -1,0 +1,90 @@
+// Exclusive prefix sums over u32 counts (cell counts, hit flags), shared by
+// the 2D and 3D engines.  4096 elements per block; the block totals are
+// scanned by one block (so up to 16M elements), then a downsweep writes the
+// exclusive prefix and the grand total at out[n], optionally clearing the
+// counts for the next use.  Integer adds in a fixed tree: exact, and the
+// result does not depend on scheduling.
+#pragma once
+#include <hip/hip_runtime.h>
+#include <stdint.h>
+
+namespace {
+// Exclusive prefix of cellCount into cellStart: 4096 cells per block,
+// block totals scanned by one block, then a downsweep that also clears the
+// counts for the next substep.  Integer adds in a fixed tree: exact.
+constexpr uint32_t SCAN_PER_THREAD = 16, SCAN_BLOCK = 256 * SCAN_PER_THREAD;
+
+__device__ __forceinline__ uint32_t block_excl_scan(uint32_t v, uint32_t* sh, uint32_t& total) {
+  // 256 threads, Hillis-Steele in LDS
+  uint32_t t = threadIdx.x;
+  sh[t] = v;
+  __syncthreads();
+  for (uint32_t off = 1; off < 256; off <<= 1) {
+    uint32_t add = t >= off ? sh[t - off] : 0u;
+    __syncthreads();
+    sh[t] += add;
+    __syncthreads();
+  }
+  total = sh[255];
+  uint32_t incl = sh[t];
+  __syncthreads();
+  return incl - v;
+}
+
+__global__ __launch_bounds__(256) void k_scan_reduce(const uint32_t* in, uint32_t n, uint32_t* blockSums) {
+  __shared__ uint32_t sh[256];
+  uint32_t base = blockIdx.x * SCAN_BLOCK + threadIdx.x * SCAN_PER_THREAD;
+  uint32_t s = 0;
+#pragma unroll
+  for (uint32_t k = 0; k < SCAN_PER_THREAD; ++k) s += (base + k < n) ? in[base + k] : 0u;
+  uint32_t total;
+  block_excl_scan(s, sh, total);
+  if (threadIdx.x == 0) blockSums[blockIdx.x] = total;
+}
+
+__global__ __launch_bounds__(256) void k_scan_top(uint32_t* blockSums, uint32_t nb) {
+  __shared__ uint32_t sh[256];
+  uint32_t base = threadIdx.x * SCAN_PER_THREAD;
+  uint32_t v[SCAN_PER_THREAD];
+  uint32_t s = 0;
+#pragma unroll
+  for (uint32_t k = 0; k < SCAN_PER_THREAD; ++k) {
+    v[k] = (base + k < nb) ? blockSums[base + k] : 0u;
+    s += v[k];
+  }
+  uint32_t total;
+  uint32_t run = block_excl_scan(s, sh, total);
+#pragma unroll
+  for (uint32_t k = 0; k < SCAN_PER_THREAD; ++k) {
+    if (base + k < nb) blockSums[base + k] = run;
+    run += v[k];
+  }
+}
+
+__global__ __launch_bounds__(256) void k_scan_down(uint32_t* inout_count, uint32_t n, const uint32_t* blockSums,
+                                                   uint32_t* out, bool clear) {
+  __shared__ uint32_t sh[256];
+  uint32_t base = blockIdx.x * SCAN_BLOCK + threadIdx.x * SCAN_PER_THREAD;
+  uint32_t v[SCAN_PER_THREAD];
+  uint32_t s = 0;
+#pragma unroll
+  for (uint32_t k = 0; k < SCAN_PER_THREAD; ++k) {
+    v[k] = (base + k < n) ? inout_count[base + k] : 0u;
+    s += v[k];
+  }
+  uint32_t total;
+  uint32_t run = block_excl_scan(s, sh, total) + blockSums[blockIdx.x];
+#pragma unroll
+  for (uint32_t k = 0; k < SCAN_PER_THREAD; ++k) {
+    if (base + k < n) {
+      out[base + k] = run;
+      if (clear) inout_count[base + k] = 0u;
+    }
+    run += v[k];
+  }
+  // the running sum after the last element of the last block is the total
+  if (blockIdx.x == gridDim.x - 1 && threadIdx.x == 255) out[n] = run;
+}
+
+
+}  // namespace
