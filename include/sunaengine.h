@@ -136,6 +136,54 @@ int sn_set_active(sn_engine* e, const uint8_t* active, uint32_t count);
  * [8] surface tension + thermal + hash, [9] chain fold. */
 int sn_profile_step(sn_engine* e, uint32_t count, float out[10]);
 
+/* ===================================================================== 3D
+ * The 3D solver (3d/sim3d.wgsl, host 3d/app.js:606-743): App.runSubsteps(k)
+ * and App.chain().  A 3D record is 16 int32 words {p.xyzw, v.xyzw, pr.xyzw,
+ * dp.xyzw} with the material index in p.w (3d/sim3d.wgsl:46-51); the world
+ * is gx x gy x gz cells of one kernel radius (2 wu).  H_set/H_slot per
+ * substep is digestState over p.xyzw, v.xyzw (the reference has no 3D state
+ * hash of its own); sn3_fnv_digest returns the reference's striped FNV. */
+typedef struct sn3_engine sn3_engine;
+typedef struct sn3_scene {
+  uint32_t n;
+  const int32_t* parts16; /* n * 16 words */
+  uint32_t gx, gy, gz;
+} sn3_scene;
+/* luts3 = the 3D kernel/wall/ball/material image (3d/app.js:85-231). */
+int sn3_create(const sn_opts* opts, const int32_t* luts3, uint32_t lut_words, sn3_engine** out);
+void sn3_destroy(sn3_engine* e);
+const char* sn3_last_error(const sn3_engine* e);
+int sn3_load(sn3_engine* e, const sn3_scene* scenes, uint32_t count);
+/* Params of App.writeParams (3d/app.js:606-630): stir amplitude and phase,
+ * cursor ball on/off, position and reach (all Q16.16 ticks). */
+int sn3_set_inputs(sn3_engine* e, int32_t stir_amp, int32_t stir_phase, uint32_t ball_on, const int32_t* ball3,
+                   int32_t ball_reach);
+int sn3_step(sn3_engine* e, uint32_t count);
+int sn3_sync(sn3_engine* e);
+int sn3_read_hash(sn3_engine* e, uint32_t scene, uint32_t acc_out[8]);
+int sn3_read_stream(sn3_engine* e, uint32_t scene, uint32_t* out, size_t max_substeps, size_t* n_out);
+int sn3_reset_stream(sn3_engine* e);
+int sn3_read_chain(sn3_engine* e, uint32_t scene, uint32_t chain_out[8]);
+int sn3_read_state(sn3_engine* e, uint32_t scene, int32_t* out16, size_t words);
+int sn3_debug_counters(sn3_engine* e, uint32_t scene, uint32_t out[16]);
+/* App.chain() (3d/app.js:717-743): the 8 FNV lanes of the striped digest. */
+int sn3_fnv_digest(sn3_engine* e, uint32_t scene, uint32_t lanes_out[8]);
+/* 3D stir events (this build's definition — the reference has no 3D
+ * impulse): fluid slots in ascending order within r of (cx, cy, cz), at most
+ * `limit`, get (ix, iy, iz) added to v, clamped to +-65536, immediately
+ * before the scene's substep `substep`. */
+typedef struct sn3_event {
+  uint32_t scene;
+  uint32_t substep;
+  int32_t cx, cy, cz, r, ix, iy, iz;
+  uint32_t limit;
+} sn3_event;
+int sn3_set_events(sn3_engine* e, const sn3_event* events, uint32_t count);
+int sn3_set_active(sn3_engine* e, const uint8_t* active, uint32_t count);
+/* out: [0] substep, [1] predict+sort, [2] neighbour build + density 0,
+ * [3] solve iterations, [4] chain fold (device ms summed over count). */
+int sn3_profile_step(sn3_engine* e, uint32_t count, float out[5]);
+
 #ifdef __cplusplus
 }
 #endif

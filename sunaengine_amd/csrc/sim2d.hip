@@ -189,6 +189,7 @@ struct Dev {
   uint32_t* hitScan;
   uint32_t N, C, S, ringCap;
   uint32_t nEvents;
+  uint32_t NS;  // neighbour-list stride: nbr[k * NS + p]
 };
 
 __device__ __forceinline__ const int32_t* MAT(const Dev& D, uint32_t m) { return D.mats + m * MAT_WORDS; }
@@ -381,7 +382,7 @@ __global__ __launch_bounds__(256) void k_nbr_solveA(Dev D) {
       int32_t cx = (int32_t)(lc % P.gridW), cy = (int32_t)(lc / P.gridW);
       int32_t xlo = maxi(cx - 1, 0), xhi = mini(cx + 1, (int32_t)P.gridW - 1);
       bool sealSeen = false, congealSeen = false;
-      uint32_t* out = D.nbr + (size_t)p * MAXNBR;
+      uint32_t* out = D.nbr + p;
       for (int32_t dy = -1; dy <= 1; ++dy) {
         int32_t gy = cy + dy;
         if (gy < 0 || gy >= (int32_t)P.gridH) continue;
@@ -398,7 +399,7 @@ __global__ __launch_bounds__(256) void k_nbr_solveA(Dev D) {
           if (iq & INF_SEAL) sealSeen = true;
           if (iq & INF_CONGEAL) congealSeen = true;
           if (cnt >= MAXNBR) { dropped++; continue; }
-          out[cnt] = q | ((iq & INF_FLUID) ? 0u : NBR_BND);
+          out[(size_t)cnt * D.NS] = q | ((iq & INF_FLUID) ? 0u : NBR_BND);
           cnt++;
           if (fluid) solveA_term(acc, d, D.lut4[idx], g);
         }
@@ -418,10 +419,10 @@ __global__ __launch_bounds__(256) void k_solveA(Dev D) {
   if (s != INACTIVE && (D.info[p] & INF_FLUID)) {
     int2 pi = D.pred[p];
     uint32_t cnt = D.nbrN[p] & 0x1fffffffu;
-    const uint32_t* nb = D.nbr + (size_t)p * MAXNBR;
+    const uint32_t* nb = D.nbr + p;
     SolveAAcc acc{D.lut4[0].x, 0, 0, 0};
     for (uint32_t k = 0; k < cnt; ++k) {
-      uint32_t q = nb[k] & ~NBR_BND;
+      uint32_t q = nb[(size_t)k * D.NS] & ~NBR_BND;
       int2 d = pred_delta(pi, D.pred[q], g);
       int32_t idx = lut_index(d, g);
       if (idx < 0) continue;
@@ -444,10 +445,10 @@ __device__ __forceinline__ int2 solveB_at(const Dev& D, uint32_t p, Dbg& g) {
   int2 pi = D.pred[p];
   int32_t li = D.lam[p];
   uint32_t cnt = D.nbrN[p] & 0x1fffffffu;
-  const uint32_t* nb = D.nbr + (size_t)p * MAXNBR;
+  const uint32_t* nb = D.nbr + p;
   int32_t dpx = 0, dpy = 0;
   for (uint32_t k = 0; k < cnt; ++k) {
-    uint32_t e = nb[k];
+    uint32_t e = nb[(size_t)k * D.NS];
     uint32_t q = e & ~NBR_BND;
     int2 d = pred_delta(pi, D.pred[q], g);
     int32_t idx = lut_index(d, g);
@@ -550,10 +551,10 @@ __global__ __launch_bounds__(256) void k_xnc(Dev D) {
       int2 pi = D.pred[p], vi = D.vtmp[p];
       int32_t rhoi = D.rho[p];
       uint32_t cnt = D.nbrN[p] & 0x1fffffffu;
-      const uint32_t* nb = D.nbr + (size_t)p * MAXNBR;
+      const uint32_t* nb = D.nbr + p;
       int32_t ax = 0, ay = 0, sx = 0, sy = 0, q = 0, c = 0;
       for (uint32_t k = 0; k < cnt; ++k) {
-        uint32_t e = nb[k];
+        uint32_t e = nb[(size_t)k * D.NS];
         if (e & NBR_BND) continue;
         int2 d = pred_delta(pi, D.pred[e], g);
         int32_t idx = lut_index(d, g);
@@ -624,10 +625,10 @@ __global__ __launch_bounds__(256) void k_st_thermal_hash(Dev D) {
       int2 pi = D.pred[p], ni = D.nrm[p];
       int32_t rhoi = D.rho[p];
       uint32_t cnt = D.nbrN[p] & 0x1fffffffu;
-      const uint32_t* nb = D.nbr + (size_t)p * MAXNBR;
+      const uint32_t* nb = D.nbr + p;
       int32_t ax = 0, ay = 0;
       for (uint32_t k = 0; k < cnt; ++k) {
-        uint32_t e = nb[k];
+        uint32_t e = nb[(size_t)k * D.NS];
         uint32_t q = e & ~NBR_BND;
         int2 d = pred_delta(pi, D.pred[q], g);
         int32_t idx = lut_index(d, g);
@@ -1106,6 +1107,7 @@ int sn_create(const sn_opts* opts, const int32_t* luts, const int32_t* mats, sn_
     D.heat = dalloc<int32_t>(e, NC);
     D.corr = dalloc<int32_t>(e, NC);
     D.nbr = dalloc<uint32_t>(e, (size_t)NC * MAXNBR);
+    D.NS = NC;
     D.nbrN = dalloc<uint32_t>(e, NC);
     D.hitFlag = dalloc<uint32_t>(e, NC);
     D.hitScan = dalloc<uint32_t>(e, (size_t)NC + 1);

@@ -117,6 +117,7 @@ struct Dev3 {
   uint32_t* hitFlag;
   uint32_t* hitScan;
   uint32_t N, C, S, ringCap;
+  uint32_t NS;  // neighbour-list stride: nbr[k * NS + p]
 };
 
 __device__ __forceinline__ int32_t mat_grav(const Dev3& D, uint32_t m) { return D.luts[OFF_MAT + m * MAT_WORDS]; }
@@ -290,7 +291,7 @@ __global__ __launch_bounds__(256) void k3_nbr_density(Dev3 D) {
     int32_t xlo = maxi(cx - 1, 0), xhi = mini(cx + 1, (int32_t)P.gx - 1);
     DensAcc A{wmul(D.luts[OFF_W], wadd(256, mat_rhoW(D, D.matY[p]))) >> 8, 0, 0, 0, 0};
     uint32_t cnt = 0;
-    uint32_t* out = D.nbr + (size_t)p * MAXNBR;
+    uint32_t* out = D.nbr + p;
     for (int32_t dz = -1; dz <= 1; ++dz) {
       int32_t gz = cz + dz;
       if (gz < 0 || gz >= (int32_t)P.gz) continue;
@@ -308,7 +309,7 @@ __global__ __launch_bounds__(256) void k3_nbr_density(Dev3 D) {
           int32_t r2q = r2q_of(d);
           if (r2q >= H2Q) continue;
           if (cnt >= MAXNBR) { dropped++; continue; }
-          out[cnt++] = q;
+          out[(size_t)(cnt++) * D.NS] = q;
           dens_term(D, A, d, (uint32_t)r2q >> 7, D.matY[q], g);
         }
       }
@@ -329,9 +330,9 @@ __global__ __launch_bounds__(256) void k3_density(Dev3 D, int4* pr) {
     int3 pi = make_int3(pr4.x, pr4.y, pr4.z);
     DensAcc A{wmul(D.luts[OFF_W], wadd(256, mat_rhoW(D, D.matY[p]))) >> 8, 0, 0, 0, 0};
     uint32_t cnt = D.nbrN[p];
-    const uint32_t* nb = D.nbr + (size_t)p * MAXNBR;
+    const uint32_t* nb = D.nbr + p;
     for (uint32_t k = 0; k < cnt; ++k) {
-      uint32_t q = nb[k];
+      uint32_t q = nb[(size_t)k * D.NS];
       int4 pq = pr[q];
       int3 d = make_int3(wsub(pq.x, pi.x), wsub(pq.y, pi.y), wsub(pq.z, pi.z));
       int32_t r2q = r2q_of(d);
@@ -356,10 +357,10 @@ __global__ __launch_bounds__(256) void k3_solve_apply(Dev3 D, const int4* pr, in
     int3 pi = make_int3(pr4.x, pr4.y, pr4.z);
     int32_t lami = pr4.w;
     uint32_t cnt = D.nbrN[p];
-    const uint32_t* nb = D.nbr + (size_t)p * MAXNBR;
+    const uint32_t* nb = D.nbr + p;
     int32_t ax = 0, ay = 0, az = 0;
     for (uint32_t k = 0; k < cnt; ++k) {
-      uint32_t q = nb[k];
+      uint32_t q = nb[(size_t)k * D.NS];
       int4 pq = pr[q];
       int3 d = make_int3(wsub(pq.x, pi.x), wsub(pq.y, pi.y), wsub(pq.z, pi.z));
       int32_t r2q = r2q_of(d);
@@ -716,6 +717,7 @@ int sn3_create(const sn_opts* opts, const int32_t* luts3, uint32_t lut_words, sn
     D.prB = dalloc3<int4>(e, NC);
     D.aux = dalloc3<int4>(e, NC);
     D.nbr = dalloc3<uint32_t>(e, (size_t)NC * MAXNBR);
+    D.NS = NC;
     D.nbrN = dalloc3<uint32_t>(e, NC);
     D.hitFlag = dalloc3<uint32_t>(e, NC);
     D.hitScan = dalloc3<uint32_t>(e, (size_t)NC + 1);

@@ -21,6 +21,10 @@ EXPORTS = [
     "sn_sync", "sn_read_hash", "sn_read_hash_stream", "sn_reset_stream", "sn_read_chain",
     "sn_read_state", "sn_read_derived", "sn_debug_counters", "sn_substep_count",
     "sn_apply_impulse", "sn_spawn_fluid", "sn_set_events", "sn_set_active", "sn_profile_step",
+    "sn3_create", "sn3_destroy", "sn3_last_error", "sn3_load", "sn3_set_inputs", "sn3_step",
+    "sn3_sync", "sn3_read_hash", "sn3_read_stream", "sn3_reset_stream", "sn3_read_chain",
+    "sn3_read_state", "sn3_debug_counters", "sn3_fnv_digest", "sn3_set_events", "sn3_set_active",
+    "sn3_profile_step",
 ]
 
 
@@ -46,6 +50,17 @@ class sn_scene(C.Structure):
     _fields_ = [("n", C.c_uint32), ("n_fluid", C.c_uint32), ("state8", C.c_void_p),
                 ("dom_w", C.c_int32), ("dom_h", C.c_int32), ("cell_shift", C.c_uint32),
                 ("st_acc", C.c_void_p)]
+
+
+class sn3_scene(C.Structure):
+    _fields_ = [("n", C.c_uint32), ("parts16", C.c_void_p), ("gx", C.c_uint32),
+                ("gy", C.c_uint32), ("gz", C.c_uint32)]
+
+
+class sn3_event(C.Structure):
+    _fields_ = [("scene", C.c_uint32), ("substep", C.c_uint32), ("cx", C.c_int32),
+                ("cy", C.c_int32), ("cz", C.c_int32), ("r", C.c_int32), ("ix", C.c_int32),
+                ("iy", C.c_int32), ("iz", C.c_int32), ("limit", C.c_uint32)]
 
 
 class sn_event(C.Structure):
@@ -90,6 +105,23 @@ def lib():
         "sn_set_events": (C.c_int, [P, P, u32]),
         "sn_set_active": (C.c_int, [P, P, u32]),
         "sn_profile_step": (C.c_int, [P, u32, P]),
+        "sn3_create": (C.c_int, [C.POINTER(sn_opts), P, u32, C.POINTER(P)]),
+        "sn3_destroy": (None, [P]),
+        "sn3_last_error": (C.c_char_p, [P]),
+        "sn3_load": (C.c_int, [P, P, u32]),
+        "sn3_set_inputs": (C.c_int, [P, i32, i32, u32, P, i32]),
+        "sn3_step": (C.c_int, [P, u32]),
+        "sn3_sync": (C.c_int, [P]),
+        "sn3_read_hash": (C.c_int, [P, u32, P]),
+        "sn3_read_stream": (C.c_int, [P, u32, P, sz, C.POINTER(sz)]),
+        "sn3_reset_stream": (C.c_int, [P]),
+        "sn3_read_chain": (C.c_int, [P, u32, P]),
+        "sn3_read_state": (C.c_int, [P, u32, P, sz]),
+        "sn3_debug_counters": (C.c_int, [P, u32, P]),
+        "sn3_fnv_digest": (C.c_int, [P, u32, P]),
+        "sn3_set_events": (C.c_int, [P, P, u32]),
+        "sn3_set_active": (C.c_int, [P, P, u32]),
+        "sn3_profile_step": (C.c_int, [P, u32, P]),
     }
     for name, (res, args) in sig.items():
         f = getattr(L, name)
@@ -99,7 +131,7 @@ def lib():
     return L
 
 
-def check(handle, rc: int) -> None:
+def check(handle, rc: int, three_d: bool = False) -> None:
     if rc != SN_OK:
-        msg = lib().sn_last_error(handle)
+        msg = (lib().sn3_last_error if three_d else lib().sn_last_error)(handle)
         raise SnError(rc, (msg or b"").decode() or f"sn error {rc}")
