@@ -87,6 +87,8 @@ constexpr uint32_t MAT_WORDS = 48, MAT_MAX = 64;
 constexpr uint32_t MAT_PHOTO_ROW = 15;
 constexpr uint32_t BOND_PERIOD = 4;
 constexpr uint32_t INACTIVE = 0xffffffffu;
+constexpr uint32_t MIXED = 0xfffffffeu;     // a wave/block spanning scenes
+constexpr int HPART_WORDS = 9;             // 8 hash words + truncation count
 // per-particle info bits, rebuilt every substep by canon_gather
 constexpr uint32_t INF_DEAD = 1, INF_FLUID = 2, INF_SEAL = 4, INF_CONGEAL = 8;
 // scene fault bits
@@ -177,6 +179,8 @@ struct Dev {
   int32_t* corr;
   uint32_t* nbr;
   uint32_t* nbrN;
+  uint32_t* drop;         // candidates dropped by truncation, per particle
+  uint32_t* hpart;        // per-block partial sums [block][HPART_WORDS]
   const int4* lut4;       // {W, G, C, S} per LUT bin
   const int32_t* lutA;    // adhesion table
   const int32_t* mats;
@@ -244,18 +248,44 @@ __device__ __forceinline__ bool supported_row(const int32_t* m) {
   return m[M_FRIC] == 0 && m[M_BONDK] == 0;
 }
 
-// Fold the per-thread debug counters into the scene's dbg words: one atomic
-// per nonzero counter per wave when the wave is uniform in scene.
+// Wrapping u32 sum over the 64 lanes (every lane must be active).
+__device__ __forceinline__ uint32_t wave_sum(uint32_t v) {
+#pragma unroll
+  for (int off = 32; off >= 1; off >>= 1) v += (uint32_t)__shfl_xor((int)v, off);
+  return v;
+}
+
+// Fold the per-thread debug counters into the scene's dbg words: the wave
+// sums each nonzero counter first and issues one atomic for it when the wave
+// is uniform in scene (the counters are order-free u32 sums).  Must be called
+// with every lane of the wave active.
 __device__ __forceinline__ void flush_dbg(const Dev& D, uint32_t s, const Dbg& g, uint32_t trunc) {
   uint32_t v[5] = {g.mul, g.div, g.div0, g.add, trunc};
   const uint32_t slot[5] = {0, 1, 2, 5, 6};
+  const uint32_t s0 = (uint32_t)__builtin_amdgcn_readfirstlane((int)s);
+  const bool uniform = __all(s == s0);
 #pragma unroll
   for (int k = 0; k < 5; ++k) {
     uint32_t x = v[k];
-    if (__any(x != 0)) {
-      if (s != INACTIVE && x) atomicAdd(&D.sd[s].dbg[slot[k]], x);
+    if (!__any(x != 0)) continue;
+    if (uniform) {
+      x = wave_sum(x);
+      if ((threadIdx.x & 63u) == 0u && s0 != INACTIVE) atomicAdd(&D.sd[s0].dbg[slot[k]], x);
+    } else if (s != INACTIVE && x) {
+      atomicAdd(&D.sd[s].dbg[slot[k]], x);
     }
   }
+}
+
+// Workgroup b is dispatched to XCD b % 8 (8 XCDs, each with its own L2).
+// Map hardware blocks so that each XCD walks one contiguous run of logical
+// blocks — a contiguous slab of the cell-sorted particle order — and its L2
+// holds that slab's neighbourhood.  A bijection on [0, gridDim.x).
+__device__ __forceinline__ uint32_t xcd_block() {
+  uint32_t b = blockIdx.x, nb = gridDim.x;
+  uint32_t full = nb & ~7u;
+  if (b >= full) return b;
+  return (b & 7u) * (full >> 3) + (b >> 3);
 }
 
 // ===================================================================== predict
@@ -263,35 +293,35 @@ __device__ __forceinline__ void flush_dbg(const Dev& D, uint32_t s, const Dbg& g
 // counter's atomic return value is the particle's arrival rank in its cell,
 // which replaces the reference's second atomic pass in `scatter`.
 __global__ __launch_bounds__(256) void k_predict_count(Dev D) {
-  uint32_t p = blockIdx.x * 256u + threadIdx.x;
-  if (p >= D.N) return;
-  uint32_t s = D.sidOf[p];
-  if (s == INACTIVE) return;
-  const SceneP& P = D.sp[s];
+  uint32_t p = xcd_block() * 256u + threadIdx.x;
+  uint32_t s = p < D.N ? D.sidOf[p] : INACTIVE;
   Dbg g;
-  int4 a = D.ps[p];
-  int4 b = D.pt[p];
-  uint32_t id = D.id[p];
-  int2 pos = make_int2(a.x, a.y);
-  int2 pr = pos;
-  if (id < P.nFluid) {
-    const int32_t* M = MAT(D, (uint32_t)b.x);
-    uint32_t fl = (uint32_t)b.y;
-    if (M[M_PHANTOM] == 0 && (fl & FLAG_ANCHOR) == 0u) {
-      int2 st = D.st[p];
-      int32_t vmaxN = fp_neg_sat(P.vmax, g);
-      int32_t vx = fp_add_sat(fp_add_sat(a.z, st.x, g), 0, g);
-      int32_t vy = fp_add_sat(fp_add_sat(fp_add_sat(a.w, st.y, g), M[M_GRAV], g), 0, g);
-      vx = clampi(vx, vmaxN, P.vmax);
-      vy = clampi(vy, vmaxN, P.vmax);
-      pr = wall_clamp(P, make_int2(fp_add_sat(pos.x, vx, g), fp_add_sat(pos.y, vy, g)));
+  if (s != INACTIVE) {
+    const SceneP& P = D.sp[s];
+    int4 a = D.ps[p];
+    int4 b = D.pt[p];
+    uint32_t id = D.id[p];
+    int2 pos = make_int2(a.x, a.y);
+    int2 pr = pos;
+    if (id < P.nFluid) {
+      const int32_t* M = MAT(D, (uint32_t)b.x);
+      uint32_t fl = (uint32_t)b.y;
+      if (M[M_PHANTOM] == 0 && (fl & FLAG_ANCHOR) == 0u) {
+        int2 st = D.st[p];
+        int32_t vmaxN = fp_neg_sat(P.vmax, g);
+        int32_t vx = fp_add_sat(fp_add_sat(a.z, st.x, g), 0, g);
+        int32_t vy = fp_add_sat(fp_add_sat(fp_add_sat(a.w, st.y, g), M[M_GRAV], g), 0, g);
+        vx = clampi(vx, vmaxN, P.vmax);
+        vy = clampi(vy, vmaxN, P.vmax);
+        pr = wall_clamp(P, make_int2(fp_add_sat(pos.x, vx, g), fp_add_sat(pos.y, vy, g)));
+      }
     }
+    CC c = cell_coord(P, pr);
+    uint32_t cell = P.cellBase + (uint32_t)c.y * P.gridW + (uint32_t)c.x;
+    D.pred0[p] = pr;
+    D.cell0[p] = cell;
+    D.rank0[p] = atomicAdd(&D.cellCount[cell], 1u);
   }
-  CC c = cell_coord(P, pr);
-  uint32_t cell = P.cellBase + (uint32_t)c.y * P.gridW + (uint32_t)c.x;
-  D.pred0[p] = pr;
-  D.cell0[p] = cell;
-  D.rank0[p] = atomicAdd(&D.cellCount[cell], 1u);
   flush_dbg(D, s, g, 0);
 }
 
@@ -300,7 +330,7 @@ __global__ __launch_bounds__(256) void k_predict_count(Dev D) {
 
 // ==================================================================== scatter
 __global__ __launch_bounds__(256) void k_scatter(Dev D) {
-  uint32_t p = blockIdx.x * 256u + threadIdx.x;
+  uint32_t p = xcd_block() * 256u + threadIdx.x;
   if (p >= D.N) return;
   uint32_t s = D.sidOf[p];
   if (s == INACTIVE) return;
@@ -316,7 +346,7 @@ __global__ __launch_bounds__(256) void k_scatter(Dev D) {
 // the particle's new position is cellStart + rank.  The whole persistent
 // record moves there, together with pred and the cell.
 __global__ __launch_bounds__(256) void k_canon_gather(Dev D) {
-  uint32_t t = blockIdx.x * 256u + threadIdx.x;
+  uint32_t t = xcd_block() * 256u + threadIdx.x;
   if (t >= D.N) return;
   uint32_t s = D.sidOf[t];
   if (s == INACTIVE) return;
@@ -363,57 +393,128 @@ __device__ __forceinline__ void solveA_finish(const Dev& D, uint32_t p, SolveAAc
   D.lam[p] = clampi(fp_neg_sat(fp_divshift(Cq, wadd(a.S, eps), 25, g), g), -LAM_MAX, LAM_MAX);
 }
 
-__global__ __launch_bounds__(256) void k_nbr_solveA(Dev D) {
-  uint32_t p = blockIdx.x * 256u + threadIdx.x;
-  uint32_t s = p < D.N ? D.sidOf[p] : INACTIVE;
+// One wave per tile of up to 8 consecutive cells of one row; up to
+// NBR_PPL particles per lane, so a tile of up to 64 * NBR_PPL particles is
+// served by one sweep.  The tile's candidates — the union of its particles'
+// 3x3 stencils: rows cy-1..cy+1, columns x0-1..x1+1, each row one contiguous
+// sorted range — are loaded 64 at a time, one per lane, and broadcast to
+// every lane with v_readlane, so the candidate loop issues no per-lane
+// memory traffic.  Walking rows in order and each row in sorted order visits
+// every particle's own candidates in exactly its canonical order (dy outer,
+// dx inner, ascending id), which is what decides truncation.
+constexpr int NBR_PPL = 2;
+
+__global__ __launch_bounds__(256) void k_nbr_tile(Dev D, const uint2* work, uint32_t nwork) {
+  const uint32_t lane = threadIdx.x & 63u;
+  uint32_t w = __builtin_amdgcn_readfirstlane(xcd_block() * 4u + (threadIdx.x >> 6));
+  if (w >= nwork) return;
+  uint2 item = work[w];
+  const uint32_t c0 = __builtin_amdgcn_readfirstlane(item.x);
+  const uint32_t yy = __builtin_amdgcn_readfirstlane(item.y);
+  const uint32_t ncell = yy & 0xffu, s = yy >> 8;
+  const SceneP& P = D.sp[s];
+  const int32_t sh = P.posShift;
+  const uint32_t gridW = P.gridW, gridH = P.gridH, cellBase = P.cellBase;
+  const uint32_t lc0 = c0 - cellBase;
+  const int32_t cy = (int32_t)(lc0 / gridW), cx0 = (int32_t)(lc0 % gridW);
+  const int32_t xlo = maxi(cx0 - 1, 0), xhi = mini(cx0 + (int32_t)ncell, (int32_t)gridW - 1);
+  const uint32_t plo = (uint32_t)((int32_t)D.cellStart[c0] + sh);
+  const uint32_t phi = (uint32_t)((int32_t)D.cellStart[c0 + ncell] + sh);
+  const uint32_t myRow = cellBase + (uint32_t)cy * gridW;
+  const int32_t W0 = D.lut4[0].x;
   Dbg g;
-  uint32_t dropped = 0;
-  if (s != INACTIVE) {
-    const SceneP& P = D.sp[s];
-    uint32_t inf = D.info[p];
-    bool fluid = (inf & INF_FLUID) != 0u;
-    SolveAAcc acc{D.lut4[0].x, 0, 0, 0};
-    uint32_t cnt = 0;
-    if (inf & INF_DEAD) {
-      D.nbrN[p] = 0u;
-    } else {
-      int2 pi = D.pred[p];
-      uint32_t lc = D.cellY[p] - P.cellBase;
-      int32_t cx = (int32_t)(lc % P.gridW), cy = (int32_t)(lc / P.gridW);
-      int32_t xlo = maxi(cx - 1, 0), xhi = mini(cx + 1, (int32_t)P.gridW - 1);
-      bool sealSeen = false, congealSeen = false;
-      uint32_t* out = D.nbr + p;
-      for (int32_t dy = -1; dy <= 1; ++dy) {
-        int32_t gy = cy + dy;
-        if (gy < 0 || gy >= (int32_t)P.gridH) continue;
-        uint32_t rowc = P.cellBase + (uint32_t)gy * P.gridW;
-        uint32_t lo = (uint32_t)((int32_t)D.cellStart[rowc + (uint32_t)xlo] + P.posShift);
-        uint32_t hi = (uint32_t)((int32_t)D.cellStart[rowc + (uint32_t)xhi + 1u] + P.posShift);
-        for (uint32_t q = lo; q < hi; ++q) {
-          if (q == p) continue;
-          uint32_t iq = D.info[q];
-          if (iq & INF_DEAD) continue;
-          int2 d = pred_delta(pi, D.pred[q], g);
-          int32_t idx = lut_index(d, g);
-          if (idx < 0) continue;
-          if (iq & INF_SEAL) sealSeen = true;
-          if (iq & INF_CONGEAL) congealSeen = true;
-          if (cnt >= MAXNBR) { dropped++; continue; }
-          out[(size_t)cnt * D.NS] = q | ((iq & INF_FLUID) ? 0u : NBR_BND);
-          cnt++;
-          if (fluid) solveA_term(acc, d, D.lut4[idx], g);
+  for (uint32_t pbase = plo; pbase < phi; pbase += 64u * NBR_PPL) {
+    uint32_t pk[NBR_PPL], inf[NBR_PPL], cnt[NBR_PPL], bits[NBR_PPL], dropped[NBR_PPL];
+    int2 pik[NBR_PPL];
+    int32_t cxk[NBR_PPL];
+    bool live[NBR_PPL];
+    SolveAAcc acc[NBR_PPL];
+#pragma unroll
+    for (int k = 0; k < NBR_PPL; ++k) {
+      pk[k] = pbase + (uint32_t)k * 64u + lane;
+      bool valid = pk[k] < phi;
+      inf[k] = valid ? D.info[pk[k]] : 0u;
+      pik[k] = valid ? D.pred[pk[k]] : make_int2(0, 0);
+      cxk[k] = valid ? (int32_t)(D.cellY[pk[k]] - myRow) : 0;
+      live[k] = valid && !(inf[k] & INF_DEAD);
+      cnt[k] = 0;
+      bits[k] = 0;
+      dropped[k] = 0;
+      acc[k] = SolveAAcc{W0, 0, 0, 0};
+    }
+    for (int32_t dy = -1; dy <= 1; ++dy) {
+      const int32_t gy = cy + dy;
+      if (gy < 0 || gy >= (int32_t)gridH) continue;
+      const uint32_t rowc = cellBase + (uint32_t)gy * gridW;
+      const uint32_t rlo = (uint32_t)((int32_t)D.cellStart[rowc + (uint32_t)xlo] + sh);
+      const uint32_t rhi = (uint32_t)((int32_t)D.cellStart[rowc + (uint32_t)xhi + 1u] + sh);
+      for (uint32_t q0 = rlo; q0 < rhi; q0 += 64u) {
+        const uint32_t qn = mini((int32_t)(rhi - q0), 64);
+        const uint32_t ql = q0 + lane;
+        int2 cp = make_int2(0, 0);
+        uint32_t ci = 0;
+        int32_t cc = 0;
+        if (lane < qn) {
+          cp = D.pred[ql];
+          ci = D.info[ql];
+          cc = (int32_t)(D.cellY[ql] - rowc);
+        }
+        for (uint32_t j = 0; j < qn; ++j) {
+          const int32_t px = __builtin_amdgcn_readlane(cp.x, j);
+          const int32_t py = __builtin_amdgcn_readlane(cp.y, j);
+          const uint32_t qi = (uint32_t)__builtin_amdgcn_readlane((int)ci, j);
+          const int32_t qc = __builtin_amdgcn_readlane(cc, j);
+          const uint32_t q = q0 + j;
+          if (qi & INF_DEAD) continue;                     // nobody sees a corpse
+#pragma unroll
+          for (int k = 0; k < NBR_PPL; ++k) {
+            if (!live[k] || q == pk[k] || qc < cxk[k] - 1 || qc > cxk[k] + 1) continue;
+            int2 d = pred_delta(pik[k], make_int2(px, py), g);
+            int32_t idx = lut_index(d, g);
+            if (idx < 0) continue;
+            bits[k] |= qi & (INF_SEAL | INF_CONGEAL);
+            if (cnt[k] >= MAXNBR) {
+              dropped[k]++;
+              continue;
+            }
+            D.nbr[(size_t)cnt[k] * D.NS + pk[k]] = q | ((qi & INF_FLUID) ? 0u : NBR_BND);
+            cnt[k]++;
+            if (inf[k] & INF_FLUID) solveA_term(acc[k], d, D.lut4[idx], g);
+          }
         }
       }
-      D.nbrN[p] = cnt | (sealSeen ? NBR_SEAL_BIT : 0u) | (congealSeen ? NBR_CONGEAL_BIT : 0u);
     }
-    if (fluid) solveA_finish(D, p, acc, MAT(D, (uint32_t)D.ptY[p].x)[M_EPS], g);
+#pragma unroll
+    for (int k = 0; k < NBR_PPL; ++k) {
+      if (pk[k] >= phi) continue;
+      D.nbrN[pk[k]] = live[k] ? (cnt[k] | ((bits[k] & INF_SEAL) ? NBR_SEAL_BIT : 0u) |
+                                 ((bits[k] & INF_CONGEAL) ? NBR_CONGEAL_BIT : 0u))
+                              : 0u;
+      D.drop[pk[k]] = dropped[k];
+      if (inf[k] & INF_FLUID) solveA_finish(D, pk[k], acc[k], MAT(D, (uint32_t)D.ptY[pk[k]].x)[M_EPS], g);
+    }
   }
-  flush_dbg(D, s, g, dropped);
+  flush_dbg(D, s, g, 0);
+}
+
+// ============================================================ neighbour loops
+// Pair loops walk a particle's list NBR_UNROLL entries at a time with every
+// load of the group issued before any term is formed, so each lane keeps
+// several gathers in flight.  Entries past the count read as the particle
+// itself (a zero offset, no saturation) and are skipped; every pair sum is a
+// wrapping u32 sum, so grouping does not change it.
+constexpr uint32_t NBR_UNROLL = 4;
+constexpr uint32_t XNC_UNROLL = 2;
+static_assert(MAXNBR % NBR_UNROLL == 0 && MAXNBR % XNC_UNROLL == 0, "list slots k + u stay below MAXNBR");
+__device__ __forceinline__ uint32_t nbr_at(const Dev& D, const uint32_t* nb, uint32_t k, uint32_t cnt,
+                                           uint32_t self) {
+  uint32_t e = nb[(size_t)k * D.NS];
+  return k < cnt ? e : self;
 }
 
 // ================================================================ solveA (k>0)
 __global__ __launch_bounds__(256) void k_solveA(Dev D) {
-  uint32_t p = blockIdx.x * 256u + threadIdx.x;
+  uint32_t p = xcd_block() * 256u + threadIdx.x;
   uint32_t s = p < D.N ? D.sidOf[p] : INACTIVE;
   Dbg g;
   if (s != INACTIVE && (D.info[p] & INF_FLUID)) {
@@ -421,12 +522,22 @@ __global__ __launch_bounds__(256) void k_solveA(Dev D) {
     uint32_t cnt = D.nbrN[p] & 0x1fffffffu;
     const uint32_t* nb = D.nbr + p;
     SolveAAcc acc{D.lut4[0].x, 0, 0, 0};
-    for (uint32_t k = 0; k < cnt; ++k) {
-      uint32_t q = nb[(size_t)k * D.NS] & ~NBR_BND;
-      int2 d = pred_delta(pi, D.pred[q], g);
-      int32_t idx = lut_index(d, g);
-      if (idx < 0) continue;
-      solveA_term(acc, d, D.lut4[idx], g);
+    for (uint32_t k = 0; k < cnt; k += NBR_UNROLL) {
+      uint32_t q[NBR_UNROLL];
+      int2 d[NBR_UNROLL];
+      int32_t idx[NBR_UNROLL];
+      int4 L[NBR_UNROLL];
+#pragma unroll
+      for (uint32_t u = 0; u < NBR_UNROLL; ++u) q[u] = nbr_at(D, nb, k + u, cnt, p);
+#pragma unroll
+      for (uint32_t u = 0; u < NBR_UNROLL; ++u) {
+        d[u] = pred_delta(pi, D.pred[q[u] & ~NBR_BND], g);
+        idx[u] = (k + u < cnt) ? lut_index(d[u], g) : -1;
+        L[u] = D.lut4[maxi(idx[u], 0)];
+      }
+#pragma unroll
+      for (uint32_t u = 0; u < NBR_UNROLL; ++u)
+        if (idx[u] >= 0) solveA_term(acc, d[u], L[u], g);
     }
     solveA_finish(D, p, acc, MAT(D, (uint32_t)D.ptY[p].x)[M_EPS], g);
   }
@@ -447,26 +558,39 @@ __device__ __forceinline__ int2 solveB_at(const Dev& D, uint32_t p, Dbg& g) {
   uint32_t cnt = D.nbrN[p] & 0x1fffffffu;
   const uint32_t* nb = D.nbr + p;
   int32_t dpx = 0, dpy = 0;
-  for (uint32_t k = 0; k < cnt; ++k) {
-    uint32_t e = nb[(size_t)k * D.NS];
-    uint32_t q = e & ~NBR_BND;
-    int2 d = pred_delta(pi, D.pred[q], g);
-    int32_t idx = lut_index(d, g);
-    if (idx < 0) continue;
-    int32_t lj = (e & NBR_BND) ? li : D.lam[q];
-    int4 L = D.lut4[idx];
-    int32_t co = fp_add_sat(fp_add_sat(li, lj, g), L.w, g);
-    dpx = wadd(dpx, clampi(fp_mul_s(co, grad_q16(L.y, d.x, g), 20 - ACC, g), -DP_TERM, DP_TERM));
-    dpy = wadd(dpy, clampi(fp_mul_s(co, grad_q16(L.y, d.y, g), 20 - ACC, g), -DP_TERM, DP_TERM));
+  for (uint32_t k = 0; k < cnt; k += NBR_UNROLL) {
+    uint32_t e[NBR_UNROLL];
+    int2 d[NBR_UNROLL];
+    int32_t idx[NBR_UNROLL], lj[NBR_UNROLL];
+    int4 L[NBR_UNROLL];
+#pragma unroll
+    for (uint32_t u = 0; u < NBR_UNROLL; ++u) e[u] = nbr_at(D, nb, k + u, cnt, p);
+#pragma unroll
+    for (uint32_t u = 0; u < NBR_UNROLL; ++u) {
+      uint32_t q = e[u] & ~NBR_BND;
+      d[u] = pred_delta(pi, D.pred[q], g);
+      lj[u] = D.lam[q];
+      idx[u] = (k + u < cnt) ? lut_index(d[u], g) : -1;
+      L[u] = D.lut4[maxi(idx[u], 0)];
+    }
+#pragma unroll
+    for (uint32_t u = 0; u < NBR_UNROLL; ++u) {
+      if (idx[u] < 0) continue;
+      int32_t co = fp_add_sat(fp_add_sat(li, (e[u] & NBR_BND) ? li : lj[u], g), L[u].w, g);
+      dpx = wadd(dpx, clampi(fp_mul_s(co, grad_q16(L[u].y, d[u].x, g), 20 - ACC, g), -DP_TERM, DP_TERM));
+      dpy = wadd(dpy, clampi(fp_mul_s(co, grad_q16(L[u].y, d[u].y, g), 20 - ACC, g), -DP_TERM, DP_TERM));
+    }
   }
   return make_int2(dpx, dpy);
 }
 
 __global__ __launch_bounds__(256) void k_colour(Dev D, const uint2* work, uint32_t nwork, uint32_t last) {
   uint32_t lane = threadIdx.x & 63u;
-  uint32_t w = blockIdx.x * 4u + (threadIdx.x >> 6);
+  uint32_t w = __builtin_amdgcn_readfirstlane(xcd_block() * 4u + (threadIdx.x >> 6));
   if (w >= nwork) return;               // whole wave exits together
   uint2 item = work[w];                 // {first global cell, ncell | scene << 8}
+  item.x = __builtin_amdgcn_readfirstlane(item.x);
+  item.y = __builtin_amdgcn_readfirstlane(item.y);
   uint32_t ncell = item.y & 0xffu;
   uint32_t s = item.y >> 8;
   const SceneP& P = D.sp[s];
@@ -533,7 +657,7 @@ __global__ __launch_bounds__(256) void k_colour(Dev D, const uint2* work, uint32
 // neighbours; each keeps its own accumulators.  Writes the final position and
 // velocity of the substep (boundary slots: position kept, velocity zeroed).
 __global__ __launch_bounds__(256) void k_xnc(Dev D) {
-  uint32_t p = blockIdx.x * 256u + threadIdx.x;
+  uint32_t p = xcd_block() * 256u + threadIdx.x;
   uint32_t s = p < D.N ? D.sidOf[p] : INACTIVE;
   Dbg g;
   if (s != INACTIVE) {
@@ -553,35 +677,52 @@ __global__ __launch_bounds__(256) void k_xnc(Dev D) {
       uint32_t cnt = D.nbrN[p] & 0x1fffffffu;
       const uint32_t* nb = D.nbr + p;
       int32_t ax = 0, ay = 0, sx = 0, sy = 0, q = 0, c = 0;
-      for (uint32_t k = 0; k < cnt; ++k) {
-        uint32_t e = nb[(size_t)k * D.NS];
-        if (e & NBR_BND) continue;
-        int2 d = pred_delta(pi, D.pred[e], g);
-        int32_t idx = lut_index(d, g);
-        if (idx < 0) continue;
-        int4 L = D.lut4[idx];
-        int4 bj = D.ptY[e];
-        const int32_t* Mj = MAT(D, (uint32_t)bj.x);
-        if (doX) {
-          int32_t K = sym_k(rhoi, D.rho[e], g);
-          int32_t w = fp_mul_s(L.x, K, 16, g);
-          int32_t cij = mat_mix(ci, Mj[M_XSPH]);
-          int2 vj = D.vtmp[e];
-          ax = wadd(ax, clampi(fp_mul_s(fp_mul_s(fp_sub_sat(vj.x, vi.x, g), w, 20 - ACC, g), cij, 16, g), -XS_TERM, XS_TERM));
-          ay = wadd(ay, clampi(fp_mul_s(fp_mul_s(fp_sub_sat(vj.y, vi.y, g), w, 20 - ACC, g), cij, 16, g), -XS_TERM, XS_TERM));
+      for (uint32_t k = 0; k < cnt; k += XNC_UNROLL) {
+        // padding entries read as boundary entries, which this loop skips;
+        // a position difference of two in-range positions never saturates
+        uint32_t e[XNC_UNROLL];
+        int2 d[XNC_UNROLL], vjs[XNC_UNROLL];
+        int32_t idx[XNC_UNROLL], rhoj[XNC_UNROLL], rjs[XNC_UNROLL];
+        int4 Ls[XNC_UNROLL], bjs[XNC_UNROLL];
+#pragma unroll
+        for (uint32_t u = 0; u < XNC_UNROLL; ++u) e[u] = nbr_at(D, nb, k + u, cnt, p | NBR_BND);
+#pragma unroll
+        for (uint32_t u = 0; u < XNC_UNROLL; ++u) {
+          uint32_t qj = e[u] & ~NBR_BND;
+          d[u] = pred_delta(pi, D.pred[qj], g);
+          idx[u] = (e[u] & NBR_BND) ? -1 : lut_index(d[u], g);
+          Ls[u] = D.lut4[maxi(idx[u], 0)];
+          bjs[u] = D.ptY[qj];
+          rhoj[u] = D.rho[qj];
+          vjs[u] = D.vtmp[qj];
+          rjs[u] = D.rinv[qj];
         }
-        int32_t rj = D.rinv[e];
-        sx = wadd(sx, fp_mul_s(grad_q16(L.y, d.x, g), rj, 16, g));
-        sy = wadd(sy, fp_mul_s(grad_q16(L.y, d.y, g), rj, 16, g));
-        int32_t cij = mat_mix(ki, Mj[M_COND]);
-        int32_t wk = fp_mul_s(L.x, cij, 16, g);
-        int32_t dT = clampi(fp_sub_sat(bj.z, Ti, g), -DT_CLAMP, DT_CLAMP);
-        q = wadd(q, clampi(fp_mul_s(dT, wk, 20 - ACC, g), -HT_TERM, HT_TERM));
-        if ((uint32_t)bj.x != mi) {
-          int32_t attack = Mj[M_CORRODE];
-          int32_t spend = (Mj[M_CORRPT] != CORR_NEVER) ? corri : 0;
-          int32_t str = wadd(attack, spend);
-          if (str != 0) c = wadd(c, clampi(fp_mul_s(L.x, str, 20 - ACC, g), 0, CORR_TERM));
+#pragma unroll
+        for (uint32_t u = 0; u < XNC_UNROLL; ++u) {
+          if (idx[u] < 0) continue;
+          const int4 L = Ls[u], bj = bjs[u];
+          const int32_t* Mj = MAT(D, (uint32_t)bj.x);
+          if (doX) {
+            int32_t K = sym_k(rhoi, rhoj[u], g);
+            int32_t w = fp_mul_s(L.x, K, 16, g);
+            int32_t cij = mat_mix(ci, Mj[M_XSPH]);
+            int2 vj = vjs[u];
+            ax = wadd(ax, clampi(fp_mul_s(fp_mul_s(fp_sub_sat(vj.x, vi.x, g), w, 20 - ACC, g), cij, 16, g), -XS_TERM, XS_TERM));
+            ay = wadd(ay, clampi(fp_mul_s(fp_mul_s(fp_sub_sat(vj.y, vi.y, g), w, 20 - ACC, g), cij, 16, g), -XS_TERM, XS_TERM));
+          }
+          int32_t rj = rjs[u];
+          sx = wadd(sx, fp_mul_s(grad_q16(L.y, d[u].x, g), rj, 16, g));
+          sy = wadd(sy, fp_mul_s(grad_q16(L.y, d[u].y, g), rj, 16, g));
+          int32_t cij = mat_mix(ki, Mj[M_COND]);
+          int32_t wk = fp_mul_s(L.x, cij, 16, g);
+          int32_t dT = clampi(fp_sub_sat(bj.z, Ti, g), -DT_CLAMP, DT_CLAMP);
+          q = wadd(q, clampi(fp_mul_s(dT, wk, 20 - ACC, g), -HT_TERM, HT_TERM));
+          if ((uint32_t)bj.x != mi) {
+            int32_t attack = Mj[M_CORRODE];
+            int32_t spend = (Mj[M_CORRPT] != CORR_NEVER) ? corri : 0;
+            int32_t str = wadd(attack, spend);
+            if (str != 0) c = wadd(c, clampi(fp_mul_s(L.x, str, 20 - ACC, g), 0, CORR_TERM));
+          }
         }
       }
       int32_t vx = vi.x, vy = vi.y;
@@ -604,7 +745,7 @@ __global__ __launch_bounds__(256) void k_xnc(Dev D) {
 // bond rows stay empty for bondless rows), then the particle's state hash
 // (src/hash/statehash.wgsl:83-159) reduced per scene.
 __global__ __launch_bounds__(256) void k_st_thermal_hash(Dev D) {
-  uint32_t p = blockIdx.x * 256u + threadIdx.x;
+  uint32_t p = xcd_block() * 256u + threadIdx.x;
   uint32_t s = p < D.N ? D.sidOf[p] : INACTIVE;
   Dbg g;
   uint32_t words[8];
@@ -627,32 +768,49 @@ __global__ __launch_bounds__(256) void k_st_thermal_hash(Dev D) {
       uint32_t cnt = D.nbrN[p] & 0x1fffffffu;
       const uint32_t* nb = D.nbr + p;
       int32_t ax = 0, ay = 0;
-      for (uint32_t k = 0; k < cnt; ++k) {
-        uint32_t e = nb[(size_t)k * D.NS];
-        uint32_t q = e & ~NBR_BND;
-        int2 d = pred_delta(pi, D.pred[q], g);
-        int32_t idx = lut_index(d, g);
-        if (idx < 0) continue;
-        if (e & NBR_BND) {
-          int32_t at = D.lutA[idx];
-          ax = wsub(ax, clampi(fp_mul_s(fp_mul_s(at, d.x, 22, g), M[M_ADH], 16 - ACC, g), -ST_TERM, ST_TERM));
-          ay = wsub(ay, clampi(fp_mul_s(fp_mul_s(at, d.y, 22, g), M[M_ADH], 16 - ACC, g), -ST_TERM, ST_TERM));
+      for (uint32_t k = 0; k < cnt; k += XNC_UNROLL) {
+        uint32_t e[XNC_UNROLL];
+        int2 d[XNC_UNROLL], njs[XNC_UNROLL];
+        int32_t idx[XNC_UNROLL], rhoj[XNC_UNROLL], cts[XNC_UNROLL], ats[XNC_UNROLL], matj[XNC_UNROLL];
+#pragma unroll
+        for (uint32_t u = 0; u < XNC_UNROLL; ++u) e[u] = nbr_at(D, nb, k + u, cnt, p);
+#pragma unroll
+        for (uint32_t u = 0; u < XNC_UNROLL; ++u) {
+          uint32_t qj = e[u] & ~NBR_BND;
+          d[u] = pred_delta(pi, D.pred[qj], g);
+          idx[u] = (k + u < cnt) ? lut_index(d[u], g) : -1;
+          uint32_t ii = (uint32_t)maxi(idx[u], 0);
+          ats[u] = D.lutA[ii];
+          cts[u] = D.lut4[ii].z;
+          rhoj[u] = D.rho[qj];
+          matj[u] = D.ptY[qj].x;
+          njs[u] = D.nrm[qj];
+        }
+#pragma unroll
+        for (uint32_t u = 0; u < XNC_UNROLL; ++u) {
+        if (idx[u] < 0) continue;
+        const int2 d1 = d[u];
+        if (e[u] & NBR_BND) {
+          int32_t at = ats[u];
+          ax = wsub(ax, clampi(fp_mul_s(fp_mul_s(at, d1.x, 22, g), M[M_ADH], 16 - ACC, g), -ST_TERM, ST_TERM));
+          ay = wsub(ay, clampi(fp_mul_s(fp_mul_s(at, d1.y, 22, g), M[M_ADH], 16 - ACC, g), -ST_TERM, ST_TERM));
           continue;
         }
-        int32_t K = sym_k(rhoi, D.rho[q], g);
-        int32_t ct = D.lut4[idx].z;
-        const int32_t* Mj = MAT(D, (uint32_t)D.ptY[q].x);
+        int32_t K = sym_k(rhoi, rhoj[u], g);
+        int32_t ct = cts[u];
+        const int32_t* Mj = MAT(D, (uint32_t)matj[u]);
         int32_t gCoh = mat_mix(M[M_GCOH], Mj[M_GCOH]);
         int32_t gCur = mat_mix(M[M_GCUR], Mj[M_GCUR]);
-        int32_t fx = fp_neg_sat(fp_mul_s(fp_mul_s(ct, d.x, 22, g), gCoh, 16, g), g);
-        int32_t fy = fp_neg_sat(fp_mul_s(fp_mul_s(ct, d.y, 22, g), gCoh, 16, g), g);
-        int2 nj = D.nrm[q];
+        int32_t fx = fp_neg_sat(fp_mul_s(fp_mul_s(ct, d1.x, 22, g), gCoh, 16, g), g);
+        int32_t fy = fp_neg_sat(fp_mul_s(fp_mul_s(ct, d1.y, 22, g), gCoh, 16, g), g);
+        int2 nj = njs[u];
         int32_t cx = fp_mul_s(fp_sub_sat(ni.x, nj.x, g), gCur, 16, g);
         int32_t cy = fp_mul_s(fp_sub_sat(ni.y, nj.y, g), gCur, 16, g);
         fx = fp_add_sat(fx, fp_neg_sat(cx, g), g);
         fy = fp_add_sat(fy, fp_neg_sat(cy, g), g);
         ax = wadd(ax, clampi(fp_mul_s(fx, K, 16 - ACC, g), -ST_TERM, ST_TERM));
         ay = wadd(ay, clampi(fp_mul_s(fy, K, 16 - ACC, g), -ST_TERM, ST_TERM));
+        }
       }
       stOut = make_int2(fp_shr_rne(ax, ACC), fp_shr_rne(ay, ACC));
       // ---- thermal
@@ -710,7 +868,7 @@ __global__ __launch_bounds__(256) void k_st_thermal_hash(Dev D) {
   // ---- state hash: 4 salted lanes over the 8 words, H_set and H_slot terms
   const uint32_t SEED[4] = {0x9e3779b1u, 0x85ebca77u, 0xc2b2ae3du, 0x27d4eb2fu};
   const uint32_t SALT[4] = {0x00000000u, 0x51633e2du, 0x9c8f2a17u, 0xd0e5b943u};
-  uint32_t hv[8];
+  uint32_t hv[HPART_WORDS];
   if (s != INACTIVE) {
 #pragma unroll
     for (int l = 0; l < 4; ++l) {
@@ -720,28 +878,38 @@ __global__ __launch_bounds__(256) void k_st_thermal_hash(Dev D) {
       hv[l] = fmix32(h ^ 8u);
       hv[4 + l] = fmix32(mm_step(h, myid) ^ 9u);
     }
+    hv[8] = D.drop[p];
   } else {
 #pragma unroll
-    for (int l = 0; l < 8; ++l) hv[l] = 0u;
+    for (int l = 0; l < HPART_WORDS; ++l) hv[l] = 0u;
   }
-  // wave-level sum when the wave is one scene, else per-lane atomics
-  uint32_t s0 = __shfl(s, 0);
-  bool uniform = __all(s == s0);
-  if (uniform) {
-    if (s0 == INACTIVE) return;
+  // Block partial (8 hash words + truncation count): a block lying wholly in
+  // one scene stores its sums in hpart[block], which k_hash_finish folds in;
+  // a block straddling scenes adds per lane with atomics.  All are wrapping
+  // u32 sums, so the result does not depend on the split.
+  __shared__ uint32_t wpart[4][HPART_WORDS];
+  __shared__ uint32_t wscene[4];
+  const uint32_t wv = threadIdx.x >> 6, lane = threadIdx.x & 63u;
+  const uint32_t s0 = (uint32_t)__builtin_amdgcn_readfirstlane((int)s);
+  const bool wuni = __all(s == s0);
 #pragma unroll
-    for (int l = 0; l < 8; ++l) {
-      uint32_t v = hv[l];
-      for (int off = 32; off >= 1; off >>= 1) v += __shfl_xor(v, off);
-      hv[l] = v;
-    }
-    if ((threadIdx.x & 63u) == 0u) {
-#pragma unroll
-      for (int l = 0; l < 8; ++l) atomicAdd(&D.sd[s0].acc[l], hv[l]);
+  for (int l = 0; l < HPART_WORDS; ++l) {
+    uint32_t v = wave_sum(hv[l]);
+    if (lane == 0u) wpart[wv][l] = v;
+  }
+  if (lane == 0u) wscene[wv] = wuni ? s0 : MIXED;
+  __syncthreads();
+  const uint32_t bs = wscene[0];
+  const bool buni = bs != MIXED && wscene[1] == bs && wscene[2] == bs && wscene[3] == bs;
+  if (buni) {
+    if (bs != INACTIVE && threadIdx.x < HPART_WORDS) {
+      uint32_t t = threadIdx.x;
+      D.hpart[(size_t)xcd_block() * HPART_WORDS + t] = wpart[0][t] + wpart[1][t] + wpart[2][t] + wpart[3][t];
     }
   } else if (s != INACTIVE) {
 #pragma unroll
     for (int l = 0; l < 8; ++l) atomicAdd(&D.sd[s].acc[l], hv[l]);
+    if (hv[8]) atomicAdd(&D.sd[s].dbg[6], hv[8]);
   }
 }
 
@@ -767,12 +935,29 @@ __global__ __launch_bounds__(256) void k_hash_state(Dev D, uint32_t s) {
 
 // Per scene: fold the chain, append this substep's digest to the H ring,
 // clear the accumulators, advance the substep counter.
-__global__ void k_hash_finish(Dev D, const uint8_t* active) {
-  uint32_t s = blockIdx.x * 64u + threadIdx.x;
-  if (s >= D.S || !active[s]) return;
+// One wave per scene: fold the scene's block partials (blocks lying wholly
+// inside its slot range) into the atomically summed remainder, then the
+// chain step, the H ring append and the substep counter.
+__global__ __launch_bounds__(64) void k_hash_finish(Dev D, const uint8_t* active) {
+  const uint32_t s = blockIdx.x;
+  if (!active[s]) return;
   SceneD& SD = D.sd[s];
+  const SceneP& P = D.sp[s];
+  const uint32_t lane = threadIdx.x;
+  const uint32_t b0 = (P.partBase + 255u) / 256u, b1 = (P.partBase + P.n) / 256u;
+  uint32_t sum[HPART_WORDS];
+#pragma unroll
+  for (int l = 0; l < HPART_WORDS; ++l) sum[l] = 0u;
+  for (uint32_t b = b0 + lane; b < b1; b += 64u) {
+#pragma unroll
+    for (int l = 0; l < HPART_WORDS; ++l) sum[l] += D.hpart[(size_t)b * HPART_WORDS + l];
+  }
+#pragma unroll
+  for (int l = 0; l < HPART_WORDS; ++l) sum[l] = wave_sum(sum[l]);
+  if (lane != 0u) return;
   uint32_t a[8];
-  for (int l = 0; l < 8; ++l) a[l] = SD.acc[l];
+  for (int l = 0; l < 8; ++l) a[l] = SD.acc[l] + sum[l];
+  SD.dbg[6] += sum[8];
   for (int j = 0; j < 8; ++j) SD.chain[j] = mm_step(SD.chain[j], a[j] ^ rotl32(a[(j + 1) & 7], 7));
   uint32_t slot = SD.streamPos % D.ringCap;
   uint32_t* r = D.hring + ((size_t)s * D.ringCap + slot) * 8;
@@ -898,9 +1083,10 @@ struct sn_engine {
   std::vector<SceneD> sdInit;
   std::vector<uint8_t> active;
   uint8_t* dActive = nullptr;
-  std::vector<uint2> cwork[4];
-  uint2* dWork[4] = {nullptr, nullptr, nullptr, nullptr};
-  uint32_t nWork[4] = {0, 0, 0, 0};
+  std::vector<uint2> cwork[5];       // [0..3] colour groups, [4] neighbour tiles
+  uint2* dWork[5] = {nullptr, nullptr, nullptr, nullptr, nullptr};
+  uint32_t nWork[5] = {0, 0, 0, 0, 0};
+  uint32_t tileG = 1;
   uint32_t S = 0, N = 0, C = 0;
   uint32_t maxEventsPerSubstep = 0;
   uint32_t colourG = 2;
@@ -973,7 +1159,8 @@ void enqueue_substep(sn_engine* e, hipEvent_t* marks) {
   mark(3);
   hipLaunchKernelGGL(k_canon_gather, dim3(gN), dim3(256), 0, st, D);
   mark(4);
-  hipLaunchKernelGGL(k_nbr_solveA, dim3(gN), dim3(256), 0, st, D);
+  hipLaunchKernelGGL(k_nbr_tile, dim3(div_up(std::max(e->nWork[4], 1u), 4)), dim3(256), 0, st, D, e->dWork[4],
+                     e->nWork[4]);
   mark(5);
   const uint32_t iters = e->opts.iters;
   for (uint32_t it = 0; it < iters; ++it) {
@@ -989,7 +1176,7 @@ void enqueue_substep(sn_engine* e, hipEvent_t* marks) {
   mark(7);
   hipLaunchKernelGGL(k_st_thermal_hash, dim3(gN), dim3(256), 0, st, D);
   mark(8);
-  hipLaunchKernelGGL(k_hash_finish, dim3(div_up(e->S, 64)), dim3(64), 0, st, D, e->dActive);
+  hipLaunchKernelGGL(k_hash_finish, dim3(e->S), dim3(64), 0, st, D, e->dActive);
   mark(9);
 }
 
@@ -1012,6 +1199,17 @@ void build_colour_work(sn_engine* e, uint32_t G) {
     }
     e->nWork[c] = (uint32_t)w.size();
   }
+  // neighbour tiles: up to tileG consecutive cells of one row, every cell
+  auto& t = e->cwork[4];
+  t.clear();
+  for (uint32_t s = 0; s < e->S; ++s) {
+    if (!e->active[s] || e->sp[s].n == 0) continue;
+    const SceneP& P = e->sp[s];
+    for (uint32_t cy = 0; cy < P.gridH; ++cy)
+      for (uint32_t cx = 0; cx < P.gridW; cx += e->tileG)
+        t.push_back(make_uint2(P.cellBase + cy * P.gridW + cx, std::min(e->tileG, P.gridW - cx) | (s << 8)));
+  }
+  e->nWork[4] = (uint32_t)t.size();
 }
 
 void upload_layout(sn_engine* e) {
@@ -1032,7 +1230,7 @@ void upload_layout(sn_engine* e) {
   HIPCHECK(hipMemcpyAsync(e->dActive, e->active.data(), e->S, hipMemcpyHostToDevice, e->stream));
   // the cell-count total of all active scenes is what the scan writes at C
   build_colour_work(e, e->colourG);
-  for (uint32_t c = 0; c < 4; ++c) {
+  for (uint32_t c = 0; c < 5; ++c) {
     if (e->nWork[c])
       HIPCHECK(hipMemcpyAsync(e->dWork[c], e->cwork[c].data(), sizeof(uint2) * e->nWork[c], hipMemcpyHostToDevice,
                               e->stream));
@@ -1109,6 +1307,8 @@ int sn_create(const sn_opts* opts, const int32_t* luts, const int32_t* mats, sn_
     D.nbr = dalloc<uint32_t>(e, (size_t)NC * MAXNBR);
     D.NS = NC;
     D.nbrN = dalloc<uint32_t>(e, NC);
+    D.drop = dalloc<uint32_t>(e, NC);
+    D.hpart = dalloc<uint32_t>(e, (size_t)div_up(NC, 256) * HPART_WORDS);
     D.hitFlag = dalloc<uint32_t>(e, NC);
     D.hitScan = dalloc<uint32_t>(e, (size_t)NC + 1);
     int4* lut4 = dalloc<int4>(e, TN);
@@ -1130,7 +1330,7 @@ int sn_create(const sn_opts* opts, const int32_t* luts, const int32_t* mats, sn_
     D.events = dalloc<Event>(e, 1);
     e->dActive = dalloc<uint8_t>(e, SC);
     // colour work tables: at most one item per cell
-    for (uint32_t c = 0; c < 4; ++c) e->dWork[c] = dalloc<uint2>(e, CC / 2 + SC * 64 + 64);
+    for (uint32_t c = 0; c < 5; ++c) e->dWork[c] = dalloc<uint2>(e, (c == 4 ? CC : CC / 2) + SC * 64 + 64);
     HIPCHECK(hipMemset(D.cellCount, 0, sizeof(uint32_t) * CC));
   } catch (const DevError& d) {
     *out = e;
@@ -1281,9 +1481,12 @@ int sn_load_scenes(sn_engine* e, const sn_scene* scenes, uint32_t count) {
         }
         parts += sc.n;
       }
-      double avg = occupied ? (double)parts / (double)occupied : 64.0;
-      uint32_t G = (uint32_t)(64.0 / std::max(avg, 1.0) + 0.5);
+      double avg = std::max(occupied ? (double)parts / (double)occupied : 64.0, 1.0);
+      uint32_t G = (uint32_t)(64.0 / avg + 0.5);
       e->colourG = std::min(8u, std::max(1u, G));
+      // a tile serves up to 64 * NBR_PPL particles in one sweep
+      uint32_t T = (uint32_t)(64.0 * NBR_PPL * 0.6 / avg);
+      e->tileG = std::min(8u, std::max(1u, T));
     }
     upload_layout(e);
     e->loaded = true;
