@@ -50,6 +50,34 @@ def algorithmic_bytes(F: int, N: int, C: int, k: int) -> dict:
     return ph
 
 
+# kernel behind each profiled class (the HIP symbol rocprofv3 reports)
+KERNEL_OF = {"predict": "k_predict_count", "scan": "k_scan_reduce+k_scan_top+k_scan_down",
+             "scatter": "k_scatter", "canon_gather": "k_canon_gather", "nbr_solveA": "k_nbr_tile",
+             "solveA": "k_solveA", "colour": "k_colour", "xnc": "k_xnc",
+             "st_thermal_hash": "k_st_thermal_hash", "hash_finish": "k_hash_finish",
+             "impulse": "k_impulse_*"}
+
+
+def class_bytes(F: int, N: int, C: int, k: int) -> dict:
+    """The same algorithmic bytes per substep, assigned to the kernel that
+    carries each phase: the sort's cell-array traffic to the scan, its
+    per-particle traffic to scatter, finalize's state write to xnc (which
+    writes the substep's final state), thermal and the hash to the fused
+    surface-tension kernel."""
+    return {
+        "predict": 32 * F + 12 * N,
+        "scan": 24 * C,
+        "scatter": 8 * N,
+        "canon_gather": 0,
+        "nbr_solveA": 16 * N + 8 * F + 4 * C,
+        "solveA": (k - 1) * (8 * N + 8 * F) + 4 * F,
+        "colour": k * (32 * N + 32 * F),
+        "xnc": 68 * F + 64 * N + 24 * F,
+        "st_thermal_hash": 12 * N + 24 * F + 40 * F + 32 * N + 4 * F,
+        "hash_finish": 0,
+    }
+
+
 def cpu_info():
     model = platform.processor() or ""
     try:
@@ -73,10 +101,9 @@ def main():
     ap.add_argument("--steps", type=int, default=100)
     ap.add_argument("--warmup", type=int, default=10)
     ap.add_argument("--side", type=int, default=1024, help="fluid block side (1024 = 1M particles)")
-    ap.add_argument("--cpu-substeps", type=int, default=3, help="CPU-baseline window in substeps")
+    ap.add_argument("--cpu-substeps", type=int, default=8, help="CPU-baseline window in substeps")
     ap.add_argument("--cpu-threads", type=int, default=0, help="0 = all available host cores")
     ap.add_argument("--no-cpu", action="store_true")
-    ap.add_argument("--profile-substeps", type=int, default=5)
     args = ap.parse_args()
     if args.gpus != 1:
         print(json.dumps({"error": "this build runs on one GPU; --gpus must be 1",
@@ -99,10 +126,12 @@ def main():
     eng.step(args.warmup)
     eng.sync()
     eng.reset_stream()
-    # ---- timed region: K substeps, inputs resident in HBM
+    # ---- timed region: K substeps, inputs resident in HBM.  Launched eagerly
+    # with a HIP event pair around every launch on the engine's stream, so the
+    # same region yields each kernel's average launch duration.
     eng.sync()
     t1 = time.perf_counter()
-    eng.step(args.steps)
+    prof = eng.profile_step(args.steps)
     eng.sync()
     t2 = time.perf_counter()
     wall = t2 - t1
@@ -114,32 +143,37 @@ def main():
     print(f"# timed {args.steps} substeps: {ms_per_step:.3f} ms/substep, {value:.4e} ps/s",
           file=sys.stderr, flush=True)
 
-    # ---- per-kernel device time (HIP events on the engine stream)
-    prof = eng.profile_step(args.profile_substeps)
-    per = {k: v / args.profile_substeps for k, v in prof.items()}
     by = algorithmic_bytes(F, N, C, iters)
-    kernel_bytes = {"predict": by["predict"], "scan": 24 * C, "scatter": 8 * N,
-                    "canon_gather": 0, "nbr_solveA": by["nbr_solveA"],
-                    "solve": by["solve"] + by["finalize"] - (64 * N + 24 * F) + 24 * F,
-                    "xnc": by["xnc"] + 32 * N, "st_thermal_hash": by["surface_tension"] + by["thermal"] + by["hash"],
-                    "hash_finish": 0}
-    dom = max((k for k in per if k != "total"), key=lambda k: per[k])
-    dom_ms = per[dom]
-    achieved_dom = kernel_bytes.get(dom, 0) / (dom_ms * 1e-3) / 1e9 if dom_ms > 0 else 0.0
+    cls_bytes = class_bytes(F, N, C, iters)
+    assert sum(cls_bytes.values()) == by["total"]
+    per_launch = {}
+    for k, v in prof.items():
+        if k == "total" or v["calls"] == 0:
+            continue
+        avg_ms = v["ms"] / v["calls"]
+        b = cls_bytes.get(k, 0) * args.steps / v["calls"]
+        per_launch[k] = {"kernel": KERNEL_OF.get(k, k), "launches_per_substep": v["calls"] / args.steps,
+                         "avg_launch_us": round(avg_ms * 1e3, 2), "ms_per_substep": round(v["ms"] / args.steps, 4),
+                         "algorithmic_bytes_per_launch": int(b),
+                         "achieved_GBs": round(b / (avg_ms * 1e-3) / 1e9, 2) if avg_ms > 0 else 0.0}
+    dom = max(per_launch, key=lambda k: per_launch[k]["ms_per_substep"])
+    d = per_launch[dom]
     achieved_all = by["total"] / (ms_per_step * 1e-3) / 1e9
     roofline = {
         "bound": "hbm",
-        "achieved": round(achieved_all, 2),
+        "achieved": d["achieved_GBs"],
         "peak": HBM_PEAK_GBS,
         "unit": "GB/s",
-        "frac": round(achieved_all / HBM_PEAK_GBS, 5),
+        "frac": round(d["achieved_GBs"] / HBM_PEAK_GBS, 6),
         "traffic": None,
-        "kernel": "whole substep (graph of %d kernels)" % (9 + 4 * iters + iters - 1 + 2),
-        "bytes_per_fluid_particle_substep": round(by["total"] / F, 1),
-        "dominant_kernel": {"name": dom, "ms": round(dom_ms, 4),
-                            "algorithmic_bytes": kernel_bytes.get(dom, 0),
-                            "achieved_GBs": round(achieved_dom, 2)},
-        "kernel_ms_per_substep": {k: round(v, 4) for k, v in per.items()},
+        "kernel": d["kernel"],
+        "kernel_avg_launch_us": d["avg_launch_us"],
+        "kernel_algorithmic_bytes_per_launch": d["algorithmic_bytes_per_launch"],
+        "whole_substep": {"algorithmic_bytes": by["total"],
+                          "bytes_per_fluid_particle_substep": round(by["total"] / F, 1),
+                          "achieved_GBs": round(achieved_all, 2),
+                          "frac": round(achieved_all / HBM_PEAK_GBS, 6)},
+        "per_kernel": per_launch,
     }
 
     # ---- CPU baseline: the C++ restatement on this host's cores, same scene
@@ -199,6 +233,7 @@ def main():
             "cells": C,
             "iters": iters,
             "dbg_truncation_events": int(dbg[6]),
+            "dbg": [int(x) for x in dbg],
         },
         "roofline": roofline,
         "cpu_baseline": cpu_baseline,

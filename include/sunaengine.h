@@ -128,13 +128,19 @@ int sn_set_events(sn_engine* e, const sn_event* events, uint32_t count);
  * state and are not advanced by sn_step. */
 int sn_set_active(sn_engine* e, const uint8_t* active, uint32_t count);
 
-/* Run `count` substeps one at a time with HIP events around each kernel
- * class on the engine's stream; out (device ms summed over the substeps):
- * [0] whole substep, [1] predict+count, [2] cell scan, [3] scatter,
+/* Profiling form of sn_step (no counterpart in the reference, which has no
+ * device timers): enqueue `count` substeps eagerly with a HIP event pair
+ * around every launch on the engine's stream, wait, and return per kernel
+ * class the summed device ms and the launch count.  Slot 0 is the whole
+ * span (first launch start to last launch end; calls = substeps); slots
+ * 1.. are: [1] predict + cell count, [2] cell scan, [3] scatter,
  * [4] canonical gather, [5] neighbour build + iteration-0 density,
- * [6] density iterations 1.. + colour passes, [7] xsph+normals+conduct,
- * [8] surface tension + thermal + hash, [9] chain fold. */
-int sn_profile_step(sn_engine* e, uint32_t count, float out[10]);
+ * [6] density iterations 1.., [7] colour passes (solveB + applyDp, finalize
+ * on the last iteration), [8] xsph + normals + conduct, [9] surface tension
+ * + thermal + hash, [10] chain fold, [11] stir events. */
+#define SN_PROF_SLOTS 16
+int sn_profile_step(sn_engine* e, uint32_t count, float out_ms[SN_PROF_SLOTS],
+                    uint32_t out_calls[SN_PROF_SLOTS]);
 
 /* ===================================================================== 3D
  * The 3D solver (3d/sim3d.wgsl, host 3d/app.js:606-743): App.runSubsteps(k)

@@ -39,16 +39,19 @@ SN_HD int32_t clampi(int32_t x, int32_t lo, int32_t hi) { return x < lo ? lo : (
 SN_HD int32_t maxi(int32_t a, int32_t b) { return a > b ? a : b; }
 SN_HD int32_t mini(int32_t a, int32_t b) { return a < b ? a : b; }
 
+// Every operation below is branch-free: the saturated result is selected
+// and the debug counter advanced by the overflow predicate, so a wave never
+// splits on the (almost always false) saturation test.
+SN_HD int32_t sat_of(int32_t sign_src) { return (sign_src >> 31) ^ I32_MAX; }  // I32_MIN if < 0 else I32_MAX
+
 // RNE(v / 2^s) saturated to i32, s in 1..31.  floor-shift of v biased by
 // (2^(s-1) - 1 + lsb(floor(v / 2^s))) is exactly round-half-to-even.
 SN_HD int32_t shr_rne_sat64(int64_t v, unsigned s, Dbg& g) {
   int64_t odd = (v >> s) & 1;
   int64_t q = (v + ((int64_t)1 << (s - 1)) - 1 + odd) >> s;
-  if ((int64_t)(int32_t)q != q) {
-    g.mul++;
-    return v < 0 ? I32_MIN : I32_MAX;
-  }
-  return (int32_t)q;
+  const bool ovf = (int64_t)(int32_t)q != q;
+  g.mul += ovf ? 1u : 0u;
+  return ovf ? sat_of((int32_t)(v >> 32)) : (int32_t)q;
 }
 
 SN_HD int32_t fp_mul_s(int32_t a, int32_t b, unsigned s, Dbg& g) {
@@ -56,20 +59,27 @@ SN_HD int32_t fp_mul_s(int32_t a, int32_t b, unsigned s, Dbg& g) {
 }
 
 SN_HD int32_t fp_add_sat(int32_t a, int32_t b, Dbg& g) {
-  int32_t s = wadd(a, b);
-  if (((a ^ s) & (b ^ s)) < 0) { g.add++; return a < 0 ? I32_MIN : I32_MAX; }
-  return s;
+  const int32_t s = wadd(a, b);
+  const bool ovf = ((a ^ s) & (b ^ s)) < 0;
+  g.add += ovf ? 1u : 0u;
+  return ovf ? sat_of(a) : s;
 }
 SN_HD int32_t fp_sub_sat(int32_t a, int32_t b, Dbg& g) {
-  int32_t s = wsub(a, b);
-  if (((a ^ b) & (a ^ s)) < 0) { g.add++; return a < 0 ? I32_MIN : I32_MAX; }
-  return s;
+  const int32_t s = wsub(a, b);
+  const bool ovf = ((a ^ b) & (a ^ s)) < 0;
+  g.add += ovf ? 1u : 0u;
+  return ovf ? sat_of(a) : s;
 }
 SN_HD int32_t fp_neg_sat(int32_t a, Dbg& g) {
-  if (a == I32_MIN) { g.add++; return I32_MAX; }
-  return -a;
+  const bool ovf = a == I32_MIN;
+  g.add += ovf ? 1u : 0u;
+  return ovf ? I32_MAX : wsub(0, a);
 }
-SN_HD int32_t fp_abs(int32_t a, Dbg& g) { return a < 0 ? fp_neg_sat(a, g) : a; }
+SN_HD int32_t fp_abs(int32_t a, Dbg& g) {
+  const bool ovf = a == I32_MIN;
+  g.add += ovf ? 1u : 0u;
+  return ovf ? I32_MAX : (a < 0 ? wsub(0, a) : a);
+}
 
 // RNE arithmetic shift of an i32 (|result| <= 2^30, cannot overflow).
 SN_HD int32_t fp_shr_rne(int32_t x, unsigned s) {

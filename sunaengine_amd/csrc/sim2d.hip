@@ -181,6 +181,11 @@ struct Dev {
   uint32_t* nbrN;
   uint32_t* drop;         // candidates dropped by truncation, per particle
   uint32_t* hpart;        // per-block partial sums [block][HPART_WORDS]
+  const uint32_t* cellScene;  // global cell -> scene, INACTIVE if its scene is inactive
+  const uint2* tileAll;   // every neighbour tile of the active scenes {first cell, ncell | scene << 8}
+  uint2* dlist[5];        // this substep's occupied cells per colour [0..3], occupied tiles [4]
+  uint32_t* listCount;    // [5] entries of dlist (cleared by predict)
+  uint32_t nTiles;
   const int4* lut4;       // {W, G, C, S} per LUT bin
   const int32_t* lutA;    // adhesion table
   const int32_t* mats;
@@ -294,6 +299,7 @@ __device__ __forceinline__ uint32_t xcd_block() {
 // which replaces the reference's second atomic pass in `scatter`.
 __global__ __launch_bounds__(256) void k_predict_count(Dev D) {
   uint32_t p = xcd_block() * 256u + threadIdx.x;
+  if (blockIdx.x == 0 && threadIdx.x < 5u) D.listCount[threadIdx.x] = 0u;  // rebuilt after the scan
   uint32_t s = p < D.N ? D.sidOf[p] : INACTIVE;
   Dbg g;
   if (s != INACTIVE) {
@@ -393,108 +399,183 @@ __device__ __forceinline__ void solveA_finish(const Dev& D, uint32_t p, SolveAAc
   D.lam[p] = clampi(fp_neg_sat(fp_divshift(Cq, wadd(a.S, eps), 25, g), g), -LAM_MAX, LAM_MAX);
 }
 
-// One wave per tile of up to 8 consecutive cells of one row; up to
+// ============================================================ per-substep work
+// After the scan: the occupied cells of each colour and the occupied
+// neighbour tiles, appended with wave-aggregated atomics.  Every item is
+// processed independently of the others, so the order of a list (which the
+// atomics leave open) affects no result.
+__device__ __forceinline__ void append_item(bool want, uint32_t* counter, uint2* list, uint2 item) {
+  const uint64_t m = __ballot(want);
+  if (m == 0ull) return;
+  const uint32_t lane = threadIdx.x & 63u;
+  const uint32_t leader = (uint32_t)__ffsll((unsigned long long)m) - 1u;
+  uint32_t base = 0;
+  if (lane == leader) base = atomicAdd(counter, (uint32_t)__popcll(m));
+  base = (uint32_t)__shfl((int)base, (int)leader);
+  if (want) list[base + (uint32_t)__popcll(m & ((1ull << lane) - 1ull))] = item;
+}
+
+__global__ __launch_bounds__(256) void k_build_lists(Dev D) {
+  const uint32_t t = blockIdx.x * 256u + threadIdx.x;
+  {
+    bool occ = false;
+    uint32_t col = 0, s = 0;
+    if (t < D.C) {
+      s = D.cellScene[t];
+      if (s != INACTIVE) {
+        occ = D.cellStart[t + 1] != D.cellStart[t];
+        const SceneP& P = D.sp[s];
+        const uint32_t lc = t - P.cellBase;
+        col = ((lc % P.gridW) & 1u) | (((lc / P.gridW) & 1u) << 1);
+      }
+    }
+#pragma unroll
+    for (uint32_t c = 0; c < 4; ++c)
+      append_item(occ && col == c, &D.listCount[c], D.dlist[c], make_uint2(t, 1u | (s << 8)));
+  }
+  {
+    bool occ = false;
+    uint2 it = make_uint2(0u, 0u);
+    if (t < D.nTiles) {
+      it = D.tileAll[t];
+      occ = D.cellStart[it.x + (it.y & 0xffu)] != D.cellStart[it.x];
+    }
+    append_item(occ, &D.listCount[4], D.dlist[4], it);
+  }
+}
+
+// Persistent kernels walk a device list: XCD x (= blockIdx.x % 8) takes the
+// x-th eighth of the list, so the items one XCD's L2 serves stay together.
+struct ListRange {
+  uint32_t lo, hi, first, stride;
+};
+__device__ __forceinline__ ListRange xcd_range(uint32_t n, uint32_t unitsPerBlock, uint32_t unit) {
+  const uint32_t xcd = blockIdx.x & 7u, nb = gridDim.x >> 3, lb = blockIdx.x >> 3;
+  ListRange r;
+  r.lo = (uint32_t)(((uint64_t)n * xcd) >> 3);
+  r.hi = (uint32_t)(((uint64_t)n * (xcd + 1u)) >> 3);
+  r.first = r.lo + lb * unitsPerBlock + unit;
+  r.stride = nb * unitsPerBlock;
+  return r;
+}
+
+// One wave per occupied tile of up to 8 consecutive cells of one row; up to
 // NBR_PPL particles per lane, so a tile of up to 64 * NBR_PPL particles is
 // served by one sweep.  The tile's candidates — the union of its particles'
 // 3x3 stencils: rows cy-1..cy+1, columns x0-1..x1+1, each row one contiguous
 // sorted range — are loaded 64 at a time, one per lane, and broadcast to
 // every lane with v_readlane, so the candidate loop issues no per-lane
-// memory traffic.  Walking rows in order and each row in sorted order visits
-// every particle's own candidates in exactly its canonical order (dy outer,
-// dx inner, ascending id), which is what decides truncation.
+// loads.  Walking rows in order and each row in sorted order visits every
+// particle's own candidates in exactly its canonical order (dy outer, dx
+// inner, ascending id), which is what decides truncation.  Iteration 0 of
+// solveA is formed in the same sweep from (W, G) staged in LDS.
 constexpr int NBR_PPL = 2;
 
-__global__ __launch_bounds__(256) void k_nbr_tile(Dev D, const uint2* work, uint32_t nwork) {
+__global__ __launch_bounds__(256) void k_nbr_tile(Dev D) {
+  __shared__ int2 lutWG[TN];
+  for (uint32_t i = threadIdx.x; i < TN; i += 256u) {
+    const int4 L = D.lut4[i];
+    lutWG[i] = make_int2(L.x, L.y);
+  }
+  __syncthreads();
   const uint32_t lane = threadIdx.x & 63u;
-  uint32_t w = __builtin_amdgcn_readfirstlane(xcd_block() * 4u + (threadIdx.x >> 6));
-  if (w >= nwork) return;
-  uint2 item = work[w];
-  const uint32_t c0 = __builtin_amdgcn_readfirstlane(item.x);
-  const uint32_t yy = __builtin_amdgcn_readfirstlane(item.y);
-  const uint32_t ncell = yy & 0xffu, s = yy >> 8;
-  const SceneP& P = D.sp[s];
-  const int32_t sh = P.posShift;
-  const uint32_t gridW = P.gridW, gridH = P.gridH, cellBase = P.cellBase;
-  const uint32_t lc0 = c0 - cellBase;
-  const int32_t cy = (int32_t)(lc0 / gridW), cx0 = (int32_t)(lc0 % gridW);
-  const int32_t xlo = maxi(cx0 - 1, 0), xhi = mini(cx0 + (int32_t)ncell, (int32_t)gridW - 1);
-  const uint32_t plo = (uint32_t)((int32_t)D.cellStart[c0] + sh);
-  const uint32_t phi = (uint32_t)((int32_t)D.cellStart[c0 + ncell] + sh);
-  const uint32_t myRow = cellBase + (uint32_t)cy * gridW;
-  const int32_t W0 = D.lut4[0].x;
-  Dbg g;
-  for (uint32_t pbase = plo; pbase < phi; pbase += 64u * NBR_PPL) {
-    uint32_t pk[NBR_PPL], inf[NBR_PPL], cnt[NBR_PPL], bits[NBR_PPL], dropped[NBR_PPL];
-    int2 pik[NBR_PPL];
-    int32_t cxk[NBR_PPL];
-    bool live[NBR_PPL];
-    SolveAAcc acc[NBR_PPL];
+  const uint32_t nwork = (uint32_t)__builtin_amdgcn_readfirstlane((int)D.listCount[4]);
+  const ListRange R = xcd_range(nwork, 4u, threadIdx.x >> 6);
+  const int32_t W0 = lutWG[0].x;
+  for (uint32_t w = R.first; w < R.hi; w += R.stride) {
+    const uint2 item = D.dlist[4][w];
+    const uint32_t c0 = __builtin_amdgcn_readfirstlane(item.x);
+    const uint32_t yy = __builtin_amdgcn_readfirstlane(item.y);
+    const uint32_t ncell = yy & 0xffu, s = yy >> 8;
+    const SceneP& P = D.sp[s];
+    const int32_t sh = P.posShift;
+    const uint32_t gridW = P.gridW, gridH = P.gridH, cellBase = P.cellBase;
+    const uint32_t lc0 = c0 - cellBase;
+    const int32_t cy = (int32_t)(lc0 / gridW), cx0 = (int32_t)(lc0 % gridW);
+    const int32_t xlo = maxi(cx0 - 1, 0), xhi = mini(cx0 + (int32_t)ncell, (int32_t)gridW - 1);
+    const uint32_t plo = (uint32_t)((int32_t)D.cellStart[c0] + sh);
+    const uint32_t phi = (uint32_t)((int32_t)D.cellStart[c0 + ncell] + sh);
+    const uint32_t myRow = cellBase + (uint32_t)cy * gridW;
+    Dbg g;
+    for (uint32_t pbase = plo; pbase < phi; pbase += 64u * NBR_PPL) {
+      uint32_t pk[NBR_PPL], inf[NBR_PPL], cnt[NBR_PPL], bits[NBR_PPL], dropped[NBR_PPL];
+      int2 pik[NBR_PPL];
+      int32_t cxk[NBR_PPL];
+      bool live[NBR_PPL];
+      SolveAAcc acc[NBR_PPL];
 #pragma unroll
-    for (int k = 0; k < NBR_PPL; ++k) {
-      pk[k] = pbase + (uint32_t)k * 64u + lane;
-      bool valid = pk[k] < phi;
-      inf[k] = valid ? D.info[pk[k]] : 0u;
-      pik[k] = valid ? D.pred[pk[k]] : make_int2(0, 0);
-      cxk[k] = valid ? (int32_t)(D.cellY[pk[k]] - myRow) : 0;
-      live[k] = valid && !(inf[k] & INF_DEAD);
-      cnt[k] = 0;
-      bits[k] = 0;
-      dropped[k] = 0;
-      acc[k] = SolveAAcc{W0, 0, 0, 0};
-    }
-    for (int32_t dy = -1; dy <= 1; ++dy) {
-      const int32_t gy = cy + dy;
-      if (gy < 0 || gy >= (int32_t)gridH) continue;
-      const uint32_t rowc = cellBase + (uint32_t)gy * gridW;
-      const uint32_t rlo = (uint32_t)((int32_t)D.cellStart[rowc + (uint32_t)xlo] + sh);
-      const uint32_t rhi = (uint32_t)((int32_t)D.cellStart[rowc + (uint32_t)xhi + 1u] + sh);
-      for (uint32_t q0 = rlo; q0 < rhi; q0 += 64u) {
-        const uint32_t qn = mini((int32_t)(rhi - q0), 64);
-        const uint32_t ql = q0 + lane;
-        int2 cp = make_int2(0, 0);
-        uint32_t ci = 0;
-        int32_t cc = 0;
-        if (lane < qn) {
-          cp = D.pred[ql];
-          ci = D.info[ql];
-          cc = (int32_t)(D.cellY[ql] - rowc);
-        }
-        for (uint32_t j = 0; j < qn; ++j) {
-          const int32_t px = __builtin_amdgcn_readlane(cp.x, j);
-          const int32_t py = __builtin_amdgcn_readlane(cp.y, j);
-          const uint32_t qi = (uint32_t)__builtin_amdgcn_readlane((int)ci, j);
-          const int32_t qc = __builtin_amdgcn_readlane(cc, j);
-          const uint32_t q = q0 + j;
-          if (qi & INF_DEAD) continue;                     // nobody sees a corpse
+      for (int k = 0; k < NBR_PPL; ++k) {
+        pk[k] = pbase + (uint32_t)k * 64u + lane;
+        const bool valid = pk[k] < phi;
+        inf[k] = valid ? D.info[pk[k]] : 0u;
+        pik[k] = valid ? D.pred[pk[k]] : make_int2(0, 0);
+        cxk[k] = valid ? (int32_t)(D.cellY[pk[k]] - myRow) : 0;
+        live[k] = valid && !(inf[k] & INF_DEAD);
+        cnt[k] = 0;
+        bits[k] = 0;
+        dropped[k] = 0;
+        acc[k] = SolveAAcc{W0, 0, 0, 0};
+      }
+      for (int32_t dy = -1; dy <= 1; ++dy) {
+        const int32_t gy = cy + dy;
+        if (gy < 0 || gy >= (int32_t)gridH) continue;
+        const uint32_t rowc = cellBase + (uint32_t)gy * gridW;
+        const uint32_t rlo = (uint32_t)((int32_t)D.cellStart[rowc + (uint32_t)xlo] + sh);
+        const uint32_t rhi = (uint32_t)((int32_t)D.cellStart[rowc + (uint32_t)xhi + 1u] + sh);
+        for (uint32_t q0 = rlo; q0 < rhi; q0 += 64u) {
+          const uint32_t qn = (uint32_t)mini((int32_t)(rhi - q0), 64);
+          const uint32_t ql = q0 + lane;
+          int2 cp = make_int2(0, 0);
+          uint32_t ci = 0;
+          int32_t cc = 0;
+          if (lane < qn) {
+            cp = D.pred[ql];
+            ci = D.info[ql];
+            cc = (int32_t)(D.cellY[ql] - rowc);
+          }
+          // wait for this block's candidates here, so that the candidate
+          // loop below never waits on the neighbour-list stores it issues
+          __builtin_amdgcn_s_waitcnt(0x0F70);  // vmcnt(0)
+          for (uint32_t j = 0; j < qn; ++j) {
+            const int32_t px = __builtin_amdgcn_readlane(cp.x, j);
+            const int32_t py = __builtin_amdgcn_readlane(cp.y, j);
+            const uint32_t qi = (uint32_t)__builtin_amdgcn_readlane((int)ci, j);
+            const int32_t qc = __builtin_amdgcn_readlane(cc, j);
+            const uint32_t q = q0 + j;
+            if (qi & INF_DEAD) continue;                     // nobody sees a corpse
 #pragma unroll
-          for (int k = 0; k < NBR_PPL; ++k) {
-            if (!live[k] || q == pk[k] || qc < cxk[k] - 1 || qc > cxk[k] + 1) continue;
-            int2 d = pred_delta(pik[k], make_int2(px, py), g);
-            int32_t idx = lut_index(d, g);
-            if (idx < 0) continue;
-            bits[k] |= qi & (INF_SEAL | INF_CONGEAL);
-            if (cnt[k] >= MAXNBR) {
-              dropped[k]++;
-              continue;
+            for (int k = 0; k < NBR_PPL; ++k) {
+              if (!live[k] || q == pk[k] || qc < cxk[k] - 1 || qc > cxk[k] + 1) continue;
+              const int2 d = pred_delta(pik[k], make_int2(px, py), g);
+              const int32_t idx = lut_index(d, g);
+              if (idx < 0) continue;
+              bits[k] |= qi & (INF_SEAL | INF_CONGEAL);
+              if (cnt[k] >= MAXNBR) {
+                dropped[k]++;
+                continue;
+              }
+              D.nbr[(size_t)cnt[k] * D.NS + pk[k]] = q | ((qi & INF_FLUID) ? 0u : NBR_BND);
+              cnt[k]++;
+              if (inf[k] & INF_FLUID) {
+                const int2 L = lutWG[idx];
+                solveA_term(acc[k], d, make_int4(L.x, L.y, 0, 0), g);
+              }
             }
-            D.nbr[(size_t)cnt[k] * D.NS + pk[k]] = q | ((qi & INF_FLUID) ? 0u : NBR_BND);
-            cnt[k]++;
-            if (inf[k] & INF_FLUID) solveA_term(acc[k], d, D.lut4[idx], g);
           }
         }
       }
-    }
 #pragma unroll
-    for (int k = 0; k < NBR_PPL; ++k) {
-      if (pk[k] >= phi) continue;
-      D.nbrN[pk[k]] = live[k] ? (cnt[k] | ((bits[k] & INF_SEAL) ? NBR_SEAL_BIT : 0u) |
-                                 ((bits[k] & INF_CONGEAL) ? NBR_CONGEAL_BIT : 0u))
-                              : 0u;
-      D.drop[pk[k]] = dropped[k];
-      if (inf[k] & INF_FLUID) solveA_finish(D, pk[k], acc[k], MAT(D, (uint32_t)D.ptY[pk[k]].x)[M_EPS], g);
+      for (int k = 0; k < NBR_PPL; ++k) {
+        if (pk[k] >= phi) continue;
+        D.nbrN[pk[k]] = live[k] ? (cnt[k] | ((bits[k] & INF_SEAL) ? NBR_SEAL_BIT : 0u) |
+                                   ((bits[k] & INF_CONGEAL) ? NBR_CONGEAL_BIT : 0u))
+                                : 0u;
+        D.drop[pk[k]] = dropped[k];
+        if (inf[k] & INF_FLUID) solveA_finish(D, pk[k], acc[k], MAT(D, (uint32_t)D.ptY[pk[k]].x)[M_EPS], g);
+      }
     }
+    flush_dbg(D, s, g, 0);
   }
-  flush_dbg(D, s, g, 0);
 }
 
 // ============================================================ neighbour loops
@@ -545,111 +626,112 @@ __global__ __launch_bounds__(256) void k_solveA(Dev D) {
 }
 
 // ================================================================ colour pass
-// solveB_at + applyDp_at for one colour (src/sim.wgsl:1337-1474).  One wave
-// per group of up to 8 same-row cells of that colour.  Same-colour
-// neighbours always share a cell (cells of one colour are >= 2 cells apart
-// and the stencil is 3x3), and no other wave reads this group's positions,
-// so computing every dp of the group before writing any pred reproduces the
-// reference's two-dispatch deferral exactly.  On the last iteration the
-// finalize of these particles (src/sim.wgsl:2245-2265) follows in place.
-__device__ __forceinline__ int2 solveB_at(const Dev& D, uint32_t p, Dbg& g) {
-  int2 pi = D.pred[p];
-  int32_t li = D.lam[p];
-  uint32_t cnt = D.nbrN[p] & 0x1fffffffu;
+// solveB_at + applyDp_at for one colour (src/sim.wgsl:1337-1474).  One
+// workgroup per occupied cell of that colour.  Same-colour neighbours always
+// share a cell (cells of one colour are >= 2 cells apart and the stencil is
+// 3x3), and no other workgroup reads this cell's positions, so computing
+// every dp of the cell before writing any pred reproduces the reference's
+// two-dispatch deferral exactly.  Each particle's neighbour list is split
+// over L lanes (entries k = sub, sub + L, ...; L = 256 / particles, a power
+// of two up to 16) whose wrapping partial sums are combined with xor
+// shuffles, so a cell's 48-deep gather chains run L-fold shorter.  The
+// (G, S) kernel tables sit in LDS.  On the last iteration the finalize of
+// these particles (src/sim.wgsl:2245-2265) follows in place.
+__device__ __forceinline__ int2 solveB_part(const Dev& D, uint32_t p, uint32_t sub, uint32_t L,
+                                            const int2* lutGS, Dbg& g) {
+  const int2 pi = D.pred[p];
+  const int32_t li = D.lam[p];
+  const uint32_t cnt = D.nbrN[p] & 0x1fffffffu;
   const uint32_t* nb = D.nbr + p;
   int32_t dpx = 0, dpy = 0;
-  for (uint32_t k = 0; k < cnt; k += NBR_UNROLL) {
-    uint32_t e[NBR_UNROLL];
-    int2 d[NBR_UNROLL];
-    int32_t idx[NBR_UNROLL], lj[NBR_UNROLL];
-    int4 L[NBR_UNROLL];
-#pragma unroll
-    for (uint32_t u = 0; u < NBR_UNROLL; ++u) e[u] = nbr_at(D, nb, k + u, cnt, p);
-#pragma unroll
-    for (uint32_t u = 0; u < NBR_UNROLL; ++u) {
-      uint32_t q = e[u] & ~NBR_BND;
-      d[u] = pred_delta(pi, D.pred[q], g);
-      lj[u] = D.lam[q];
-      idx[u] = (k + u < cnt) ? lut_index(d[u], g) : -1;
-      L[u] = D.lut4[maxi(idx[u], 0)];
+  for (uint32_t k = sub; k < cnt; k += 2u * L) {
+    const uint32_t k1 = k + L;
+    const uint32_t e0 = nb[(size_t)k * D.NS];
+    const uint32_t e1 = k1 < cnt ? nb[(size_t)k1 * D.NS] : p;
+    const uint32_t q0 = e0 & ~NBR_BND, q1 = e1 & ~NBR_BND;
+    const int2 pj0 = D.pred[q0], pj1 = D.pred[q1];
+    const int32_t lj0 = D.lam[q0], lj1 = D.lam[q1];
+    const int2 d0 = pred_delta(pi, pj0, g), d1 = pred_delta(pi, pj1, g);
+    const int32_t i0 = lut_index(d0, g);
+    const int32_t i1 = k1 < cnt ? lut_index(d1, g) : -1;
+    if (i0 >= 0) {
+      const int2 T = lutGS[i0];
+      const int32_t co = fp_add_sat(fp_add_sat(li, (e0 & NBR_BND) ? li : lj0, g), T.y, g);
+      dpx = wadd(dpx, clampi(fp_mul_s(co, grad_q16(T.x, d0.x, g), 20 - ACC, g), -DP_TERM, DP_TERM));
+      dpy = wadd(dpy, clampi(fp_mul_s(co, grad_q16(T.x, d0.y, g), 20 - ACC, g), -DP_TERM, DP_TERM));
     }
-#pragma unroll
-    for (uint32_t u = 0; u < NBR_UNROLL; ++u) {
-      if (idx[u] < 0) continue;
-      int32_t co = fp_add_sat(fp_add_sat(li, (e[u] & NBR_BND) ? li : lj[u], g), L[u].w, g);
-      dpx = wadd(dpx, clampi(fp_mul_s(co, grad_q16(L[u].y, d[u].x, g), 20 - ACC, g), -DP_TERM, DP_TERM));
-      dpy = wadd(dpy, clampi(fp_mul_s(co, grad_q16(L[u].y, d[u].y, g), 20 - ACC, g), -DP_TERM, DP_TERM));
+    if (i1 >= 0) {
+      const int2 T = lutGS[i1];
+      const int32_t co = fp_add_sat(fp_add_sat(li, (e1 & NBR_BND) ? li : lj1, g), T.y, g);
+      dpx = wadd(dpx, clampi(fp_mul_s(co, grad_q16(T.x, d1.x, g), 20 - ACC, g), -DP_TERM, DP_TERM));
+      dpy = wadd(dpy, clampi(fp_mul_s(co, grad_q16(T.x, d1.y, g), 20 - ACC, g), -DP_TERM, DP_TERM));
     }
   }
   return make_int2(dpx, dpy);
 }
 
-__global__ __launch_bounds__(256) void k_colour(Dev D, const uint2* work, uint32_t nwork, uint32_t last) {
-  uint32_t lane = threadIdx.x & 63u;
-  uint32_t w = __builtin_amdgcn_readfirstlane(xcd_block() * 4u + (threadIdx.x >> 6));
-  if (w >= nwork) return;               // whole wave exits together
-  uint2 item = work[w];                 // {first global cell, ncell | scene << 8}
-  item.x = __builtin_amdgcn_readfirstlane(item.x);
-  item.y = __builtin_amdgcn_readfirstlane(item.y);
-  uint32_t ncell = item.y & 0xffu;
-  uint32_t s = item.y >> 8;
-  const SceneP& P = D.sp[s];
-  Dbg g;
-  // lanes < ncell own one cell of the group: its sorted range
-  uint32_t myLo = 0, myCnt = 0;
-  if (lane < ncell) {
-    uint32_t c = item.x + 2u * lane;
-    myLo = (uint32_t)((int32_t)D.cellStart[c] + P.posShift);
-    myCnt = (uint32_t)((int32_t)D.cellStart[c + 1] + P.posShift) - myLo;
+__global__ __launch_bounds__(256) void k_colour(Dev D, uint32_t colour, uint32_t last) {
+  __shared__ int2 lutGS[TN];
+  for (uint32_t i = threadIdx.x; i < TN; i += 256u) {
+    const int4 L = D.lut4[i];
+    lutGS[i] = make_int2(L.y, L.w);
   }
-  // exclusive prefix over the group's cells (ncell <= 8)
-  uint32_t T = 0, myEx = 0;
-  for (uint32_t k = 0; k < ncell; ++k) {
-    uint32_t ck = __shfl(myCnt, (int)k);
-    if (k < lane) myEx += ck;
-    T += ck;
+  __syncthreads();
+  const uint32_t n = (uint32_t)__builtin_amdgcn_readfirstlane((int)D.listCount[colour]);
+  const uint2* list = D.dlist[colour];
+  const ListRange R = xcd_range(n, 1u, 0u);
+  for (uint32_t w = R.first; w < R.hi; w += R.stride) {
+    const uint2 item = list[w];
+    const uint32_t c = __builtin_amdgcn_readfirstlane(item.x);
+    const uint32_t s = __builtin_amdgcn_readfirstlane(item.y) >> 8;
+    const SceneP& P = D.sp[s];
+    const uint32_t plo = (uint32_t)((int32_t)D.cellStart[c] + P.posShift);
+    const uint32_t cnt = (uint32_t)__builtin_amdgcn_readfirstlane((int)(D.cellStart[c + 1] - D.cellStart[c]));
+    Dbg g;
+    uint32_t L = 16;
+    while (L > 1u && L * cnt > 256u) L >>= 1;
+    const uint32_t per = 256u / L, sub = threadIdx.x & (L - 1u);
+    // pass 1: dp of every particle of the cell
+    for (uint32_t base = 0; base < cnt; base += per) {
+      const uint32_t pl = base + threadIdx.x / L;
+      const uint32_t p = plo + pl;
+      const bool act = pl < cnt && (D.info[p] & INF_FLUID);
+      int2 dp = make_int2(0, 0);
+      if (act) dp = solveB_part(D, p, sub, L, lutGS, g);
+      for (uint32_t off = 1; off < L; off <<= 1) {
+        dp.x = wadd(dp.x, __shfl_xor(dp.x, (int)off));
+        dp.y = wadd(dp.y, __shfl_xor(dp.y, (int)off));
+      }
+      if (act && sub == 0u) D.dp[p] = dp;
+    }
+    __syncthreads();
+    // pass 2: apply (and finalize on the last iteration)
+    for (uint32_t pl = threadIdx.x; pl < cnt; pl += 256u) {
+      const uint32_t p = plo + pl;
+      if (!(D.info[p] & INF_FLUID)) continue;
+      const int4 b = D.ptY[p];
+      const uint32_t fl = (uint32_t)b.y;
+      int2 pr = D.pred[p];
+      if ((fl & (FLAG_RIGID | FLAG_ANCHOR)) == 0u) {
+        const int32_t* M = MAT(D, (uint32_t)b.x);
+        const int32_t dm = M[M_DPMAX], dmN = fp_neg_sat(dm, g);
+        const int2 dp = D.dp[p];
+        const int32_t ddx = clampi(fp_shr_rne(dp.x, ACC), dmN, dm);
+        const int32_t ddy = clampi(fp_shr_rne(dp.y, ACC), dmN, dm);
+        pr = wall_clamp(P, make_int2(fp_add_sat(pr.x, ddx, g), fp_add_sat(pr.y, ddy, g)));
+        D.pred[p] = pr;
+      }
+      if (last) {
+        const int4 a = D.psY[p];
+        const int32_t vmaxN = fp_neg_sat(P.vmax, g);
+        const int32_t vx = clampi(fp_sub_sat(pr.x, a.x, g), vmaxN, P.vmax);
+        const int32_t vy = clampi(fp_sub_sat(pr.y, a.y, g), vmaxN, P.vmax);
+        D.vtmp[p] = make_int2(vx, vy);
+        D.rinv[p] = clampi(fp_divshift(RHO0, maxi(D.rho[p], RHO_FLOOR_RINV), 16, g), 0, RINV_MAX);
+      }
+    }
+    flush_dbg(D, s, g, 0);
   }
-  // pass 1: dp of every particle of the group
-  for (uint32_t f0 = 0; f0 < T; f0 += 64u) {
-    uint32_t f = f0 + lane;
-    uint32_t p = 0xffffffffu;
-    for (uint32_t k = 0; k < ncell; ++k) {
-      uint32_t ek = __shfl(myEx, (int)k), ck = __shfl(myCnt, (int)k), lk = __shfl(myLo, (int)k);
-      if (f >= ek && f < ek + ck) p = lk + (f - ek);
-    }
-    if (p != 0xffffffffu && (D.info[p] & INF_FLUID)) D.dp[p] = solveB_at(D, p, g);
-  }
-  // pass 2: apply (and finalize on the last iteration)
-  for (uint32_t f0 = 0; f0 < T; f0 += 64u) {
-    uint32_t f = f0 + lane;
-    uint32_t p = 0xffffffffu;
-    for (uint32_t k = 0; k < ncell; ++k) {
-      uint32_t ek = __shfl(myEx, (int)k), ck = __shfl(myCnt, (int)k), lk = __shfl(myLo, (int)k);
-      if (f >= ek && f < ek + ck) p = lk + (f - ek);
-    }
-    if (p == 0xffffffffu || !(D.info[p] & INF_FLUID)) continue;
-    int4 b = D.ptY[p];
-    uint32_t fl = (uint32_t)b.y;
-    int2 pr = D.pred[p];
-    if ((fl & (FLAG_RIGID | FLAG_ANCHOR)) == 0u) {
-      const int32_t* M = MAT(D, (uint32_t)b.x);
-      int32_t dm = M[M_DPMAX], dmN = fp_neg_sat(dm, g);
-      int2 dp = D.dp[p];
-      int32_t ddx = clampi(fp_shr_rne(dp.x, ACC), dmN, dm);
-      int32_t ddy = clampi(fp_shr_rne(dp.y, ACC), dmN, dm);
-      pr = wall_clamp(P, make_int2(fp_add_sat(pr.x, ddx, g), fp_add_sat(pr.y, ddy, g)));
-      D.pred[p] = pr;
-    }
-    if (last) {
-      int4 a = D.psY[p];
-      int32_t vmaxN = fp_neg_sat(P.vmax, g);
-      int32_t vx = clampi(fp_sub_sat(pr.x, a.x, g), vmaxN, P.vmax);
-      int32_t vy = clampi(fp_sub_sat(pr.y, a.y, g), vmaxN, P.vmax);
-      D.vtmp[p] = make_int2(vx, vy);
-      D.rinv[p] = clampi(fp_divshift(RHO0, maxi(D.rho[p], RHO_FLOOR_RINV), 16, g), 0, RINV_MAX);
-    }
-  }
-  flush_dbg(D, s, g, 0);
 }
 
 // ========================================================= xsph+normals+conduct
@@ -1083,21 +1165,22 @@ struct sn_engine {
   std::vector<SceneD> sdInit;
   std::vector<uint8_t> active;
   uint8_t* dActive = nullptr;
-  std::vector<uint2> cwork[5];       // [0..3] colour groups, [4] neighbour tiles
-  uint2* dWork[5] = {nullptr, nullptr, nullptr, nullptr, nullptr};
-  uint32_t nWork[5] = {0, 0, 0, 0, 0};
+  std::vector<uint2> tiles;          // every neighbour tile of the active scenes
   uint32_t tileG = 1;
+  uint32_t nbrBlocks = 0, colourBlocks = 0;  // persistent grid sizes
   uint32_t S = 0, N = 0, C = 0;
   uint32_t maxEventsPerSubstep = 0;
-  uint32_t colourG = 2;
   Event* dEvents = nullptr;
   uint32_t eventCap = 0;
   std::vector<void*> allocs;
   hipGraphExec_t graph = nullptr;
   uint32_t graphSubsteps = 0;
   bool loaded = false;
-  hipEvent_t ev[10];
-  // last sn_profile_step timings
+  // profiling: an event pair around every launch, tagged with its class
+  std::vector<hipEvent_t> evPool;
+  std::vector<int> evClass;
+  uint32_t evUsed = 0;
+  bool profiling = false;
 };
 
 namespace {
@@ -1131,76 +1214,104 @@ void enqueue_scan(sn_engine* e, uint32_t* count, uint32_t n, uint32_t* out, bool
 }
 
 // One substep of every active scene, enqueued on the engine stream.
-void enqueue_substep(sn_engine* e, hipEvent_t* marks) {
+// Kernel classes timed by sn_profile_step (index into its outputs, + 1).
+enum ProfClass {
+  PC_PREDICT = 0, PC_SCAN, PC_SCATTER, PC_CANON, PC_NBR, PC_SOLVEA, PC_COLOUR, PC_XNC, PC_STH, PC_FINISH,
+  PC_IMPULSE, PC_LISTS, PC_COUNT
+};
+
+// Record an event before (begin) / after (end) one launch while profiling.
+struct ProfScope {
+  sn_engine* e;
+  int cls;
+  ProfScope(sn_engine* e_, int cls_) : e(e_), cls(cls_) {
+    if (e->profiling) record();
+  }
+  ~ProfScope() {
+    if (e->profiling) record();
+  }
+  void record() {
+    if (e->evUsed >= e->evPool.size()) {
+      hipEvent_t ev;
+      HIPCHECK(hipEventCreate(&ev));
+      e->evPool.push_back(ev);
+      e->evClass.push_back(0);
+    }
+    e->evClass[e->evUsed] = cls;
+    HIPCHECK(hipEventRecord(e->evPool[e->evUsed], e->stream));
+    e->evUsed++;
+  }
+};
+
+void enqueue_substep(sn_engine* e) {
   Dev& D = e->D;
   const uint32_t gN = div_up(std::max(D.N, 1u), 256);
   hipStream_t st = e->stream;
   // scheduled stir events due at this substep (applied in arming order)
   for (uint32_t gi = 0; gi < e->maxEventsPerSubstep; ++gi) {
+    ProfScope ps(e, PC_IMPULSE);
     hipLaunchKernelGGL(k_impulse_mark, dim3(gN), dim3(256), 0, st, D, gi);
     enqueue_scan(e, D.hitFlag, D.N, D.hitScan, false);
     hipLaunchKernelGGL(k_impulse_apply, dim3(gN), dim3(256), 0, st, D, gi);
   }
-  if (e->maxEventsPerSubstep)
+  if (e->maxEventsPerSubstep) {
+    ProfScope ps(e, PC_IMPULSE);
     hipLaunchKernelGGL(k_impulse_advance, dim3(div_up(e->S, 64)), dim3(64), 0, st, D, e->dActive,
                        e->maxEventsPerSubstep);
-  // marks (profiling only): 0 predict | 1 scan | 2 scatter | 3 canon_gather |
-  // 4 nbr_solveA | 5 solveA(k>0) + colour passes | 6 xnc | 7 st_thermal_hash |
-  // 8 hash_finish | 9 end
-  auto mark = [&](int k) {
-    if (marks) hipEventRecord(marks[k], st);
-  };
-  mark(0);
-  hipLaunchKernelGGL(k_predict_count, dim3(gN), dim3(256), 0, st, D);
-  mark(1);
-  enqueue_scan(e, D.cellCount, D.C, D.cellStart, true);
-  mark(2);
-  hipLaunchKernelGGL(k_scatter, dim3(gN), dim3(256), 0, st, D);
-  mark(3);
-  hipLaunchKernelGGL(k_canon_gather, dim3(gN), dim3(256), 0, st, D);
-  mark(4);
-  hipLaunchKernelGGL(k_nbr_tile, dim3(div_up(std::max(e->nWork[4], 1u), 4)), dim3(256), 0, st, D, e->dWork[4],
-                     e->nWork[4]);
-  mark(5);
+  }
+  {
+    ProfScope ps(e, PC_PREDICT);
+    hipLaunchKernelGGL(k_predict_count, dim3(gN), dim3(256), 0, st, D);
+  }
+  {
+    ProfScope ps(e, PC_SCAN);
+    enqueue_scan(e, D.cellCount, D.C, D.cellStart, true);
+  }
+  {
+    ProfScope ps(e, PC_SCATTER);
+    hipLaunchKernelGGL(k_scatter, dim3(gN), dim3(256), 0, st, D);
+  }
+  {
+    ProfScope ps(e, PC_CANON);
+    hipLaunchKernelGGL(k_canon_gather, dim3(gN), dim3(256), 0, st, D);
+  }
+  {
+    ProfScope ps(e, PC_LISTS);
+    hipLaunchKernelGGL(k_build_lists, dim3(div_up(std::max(std::max(D.C, D.nTiles), 1u), 256)), dim3(256), 0, st, D);
+  }
+  {
+    ProfScope ps(e, PC_NBR);
+    hipLaunchKernelGGL(k_nbr_tile, dim3(e->nbrBlocks), dim3(256), 0, st, D);
+  }
   const uint32_t iters = e->opts.iters;
   for (uint32_t it = 0; it < iters; ++it) {
-    if (it > 0) hipLaunchKernelGGL(k_solveA, dim3(gN), dim3(256), 0, st, D);
+    if (it > 0) {
+      ProfScope ps(e, PC_SOLVEA);
+      hipLaunchKernelGGL(k_solveA, dim3(gN), dim3(256), 0, st, D);
+    }
     for (uint32_t c = 0; c < 4; ++c) {
-      if (e->nWork[c] == 0) continue;
-      hipLaunchKernelGGL(k_colour, dim3(div_up(e->nWork[c], 4)), dim3(256), 0, st, D, e->dWork[c],
-                         e->nWork[c], (uint32_t)(it + 1 == iters));
+      ProfScope ps(e, PC_COLOUR);
+      hipLaunchKernelGGL(k_colour, dim3(e->colourBlocks), dim3(256), 0, st, D, c, (uint32_t)(it + 1 == iters));
     }
   }
-  mark(6);
-  hipLaunchKernelGGL(k_xnc, dim3(gN), dim3(256), 0, st, D);
-  mark(7);
-  hipLaunchKernelGGL(k_st_thermal_hash, dim3(gN), dim3(256), 0, st, D);
-  mark(8);
-  hipLaunchKernelGGL(k_hash_finish, dim3(e->S), dim3(64), 0, st, D, e->dActive);
-  mark(9);
+  {
+    ProfScope ps(e, PC_XNC);
+    hipLaunchKernelGGL(k_xnc, dim3(gN), dim3(256), 0, st, D);
+  }
+  {
+    ProfScope ps(e, PC_STH);
+    hipLaunchKernelGGL(k_st_thermal_hash, dim3(gN), dim3(256), 0, st, D);
+  }
+  {
+    ProfScope ps(e, PC_FINISH);
+    hipLaunchKernelGGL(k_hash_finish, dim3(e->S), dim3(64), 0, st, D, e->dActive);
+  }
 }
 
-// Colour work: every active scene's cells of colour c, grouped by up to G
-// consecutive same-colour cells of one row.
-void build_colour_work(sn_engine* e, uint32_t G) {
-  for (uint32_t c = 0; c < 4; ++c) {
-    auto& w = e->cwork[c];
-    w.clear();
-    uint32_t a = c & 1u, b = c >> 1;
-    for (uint32_t s = 0; s < e->S; ++s) {
-      if (!e->active[s] || e->sp[s].n == 0) continue;
-      const SceneP& P = e->sp[s];
-      for (uint32_t cy = b; cy < P.gridH; cy += 2) {
-        for (uint32_t cx = a; cx < P.gridW; cx += 2 * G) {
-          uint32_t ncell = std::min(G, (P.gridW - cx + 1) / 2);
-          w.push_back(make_uint2(P.cellBase + cy * P.gridW + cx, ncell | (s << 8)));
-        }
-      }
-    }
-    e->nWork[c] = (uint32_t)w.size();
-  }
-  // neighbour tiles: up to tileG consecutive cells of one row, every cell
-  auto& t = e->cwork[4];
+// Every neighbour tile of the active scenes: up to tileG consecutive cells of
+// one row; k_build_lists keeps the occupied ones each substep.
+void build_tiles(sn_engine* e) {
+  auto& t = e->tiles;
   t.clear();
   for (uint32_t s = 0; s < e->S; ++s) {
     if (!e->active[s] || e->sp[s].n == 0) continue;
@@ -1209,7 +1320,6 @@ void build_colour_work(sn_engine* e, uint32_t G) {
       for (uint32_t cx = 0; cx < P.gridW; cx += e->tileG)
         t.push_back(make_uint2(P.cellBase + cy * P.gridW + cx, std::min(e->tileG, P.gridW - cx) | (s << 8)));
   }
-  e->nWork[4] = (uint32_t)t.size();
 }
 
 void upload_layout(sn_engine* e) {
@@ -1228,13 +1338,20 @@ void upload_layout(sn_engine* e) {
   HIPCHECK(hipMemcpyAsync((void*)e->D.sidOf, sid.data(), sizeof(uint32_t) * e->N, hipMemcpyHostToDevice, e->stream));
   HIPCHECK(hipMemcpyAsync((void*)e->D.sp, e->sp.data(), sizeof(SceneP) * e->S, hipMemcpyHostToDevice, e->stream));
   HIPCHECK(hipMemcpyAsync(e->dActive, e->active.data(), e->S, hipMemcpyHostToDevice, e->stream));
-  // the cell-count total of all active scenes is what the scan writes at C
-  build_colour_work(e, e->colourG);
-  for (uint32_t c = 0; c < 5; ++c) {
-    if (e->nWork[c])
-      HIPCHECK(hipMemcpyAsync(e->dWork[c], e->cwork[c].data(), sizeof(uint2) * e->nWork[c], hipMemcpyHostToDevice,
-                              e->stream));
+  // cell -> scene for the cells of active scenes
+  std::vector<uint32_t> cs(std::max(e->C, 1u), INACTIVE);
+  for (uint32_t s = 0; s < e->S; ++s) {
+    if (!e->active[s]) continue;
+    const SceneP& P = e->sp[s];
+    for (uint32_t k = 0; k < P.cellTotal; ++k) cs[P.cellBase + k] = s;
   }
+  HIPCHECK(hipMemcpyAsync((void*)e->D.cellScene, cs.data(), sizeof(uint32_t) * e->C, hipMemcpyHostToDevice,
+                          e->stream));
+  build_tiles(e);
+  e->D.nTiles = (uint32_t)e->tiles.size();
+  if (e->D.nTiles)
+    HIPCHECK(hipMemcpyAsync((void*)e->D.tileAll, e->tiles.data(), sizeof(uint2) * e->D.nTiles, hipMemcpyHostToDevice,
+                            e->stream));
   HIPCHECK(hipStreamSynchronize(e->stream));
 }
 
@@ -1273,7 +1390,6 @@ int sn_create(const sn_opts* opts, const int32_t* luts, const int32_t* mats, sn_
   try {
     HIPCHECK(hipSetDevice((int)e->opts.device));
     HIPCHECK(hipStreamCreateWithFlags(&e->stream, hipStreamNonBlocking));
-    for (auto& ev : e->ev) HIPCHECK(hipEventCreate(&ev));
     const uint32_t NC = e->opts.max_particles, CC = e->opts.max_cells, SC = e->opts.max_scenes;
     Dev& D = e->D;
     D.ps = dalloc<int4>(e, NC);
@@ -1329,8 +1445,22 @@ int sn_create(const sn_opts* opts, const int32_t* luts, const int32_t* mats, sn_
     D.ringCap = e->opts.stream_cap;
     D.events = dalloc<Event>(e, 1);
     e->dActive = dalloc<uint8_t>(e, SC);
-    // colour work tables: at most one item per cell
-    for (uint32_t c = 0; c < 5; ++c) e->dWork[c] = dalloc<uint2>(e, (c == 4 ? CC : CC / 2) + SC * 64 + 64);
+    // per-substep work lists: at most one item per cell (tiles <= cells)
+    D.cellScene = dalloc<uint32_t>(e, CC);
+    D.tileAll = dalloc<uint2>(e, CC);
+    for (uint32_t c = 0; c < 5; ++c) D.dlist[c] = dalloc<uint2>(e, CC);
+    D.listCount = dalloc<uint32_t>(e, 8);
+    // persistent grids: as many workgroups as can be resident, a multiple of 8 (XCDs)
+    {
+      hipDeviceProp_t prop;
+      HIPCHECK(hipGetDeviceProperties(&prop, (int)e->opts.device));
+      int bN = 0, bC = 0;
+      HIPCHECK(hipOccupancyMaxActiveBlocksPerMultiprocessor(&bN, reinterpret_cast<const void*>(k_nbr_tile), 256, 0));
+      HIPCHECK(hipOccupancyMaxActiveBlocksPerMultiprocessor(&bC, reinterpret_cast<const void*>(k_colour), 256, 0));
+      const uint32_t cus = (uint32_t)std::max(prop.multiProcessorCount, 1);
+      e->nbrBlocks = std::max(8u, (cus * (uint32_t)std::max(bN, 1)) & ~7u);
+      e->colourBlocks = std::max(8u, (cus * (uint32_t)std::max(bC, 1)) & ~7u);
+    }
     HIPCHECK(hipMemset(D.cellCount, 0, sizeof(uint32_t) * CC));
   } catch (const DevError& d) {
     *out = e;
@@ -1347,8 +1477,7 @@ void sn_destroy(sn_engine* e) {
   drop_graph(e);
   for (void* p : e->allocs) hipFree(p);
   if (e->dEvents) hipFree(e->dEvents);
-  for (auto& ev : e->ev)
-    if (ev) hipEventDestroy(ev);
+  for (hipEvent_t ev : e->evPool) hipEventDestroy(ev);
   if (e->stream) hipStreamDestroy(e->stream);
   delete e;
 }
@@ -1464,8 +1593,7 @@ int sn_load_scenes(sn_engine* e, const sn_scene* scenes, uint32_t count) {
     HIPCHECK(hipMemcpyAsync(D.sd, sd.data(), sizeof(SceneD) * count, hipMemcpyHostToDevice, e->stream));
     HIPCHECK(hipMemsetAsync(D.cellCount, 0, sizeof(uint32_t) * std::max(e->C, 1u), e->stream));
     e->maxEventsPerSubstep = 0;
-    // colour group size: about one wave of particles per group at the
-    // initial occupancy of occupied cells
+    // neighbour tile width from the initial occupancy of occupied cells
     {
       uint64_t occupied = 0, parts = 0;
       for (uint32_t s = 0; s < count; ++s) {
@@ -1482,8 +1610,6 @@ int sn_load_scenes(sn_engine* e, const sn_scene* scenes, uint32_t count) {
         parts += sc.n;
       }
       double avg = std::max(occupied ? (double)parts / (double)occupied : 64.0, 1.0);
-      uint32_t G = (uint32_t)(64.0 / avg + 0.5);
-      e->colourG = std::min(8u, std::max(1u, G));
       // a tile serves up to 64 * NBR_PPL particles in one sweep
       uint32_t T = (uint32_t)(64.0 * NBR_PPL * 0.6 / avg);
       e->tileG = std::min(8u, std::max(1u, T));
@@ -1508,7 +1634,7 @@ int sn_step(sn_engine* e, uint32_t count) {
     if (!e->graph) {
       hipGraph_t g;
       HIPCHECK(hipStreamBeginCapture(e->stream, hipStreamCaptureModeThreadLocal));
-      for (uint32_t k = 0; k < G; ++k) enqueue_substep(e, nullptr);
+      for (uint32_t k = 0; k < G; ++k) enqueue_substep(e);
       HIPCHECK(hipStreamEndCapture(e->stream, &g));
       HIPCHECK(hipGraphInstantiate(&e->graph, g, nullptr, nullptr, 0));
       HIPCHECK(hipGraphDestroy(g));
@@ -1519,7 +1645,7 @@ int sn_step(sn_engine* e, uint32_t count) {
       HIPCHECK(hipGraphLaunch(e->graph, e->stream));
       done += e->graphSubsteps;
     }
-    for (; done < count; ++done) enqueue_substep(e, nullptr);
+    for (; done < count; ++done) enqueue_substep(e);
     HIPCHECK(hipGetLastError());
   } catch (const DevError& d) {
     return fail(e, SN_E_DEVICE, d.msg);
@@ -1539,23 +1665,37 @@ int sn_sync(sn_engine* e) {
   }
 }
 
-int sn_profile_step(sn_engine* e, uint32_t count, float out[10]) {
-  if (!e || !out) return SN_E_ARG;
+int sn_profile_step(sn_engine* e, uint32_t count, float out_ms[SN_PROF_SLOTS], uint32_t out_calls[SN_PROF_SLOTS]) {
+  if (!e || !out_ms || !out_calls) return SN_E_ARG;
   if (!e->loaded) return fail(e, SN_E_STATE, "[aether] step() called before loadScene()");
   try {
     HIPCHECK(hipSetDevice((int)e->opts.device));
-    float acc[10] = {0, 0, 0, 0, 0, 0, 0, 0, 0, 0};
-    for (uint32_t k = 0; k < count; ++k) {
-      enqueue_substep(e, e->ev);
-      HIPCHECK(hipEventSynchronize(e->ev[9]));
-      for (int m = 0; m < 9; ++m) {
-        float t;
-        HIPCHECK(hipEventElapsedTime(&t, e->ev[m], e->ev[m + 1]));
-        acc[1 + m] += t;
-        acc[0] += t;
-      }
+    for (int k = 0; k < SN_PROF_SLOTS; ++k) {
+      out_ms[k] = 0.0f;
+      out_calls[k] = 0;
     }
-    for (int k = 0; k < 10; ++k) out[k] = acc[k];
+    if (count == 0) return SN_OK;
+    e->profiling = true;
+    e->evUsed = 0;
+    try {
+      for (uint32_t k = 0; k < count; ++k) enqueue_substep(e);
+    } catch (...) {
+      e->profiling = false;
+      throw;
+    }
+    e->profiling = false;
+    const uint32_t used = e->evUsed;
+    HIPCHECK(hipEventSynchronize(e->evPool[used - 1]));
+    for (uint32_t i = 0; i + 1 < used; i += 2) {
+      float t;
+      HIPCHECK(hipEventElapsedTime(&t, e->evPool[i], e->evPool[i + 1]));
+      out_ms[1 + e->evClass[i]] += t;
+      out_calls[1 + e->evClass[i]] += 1;
+    }
+    float tot;
+    HIPCHECK(hipEventElapsedTime(&tot, e->evPool[0], e->evPool[used - 1]));
+    out_ms[0] = tot;
+    out_calls[0] = count;
   } catch (const DevError& d) {
     return fail(e, SN_E_DEVICE, d.msg);
   }

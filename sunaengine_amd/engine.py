@@ -225,14 +225,20 @@ class Engine:
         m = np.ascontiguousarray(np.asarray(mask, dtype=np.uint8))
         self._check(native.lib().sn_set_active(self._h, m.ctypes.data, m.size))
 
-    PROFILE_KEYS = ("total", "predict", "scan", "scatter", "canon_gather", "nbr_solveA",
-                    "solve", "xnc", "st_thermal_hash", "hash_finish")
+    # kernel classes of sn_profile_step, slot 0 = the whole span
+    PROFILE_KEYS = ("total", "predict", "scan", "scatter", "canon_gather", "nbr_solveA", "solveA",
+                    "colour", "xnc", "st_thermal_hash", "hash_finish", "impulse")
 
     def profile_step(self, count: int = 1) -> dict:
-        """Device ms per kernel class, summed over `count` substeps."""
-        out = np.zeros(10, np.float32)
-        self._check(native.lib().sn_profile_step(self._h, int(count), out.ctypes.data))
-        return {k: float(v) for k, v in zip(self.PROFILE_KEYS, out)}
+        """Run `count` substeps eagerly with an event pair around every launch.
+
+        Returns {class: {"ms": summed device ms, "calls": launches}}; "total"
+        is the span from the first launch's start to the last one's end.
+        """
+        ms = np.zeros(16, np.float32)
+        calls = np.zeros(16, np.uint32)
+        self._check(native.lib().sn_profile_step(self._h, int(count), ms.ctypes.data, calls.ctypes.data))
+        return {k: {"ms": float(ms[i]), "calls": int(calls[i])} for i, k in enumerate(self.PROFILE_KEYS)}
 
 
 def create_engine(**opts) -> Engine:

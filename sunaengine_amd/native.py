@@ -104,7 +104,7 @@ def lib():
         "sn_spawn_fluid": (C.c_int, [P, P, u32, P, u32, C.POINTER(u32)]),
         "sn_set_events": (C.c_int, [P, P, u32]),
         "sn_set_active": (C.c_int, [P, P, u32]),
-        "sn_profile_step": (C.c_int, [P, u32, P]),
+        "sn_profile_step": (C.c_int, [P, u32, P, P]),
         "sn3_create": (C.c_int, [C.POINTER(sn_opts), P, u32, C.POINTER(P)]),
         "sn3_destroy": (None, [P]),
         "sn3_last_error": (C.c_char_p, [P]),
