@@ -135,9 +135,10 @@ int sn_set_active(sn_engine* e, const uint8_t* active, uint32_t count);
  * span (first launch start to last launch end; calls = substeps); slots
  * 1.. are: [1] predict + cell count, [2] cell scan, [3] scatter,
  * [4] canonical gather, [5] neighbour build + iteration-0 density,
- * [6] density iterations 1.., [7] colour passes (solveB + applyDp, finalize
- * on the last iteration), [8] xsph + normals + conduct, [9] surface tension
- * + thermal + hash, [10] chain fold, [11] stir events. */
+ * [6] density iterations 1.., [7] solveB per colour, [8] xsph + normals +
+ * conduct, [9] surface tension + thermal + hash, [10] chain fold, [11] stir
+ * events, [12] work-list build, [13] applyDp per colour (finalize on the
+ * last iteration). */
 #define SN_PROF_SLOTS 16
 int sn_profile_step(sn_engine* e, uint32_t count, float out_ms[SN_PROF_SLOTS],
                     uint32_t out_calls[SN_PROF_SLOTS]);

@@ -400,47 +400,60 @@ __device__ __forceinline__ void solveA_finish(const Dev& D, uint32_t p, SolveAAc
 }
 
 // ============================================================ per-substep work
-// After the scan: the occupied cells of each colour and the occupied
-// neighbour tiles, appended with wave-aggregated atomics.  Every item is
-// processed independently of the others, so the order of a list (which the
-// atomics leave open) affects no result.
-__device__ __forceinline__ void append_item(bool want, uint32_t* counter, uint2* list, uint2 item) {
-  const uint64_t m = __ballot(want);
-  if (m == 0ull) return;
+// After the scan: work lists of this substep.  [0..3]: every occupied cell
+// of colour c, one item {cell, chunk} per CELL_CHUNK of its particles; [4]:
+// every occupied neighbour tile, one item {tile, chunk} per TILE_CHUNK of its
+// particles.  Chunking bounds every item's work, so a cell that walls or
+// collisions have packed with hundreds of particles is spread over many
+// waves instead of stalling one.  Items are appended with wave-aggregated
+// atomics; every item is processed independently of the others, so list
+// order (which the atomics leave open) affects no result.
+constexpr int NBR_PPL = 2;             // particles per lane in the neighbour sweep
+constexpr uint32_t CELL_CHUNK = 64;
+constexpr uint32_t TILE_CHUNK = 128;   // 64 lanes x NBR_PPL
+
+__device__ __forceinline__ void append_chunks(uint32_t n, uint32_t* counter, uint2* list, uint32_t key) {
   const uint32_t lane = threadIdx.x & 63u;
-  const uint32_t leader = (uint32_t)__ffsll((unsigned long long)m) - 1u;
+  uint32_t incl = n;
+#pragma unroll
+  for (uint32_t off = 1; off < 64u; off <<= 1) {
+    const uint32_t v = (uint32_t)__shfl_up((int)incl, off);
+    if (lane >= off) incl += v;
+  }
+  const uint32_t total = (uint32_t)__shfl((int)incl, 63);
+  if (total == 0u) return;
   uint32_t base = 0;
-  if (lane == leader) base = atomicAdd(counter, (uint32_t)__popcll(m));
-  base = (uint32_t)__shfl((int)base, (int)leader);
-  if (want) list[base + (uint32_t)__popcll(m & ((1ull << lane) - 1ull))] = item;
+  if (lane == 0u) base = atomicAdd(counter, total);
+  base = (uint32_t)__shfl((int)base, 0) + incl - n;
+  for (uint32_t k = 0; k < n; ++k) list[base + k] = make_uint2(key, k);
 }
 
+// Called with every lane of every wave active.
 __global__ __launch_bounds__(256) void k_build_lists(Dev D) {
   const uint32_t t = blockIdx.x * 256u + threadIdx.x;
   {
-    bool occ = false;
-    uint32_t col = 0, s = 0;
+    uint32_t cnt = 0, col = 0;
     if (t < D.C) {
-      s = D.cellScene[t];
+      const uint32_t s = D.cellScene[t];
       if (s != INACTIVE) {
-        occ = D.cellStart[t + 1] != D.cellStart[t];
+        cnt = D.cellStart[t + 1] - D.cellStart[t];
         const SceneP& P = D.sp[s];
         const uint32_t lc = t - P.cellBase;
         col = ((lc % P.gridW) & 1u) | (((lc / P.gridW) & 1u) << 1);
       }
     }
+    const uint32_t nch = (cnt + CELL_CHUNK - 1u) / CELL_CHUNK;
 #pragma unroll
-    for (uint32_t c = 0; c < 4; ++c)
-      append_item(occ && col == c, &D.listCount[c], D.dlist[c], make_uint2(t, 1u | (s << 8)));
+    for (uint32_t c = 0; c < 4; ++c) append_chunks(col == c ? nch : 0u, &D.listCount[c], D.dlist[c], t);
   }
   {
-    bool occ = false;
-    uint2 it = make_uint2(0u, 0u);
+    uint32_t nch = 0;
     if (t < D.nTiles) {
-      it = D.tileAll[t];
-      occ = D.cellStart[it.x + (it.y & 0xffu)] != D.cellStart[it.x];
+      const uint2 it = D.tileAll[t];
+      const uint32_t cnt = D.cellStart[it.x + (it.y & 0xffu)] - D.cellStart[it.x];
+      nch = (cnt + TILE_CHUNK - 1u) / TILE_CHUNK;
     }
-    append_item(occ, &D.listCount[4], D.dlist[4], it);
+    append_chunks(nch, &D.listCount[4], D.dlist[4], t);
   }
 }
 
@@ -469,8 +482,6 @@ __device__ __forceinline__ ListRange xcd_range(uint32_t n, uint32_t unitsPerBloc
 // particle's own candidates in exactly its canonical order (dy outer, dx
 // inner, ascending id), which is what decides truncation.  Iteration 0 of
 // solveA is formed in the same sweep from (W, G) staged in LDS.
-constexpr int NBR_PPL = 2;
-
 __global__ __launch_bounds__(256) void k_nbr_tile(Dev D) {
   __shared__ int2 lutWG[TN];
   for (uint32_t i = threadIdx.x; i < TN; i += 256u) {
@@ -484,8 +495,11 @@ __global__ __launch_bounds__(256) void k_nbr_tile(Dev D) {
   const int32_t W0 = lutWG[0].x;
   for (uint32_t w = R.first; w < R.hi; w += R.stride) {
     const uint2 item = D.dlist[4][w];
-    const uint32_t c0 = __builtin_amdgcn_readfirstlane(item.x);
-    const uint32_t yy = __builtin_amdgcn_readfirstlane(item.y);
+    const uint32_t tix = __builtin_amdgcn_readfirstlane(item.x);
+    const uint32_t chunk = __builtin_amdgcn_readfirstlane(item.y);
+    const uint2 tile = D.tileAll[tix];
+    const uint32_t c0 = __builtin_amdgcn_readfirstlane(tile.x);
+    const uint32_t yy = __builtin_amdgcn_readfirstlane(tile.y);
     const uint32_t ncell = yy & 0xffu, s = yy >> 8;
     const SceneP& P = D.sp[s];
     const int32_t sh = P.posShift;
@@ -493,10 +507,13 @@ __global__ __launch_bounds__(256) void k_nbr_tile(Dev D) {
     const uint32_t lc0 = c0 - cellBase;
     const int32_t cy = (int32_t)(lc0 / gridW), cx0 = (int32_t)(lc0 % gridW);
     const int32_t xlo = maxi(cx0 - 1, 0), xhi = mini(cx0 + (int32_t)ncell, (int32_t)gridW - 1);
-    const uint32_t plo = (uint32_t)((int32_t)D.cellStart[c0] + sh);
-    const uint32_t phi = (uint32_t)((int32_t)D.cellStart[c0 + ncell] + sh);
+    const uint32_t tlo = (uint32_t)((int32_t)D.cellStart[c0] + sh);
+    const uint32_t thi = (uint32_t)((int32_t)D.cellStart[c0 + ncell] + sh);
+    const uint32_t plo = tlo + chunk * TILE_CHUNK;
+    const uint32_t phi = (uint32_t)mini((int32_t)thi, (int32_t)(plo + TILE_CHUNK));
     const uint32_t myRow = cellBase + (uint32_t)cy * gridW;
     Dbg g;
+    static_assert(TILE_CHUNK == 64u * NBR_PPL, "one sweep per tile chunk");
     for (uint32_t pbase = plo; pbase < phi; pbase += 64u * NBR_PPL) {
       uint32_t pk[NBR_PPL], inf[NBR_PPL], cnt[NBR_PPL], bits[NBR_PPL], dropped[NBR_PPL];
       int2 pik[NBR_PPL];
@@ -626,17 +643,16 @@ __global__ __launch_bounds__(256) void k_solveA(Dev D) {
 }
 
 // ================================================================ colour pass
-// solveB_at + applyDp_at for one colour (src/sim.wgsl:1337-1474).  One
-// workgroup per occupied cell of that colour.  Same-colour neighbours always
-// share a cell (cells of one colour are >= 2 cells apart and the stencil is
-// 3x3), and no other workgroup reads this cell's positions, so computing
-// every dp of the cell before writing any pred reproduces the reference's
-// two-dispatch deferral exactly.  Each particle's neighbour list is split
-// over L lanes (entries k = sub, sub + L, ...; L = 256 / particles, a power
-// of two up to 16) whose wrapping partial sums are combined with xor
-// shuffles, so a cell's 48-deep gather chains run L-fold shorter.  The
-// (G, S) kernel tables sit in LDS.  On the last iteration the finalize of
-// these particles (src/sim.wgsl:2245-2265) follows in place.
+// solveB_c then applyDp_c (src/sim.wgsl:1337-1474) for one colour, as two
+// launches over the colour's cell chunks, like the reference's two
+// dispatches: every dp of the colour is formed before any pred moves.
+// k_solveB gives each chunk one workgroup and splits each particle's
+// neighbour list over L lanes (entries k = sub, sub + L, ...; L = 256 /
+// particles, a power of two up to 16) whose wrapping partial sums are
+// combined with xor shuffles, so the 48-deep gather chains run L-fold
+// shorter; the (G, S) kernel tables sit in LDS.  k_apply moves the chunk's
+// particles and, on the last iteration, finalizes them
+// (src/sim.wgsl:2245-2265).
 __device__ __forceinline__ int2 solveB_part(const Dev& D, uint32_t p, uint32_t sub, uint32_t L,
                                             const int2* lutGS, Dbg& g) {
   const int2 pi = D.pred[p];
@@ -670,7 +686,17 @@ __device__ __forceinline__ int2 solveB_part(const Dev& D, uint32_t p, uint32_t s
   return make_int2(dpx, dpy);
 }
 
-__global__ __launch_bounds__(256) void k_colour(Dev D, uint32_t colour, uint32_t last) {
+__device__ __forceinline__ void chunk_range(const Dev& D, uint2 item, uint32_t& s, uint32_t& lo, uint32_t& m) {
+  const uint32_t c = item.x;
+  s = D.cellScene[c];
+  const int32_t sh = D.sp[s].posShift;
+  const uint32_t clo = (uint32_t)((int32_t)D.cellStart[c] + sh);
+  const uint32_t chi = (uint32_t)((int32_t)D.cellStart[c + 1] + sh);
+  lo = clo + item.y * CELL_CHUNK;
+  m = (uint32_t)mini((int32_t)(chi - lo), (int32_t)CELL_CHUNK);
+}
+
+__global__ __launch_bounds__(256) void k_solveB(Dev D, uint32_t colour) {
   __shared__ int2 lutGS[TN];
   for (uint32_t i = threadIdx.x; i < TN; i += 256u) {
     const int4 L = D.lut4[i];
@@ -681,34 +707,46 @@ __global__ __launch_bounds__(256) void k_colour(Dev D, uint32_t colour, uint32_t
   const uint2* list = D.dlist[colour];
   const ListRange R = xcd_range(n, 1u, 0u);
   for (uint32_t w = R.first; w < R.hi; w += R.stride) {
-    const uint2 item = list[w];
-    const uint32_t c = __builtin_amdgcn_readfirstlane(item.x);
-    const uint32_t s = __builtin_amdgcn_readfirstlane(item.y) >> 8;
-    const SceneP& P = D.sp[s];
-    const uint32_t plo = (uint32_t)((int32_t)D.cellStart[c] + P.posShift);
-    const uint32_t cnt = (uint32_t)__builtin_amdgcn_readfirstlane((int)(D.cellStart[c + 1] - D.cellStart[c]));
+    uint2 item = list[w];
+    item.x = __builtin_amdgcn_readfirstlane(item.x);
+    item.y = __builtin_amdgcn_readfirstlane(item.y);
+    uint32_t s, lo, m;
+    chunk_range(D, item, s, lo, m);
+    s = __builtin_amdgcn_readfirstlane(s);
+    m = __builtin_amdgcn_readfirstlane(m);
     Dbg g;
     uint32_t L = 16;
-    while (L > 1u && L * cnt > 256u) L >>= 1;
-    const uint32_t per = 256u / L, sub = threadIdx.x & (L - 1u);
-    // pass 1: dp of every particle of the cell
-    for (uint32_t base = 0; base < cnt; base += per) {
-      const uint32_t pl = base + threadIdx.x / L;
-      const uint32_t p = plo + pl;
-      const bool act = pl < cnt && (D.info[p] & INF_FLUID);
-      int2 dp = make_int2(0, 0);
-      if (act) dp = solveB_part(D, p, sub, L, lutGS, g);
-      for (uint32_t off = 1; off < L; off <<= 1) {
-        dp.x = wadd(dp.x, __shfl_xor(dp.x, (int)off));
-        dp.y = wadd(dp.y, __shfl_xor(dp.y, (int)off));
-      }
-      if (act && sub == 0u) D.dp[p] = dp;
+    while (L > 1u && L * m > 256u) L >>= 1;
+    const uint32_t pl = threadIdx.x / L, sub = threadIdx.x & (L - 1u);
+    const uint32_t p = lo + pl;
+    const bool act = pl < m && (D.info[p] & INF_FLUID);
+    int2 dp = make_int2(0, 0);
+    if (act) dp = solveB_part(D, p, sub, L, lutGS, g);
+    for (uint32_t off = 1; off < L; off <<= 1) {
+      dp.x = wadd(dp.x, __shfl_xor(dp.x, (int)off));
+      dp.y = wadd(dp.y, __shfl_xor(dp.y, (int)off));
     }
-    __syncthreads();
-    // pass 2: apply (and finalize on the last iteration)
-    for (uint32_t pl = threadIdx.x; pl < cnt; pl += 256u) {
-      const uint32_t p = plo + pl;
-      if (!(D.info[p] & INF_FLUID)) continue;
+    if (act && sub == 0u) D.dp[p] = dp;
+    flush_dbg(D, s, g, 0);
+  }
+}
+
+__global__ __launch_bounds__(256) void k_apply(Dev D, uint32_t colour, uint32_t last) {
+  const uint32_t lane = threadIdx.x & 63u;
+  const uint32_t n = (uint32_t)__builtin_amdgcn_readfirstlane((int)D.listCount[colour]);
+  const uint2* list = D.dlist[colour];
+  const ListRange R = xcd_range(n, 4u, threadIdx.x >> 6);
+  for (uint32_t w = R.first; w < R.hi; w += R.stride) {
+    uint2 item = list[w];
+    item.x = __builtin_amdgcn_readfirstlane(item.x);
+    item.y = __builtin_amdgcn_readfirstlane(item.y);
+    uint32_t s, lo, m;
+    chunk_range(D, item, s, lo, m);
+    s = __builtin_amdgcn_readfirstlane(s);
+    const SceneP& P = D.sp[s];
+    Dbg g;
+    const uint32_t p = lo + lane;
+    if (lane < m && (D.info[p] & INF_FLUID)) {
       const int4 b = D.ptY[p];
       const uint32_t fl = (uint32_t)b.y;
       int2 pr = D.pred[p];
@@ -1167,7 +1205,7 @@ struct sn_engine {
   uint8_t* dActive = nullptr;
   std::vector<uint2> tiles;          // every neighbour tile of the active scenes
   uint32_t tileG = 1;
-  uint32_t nbrBlocks = 0, colourBlocks = 0;  // persistent grid sizes
+  uint32_t nbrBlocks = 0, colourBlocks = 0, applyBlocks = 0;  // persistent grid sizes
   uint32_t S = 0, N = 0, C = 0;
   uint32_t maxEventsPerSubstep = 0;
   Event* dEvents = nullptr;
@@ -1217,7 +1255,7 @@ void enqueue_scan(sn_engine* e, uint32_t* count, uint32_t n, uint32_t* out, bool
 // Kernel classes timed by sn_profile_step (index into its outputs, + 1).
 enum ProfClass {
   PC_PREDICT = 0, PC_SCAN, PC_SCATTER, PC_CANON, PC_NBR, PC_SOLVEA, PC_COLOUR, PC_XNC, PC_STH, PC_FINISH,
-  PC_IMPULSE, PC_LISTS, PC_COUNT
+  PC_IMPULSE, PC_LISTS, PC_APPLY, PC_COUNT
 };
 
 // Record an event before (begin) / after (end) one launch while profiling.
@@ -1290,8 +1328,12 @@ void enqueue_substep(sn_engine* e) {
       hipLaunchKernelGGL(k_solveA, dim3(gN), dim3(256), 0, st, D);
     }
     for (uint32_t c = 0; c < 4; ++c) {
-      ProfScope ps(e, PC_COLOUR);
-      hipLaunchKernelGGL(k_colour, dim3(e->colourBlocks), dim3(256), 0, st, D, c, (uint32_t)(it + 1 == iters));
+      {
+        ProfScope ps(e, PC_COLOUR);
+        hipLaunchKernelGGL(k_solveB, dim3(e->colourBlocks), dim3(256), 0, st, D, c);
+      }
+      ProfScope ps(e, PC_APPLY);
+      hipLaunchKernelGGL(k_apply, dim3(e->applyBlocks), dim3(256), 0, st, D, c, (uint32_t)(it + 1 == iters));
     }
   }
   {
@@ -1448,18 +1490,21 @@ int sn_create(const sn_opts* opts, const int32_t* luts, const int32_t* mats, sn_
     // per-substep work lists: at most one item per cell (tiles <= cells)
     D.cellScene = dalloc<uint32_t>(e, CC);
     D.tileAll = dalloc<uint2>(e, CC);
-    for (uint32_t c = 0; c < 5; ++c) D.dlist[c] = dalloc<uint2>(e, CC);
+    // a list holds at most one item per cell (tile) plus one per full chunk
+    for (uint32_t c = 0; c < 5; ++c) D.dlist[c] = dalloc<uint2>(e, (size_t)CC + NC / CELL_CHUNK + 64);
     D.listCount = dalloc<uint32_t>(e, 8);
     // persistent grids: as many workgroups as can be resident, a multiple of 8 (XCDs)
     {
       hipDeviceProp_t prop;
       HIPCHECK(hipGetDeviceProperties(&prop, (int)e->opts.device));
-      int bN = 0, bC = 0;
+      int bN = 0, bC = 0, bA = 0;
       HIPCHECK(hipOccupancyMaxActiveBlocksPerMultiprocessor(&bN, reinterpret_cast<const void*>(k_nbr_tile), 256, 0));
-      HIPCHECK(hipOccupancyMaxActiveBlocksPerMultiprocessor(&bC, reinterpret_cast<const void*>(k_colour), 256, 0));
+      HIPCHECK(hipOccupancyMaxActiveBlocksPerMultiprocessor(&bC, reinterpret_cast<const void*>(k_solveB), 256, 0));
+      HIPCHECK(hipOccupancyMaxActiveBlocksPerMultiprocessor(&bA, reinterpret_cast<const void*>(k_apply), 256, 0));
       const uint32_t cus = (uint32_t)std::max(prop.multiProcessorCount, 1);
       e->nbrBlocks = std::max(8u, (cus * (uint32_t)std::max(bN, 1)) & ~7u);
       e->colourBlocks = std::max(8u, (cus * (uint32_t)std::max(bC, 1)) & ~7u);
+      e->applyBlocks = std::max(8u, (cus * (uint32_t)std::max(bA, 1)) & ~7u);
     }
     HIPCHECK(hipMemset(D.cellCount, 0, sizeof(uint32_t) * CC));
   } catch (const DevError& d) {

@@ -227,7 +227,7 @@ class Engine:
 
     # kernel classes of sn_profile_step, slot 0 = the whole span
     PROFILE_KEYS = ("total", "predict", "scan", "scatter", "canon_gather", "nbr_solveA", "solveA",
-                    "colour", "xnc", "st_thermal_hash", "hash_finish", "impulse")
+                    "solveB", "xnc", "st_thermal_hash", "hash_finish", "impulse", "lists", "apply")
 
     def profile_step(self, count: int = 1) -> dict:
         """Run `count` substeps eagerly with an event pair around every launch.
