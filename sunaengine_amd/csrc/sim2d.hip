@@ -122,7 +122,8 @@ struct SceneP {
   uint32_t n, nFluid, gridW, gridH;
   uint32_t cellShift, cellBase, cellTotal, partBase;
   int32_t domW, domH, vmax, posShift;   // posShift = partBase - (particles of earlier scenes)
-  uint32_t weldHold, cap, cellCap, pad;
+  uint32_t weldHold, cap, cellCap;
+  uint32_t bounded;   // every position of the scene stays within +-2^28 (see pdelta)
 };
 struct SceneD {
   uint32_t substep;    // substeps encoded since load (`_substep`, src/engine.js:3811)
@@ -220,6 +221,31 @@ __device__ __forceinline__ int32_t lut_index(int2 d, Dbg& g) {
 }
 __device__ __forceinline__ int2 pred_delta(int2 pi, int2 pj, Dbg& g) {
   return make_int2(fp_sub_sat(pi.x, pj.x, g), fp_sub_sat(pi.y, pj.y, g));
+}
+// Pair geometry.  In a bounded scene every position lies within +-2^28: the
+// domain and every loaded or spawned position are checked on the host, fluid
+// predictions are wall-clamped into the domain and boundary slots never
+// move.  A difference of two positions then never overflows and its absolute
+// value is exact, so the saturating forms above reduce to plain arithmetic
+// and count no event; and with ax, ay <= HS the squared radius fits the
+// 24-bit multiplier.  Scenes that are not bounded take the exact saturating
+// forms.
+constexpr int32_t POS_BOUND = 1 << 28;
+template <bool BND>
+__device__ __forceinline__ int2 pdelta(int2 pi, int2 pj, Dbg& g) {
+  if constexpr (BND) return make_int2(pi.x - pj.x, pi.y - pj.y);
+  else return pred_delta(pi, pj, g);
+}
+template <bool BND>
+__device__ __forceinline__ int32_t lut_idx(int2 d, Dbg& g) {
+  if constexpr (BND) {
+    const uint32_t ax = ((uint32_t)abs(d.x) + 8u) >> 4, ay = ((uint32_t)abs(d.y) + 8u) >> 4;
+    const uint32_t axc = min(ax, (uint32_t)HS + 1u), ayc = min(ay, (uint32_t)HS + 1u);
+    const uint32_t r2 = __umul24(axc, axc) + __umul24(ayc, ayc);
+    return (ax > (uint32_t)HS || ay > (uint32_t)HS || r2 >= (uint32_t)H2S) ? -1 : (int32_t)(r2 >> 15);
+  } else {
+    return lut_index(d, g);
+  }
 }
 __device__ __forceinline__ int32_t grad_q16(int32_t gt, int32_t d, Dbg& g) { return fp_mul_s(gt, d, 26, g); }
 __device__ __forceinline__ int2 wall_clamp(const SceneP& P, int2 p) {
@@ -482,6 +508,105 @@ __device__ __forceinline__ ListRange xcd_range(uint32_t n, uint32_t unitsPerBloc
 // particle's own candidates in exactly its canonical order (dy outer, dx
 // inner, ascending id), which is what decides truncation.  Iteration 0 of
 // solveA is formed in the same sweep from (W, G) staged in LDS.
+// One chunk of one tile: the sweep described above.  BND selects the bounded
+// pair geometry; WIN is false for one-cell tiles, whose candidate union is
+// exactly every particle's own 3x3 stencil, so no per-lane column window
+// applies.
+template <bool BND, bool WIN>
+__device__ __forceinline__ void nbr_chunk(const Dev& D, const int2* lutWG, uint32_t c0, uint32_t ncell, uint32_t s,
+                                          uint32_t chunk, uint32_t lane, int32_t W0, Dbg& g) {
+  const SceneP& P = D.sp[s];
+  const int32_t sh = P.posShift;
+  const uint32_t gridW = P.gridW, gridH = P.gridH, cellBase = P.cellBase;
+  const uint32_t lc0 = c0 - cellBase;
+  const int32_t cy = (int32_t)(lc0 / gridW), cx0 = (int32_t)(lc0 % gridW);
+  const int32_t xlo = maxi(cx0 - 1, 0), xhi = mini(cx0 + (int32_t)ncell, (int32_t)gridW - 1);
+  const uint32_t tlo = (uint32_t)((int32_t)D.cellStart[c0] + sh);
+  const uint32_t thi = (uint32_t)((int32_t)D.cellStart[c0 + ncell] + sh);
+  const uint32_t plo = tlo + chunk * TILE_CHUNK;
+  const uint32_t phi = (uint32_t)mini((int32_t)thi, (int32_t)(plo + TILE_CHUNK));
+  const uint32_t myRow = cellBase + (uint32_t)cy * gridW;
+  static_assert(TILE_CHUNK == 64u * NBR_PPL, "one sweep per tile chunk");
+  for (uint32_t pbase = plo; pbase < phi; pbase += 64u * NBR_PPL) {
+    uint32_t pk[NBR_PPL], inf[NBR_PPL], cnt[NBR_PPL], bits[NBR_PPL], dropped[NBR_PPL];
+    int2 pik[NBR_PPL];
+    int32_t cxk[NBR_PPL];
+    bool live[NBR_PPL];
+    SolveAAcc acc[NBR_PPL];
+#pragma unroll
+    for (int k = 0; k < NBR_PPL; ++k) {
+      pk[k] = pbase + (uint32_t)k * 64u + lane;
+      const bool valid = pk[k] < phi;
+      inf[k] = valid ? D.info[pk[k]] : 0u;
+      pik[k] = valid ? D.pred[pk[k]] : make_int2(0, 0);
+      cxk[k] = valid ? (int32_t)(D.cellY[pk[k]] - myRow) : 0;
+      live[k] = valid && !(inf[k] & INF_DEAD);
+      cnt[k] = 0;
+      bits[k] = 0;
+      dropped[k] = 0;
+      acc[k] = SolveAAcc{W0, 0, 0, 0};
+    }
+    for (int32_t dy = -1; dy <= 1; ++dy) {
+      const int32_t gy = cy + dy;
+      if (gy < 0 || gy >= (int32_t)gridH) continue;
+      const uint32_t rowc = cellBase + (uint32_t)gy * gridW;
+      const uint32_t rlo = (uint32_t)((int32_t)D.cellStart[rowc + (uint32_t)xlo] + sh);
+      const uint32_t rhi = (uint32_t)((int32_t)D.cellStart[rowc + (uint32_t)xhi + 1u] + sh);
+      for (uint32_t q0 = rlo; q0 < rhi; q0 += 64u) {
+        const uint32_t qn = (uint32_t)mini((int32_t)(rhi - q0), 64);
+        const uint32_t ql = q0 + lane;
+        int2 cp = make_int2(0, 0);
+        uint32_t ci = 0;
+        int32_t cc = 0;
+        if (lane < qn) {
+          cp = D.pred[ql];
+          ci = D.info[ql];
+          cc = (int32_t)(D.cellY[ql] - rowc);
+        }
+        // wait for this block's candidates here, so that the candidate
+        // loop below never waits on the neighbour-list stores it issues
+        __builtin_amdgcn_s_waitcnt(0x0F70);  // vmcnt(0)
+        for (uint32_t j = 0; j < qn; ++j) {
+          const int32_t px = __builtin_amdgcn_readlane(cp.x, j);
+          const int32_t py = __builtin_amdgcn_readlane(cp.y, j);
+          const uint32_t qi = (uint32_t)__builtin_amdgcn_readlane((int)ci, j);
+          const int32_t qc = __builtin_amdgcn_readlane(cc, j);
+          const uint32_t q = q0 + j;
+          if (qi & INF_DEAD) continue;                     // nobody sees a corpse
+#pragma unroll
+          for (int k = 0; k < NBR_PPL; ++k) {
+            if (!live[k] || q == pk[k]) continue;
+            if (WIN && (qc < cxk[k] - 1 || qc > cxk[k] + 1)) continue;
+            const int2 d = pdelta<BND>(pik[k], make_int2(px, py), g);
+            const int32_t idx = lut_idx<BND>(d, g);
+            if (idx < 0) continue;
+            bits[k] |= qi & (INF_SEAL | INF_CONGEAL);
+            if (cnt[k] >= MAXNBR) {
+              dropped[k]++;
+              continue;
+            }
+            D.nbr[(size_t)cnt[k] * D.NS + pk[k]] = q | ((qi & INF_FLUID) ? 0u : NBR_BND);
+            cnt[k]++;
+            if (inf[k] & INF_FLUID) {
+              const int2 L = lutWG[idx];
+              solveA_term(acc[k], d, make_int4(L.x, L.y, 0, 0), g);
+            }
+          }
+        }
+      }
+    }
+#pragma unroll
+    for (int k = 0; k < NBR_PPL; ++k) {
+      if (pk[k] >= phi) continue;
+      D.nbrN[pk[k]] = live[k] ? (cnt[k] | ((bits[k] & INF_SEAL) ? NBR_SEAL_BIT : 0u) |
+                                 ((bits[k] & INF_CONGEAL) ? NBR_CONGEAL_BIT : 0u))
+                              : 0u;
+      D.drop[pk[k]] = dropped[k];
+      if (inf[k] & INF_FLUID) solveA_finish(D, pk[k], acc[k], MAT(D, (uint32_t)D.ptY[pk[k]].x)[M_EPS], g);
+    }
+  }
+}
+
 __global__ __launch_bounds__(256) void k_nbr_tile(Dev D) {
   __shared__ int2 lutWG[TN];
   for (uint32_t i = threadIdx.x; i < TN; i += 256u) {
@@ -501,95 +626,13 @@ __global__ __launch_bounds__(256) void k_nbr_tile(Dev D) {
     const uint32_t c0 = __builtin_amdgcn_readfirstlane(tile.x);
     const uint32_t yy = __builtin_amdgcn_readfirstlane(tile.y);
     const uint32_t ncell = yy & 0xffu, s = yy >> 8;
-    const SceneP& P = D.sp[s];
-    const int32_t sh = P.posShift;
-    const uint32_t gridW = P.gridW, gridH = P.gridH, cellBase = P.cellBase;
-    const uint32_t lc0 = c0 - cellBase;
-    const int32_t cy = (int32_t)(lc0 / gridW), cx0 = (int32_t)(lc0 % gridW);
-    const int32_t xlo = maxi(cx0 - 1, 0), xhi = mini(cx0 + (int32_t)ncell, (int32_t)gridW - 1);
-    const uint32_t tlo = (uint32_t)((int32_t)D.cellStart[c0] + sh);
-    const uint32_t thi = (uint32_t)((int32_t)D.cellStart[c0 + ncell] + sh);
-    const uint32_t plo = tlo + chunk * TILE_CHUNK;
-    const uint32_t phi = (uint32_t)mini((int32_t)thi, (int32_t)(plo + TILE_CHUNK));
-    const uint32_t myRow = cellBase + (uint32_t)cy * gridW;
+    const bool bnd = D.sp[s].bounded != 0u;
     Dbg g;
-    static_assert(TILE_CHUNK == 64u * NBR_PPL, "one sweep per tile chunk");
-    for (uint32_t pbase = plo; pbase < phi; pbase += 64u * NBR_PPL) {
-      uint32_t pk[NBR_PPL], inf[NBR_PPL], cnt[NBR_PPL], bits[NBR_PPL], dropped[NBR_PPL];
-      int2 pik[NBR_PPL];
-      int32_t cxk[NBR_PPL];
-      bool live[NBR_PPL];
-      SolveAAcc acc[NBR_PPL];
-#pragma unroll
-      for (int k = 0; k < NBR_PPL; ++k) {
-        pk[k] = pbase + (uint32_t)k * 64u + lane;
-        const bool valid = pk[k] < phi;
-        inf[k] = valid ? D.info[pk[k]] : 0u;
-        pik[k] = valid ? D.pred[pk[k]] : make_int2(0, 0);
-        cxk[k] = valid ? (int32_t)(D.cellY[pk[k]] - myRow) : 0;
-        live[k] = valid && !(inf[k] & INF_DEAD);
-        cnt[k] = 0;
-        bits[k] = 0;
-        dropped[k] = 0;
-        acc[k] = SolveAAcc{W0, 0, 0, 0};
-      }
-      for (int32_t dy = -1; dy <= 1; ++dy) {
-        const int32_t gy = cy + dy;
-        if (gy < 0 || gy >= (int32_t)gridH) continue;
-        const uint32_t rowc = cellBase + (uint32_t)gy * gridW;
-        const uint32_t rlo = (uint32_t)((int32_t)D.cellStart[rowc + (uint32_t)xlo] + sh);
-        const uint32_t rhi = (uint32_t)((int32_t)D.cellStart[rowc + (uint32_t)xhi + 1u] + sh);
-        for (uint32_t q0 = rlo; q0 < rhi; q0 += 64u) {
-          const uint32_t qn = (uint32_t)mini((int32_t)(rhi - q0), 64);
-          const uint32_t ql = q0 + lane;
-          int2 cp = make_int2(0, 0);
-          uint32_t ci = 0;
-          int32_t cc = 0;
-          if (lane < qn) {
-            cp = D.pred[ql];
-            ci = D.info[ql];
-            cc = (int32_t)(D.cellY[ql] - rowc);
-          }
-          // wait for this block's candidates here, so that the candidate
-          // loop below never waits on the neighbour-list stores it issues
-          __builtin_amdgcn_s_waitcnt(0x0F70);  // vmcnt(0)
-          for (uint32_t j = 0; j < qn; ++j) {
-            const int32_t px = __builtin_amdgcn_readlane(cp.x, j);
-            const int32_t py = __builtin_amdgcn_readlane(cp.y, j);
-            const uint32_t qi = (uint32_t)__builtin_amdgcn_readlane((int)ci, j);
-            const int32_t qc = __builtin_amdgcn_readlane(cc, j);
-            const uint32_t q = q0 + j;
-            if (qi & INF_DEAD) continue;                     // nobody sees a corpse
-#pragma unroll
-            for (int k = 0; k < NBR_PPL; ++k) {
-              if (!live[k] || q == pk[k] || qc < cxk[k] - 1 || qc > cxk[k] + 1) continue;
-              const int2 d = pred_delta(pik[k], make_int2(px, py), g);
-              const int32_t idx = lut_index(d, g);
-              if (idx < 0) continue;
-              bits[k] |= qi & (INF_SEAL | INF_CONGEAL);
-              if (cnt[k] >= MAXNBR) {
-                dropped[k]++;
-                continue;
-              }
-              D.nbr[(size_t)cnt[k] * D.NS + pk[k]] = q | ((qi & INF_FLUID) ? 0u : NBR_BND);
-              cnt[k]++;
-              if (inf[k] & INF_FLUID) {
-                const int2 L = lutWG[idx];
-                solveA_term(acc[k], d, make_int4(L.x, L.y, 0, 0), g);
-              }
-            }
-          }
-        }
-      }
-#pragma unroll
-      for (int k = 0; k < NBR_PPL; ++k) {
-        if (pk[k] >= phi) continue;
-        D.nbrN[pk[k]] = live[k] ? (cnt[k] | ((bits[k] & INF_SEAL) ? NBR_SEAL_BIT : 0u) |
-                                   ((bits[k] & INF_CONGEAL) ? NBR_CONGEAL_BIT : 0u))
-                                : 0u;
-        D.drop[pk[k]] = dropped[k];
-        if (inf[k] & INF_FLUID) solveA_finish(D, pk[k], acc[k], MAT(D, (uint32_t)D.ptY[pk[k]].x)[M_EPS], g);
-      }
+    if (bnd) {
+      if (ncell == 1u) nbr_chunk<true, false>(D, lutWG, c0, ncell, s, chunk, lane, W0, g);
+      else nbr_chunk<true, true>(D, lutWG, c0, ncell, s, chunk, lane, W0, g);
+    } else {
+      nbr_chunk<false, true>(D, lutWG, c0, ncell, s, chunk, lane, W0, g);
     }
     flush_dbg(D, s, g, 0);
   }
@@ -611,33 +654,45 @@ __device__ __forceinline__ uint32_t nbr_at(const Dev& D, const uint32_t* nb, uin
 }
 
 // ================================================================ solveA (k>0)
-__global__ __launch_bounds__(256) void k_solveA(Dev D) {
-  uint32_t p = xcd_block() * 256u + threadIdx.x;
-  uint32_t s = p < D.N ? D.sidOf[p] : INACTIVE;
-  Dbg g;
-  if (s != INACTIVE && (D.info[p] & INF_FLUID)) {
-    int2 pi = D.pred[p];
-    uint32_t cnt = D.nbrN[p] & 0x1fffffffu;
-    const uint32_t* nb = D.nbr + p;
-    SolveAAcc acc{D.lut4[0].x, 0, 0, 0};
-    for (uint32_t k = 0; k < cnt; k += NBR_UNROLL) {
-      uint32_t q[NBR_UNROLL];
-      int2 d[NBR_UNROLL];
-      int32_t idx[NBR_UNROLL];
-      int4 L[NBR_UNROLL];
+template <bool BND>
+__device__ __forceinline__ void solveA_at(const Dev& D, uint32_t p, Dbg& g) {
+  const int2 pi = D.pred[p];
+  const uint32_t cnt = D.nbrN[p] & 0x1fffffffu;
+  const uint32_t* nb = D.nbr + p;
+  SolveAAcc acc{D.lut4[0].x, 0, 0, 0};
+  for (uint32_t k = 0; k < cnt; k += NBR_UNROLL) {
+    uint32_t q[NBR_UNROLL];
+    int2 d[NBR_UNROLL];
+    int32_t idx[NBR_UNROLL];
+    int4 L[NBR_UNROLL];
 #pragma unroll
-      for (uint32_t u = 0; u < NBR_UNROLL; ++u) q[u] = nbr_at(D, nb, k + u, cnt, p);
+    for (uint32_t u = 0; u < NBR_UNROLL; ++u) q[u] = nbr_at(D, nb, k + u, cnt, p);
 #pragma unroll
-      for (uint32_t u = 0; u < NBR_UNROLL; ++u) {
-        d[u] = pred_delta(pi, D.pred[q[u] & ~NBR_BND], g);
-        idx[u] = (k + u < cnt) ? lut_index(d[u], g) : -1;
-        L[u] = D.lut4[maxi(idx[u], 0)];
-      }
-#pragma unroll
-      for (uint32_t u = 0; u < NBR_UNROLL; ++u)
-        if (idx[u] >= 0) solveA_term(acc, d[u], L[u], g);
+    for (uint32_t u = 0; u < NBR_UNROLL; ++u) {
+      d[u] = pdelta<BND>(pi, D.pred[q[u] & ~NBR_BND], g);
+      idx[u] = (k + u < cnt) ? lut_idx<BND>(d[u], g) : -1;
+      L[u] = D.lut4[maxi(idx[u], 0)];
     }
-    solveA_finish(D, p, acc, MAT(D, (uint32_t)D.ptY[p].x)[M_EPS], g);
+#pragma unroll
+    for (uint32_t u = 0; u < NBR_UNROLL; ++u)
+      if (idx[u] >= 0) solveA_term(acc, d[u], L[u], g);
+  }
+  solveA_finish(D, p, acc, MAT(D, (uint32_t)D.ptY[p].x)[M_EPS], g);
+}
+
+// True on every lane when the wave's active lanes all belong to bounded scenes.
+__device__ __forceinline__ bool wave_bounded(const Dev& D, uint32_t s) {
+  return __all(s == INACTIVE || D.sp[s].bounded != 0u);
+}
+
+__global__ __launch_bounds__(256) void k_solveA(Dev D) {
+  const uint32_t p = xcd_block() * 256u + threadIdx.x;
+  const uint32_t s = p < D.N ? D.sidOf[p] : INACTIVE;
+  Dbg g;
+  const bool bnd = wave_bounded(D, s);
+  if (s != INACTIVE && (D.info[p] & INF_FLUID)) {
+    if (bnd) solveA_at<true>(D, p, g);
+    else solveA_at<false>(D, p, g);
   }
   flush_dbg(D, s, g, 0);
 }
@@ -653,6 +708,7 @@ __global__ __launch_bounds__(256) void k_solveA(Dev D) {
 // shorter; the (G, S) kernel tables sit in LDS.  k_apply moves the chunk's
 // particles and, on the last iteration, finalizes them
 // (src/sim.wgsl:2245-2265).
+template <bool BND>
 __device__ __forceinline__ int2 solveB_part(const Dev& D, uint32_t p, uint32_t sub, uint32_t L,
                                             const int2* lutGS, Dbg& g) {
   const int2 pi = D.pred[p];
@@ -667,9 +723,9 @@ __device__ __forceinline__ int2 solveB_part(const Dev& D, uint32_t p, uint32_t s
     const uint32_t q0 = e0 & ~NBR_BND, q1 = e1 & ~NBR_BND;
     const int2 pj0 = D.pred[q0], pj1 = D.pred[q1];
     const int32_t lj0 = D.lam[q0], lj1 = D.lam[q1];
-    const int2 d0 = pred_delta(pi, pj0, g), d1 = pred_delta(pi, pj1, g);
-    const int32_t i0 = lut_index(d0, g);
-    const int32_t i1 = k1 < cnt ? lut_index(d1, g) : -1;
+    const int2 d0 = pdelta<BND>(pi, pj0, g), d1 = pdelta<BND>(pi, pj1, g);
+    const int32_t i0 = lut_idx<BND>(d0, g);
+    const int32_t i1 = k1 < cnt ? lut_idx<BND>(d1, g) : -1;
     if (i0 >= 0) {
       const int2 T = lutGS[i0];
       const int32_t co = fp_add_sat(fp_add_sat(li, (e0 & NBR_BND) ? li : lj0, g), T.y, g);
@@ -721,7 +777,8 @@ __global__ __launch_bounds__(256) void k_solveB(Dev D, uint32_t colour) {
     const uint32_t p = lo + pl;
     const bool act = pl < m && (D.info[p] & INF_FLUID);
     int2 dp = make_int2(0, 0);
-    if (act) dp = solveB_part(D, p, sub, L, lutGS, g);
+    if (act) dp = D.sp[s].bounded ? solveB_part<true>(D, p, sub, L, lutGS, g)
+                                  : solveB_part<false>(D, p, sub, L, lutGS, g);
     for (uint32_t off = 1; off < L; off <<= 1) {
       dp.x = wadd(dp.x, __shfl_xor(dp.x, (int)off));
       dp.y = wadd(dp.y, __shfl_xor(dp.y, (int)off));
@@ -776,87 +833,154 @@ __global__ __launch_bounds__(256) void k_apply(Dev D, uint32_t colour, uint32_t 
 // src/sim.wgsl:2300-2384 and :2528-2582 share one gather over fluid
 // neighbours; each keeps its own accumulators.  Writes the final position and
 // velocity of the substep (boundary slots: position kept, velocity zeroed).
+template <bool BND>
+__device__ __forceinline__ void xnc_fluid(const Dev& D, uint32_t p, Dbg& g) {
+  int4 b = D.ptY[p];
+  uint32_t mi = (uint32_t)b.x;
+  const int32_t* Mi = MAT(D, mi);
+  uint32_t fl = (uint32_t)b.y;
+  bool doX = (fl & (FLAG_RIGID | FLAG_ANCHOR)) == 0u;
+  int32_t ci = Mi[M_XSPH], ki = Mi[M_COND], Ti = b.z, corri = Mi[M_CORRODE];
+  int2 pi = D.pred[p], vi = D.vtmp[p];
+  int32_t rhoi = D.rho[p];
+  uint32_t cnt = D.nbrN[p] & 0x1fffffffu;
+  const uint32_t* nb = D.nbr + p;
+  int32_t ax = 0, ay = 0, sx = 0, sy = 0, q = 0, c = 0;
+  for (uint32_t k = 0; k < cnt; k += XNC_UNROLL) {
+    // padding entries read as boundary entries, which this loop skips
+    uint32_t e[XNC_UNROLL];
+    int2 d[XNC_UNROLL], vjs[XNC_UNROLL];
+    int32_t idx[XNC_UNROLL], rhoj[XNC_UNROLL], rjs[XNC_UNROLL];
+    int4 Ls[XNC_UNROLL], bjs[XNC_UNROLL];
+#pragma unroll
+    for (uint32_t u = 0; u < XNC_UNROLL; ++u) e[u] = nbr_at(D, nb, k + u, cnt, p | NBR_BND);
+#pragma unroll
+    for (uint32_t u = 0; u < XNC_UNROLL; ++u) {
+      // a skipped (boundary or padding) entry is measured against the
+      // particle itself: a zero offset, no saturation event
+      const bool skip = (e[u] & NBR_BND) != 0u;
+      const uint32_t qj = e[u] & ~NBR_BND;
+      d[u] = pdelta<BND>(pi, skip ? pi : D.pred[qj], g);
+      idx[u] = skip ? -1 : lut_idx<BND>(d[u], g);
+      Ls[u] = D.lut4[maxi(idx[u], 0)];
+      bjs[u] = D.ptY[qj];
+      rhoj[u] = D.rho[qj];
+      vjs[u] = D.vtmp[qj];
+      rjs[u] = D.rinv[qj];
+    }
+#pragma unroll
+    for (uint32_t u = 0; u < XNC_UNROLL; ++u) {
+      if (idx[u] < 0) continue;
+      const int4 L = Ls[u], bj = bjs[u];
+      const int32_t* Mj = MAT(D, (uint32_t)bj.x);
+      if (doX) {
+        int32_t K = sym_k(rhoi, rhoj[u], g);
+        int32_t w = fp_mul_s(L.x, K, 16, g);
+        int32_t cij = mat_mix(ci, Mj[M_XSPH]);
+        int2 vj = vjs[u];
+        ax = wadd(ax, clampi(fp_mul_s(fp_mul_s(fp_sub_sat(vj.x, vi.x, g), w, 20 - ACC, g), cij, 16, g), -XS_TERM, XS_TERM));
+        ay = wadd(ay, clampi(fp_mul_s(fp_mul_s(fp_sub_sat(vj.y, vi.y, g), w, 20 - ACC, g), cij, 16, g), -XS_TERM, XS_TERM));
+      }
+      int32_t rj = rjs[u];
+      sx = wadd(sx, fp_mul_s(grad_q16(L.y, d[u].x, g), rj, 16, g));
+      sy = wadd(sy, fp_mul_s(grad_q16(L.y, d[u].y, g), rj, 16, g));
+      int32_t cij = mat_mix(ki, Mj[M_COND]);
+      int32_t wk = fp_mul_s(L.x, cij, 16, g);
+      int32_t dT = clampi(fp_sub_sat(bj.z, Ti, g), -DT_CLAMP, DT_CLAMP);
+      q = wadd(q, clampi(fp_mul_s(dT, wk, 20 - ACC, g), -HT_TERM, HT_TERM));
+      if ((uint32_t)bj.x != mi) {
+        int32_t attack = Mj[M_CORRODE];
+        int32_t spend = (Mj[M_CORRPT] != CORR_NEVER) ? corri : 0;
+        int32_t str = wadd(attack, spend);
+        if (str != 0) c = wadd(c, clampi(fp_mul_s(L.x, str, 20 - ACC, g), 0, CORR_TERM));
+      }
+    }
+  }
+  int32_t vx = vi.x, vy = vi.y;
+  if (doX) {
+    vx = fp_add_sat(vx, fp_shr_rne(ax, ACC), g);
+    vy = fp_add_sat(vy, fp_shr_rne(ay, ACC), g);
+  }
+  D.ps[p] = make_int4(pi.x, pi.y, vx, vy);
+  D.nrm[p] = make_int2(fp_mul_s(sx, H, 16, g), fp_mul_s(sy, H, 16, g));
+  D.heat[p] = q;
+  D.corr[p] = c;
+}
+
 __global__ __launch_bounds__(256) void k_xnc(Dev D) {
   uint32_t p = xcd_block() * 256u + threadIdx.x;
   uint32_t s = p < D.N ? D.sidOf[p] : INACTIVE;
   Dbg g;
+  const bool bnd = wave_bounded(D, s);
   if (s != INACTIVE) {
     uint32_t inf = D.info[p];
     if (!(inf & INF_FLUID)) {
       int4 a = D.psY[p];
       D.ps[p] = make_int4(a.x, a.y, 0, 0);
+    } else if (bnd) {
+      xnc_fluid<true>(D, p, g);
     } else {
-      int4 b = D.ptY[p];
-      uint32_t mi = (uint32_t)b.x;
-      const int32_t* Mi = MAT(D, mi);
-      uint32_t fl = (uint32_t)b.y;
-      bool doX = (fl & (FLAG_RIGID | FLAG_ANCHOR)) == 0u;
-      int32_t ci = Mi[M_XSPH], ki = Mi[M_COND], Ti = b.z, corri = Mi[M_CORRODE];
-      int2 pi = D.pred[p], vi = D.vtmp[p];
-      int32_t rhoi = D.rho[p];
-      uint32_t cnt = D.nbrN[p] & 0x1fffffffu;
-      const uint32_t* nb = D.nbr + p;
-      int32_t ax = 0, ay = 0, sx = 0, sy = 0, q = 0, c = 0;
-      for (uint32_t k = 0; k < cnt; k += XNC_UNROLL) {
-        // padding entries read as boundary entries, which this loop skips;
-        // a position difference of two in-range positions never saturates
-        uint32_t e[XNC_UNROLL];
-        int2 d[XNC_UNROLL], vjs[XNC_UNROLL];
-        int32_t idx[XNC_UNROLL], rhoj[XNC_UNROLL], rjs[XNC_UNROLL];
-        int4 Ls[XNC_UNROLL], bjs[XNC_UNROLL];
-#pragma unroll
-        for (uint32_t u = 0; u < XNC_UNROLL; ++u) e[u] = nbr_at(D, nb, k + u, cnt, p | NBR_BND);
-#pragma unroll
-        for (uint32_t u = 0; u < XNC_UNROLL; ++u) {
-          uint32_t qj = e[u] & ~NBR_BND;
-          d[u] = pred_delta(pi, D.pred[qj], g);
-          idx[u] = (e[u] & NBR_BND) ? -1 : lut_index(d[u], g);
-          Ls[u] = D.lut4[maxi(idx[u], 0)];
-          bjs[u] = D.ptY[qj];
-          rhoj[u] = D.rho[qj];
-          vjs[u] = D.vtmp[qj];
-          rjs[u] = D.rinv[qj];
-        }
-#pragma unroll
-        for (uint32_t u = 0; u < XNC_UNROLL; ++u) {
-          if (idx[u] < 0) continue;
-          const int4 L = Ls[u], bj = bjs[u];
-          const int32_t* Mj = MAT(D, (uint32_t)bj.x);
-          if (doX) {
-            int32_t K = sym_k(rhoi, rhoj[u], g);
-            int32_t w = fp_mul_s(L.x, K, 16, g);
-            int32_t cij = mat_mix(ci, Mj[M_XSPH]);
-            int2 vj = vjs[u];
-            ax = wadd(ax, clampi(fp_mul_s(fp_mul_s(fp_sub_sat(vj.x, vi.x, g), w, 20 - ACC, g), cij, 16, g), -XS_TERM, XS_TERM));
-            ay = wadd(ay, clampi(fp_mul_s(fp_mul_s(fp_sub_sat(vj.y, vi.y, g), w, 20 - ACC, g), cij, 16, g), -XS_TERM, XS_TERM));
-          }
-          int32_t rj = rjs[u];
-          sx = wadd(sx, fp_mul_s(grad_q16(L.y, d[u].x, g), rj, 16, g));
-          sy = wadd(sy, fp_mul_s(grad_q16(L.y, d[u].y, g), rj, 16, g));
-          int32_t cij = mat_mix(ki, Mj[M_COND]);
-          int32_t wk = fp_mul_s(L.x, cij, 16, g);
-          int32_t dT = clampi(fp_sub_sat(bj.z, Ti, g), -DT_CLAMP, DT_CLAMP);
-          q = wadd(q, clampi(fp_mul_s(dT, wk, 20 - ACC, g), -HT_TERM, HT_TERM));
-          if ((uint32_t)bj.x != mi) {
-            int32_t attack = Mj[M_CORRODE];
-            int32_t spend = (Mj[M_CORRPT] != CORR_NEVER) ? corri : 0;
-            int32_t str = wadd(attack, spend);
-            if (str != 0) c = wadd(c, clampi(fp_mul_s(L.x, str, 20 - ACC, g), 0, CORR_TERM));
-          }
-        }
-      }
-      int32_t vx = vi.x, vy = vi.y;
-      if (doX) {
-        vx = fp_add_sat(vx, fp_shr_rne(ax, ACC), g);
-        vy = fp_add_sat(vy, fp_shr_rne(ay, ACC), g);
-      }
-      D.ps[p] = make_int4(pi.x, pi.y, vx, vy);
-      D.nrm[p] = make_int2(fp_mul_s(sx, H, 16, g), fp_mul_s(sy, H, 16, g));
-      D.heat[p] = q;
-      D.corr[p] = c;
+      xnc_fluid<false>(D, p, g);
     }
   }
   flush_dbg(D, s, g, 0);
+}
+
+// surfaceTension's pair sum (src/sim.wgsl:2409-2494) for fluid particle p of
+// material row M: cohesion and curvature against fluid neighbours, adhesion
+// against boundary ones.
+template <bool BND>
+__device__ __forceinline__ int2 st_sum(const Dev& D, uint32_t p, const int32_t* M, Dbg& g) {
+  int2 pi = D.pred[p], ni = D.nrm[p];
+  int32_t rhoi = D.rho[p];
+  uint32_t cnt = D.nbrN[p] & 0x1fffffffu;
+  const uint32_t* nb = D.nbr + p;
+  int32_t ax = 0, ay = 0;
+  for (uint32_t k = 0; k < cnt; k += XNC_UNROLL) {
+    uint32_t e[XNC_UNROLL];
+    int2 d[XNC_UNROLL], njs[XNC_UNROLL];
+    int32_t idx[XNC_UNROLL], rhoj[XNC_UNROLL], cts[XNC_UNROLL], ats[XNC_UNROLL], matj[XNC_UNROLL];
+#pragma unroll
+    for (uint32_t u = 0; u < XNC_UNROLL; ++u) e[u] = nbr_at(D, nb, k + u, cnt, p);
+#pragma unroll
+    for (uint32_t u = 0; u < XNC_UNROLL; ++u) {
+      uint32_t qj = e[u] & ~NBR_BND;
+      d[u] = pdelta<BND>(pi, D.pred[qj], g);
+      idx[u] = (k + u < cnt) ? lut_idx<BND>(d[u], g) : -1;
+      uint32_t ii = (uint32_t)maxi(idx[u], 0);
+      ats[u] = D.lutA[ii];
+      cts[u] = D.lut4[ii].z;
+      rhoj[u] = D.rho[qj];
+      matj[u] = D.ptY[qj].x;
+      njs[u] = D.nrm[qj];
+    }
+#pragma unroll
+    for (uint32_t u = 0; u < XNC_UNROLL; ++u) {
+    if (idx[u] < 0) continue;
+    const int2 d1 = d[u];
+    if (e[u] & NBR_BND) {
+      int32_t at = ats[u];
+      ax = wsub(ax, clampi(fp_mul_s(fp_mul_s(at, d1.x, 22, g), M[M_ADH], 16 - ACC, g), -ST_TERM, ST_TERM));
+      ay = wsub(ay, clampi(fp_mul_s(fp_mul_s(at, d1.y, 22, g), M[M_ADH], 16 - ACC, g), -ST_TERM, ST_TERM));
+      continue;
+    }
+    int32_t K = sym_k(rhoi, rhoj[u], g);
+    int32_t ct = cts[u];
+    const int32_t* Mj = MAT(D, (uint32_t)matj[u]);
+    int32_t gCoh = mat_mix(M[M_GCOH], Mj[M_GCOH]);
+    int32_t gCur = mat_mix(M[M_GCUR], Mj[M_GCUR]);
+    int32_t fx = fp_neg_sat(fp_mul_s(fp_mul_s(ct, d1.x, 22, g), gCoh, 16, g), g);
+    int32_t fy = fp_neg_sat(fp_mul_s(fp_mul_s(ct, d1.y, 22, g), gCoh, 16, g), g);
+    int2 nj = njs[u];
+    int32_t cx = fp_mul_s(fp_sub_sat(ni.x, nj.x, g), gCur, 16, g);
+    int32_t cy = fp_mul_s(fp_sub_sat(ni.y, nj.y, g), gCur, 16, g);
+    fx = fp_add_sat(fx, fp_neg_sat(cx, g), g);
+    fy = fp_add_sat(fy, fp_neg_sat(cy, g), g);
+    ax = wadd(ax, clampi(fp_mul_s(fx, K, 16 - ACC, g), -ST_TERM, ST_TERM));
+    ay = wadd(ay, clampi(fp_mul_s(fy, K, 16 - ACC, g), -ST_TERM, ST_TERM));
+    }
+  }
+  return make_int2(ax, ay);
 }
 
 // ==================================================== surfaceTension + thermal
@@ -870,6 +994,7 @@ __global__ __launch_bounds__(256) void k_st_thermal_hash(Dev D) {
   Dbg g;
   uint32_t words[8];
   uint32_t myid = 0;
+  const bool bnd = wave_bounded(D, s);
   if (s != INACTIVE) {
     const SceneP& P = D.sp[s];
     const SceneD& SD = D.sd[s];
@@ -883,55 +1008,8 @@ __global__ __launch_bounds__(256) void k_st_thermal_hash(Dev D) {
       uint32_t mat0 = (uint32_t)b.x;
       const int32_t* M = MAT(D, mat0);
       // ---- surfaceTension
-      int2 pi = D.pred[p], ni = D.nrm[p];
-      int32_t rhoi = D.rho[p];
-      uint32_t cnt = D.nbrN[p] & 0x1fffffffu;
-      const uint32_t* nb = D.nbr + p;
-      int32_t ax = 0, ay = 0;
-      for (uint32_t k = 0; k < cnt; k += XNC_UNROLL) {
-        uint32_t e[XNC_UNROLL];
-        int2 d[XNC_UNROLL], njs[XNC_UNROLL];
-        int32_t idx[XNC_UNROLL], rhoj[XNC_UNROLL], cts[XNC_UNROLL], ats[XNC_UNROLL], matj[XNC_UNROLL];
-#pragma unroll
-        for (uint32_t u = 0; u < XNC_UNROLL; ++u) e[u] = nbr_at(D, nb, k + u, cnt, p);
-#pragma unroll
-        for (uint32_t u = 0; u < XNC_UNROLL; ++u) {
-          uint32_t qj = e[u] & ~NBR_BND;
-          d[u] = pred_delta(pi, D.pred[qj], g);
-          idx[u] = (k + u < cnt) ? lut_index(d[u], g) : -1;
-          uint32_t ii = (uint32_t)maxi(idx[u], 0);
-          ats[u] = D.lutA[ii];
-          cts[u] = D.lut4[ii].z;
-          rhoj[u] = D.rho[qj];
-          matj[u] = D.ptY[qj].x;
-          njs[u] = D.nrm[qj];
-        }
-#pragma unroll
-        for (uint32_t u = 0; u < XNC_UNROLL; ++u) {
-        if (idx[u] < 0) continue;
-        const int2 d1 = d[u];
-        if (e[u] & NBR_BND) {
-          int32_t at = ats[u];
-          ax = wsub(ax, clampi(fp_mul_s(fp_mul_s(at, d1.x, 22, g), M[M_ADH], 16 - ACC, g), -ST_TERM, ST_TERM));
-          ay = wsub(ay, clampi(fp_mul_s(fp_mul_s(at, d1.y, 22, g), M[M_ADH], 16 - ACC, g), -ST_TERM, ST_TERM));
-          continue;
-        }
-        int32_t K = sym_k(rhoi, rhoj[u], g);
-        int32_t ct = cts[u];
-        const int32_t* Mj = MAT(D, (uint32_t)matj[u]);
-        int32_t gCoh = mat_mix(M[M_GCOH], Mj[M_GCOH]);
-        int32_t gCur = mat_mix(M[M_GCUR], Mj[M_GCUR]);
-        int32_t fx = fp_neg_sat(fp_mul_s(fp_mul_s(ct, d1.x, 22, g), gCoh, 16, g), g);
-        int32_t fy = fp_neg_sat(fp_mul_s(fp_mul_s(ct, d1.y, 22, g), gCoh, 16, g), g);
-        int2 nj = njs[u];
-        int32_t cx = fp_mul_s(fp_sub_sat(ni.x, nj.x, g), gCur, 16, g);
-        int32_t cy = fp_mul_s(fp_sub_sat(ni.y, nj.y, g), gCur, 16, g);
-        fx = fp_add_sat(fx, fp_neg_sat(cx, g), g);
-        fy = fp_add_sat(fy, fp_neg_sat(cy, g), g);
-        ax = wadd(ax, clampi(fp_mul_s(fx, K, 16 - ACC, g), -ST_TERM, ST_TERM));
-        ay = wadd(ay, clampi(fp_mul_s(fy, K, 16 - ACC, g), -ST_TERM, ST_TERM));
-        }
-      }
+      const int2 axy = bnd ? st_sum<true>(D, p, M, g) : st_sum<false>(D, p, M, g);
+      const int32_t ax = axy.x, ay = axy.y;
       stOut = make_int2(fp_shr_rne(ax, ACC), fp_shr_rne(ay, ACC));
       // ---- thermal
       int32_t T = fp_add_sat(b.z, fp_shr_rne(D.heat[p], ACC), g);
@@ -1552,9 +1630,12 @@ int sn_load_scenes(sn_engine* e, const sn_scene* scenes, uint32_t count) {
     uint32_t gh = (uint32_t)std::max<uint64_t>(1, ((uint64_t)sc.dom_h + cs - 1) / cs);
     if ((uint64_t)gw * gh > 1048576u)
       return fail(e, SN_E_ARG, "[aether] cellTotal exceeds the 2-level scan limit 1048576");
+    bool bounded = sc.dom_w <= POS_BOUND && sc.dom_h <= POS_BOUND;
     for (uint32_t i = 0; i < sc.n; ++i) {
-      uint32_t m = (uint32_t)sc.state8[(size_t)i * 8 + 4];
+      const int32_t* w = sc.state8 + (size_t)i * 8;
+      uint32_t m = (uint32_t)w[4];
       if (m >= MAT_MAX) return fail(e, SN_E_ARG, "[aether] matId outside the material table");
+      bounded = bounded && w[0] >= -POS_BOUND && w[0] <= POS_BOUND && w[1] >= -POS_BOUND && w[1] <= POS_BOUND;
     }
     SceneP& P = sp[s];
     std::memset(&P, 0, sizeof(P));
@@ -1572,6 +1653,7 @@ int sn_load_scenes(sn_engine* e, const sn_scene* scenes, uint32_t count) {
     P.domH = sc.dom_h;
     P.vmax = 65536;
     P.weldHold = 0;
+    P.bounded = bounded ? 1u : 0u;
     totN += sc.n;
     totC += (uint64_t)gw * gh;
   }

@@ -68,6 +68,19 @@ def test_overdense_truncation_stream():
     e.destroy()
 
 
+def test_unbounded_domain_stream():
+    """A domain wider than 2^28 ticks: the kernels take the saturating pair
+    geometry (the bounded fast path is off for this scene) and must still
+    match the oracle bit for bit."""
+    from sunaengine_amd import scenes
+    wide = scenes.dam_break_4096()
+    wide.domW = 1 << 29
+    wide.name = "dam_break_4096_wide"
+    _print("dam_break_4096 in a 2^29-tick wide domain x120")
+    e, o = _stream_parity(wide, 3, 120)
+    e.destroy()
+
+
 def test_thermal_small_stream():
     """Non-uniform temperatures: conduction and ambient relaxation."""
     from sunaengine_amd import scenes
