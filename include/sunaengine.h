@@ -128,6 +128,17 @@ int sn_set_events(sn_engine* e, const sn_event* events, uint32_t count);
  * state and are not advanced by sn_step. */
 int sn_set_active(sn_engine* e, const uint8_t* active, uint32_t count);
 
+/* Slotted worlds for continuous batching (no counterpart in the reference,
+ * whose engine holds one scene).  sn_reserve_slots replaces the world with
+ * `count` empty, inactive partitions of cap_particles[s] particle slots and
+ * cap_cells[s] grid cells.  sn_load_slot places one scene (validated as
+ * sn_load_scene validates) into partition `slot` and activates it, with its
+ * own substep counter, chain and H ring starting afresh; the other
+ * partitions keep their state.  sn_set_active(..., 0) retires a partition.
+ * SN_E_CAPACITY if the scene does not fit the partition. */
+int sn_reserve_slots(sn_engine* e, uint32_t count, const uint32_t* cap_particles, const uint32_t* cap_cells);
+int sn_load_slot(sn_engine* e, uint32_t slot, const sn_scene* scene);
+
 /* Profiling form of sn_step (no counterpart in the reference, which has no
  * device timers): enqueue `count` substeps eagerly with a HIP event pair
  * around every launch on the engine's stream, wait, and return per kernel

@@ -1292,6 +1292,7 @@ struct sn_engine {
   hipGraphExec_t graph = nullptr;
   uint32_t graphSubsteps = 0;
   bool loaded = false;
+  bool slotted = false;  // sn_reserve_slots layout: partitions keep their bases
   // profiling: an event pair around every launch, tagged with its class
   std::vector<hipEvent_t> evPool;
   std::vector<int> evClass;
@@ -1308,6 +1309,10 @@ T* dalloc(sn_engine* e, size_t count) {
   e->allocs.push_back(p);
   return static_cast<T*>(p);
 }
+
+// chain start value (CHAIN_INIT, src/hash/statehash.js:28-31)
+const uint32_t CHAIN_INIT[8] = {0x243f6a88u, 0x85a308d3u, 0x13198a2eu, 0x03707344u,
+                                 0xa4093822u, 0x299f31d0u, 0x082efa98u, 0xec4e6c89u};
 
 int fail(sn_engine* e, int code, const std::string& msg) {
   if (e) e->err = msg;
@@ -1684,6 +1689,7 @@ int sn_load_scenes(sn_engine* e, const sn_scene* scenes, uint32_t count) {
     e->C = (uint32_t)totC;
     e->sp = sp;
     e->active.assign(count, 1);
+    e->slotted = false;
     Dev& D = e->D;
     D.N = e->N;
     D.C = e->C;
@@ -1710,8 +1716,6 @@ int sn_load_scenes(sn_engine* e, const sn_scene* scenes, uint32_t count) {
       HIPCHECK(hipMemcpyAsync(D.st, st.data(), sizeof(int2) * e->N, hipMemcpyHostToDevice, e->stream));
     }
     std::vector<SceneD> sd(count);
-    static const uint32_t CHAIN_INIT[8] = {0x243f6a88u, 0x85a308d3u, 0x13198a2eu, 0x03707344u,
-                                           0xa4093822u, 0x299f31d0u, 0x082efa98u, 0xec4e6c89u};
     for (auto& d : sd) {
       std::memset(&d, 0, sizeof(d));
       d.bondDirty = 1;
@@ -1750,6 +1754,139 @@ int sn_load_scenes(sn_engine* e, const sn_scene* scenes, uint32_t count) {
 }
 
 int sn_load_scene(sn_engine* e, const sn_scene* scene) { return sn_load_scenes(e, scene, 1); }
+
+// ------------------------------------------------------------------ slots
+// A slotted world (continuous batching): `count` fixed partitions, each with
+// room for cap_particles[s] slots and cap_cells[s] grid cells, all empty and
+// inactive.  sn_load_slot then places one scene into a free partition while
+// the others keep their state, and sn_set_active(…, 0) retires it.
+int sn_reserve_slots(sn_engine* e, uint32_t count, const uint32_t* cap_particles, const uint32_t* cap_cells) {
+  if (!e || !cap_particles || !cap_cells) return SN_E_ARG;
+  if (count == 0) return fail(e, SN_E_ARG, "[aether] a world needs at least one slot");
+  if (count > e->opts.max_scenes)
+    return fail(e, SN_E_CAPACITY, "[aether] " + std::to_string(count) + " slots exceed maxScenes " +
+                                      std::to_string(e->opts.max_scenes));
+  uint64_t totN = 0, totC = 0;
+  std::vector<SceneP> sp(count);
+  for (uint32_t s = 0; s < count; ++s) {
+    SceneP& P = sp[s];
+    std::memset(&P, 0, sizeof(P));
+    P.partBase = (uint32_t)totN;
+    P.cellBase = (uint32_t)totC;
+    P.cap = cap_particles[s];
+    P.cellCap = cap_cells[s];
+    P.vmax = 65536;
+    P.domW = P.domH = 1;
+    P.gridW = P.gridH = 1;
+    P.cellShift = 18;
+    totN += cap_particles[s];
+    totC += cap_cells[s];
+  }
+  if (totN > e->opts.max_particles || totC > e->opts.max_cells)
+    return fail(e, SN_E_CAPACITY, "[aether] slot capacities exceed maxParticles / maxCells");
+  try {
+    HIPCHECK(hipSetDevice((int)e->opts.device));
+    HIPCHECK(hipStreamSynchronize(e->stream));
+    drop_graph(e);
+    e->S = count;
+    e->N = (uint32_t)totN;
+    e->C = (uint32_t)totC;
+    e->sp = sp;
+    e->active.assign(count, 0);
+    Dev& D = e->D;
+    D.N = e->N;
+    D.C = e->C;
+    D.S = e->S;
+    std::vector<SceneD> sd(count);
+    for (auto& d : sd) std::memset(&d, 0, sizeof(d));
+    HIPCHECK(hipMemcpyAsync(D.sd, sd.data(), sizeof(SceneD) * count, hipMemcpyHostToDevice, e->stream));
+    HIPCHECK(hipMemsetAsync(D.cellCount, 0, sizeof(uint32_t) * std::max(e->C, 1u), e->stream));
+    e->maxEventsPerSubstep = 0;
+    e->tileG = 1;
+    upload_layout(e);
+    e->loaded = true;
+    e->slotted = true;
+  } catch (const DevError& d) {
+    return fail(e, SN_E_DEVICE, d.msg);
+  }
+  return SN_OK;
+}
+
+int sn_load_slot(sn_engine* e, uint32_t slot, const sn_scene* sc) {
+  if (!e || !sc) return SN_E_ARG;
+  if (!e->loaded || !e->slotted) return fail(e, SN_E_STATE, "[aether] sn_load_slot needs a slotted world");
+  if (slot >= e->S) return fail(e, SN_E_ARG, "[aether] slot index out of range");
+  if (sc->n_fluid > sc->n) return fail(e, SN_E_ARG, "[aether] scene.nFluid out of range");
+  if (sc->n && !sc->state8) return fail(e, SN_E_ARG, "[aether] scene state missing");
+  if (sc->cell_shift < 1 || sc->cell_shift > 27)
+    return fail(e, SN_E_ARG, "[aether] scene.cellShift outside the 1..27 contract");
+  if ((int64_t(1) << sc->cell_shift) < H) return fail(e, SN_E_ARG, "[aether] cells smaller than the kernel radius");
+  if (sc->dom_w <= 2 * WALL || sc->dom_h <= 2 * WALL)
+    return fail(e, SN_E_ARG, "[aether] domain smaller than the wall band");
+  SceneP P = e->sp[slot];
+  const uint64_t cs = uint64_t(1) << sc->cell_shift;
+  const uint32_t gw = (uint32_t)std::max<uint64_t>(1, ((uint64_t)sc->dom_w + cs - 1) / cs);
+  const uint32_t gh = (uint32_t)std::max<uint64_t>(1, ((uint64_t)sc->dom_h + cs - 1) / cs);
+  if (sc->n > P.cap || (uint64_t)gw * gh > P.cellCap)
+    return fail(e, SN_E_CAPACITY, "[aether] scene does not fit slot " + std::to_string(slot));
+  bool bounded = sc->dom_w <= POS_BOUND && sc->dom_h <= POS_BOUND;
+  for (uint32_t i = 0; i < sc->n; ++i) {
+    const int32_t* w = sc->state8 + (size_t)i * 8;
+    if ((uint32_t)w[4] >= MAT_MAX) return fail(e, SN_E_ARG, "[aether] matId outside the material table");
+    bounded = bounded && w[0] >= -POS_BOUND && w[0] <= POS_BOUND && w[1] >= -POS_BOUND && w[1] <= POS_BOUND;
+  }
+  try {
+    HIPCHECK(hipSetDevice((int)e->opts.device));
+    HIPCHECK(hipStreamSynchronize(e->stream));
+    std::vector<int32_t> mats(MAT_MAX * MAT_WORDS);
+    HIPCHECK(hipMemcpy(mats.data(), e->D.mats, sizeof(int32_t) * mats.size(), hipMemcpyDeviceToHost));
+    for (uint32_t i = 0; i < sc->n_fluid; ++i) {
+      const int32_t* M = &mats[(uint32_t)sc->state8[(size_t)i * 8 + 4] * MAT_WORDS];
+      if (M[M_FRIC] != 0 || M[M_BONDK] != 0)
+        return fail(e, SN_E_UNSUPPORTED, "[aether] material row has friction or bonds; this build runs "
+                                         "frictionless, bondless rows");
+    }
+    drop_graph(e);
+    P.n = sc->n;
+    P.nFluid = sc->n_fluid;
+    P.gridW = gw;
+    P.gridH = gh;
+    P.cellShift = sc->cell_shift;
+    P.cellTotal = gw * gh;
+    P.domW = sc->dom_w;
+    P.domH = sc->dom_h;
+    P.bounded = bounded ? 1u : 0u;
+    e->sp[slot] = P;
+    Dev& D = e->D;
+    std::vector<int4> ps(sc->n), pt(sc->n);
+    std::vector<uint32_t> id(sc->n);
+    std::vector<int2> st(sc->n, make_int2(0, 0));
+    for (uint32_t i = 0; i < sc->n; ++i) {
+      const int32_t* w = sc->state8 + (size_t)i * 8;
+      ps[i] = make_int4(w[0], w[1], w[2], w[3]);
+      pt[i] = make_int4(w[4], w[5], w[6], w[7]);
+      id[i] = i;
+      if (sc->st_acc) st[i] = make_int2(sc->st_acc[2 * i], sc->st_acc[2 * i + 1]);
+    }
+    if (sc->n) {
+      HIPCHECK(hipMemcpyAsync(D.ps + P.partBase, ps.data(), sizeof(int4) * sc->n, hipMemcpyHostToDevice, e->stream));
+      HIPCHECK(hipMemcpyAsync(D.pt + P.partBase, pt.data(), sizeof(int4) * sc->n, hipMemcpyHostToDevice, e->stream));
+      HIPCHECK(hipMemcpyAsync(D.id + P.partBase, id.data(), sizeof(uint32_t) * sc->n, hipMemcpyHostToDevice,
+                              e->stream));
+      HIPCHECK(hipMemcpyAsync(D.st + P.partBase, st.data(), sizeof(int2) * sc->n, hipMemcpyHostToDevice, e->stream));
+    }
+    SceneD d;
+    std::memset(&d, 0, sizeof(d));
+    d.bondDirty = 1;
+    for (int j = 0; j < 8; ++j) d.chain[j] = CHAIN_INIT[j];
+    HIPCHECK(hipMemcpyAsync(D.sd + slot, &d, sizeof(SceneD), hipMemcpyHostToDevice, e->stream));
+    e->active[slot] = 1;
+    upload_layout(e);
+  } catch (const DevError& d) {
+    return fail(e, SN_E_DEVICE, d.msg);
+  }
+  return SN_OK;
+}
 
 int sn_step(sn_engine* e, uint32_t count) {
   if (!e) return SN_E_ARG;

@@ -99,6 +99,27 @@ class Engine:
         self.scenes, self.images = scenes, imgs
         return self
 
+    # ------------------------------------------------------- slotted worlds
+    def reserve_slots(self, cap_particles: Sequence[int], cap_cells: Sequence[int]):
+        """Replace the world with empty, inactive partitions (continuous batching)."""
+        cp = np.ascontiguousarray(cap_particles, np.uint32)
+        cc = np.ascontiguousarray(cap_cells, np.uint32)
+        assert cp.size == cc.size
+        self._check(native.lib().sn_reserve_slots(self._h, cp.size, cp.ctypes.data, cc.ctypes.data))
+        self.scenes = [None] * cp.size
+        self.images = [None] * cp.size
+        return self
+
+    def load_slot(self, slot: int, scene: sc.Scene):
+        """Place `scene` into free partition `slot` and activate it; the other
+        partitions keep running state."""
+        im = sc.load_image(scene, self.mats_rows, max_particles=self.max_particles, max_cells=self.max_cells)
+        st = np.ascontiguousarray(im.state, np.int32)
+        one = native.sn_scene(im.n, im.nFluid, st.ctypes.data, im.domW, im.domH, im.cellShift, None)
+        self._check(native.lib().sn_load_slot(self._h, int(slot), C.byref(one)))
+        self.scenes[slot], self.images[slot] = scene, im
+        return self
+
     @property
     def n(self) -> int:
         return self.images[0].n
