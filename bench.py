@@ -102,16 +102,194 @@ def main():
     ap.add_argument("--gpus", type=int, default=1)
     ap.add_argument("--steps", type=int, default=100)
     ap.add_argument("--warmup", type=int, default=10)
+    ap.add_argument("--workload", default="scene1m", choices=("scene1m", "batched", "serve", "3d"),
+                    help="scene1m: the 1M-particle 2D scene (default); batched: 512 uneven scenes, "
+                         "batched vs one at a time; serve: Poisson job stream with continuous batching; "
+                         "3d: the 2M-particle 3D block")
     ap.add_argument("--side", type=int, default=1024, help="fluid block side (1024 = 1M particles)")
     ap.add_argument("--cpu-substeps", type=int, default=8, help="CPU-baseline window in substeps")
     ap.add_argument("--cpu-threads", type=int, default=0, help="0 = all available host cores")
     ap.add_argument("--no-cpu", action="store_true")
+    ap.add_argument("--scenes", type=int, default=512, help="batched: scene count")
+    ap.add_argument("--solo-sample", type=int, default=16, help="batched: scenes also run one at a time")
+    ap.add_argument("--rates", default="10,50,200,500", help="serve: arrival rates in jobs/s")
+    ap.add_argument("--jobs", type=int, default=16, help="serve: jobs per arrival rate")
+    ap.add_argument("--max-steps", type=int, default=5000, help="serve: upper bound of a job's substeps")
     args = ap.parse_args()
     if args.gpus != 1:
         print(json.dumps({"error": "this build runs on one GPU; --gpus must be 1",
                           "n_gpus": args.gpus}), flush=True)
         sys.exit(2)
+    if args.workload == "batched":
+        return run_batched(args)
+    if args.workload == "serve":
+        return run_serve(args)
+    if args.workload == "3d":
+        return run_3d(args)
+    return run_scene1m(args)
 
+
+def _line(value, unit, steps, warmup, ms_per_step, config, roofline=None, cpu_baseline=None,
+          higher_is_better=True, scaling="weak"):
+    return {
+        "metric": METRIC, "value": round(value, 1), "unit": unit, "n_gpus": 1, "steps": steps,
+        "warmup": warmup, "ms_per_step": round(ms_per_step, 4), "higher_is_better": higher_is_better,
+        "scaling": scaling, "vs_baseline": None, "dtype": "int32",
+        "data": "synthetic (seeded integer scenes; no dataset)", "config": config,
+        "roofline": roofline, "cpu_baseline": cpu_baseline,
+    }
+
+
+def run_batched(args):
+    """512 independent 2D scenes (fluid counts log-uniform in [1024, 65536],
+    0.5 wu blocks at seeded positions in walled 256 x 128 wu boxes), each with
+    its own stir schedule (Bernoulli 1/50 per substep, radius 8 wu, magnitude
+    uniform in [0, 2] wu per substep), advanced together in one world by
+    hipGraph-replayed substeps, against a sample of the same scenes run one
+    at a time; every sampled scene's per-substep stream must be equal in both."""
+    from sunaengine_amd import scenes, serve
+    from sunaengine_amd.engine import Engine
+    iters = 4
+    K, W = args.steps, args.warmup
+    t0 = time.time()
+    ss = scenes.batch_scenes(count=args.scenes)
+    imgs = [scenes.load_image(s) for s in ss]
+    evs = []
+    for k, s in enumerate(ss):
+        box = (8 * 65536, s.domH // 2, s.domW - 8 * 65536, s.domH - 8 * 65536)
+        evs.append(serve.stir_events_2d(scenes.splitmix32(0xE7E7 + k), W + K, box))
+    F = sum(im.nFluid for im in imgs)
+    N = sum(im.n for im in imgs)
+    C = sum(im.cellTotal for im in imgs)
+    print(f"# {len(ss)} scenes: {F:,} fluid, {N:,} slots, {C:,} cells, "
+          f"{sum(len(e) for e in evs)} stir events ({time.time() - t0:.1f}s)", file=sys.stderr, flush=True)
+    eng = Engine(max_particles=N, max_cells=C, iters=iters, max_scenes=len(ss), stream_cap=K + W + 8)
+    eng.load_scenes(ss)
+    eng.set_events([(k,) + tuple(e) for k, es in enumerate(evs) for e in es])
+    eng.step(W)
+    eng.sync()
+    eng.reset_stream()
+    t1 = time.perf_counter()
+    eng.step(K)
+    eng.sync()
+    dt = time.perf_counter() - t1
+    agg = F * K / dt
+    print(f"# batched: {dt * 1e3 / K:.3f} ms/substep, {agg:.4e} ps/s", file=sys.stderr, flush=True)
+    order = sorted(range(len(ss)), key=lambda k: imgs[k].nFluid)
+    m = max(1, min(args.solo_sample, len(ss)))
+    pick = sorted({order[int(round(i * (len(order) - 1) / max(1, m - 1)))] for i in range(m)})
+    solo_work, solo_time, equal = 0, 0.0, True
+    for k in pick:
+        e1 = Engine(max_particles=imgs[k].n, max_cells=imgs[k].cellTotal, iters=iters, stream_cap=K + W + 8)
+        e1.load_scene(ss[k])
+        e1.set_events([(0,) + tuple(e) for e in evs[k]])
+        e1.step(W)
+        e1.sync()
+        e1.reset_stream()
+        t2 = time.perf_counter()
+        e1.step(K)
+        e1.sync()
+        solo_time += time.perf_counter() - t2
+        solo_work += imgs[k].nFluid * K
+        equal = equal and bool(np.array_equal(e1.hash_stream(), eng.hash_stream(k)))
+        e1.destroy()
+    solo = solo_work / solo_time
+    print(f"# one at a time ({len(pick)} scenes): {solo:.4e} ps/s; batched/solo {agg / solo:.2f}x; "
+          f"streams equal: {equal}", file=sys.stderr, flush=True)
+    cfg = {
+        "workload": (f"batched: {len(ss)} independent 2D scenes, fluid counts log-uniform in [1024, 65536] "
+                     f"({F:,} fluid in all), 0.5 wu blocks in walled 256x128 wu boxes, {iters} density "
+                     "iterations, per-scene stir schedules (Bernoulli 1/50 per substep, r = 8 wu, |i| in "
+                     "[0, 2] wu/substep); hipGraph-replayed substeps; unit = one fluid particle-substep"),
+        "scenes": len(ss), "n_fluid": F, "n_particles": N, "cells": C, "iters": iters,
+        "stir_events": sum(len(e) for e in evs),
+        "one_at_a_time_value": round(solo, 1),
+        "one_at_a_time_sample": f"{len(pick)} scenes spread over the size range, each alone in its own world",
+        "batched_over_one_at_a_time": round(agg / solo, 3),
+        "streams_equal_alone_and_batched": equal,
+    }
+    print(json.dumps(_line(agg, "particle-substeps/s", K, W, dt * 1e3 / K, cfg)), flush=True)
+    eng.destroy()
+
+
+def run_serve(args):
+    """A Poisson stream of replay/verify jobs (70% 2D, 30% 3D; particle counts
+    log-uniform in [4096, 262144]; substeps uniform in [100, max-steps]; stir
+    schedules as in the batched workload) served with continuous batching:
+    admission by free partition memory, 2D and 3D worlds on their own streams,
+    each job's per-substep H stream returned.  One sweep over arrival rates;
+    `value` is the aggregate particle-substeps/s over the sweep."""
+    from sunaengine_amd import serve
+    rates = [float(r) for r in args.rates.split(",")]
+    srv = serve.Server()
+    per = []
+    work = 0.0
+    busy = 0.0
+    for ri, rate in enumerate(rates):
+        jobs = serve.poisson_jobs(args.jobs, rate, seed=0x5E7E + 101 * ri, steps_hi=args.max_steps)
+        st = srv.run(jobs)
+        summ = serve.summarize(jobs, st["wall_s"])
+        summ["rate_jobs_per_s"] = rate
+        per.append(summ)
+        work += summ["particle_substeps"]
+        busy += st["wall_s"]
+        print(f"# rate {rate}/s: {summ['jobs']} jobs, {summ['throughput']:.3e} ps/s, "
+              f"p50 {summ['p50_latency_s']:.2f}s p99 {summ['p99_latency_s']:.2f}s", file=sys.stderr, flush=True)
+    srv.close()
+    value = work / busy
+    cfg = {
+        "workload": ("serving: Poisson job stream, 70% 2D / 30% 3D, particle counts log-uniform in "
+                     f"[4096, 262144], substeps uniform in [100, {args.max_steps}], stir schedules; "
+                     "continuous batching into slotted 2D and 3D worlds, admission by partition memory; "
+                     "unit = one particle-substep of a job"),
+        "rates": per,
+        "jobs_per_rate": args.jobs,
+    }
+    p99 = max(p["p99_latency_s"] for p in per)
+    print(json.dumps(_line(value, "particle-substeps/s", 0, 0, 0.0,
+                           dict(cfg, worst_p99_latency_s=p99))), flush=True)
+
+
+def run_3d(args):
+    """The 3D solver at scale: a 128^3 block (2,097,152 particles) at 0.5 wu
+    pitch with seeded jitter in a 256 x 128 x 256 wu world (128 x 64 x 128
+    cells of 2 wu), 3 density iterations; unit = one particle-substep."""
+    from sunaengine_amd import scenes3d
+    from sunaengine_amd.engine3d import Engine3D
+    iters = 3
+    t0 = time.time()
+    side = 128 if args.side == 1024 else args.side
+    s = scenes3d.block_scene(side=side)
+    print(f"# 3D block n={s.n} cells={s.cell_total} ({time.time() - t0:.1f}s)", file=sys.stderr, flush=True)
+    e = Engine3D(max_particles=s.n, max_cells=s.cell_total, iters=iters,
+                 stream_cap=max(4096, args.steps + args.warmup + 8))
+    e.load([s])
+    e.step(args.warmup)
+    e.sync()
+    e.reset_stream()
+    t1 = time.perf_counter()
+    e.step(args.steps)
+    e.sync()
+    dt = time.perf_counter() - t1
+    value = s.n * args.steps / dt
+    prof = e.profile_step(2)
+    print(f"# 3D: {dt * 1e3 / args.steps:.3f} ms/substep, {value:.4e} ps/s", file=sys.stderr, flush=True)
+    bpu = 172 + 92 * iters + 28 * s.cell_total / s.n
+    ach = value * bpu / 1e9
+    cfg = {"workload": (f"3D single scene: {s.n:,} particles ({side}^3 block at 0.5 wu, seeded jitter) in a "
+                        f"{s.gx}x{s.gy}x{s.gz}-cell world, {iters} density iterations; unit = one "
+                        "particle-substep"),
+           "n_particles": s.n, "cells": s.cell_total, "iters": iters,
+           "kernel_ms_per_2_substeps": prof}
+    roof = {"bound": "hbm", "achieved": round(ach, 2), "peak": HBM_PEAK_GBS, "unit": "GB/s",
+            "frac": round(ach / HBM_PEAK_GBS, 6), "traffic": None, "kernel": "whole 3D substep",
+            "bytes_per_particle_substep": round(bpu, 1)}
+    print(json.dumps(_line(value, "particle-substeps/s", args.steps, args.warmup, dt * 1e3 / args.steps,
+                           cfg, roofline=roof)), flush=True)
+    e.destroy()
+
+
+def run_scene1m(args):
     from sunaengine_amd import scenes
     from sunaengine_amd.engine import Engine
 

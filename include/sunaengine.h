@@ -198,6 +198,10 @@ typedef struct sn3_event {
 } sn3_event;
 int sn3_set_events(sn3_engine* e, const sn3_event* events, uint32_t count);
 int sn3_set_active(sn3_engine* e, const uint8_t* active, uint32_t count);
+/* Slotted 3D worlds, as sn_reserve_slots / sn_load_slot (no reference
+ * counterpart). */
+int sn3_reserve_slots(sn3_engine* e, uint32_t count, const uint32_t* cap_particles, const uint32_t* cap_cells);
+int sn3_load_slot(sn3_engine* e, uint32_t slot, const sn3_scene* scene);
 /* out: [0] substep, [1] predict+sort, [2] neighbour build + density 0,
  * [3] solve iterations, [4] chain fold (device ms summed over count). */
 int sn3_profile_step(sn3_engine* e, uint32_t count, float out[5]);

@@ -66,6 +66,7 @@ struct Scene3P {
   uint32_t cellBase, cellTotal, partBase;
   int32_t posShift;
   int32_t wx, wy, wz, pad;   // world extents in ticks
+  uint32_t cap, cellCap, pad2, pad3;  // partition sizes (slotted worlds)
 };
 struct Scene3D {
   uint32_t substep, fault, streamPos, pad;
@@ -574,6 +575,7 @@ struct sn3_engine {
   hipGraphExec_t graph = nullptr;
   uint32_t graphSubsteps = 0;
   bool loaded = false;
+  bool slotted = false;
   uint32_t maxEvents = 0;
   Event3* dEvents = nullptr;
   uint32_t eventCap = 0;
@@ -794,6 +796,7 @@ int sn3_load(sn3_engine* e, const sn3_scene* scenes, uint32_t count) {
     e->C = (uint32_t)totC;
     e->sp = sp;
     e->active.assign(count, 1);
+    e->slotted = false;
     Dev3& D = e->D;
     D.N = e->N;
     D.C = e->C;
@@ -825,6 +828,109 @@ int sn3_load(sn3_engine* e, const sn3_scene* scenes, uint32_t count) {
     e->maxEvents = 0;
     upload_layout3(e);
     e->loaded = true;
+  } catch (const DevError3& d) {
+    return fail3(e, SN_E_DEVICE, d.msg);
+  }
+  return SN_OK;
+}
+
+// Slotted 3D worlds (continuous batching): see sn_reserve_slots.
+int sn3_reserve_slots(sn3_engine* e, uint32_t count, const uint32_t* cap_particles, const uint32_t* cap_cells) {
+  if (!e || !cap_particles || !cap_cells || !count) return SN_E_ARG;
+  if (count > e->opts.max_scenes) return fail3(e, SN_E_CAPACITY, "[aether-3d] too many slots");
+  uint64_t totN = 0, totC = 0;
+  std::vector<Scene3P> sp(count);
+  for (uint32_t s = 0; s < count; ++s) {
+    Scene3P& P = sp[s];
+    std::memset(&P, 0, sizeof(P));
+    P.partBase = (uint32_t)totN;
+    P.cellBase = (uint32_t)totC;
+    P.cap = cap_particles[s];
+    P.cellCap = cap_cells[s];
+    P.gx = P.gy = P.gz = 1;
+    P.wx = P.wy = P.wz = H;
+    totN += cap_particles[s];
+    totC += cap_cells[s];
+  }
+  if (totN > e->opts.max_particles || totC > e->opts.max_cells)
+    return fail3(e, SN_E_CAPACITY, "[aether-3d] slot capacities exceed maxParticles / maxCells");
+  try {
+    HIPCHECK3(hipSetDevice((int)e->opts.device));
+    HIPCHECK3(hipStreamSynchronize(e->stream));
+    drop_graph3(e);
+    e->S = count;
+    e->N = (uint32_t)totN;
+    e->C = (uint32_t)totC;
+    e->sp = sp;
+    e->active.assign(count, 0);
+    Dev3& D = e->D;
+    D.N = e->N;
+    D.C = e->C;
+    D.S = e->S;
+    std::vector<Scene3D> sd(count);
+    for (auto& d : sd) std::memset(&d, 0, sizeof(d));
+    HIPCHECK3(hipMemcpy(D.sd, sd.data(), sizeof(Scene3D) * count, hipMemcpyHostToDevice));
+    HIPCHECK3(hipMemset(D.cellCount, 0, 4 * (size_t)std::max(e->C, 1u)));
+    e->maxEvents = 0;
+    upload_layout3(e);
+    e->loaded = true;
+    e->slotted = true;
+  } catch (const DevError3& d) {
+    return fail3(e, SN_E_DEVICE, d.msg);
+  }
+  return SN_OK;
+}
+
+int sn3_load_slot(sn3_engine* e, uint32_t slot, const sn3_scene* sc) {
+  if (!e || !sc) return SN_E_ARG;
+  if (!e->loaded || !e->slotted) return fail3(e, SN_E_STATE, "[aether-3d] sn3_load_slot needs a slotted world");
+  if (slot >= e->S) return fail3(e, SN_E_ARG, "[aether-3d] slot index out of range");
+  if (!sc->gx || !sc->gy || !sc->gz) return fail3(e, SN_E_ARG, "[aether-3d] empty grid");
+  if ((uint64_t)sc->gx * H > 0x7fffffffull || (uint64_t)sc->gy * H > 0x7fffffffull ||
+      (uint64_t)sc->gz * H > 0x7fffffffull)
+    return fail3(e, SN_E_ARG, "[aether-3d] world extent exceeds i32 ticks");
+  Scene3P P = e->sp[slot];
+  const uint64_t cells = (uint64_t)sc->gx * sc->gy * sc->gz;
+  if (sc->n > P.cap || cells > P.cellCap)
+    return fail3(e, SN_E_CAPACITY, "[aether-3d] scene does not fit slot " + std::to_string(slot));
+  for (uint32_t i = 0; i < sc->n; ++i)
+    if ((uint32_t)sc->parts16[(size_t)i * 16 + 3] >= NMAT3)
+      return fail3(e, SN_E_ARG, "[aether-3d] material outside the table");
+  try {
+    HIPCHECK3(hipSetDevice((int)e->opts.device));
+    HIPCHECK3(hipStreamSynchronize(e->stream));
+    drop_graph3(e);
+    P.n = sc->n;
+    P.gx = sc->gx;
+    P.gy = sc->gy;
+    P.gz = sc->gz;
+    P.cellTotal = (uint32_t)cells;
+    P.wx = (int32_t)sc->gx * H;
+    P.wy = (int32_t)sc->gy * H;
+    P.wz = (int32_t)sc->gz * H;
+    e->sp[slot] = P;
+    Dev3& D = e->D;
+    std::vector<int4> ps(sc->n), vel(sc->n);
+    std::vector<uint32_t> id(sc->n);
+    for (uint32_t i = 0; i < sc->n; ++i) {
+      const int32_t* r = sc->parts16 + (size_t)i * 16;
+      ps[i] = make_int4(r[0], r[1], r[2], r[3]);
+      vel[i] = make_int4(r[4], r[5], r[6], r[7]);
+      id[i] = i;
+    }
+    if (sc->n) {
+      HIPCHECK3(hipMemcpy(D.ps + P.partBase, ps.data(), 16 * (size_t)sc->n, hipMemcpyHostToDevice));
+      HIPCHECK3(hipMemcpy(D.vel + P.partBase, vel.data(), 16 * (size_t)sc->n, hipMemcpyHostToDevice));
+      HIPCHECK3(hipMemcpy(D.id + P.partBase, id.data(), 4 * (size_t)sc->n, hipMemcpyHostToDevice));
+    }
+    static const uint32_t CHAIN_INIT[8] = {0x243f6a88u, 0x85a308d3u, 0x13198a2eu, 0x03707344u,
+                                           0xa4093822u, 0x299f31d0u, 0x082efa98u, 0xec4e6c89u};
+    Scene3D d;
+    std::memset(&d, 0, sizeof(d));
+    for (int j = 0; j < 8; ++j) d.chain[j] = CHAIN_INIT[j];
+    HIPCHECK3(hipMemcpy(D.sd + slot, &d, sizeof(Scene3D), hipMemcpyHostToDevice));
+    e->active[slot] = 1;
+    upload_layout3(e);
   } catch (const DevError3& d) {
     return fail3(e, SN_E_DEVICE, d.msg);
   }

@@ -54,6 +54,24 @@ class Engine3D:
         self.scenes = list(scenes)
         return self
 
+    def reserve_slots(self, cap_particles: Sequence[int], cap_cells: Sequence[int]):
+        """Replace the world with empty, inactive partitions (continuous batching)."""
+        cp = np.ascontiguousarray(cap_particles, np.uint32)
+        cc = np.ascontiguousarray(cap_cells, np.uint32)
+        assert cp.size == cc.size
+        self._check(native.lib().sn3_reserve_slots(self._h, cp.size, cp.ctypes.data, cc.ctypes.data))
+        self.scenes = [None] * cp.size
+        return self
+
+    def load_slot(self, slot: int, sc: s3.Scene3):
+        """Place `sc` into free partition `slot` and activate it."""
+        s3.validate3(sc)
+        a = np.ascontiguousarray(sc.parts, np.int32)
+        one = native.sn3_scene(sc.n, a.ctypes.data, sc.gx, sc.gy, sc.gz)
+        self._check(native.lib().sn3_load_slot(self._h, int(slot), C.byref(one)))
+        self.scenes[slot] = sc
+        return self
+
     def set_inputs(self, stir_amp=0, stir_phase=0, ball_on=0, ball=(0, 0, 0), ball_reach=0):
         b = (C.c_int32 * 3)(*ball)
         self._check(native.lib().sn3_set_inputs(self._h, stir_amp, stir_phase, ball_on,

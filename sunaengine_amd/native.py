@@ -25,7 +25,7 @@ EXPORTS = [
     "sn3_create", "sn3_destroy", "sn3_last_error", "sn3_load", "sn3_set_inputs", "sn3_step",
     "sn3_sync", "sn3_read_hash", "sn3_read_stream", "sn3_reset_stream", "sn3_read_chain",
     "sn3_read_state", "sn3_debug_counters", "sn3_fnv_digest", "sn3_set_events", "sn3_set_active",
-    "sn3_profile_step",
+    "sn3_profile_step", "sn3_reserve_slots", "sn3_load_slot",
 ]
 
 
@@ -124,6 +124,8 @@ def lib():
         "sn3_fnv_digest": (C.c_int, [P, u32, P]),
         "sn3_set_events": (C.c_int, [P, P, u32]),
         "sn3_set_active": (C.c_int, [P, P, u32]),
+        "sn3_reserve_slots": (C.c_int, [P, u32, P, P]),
+        "sn3_load_slot": (C.c_int, [P, u32, P]),
         "sn3_profile_step": (C.c_int, [P, u32, P]),
     }
     for name, (res, args) in sig.items():

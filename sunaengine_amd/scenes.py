@@ -382,6 +382,34 @@ def thermal_1m(seed: int = 0x5EED01, temp_seed: int = 0x7E3317, side: int = 1024
     return s
 
 
+def fluid_block_scene(nf: int, seed: int, x_pick: int = 0, name: str = "block") -> Scene:
+    """Exactly `nf` water particles as a 0.5 wu block about 2:1 wide, standing
+    on the floor of a walled 256 x 128 wu box (scaled up by a power of two
+    until the block fits), at x position `x_pick` (in half-wu steps, taken
+    modulo the free width), jittered by its own seed."""
+    bw = int(math.ceil(math.sqrt(2 * nf)))
+    bh = -(-nf // bw)
+    k = 1
+    while 256 * k < bw * 0.5 + 16 or 128 * k < bh * 0.5 + 24:
+        k *= 2
+    dom_w, dom_h = 256 * k, 128 * k
+    width_wu = (bw * (ONE >> 1)) / ONE
+    max_x0 = int((dom_w - 6 - width_wu) * 2)       # half-wu steps
+    x0 = (3 * ONE) + (x_pick % max(1, max_x0)) * (ONE >> 1)
+    s = walled_block_scene(bw, bh, dom_w, dom_h, seed, x0=x0, name=name)
+    # trim the partial last row down to exactly nf fluid particles
+    drop = bw * bh - nf
+    if drop:
+        keep = np.ones(s.n, bool)
+        keep[nf:bw * bh] = False
+        idx = np.nonzero(keep)[0]
+        s = Scene(n=s.n - drop, nFluid=nf, pos=s.pos.reshape(-1, 2)[idx].reshape(-1).copy(),
+                  vel=s.vel.reshape(-1, 2)[idx].reshape(-1).copy(), matId=s.matId[idx].copy(),
+                  domW=s.domW, domH=s.domH, name=s.name, meta=s.meta)
+    s.meta.update(block=(bw, bh), x0=int(x0), box=(dom_w, dom_h))
+    return s
+
+
 def batch_scenes(count: int = 512, seed: int = 0xBA7C4, lo: int = 1024, hi: int = 65536):
     """`count` independent scenes in the 256 x 128 wu walled box, fluid counts
     log-uniform in [lo, hi] (rounded), each a rectangular 0.5 wu block (about

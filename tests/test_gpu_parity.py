@@ -179,3 +179,20 @@ def test_slotted_world_continuous_batching():
         assert got.shape == ref.shape
         assert first_mismatch(got, ref) is None, f"scene {k}"
     e.destroy()
+
+
+@pytest.mark.parametrize("wide", [False, True], ids=["bounded", "unbounded"])
+def test_full_scale_1m_first_substeps(wide):
+    """The 1,048,576-particle scene itself (the measured workload), first
+    substeps against the oracle on every substep; `unbounded` widens the
+    domain past 2^28 ticks so the saturating pair geometry runs at scale."""
+    from sunaengine_amd import scenes
+    from oracle.oracle import set_threads
+    set_threads(16)
+    s = scenes.thermal_1m()
+    if wide:
+        s.domW = 1 << 29
+        s.name += "_wide"
+    _print(f"{s.name}: n={s.n}, 4 substeps")
+    e, o = _stream_parity(s, 4, 4)
+    e.destroy()

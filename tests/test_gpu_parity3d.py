@@ -61,3 +61,31 @@ def test_layers_with_stir_and_ball():
     assert first_mismatch(got, ref) is None
     assert np.array_equal(e.read_state()[:, :8], o.read()[:, :8])
     e.destroy()
+
+
+def test_slotted_3d_world():
+    """3D scenes placed into and retired from partitions of a running world
+    each equal their own oracle run."""
+    from oracle import OracleWorld3
+    from sunaengine_amd import scenes3d, tables3d
+    from sunaengine_amd.engine3d import Engine3D
+    a = scenes3d.block_scene(side=10, gx=12, gy=12, gz=12, seed=5)
+    b = scenes3d.column_scene(2000)
+    c = scenes3d.block_scene(side=12, gx=12, gy=14, gz=12, seed=8)
+    caps_n = [3000, 2500]
+    caps_c = [max(a.cell_total, c.cell_total), b.cell_total]
+    e = Engine3D(max_particles=sum(caps_n), max_cells=sum(caps_c), iters=2, max_scenes=2)
+    e.reserve_slots(caps_n, caps_c)
+    e.load_slot(0, a); e.step(12)
+    e.load_slot(1, b); e.step(10)
+    got_a = e.hash_stream(0)                  # 22 substeps
+    e.set_active([0, 1]); e.step(5)
+    e.load_slot(0, c); e.step(9)
+    got_b, got_c = e.hash_stream(1), e.hash_stream(0)   # 24 and 9 substeps
+    for sc, steps, got in ((a, 22, got_a), (b, 24, got_b), (c, 9, got_c)):
+        o = OracleWorld3(sc, tables3d.luts3d(), 2)
+        ref = o.step(steps, stream=True)
+        print(f"3d slotted n={sc.n} x{steps}", flush=True)
+        assert got.shape == ref.shape
+        assert first_mismatch(got, ref) is None
+    e.destroy()
