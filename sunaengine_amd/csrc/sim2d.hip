@@ -511,30 +511,44 @@ __device__ __forceinline__ ListRange xcd_range(uint32_t n, uint32_t unitsPerBloc
 // One chunk of one tile: the sweep described above.  BND selects the bounded
 // pair geometry; WIN is false for one-cell tiles, whose candidate union is
 // exactly every particle's own 3x3 stencil, so no per-lane column window
-// applies.
-template <bool BND, bool WIN>
+// applies; PPL is the particles per lane (1 when the chunk fits one wave).
+// In a bounded scene a candidate is accepted only with |dx|, |dy| <= REACH
+// (lut_idx rejects (|d| + 8) >> 4 > HS), so a candidate outside the chunk's
+// bounding box grown by REACH is rejected by every lane and has no other
+// effect; each block of 64 candidates is reduced to a ballot of the ones
+// inside that box and the sweep visits only those, still in ascending order.
+constexpr int32_t REACH = (HS + 1) * 16 - 9;   // largest |d| with (|d| + 8) >> 4 <= HS
+
+__device__ __forceinline__ int32_t wave_min(int32_t v) {
+#pragma unroll
+  for (int off = 32; off >= 1; off >>= 1) v = mini(v, __shfl_xor(v, off));
+  return v;
+}
+__device__ __forceinline__ int32_t wave_max(int32_t v) {
+#pragma unroll
+  for (int off = 32; off >= 1; off >>= 1) v = maxi(v, __shfl_xor(v, off));
+  return v;
+}
+
+template <bool BND, bool WIN, int PPL>
 __device__ __forceinline__ void nbr_chunk(const Dev& D, const int2* lutWG, uint32_t c0, uint32_t ncell, uint32_t s,
-                                          uint32_t chunk, uint32_t lane, int32_t W0, Dbg& g) {
+                                          uint32_t plo, uint32_t phi, uint32_t lane, int32_t W0, Dbg& g) {
   const SceneP& P = D.sp[s];
   const int32_t sh = P.posShift;
   const uint32_t gridW = P.gridW, gridH = P.gridH, cellBase = P.cellBase;
   const uint32_t lc0 = c0 - cellBase;
   const int32_t cy = (int32_t)(lc0 / gridW), cx0 = (int32_t)(lc0 % gridW);
   const int32_t xlo = maxi(cx0 - 1, 0), xhi = mini(cx0 + (int32_t)ncell, (int32_t)gridW - 1);
-  const uint32_t tlo = (uint32_t)((int32_t)D.cellStart[c0] + sh);
-  const uint32_t thi = (uint32_t)((int32_t)D.cellStart[c0 + ncell] + sh);
-  const uint32_t plo = tlo + chunk * TILE_CHUNK;
-  const uint32_t phi = (uint32_t)mini((int32_t)thi, (int32_t)(plo + TILE_CHUNK));
   const uint32_t myRow = cellBase + (uint32_t)cy * gridW;
-  static_assert(TILE_CHUNK == 64u * NBR_PPL, "one sweep per tile chunk");
-  for (uint32_t pbase = plo; pbase < phi; pbase += 64u * NBR_PPL) {
-    uint32_t pk[NBR_PPL], inf[NBR_PPL], cnt[NBR_PPL], bits[NBR_PPL], dropped[NBR_PPL];
-    int2 pik[NBR_PPL];
-    int32_t cxk[NBR_PPL];
-    bool live[NBR_PPL];
-    SolveAAcc acc[NBR_PPL];
+  for (uint32_t pbase = plo; pbase < phi; pbase += 64u * PPL) {
+    uint32_t pk[PPL], inf[PPL], cnt[PPL], bits[PPL], dropped[PPL];
+    int2 pik[PPL];
+    int32_t cxk[PPL];
+    bool live[PPL];
+    SolveAAcc acc[PPL];
+    int32_t mnx = I32_MAX, mny = I32_MAX, mxx = I32_MIN, mxy = I32_MIN;
 #pragma unroll
-    for (int k = 0; k < NBR_PPL; ++k) {
+    for (int k = 0; k < PPL; ++k) {
       pk[k] = pbase + (uint32_t)k * 64u + lane;
       const bool valid = pk[k] < phi;
       inf[k] = valid ? D.info[pk[k]] : 0u;
@@ -545,6 +559,20 @@ __device__ __forceinline__ void nbr_chunk(const Dev& D, const int2* lutWG, uint3
       bits[k] = 0;
       dropped[k] = 0;
       acc[k] = SolveAAcc{W0, 0, 0, 0};
+      if (live[k]) {
+        mnx = mini(mnx, pik[k].x);
+        mxx = maxi(mxx, pik[k].x);
+        mny = mini(mny, pik[k].y);
+        mxy = maxi(mxy, pik[k].y);
+      }
+    }
+    // the chunk's reach box (bounded scenes: no overflow within +-2^28 + REACH)
+    int32_t bx0 = 0, bx1 = 0, by0 = 0, by1 = 0;
+    if constexpr (BND) {
+      bx0 = wave_min(mnx) - REACH;
+      bx1 = wave_max(mxx) + REACH;
+      by0 = wave_min(mny) - REACH;
+      by1 = wave_max(mxy) + REACH;
     }
     for (int32_t dy = -1; dy <= 1; ++dy) {
       const int32_t gy = cy + dy;
@@ -566,15 +594,19 @@ __device__ __forceinline__ void nbr_chunk(const Dev& D, const int2* lutWG, uint3
         // wait for this block's candidates here, so that the candidate
         // loop below never waits on the neighbour-list stores it issues
         __builtin_amdgcn_s_waitcnt(0x0F70);  // vmcnt(0)
-        for (uint32_t j = 0; j < qn; ++j) {
+        bool want = lane < qn && !(ci & INF_DEAD);   // nobody sees a corpse
+        if constexpr (BND) want = want && cp.x >= bx0 && cp.x <= bx1 && cp.y >= by0 && cp.y <= by1;
+        uint64_t todo = __ballot(want);
+        while (todo) {
+          const uint32_t j = (uint32_t)__builtin_ctzll(todo);
+          todo &= todo - 1ull;
           const int32_t px = __builtin_amdgcn_readlane(cp.x, j);
           const int32_t py = __builtin_amdgcn_readlane(cp.y, j);
           const uint32_t qi = (uint32_t)__builtin_amdgcn_readlane((int)ci, j);
-          const int32_t qc = __builtin_amdgcn_readlane(cc, j);
+          const int32_t qc = WIN ? __builtin_amdgcn_readlane(cc, j) : 0;
           const uint32_t q = q0 + j;
-          if (qi & INF_DEAD) continue;                     // nobody sees a corpse
 #pragma unroll
-          for (int k = 0; k < NBR_PPL; ++k) {
+          for (int k = 0; k < PPL; ++k) {
             if (!live[k] || q == pk[k]) continue;
             if (WIN && (qc < cxk[k] - 1 || qc > cxk[k] + 1)) continue;
             const int2 d = pdelta<BND>(pik[k], make_int2(px, py), g);
@@ -596,7 +628,7 @@ __device__ __forceinline__ void nbr_chunk(const Dev& D, const int2* lutWG, uint3
       }
     }
 #pragma unroll
-    for (int k = 0; k < NBR_PPL; ++k) {
+    for (int k = 0; k < PPL; ++k) {
       if (pk[k] >= phi) continue;
       D.nbrN[pk[k]] = live[k] ? (cnt[k] | ((bits[k] & INF_SEAL) ? NBR_SEAL_BIT : 0u) |
                                  ((bits[k] & INF_CONGEAL) ? NBR_CONGEAL_BIT : 0u))
@@ -627,12 +659,23 @@ __global__ __launch_bounds__(256) void k_nbr_tile(Dev D) {
     const uint32_t yy = __builtin_amdgcn_readfirstlane(tile.y);
     const uint32_t ncell = yy & 0xffu, s = yy >> 8;
     const bool bnd = D.sp[s].bounded != 0u;
+    const int32_t sh = D.sp[s].posShift;
+    const uint32_t tlo = (uint32_t)((int32_t)D.cellStart[c0] + sh);
+    const uint32_t thi = (uint32_t)((int32_t)D.cellStart[c0 + ncell] + sh);
+    const uint32_t plo = tlo + chunk * TILE_CHUNK;
+    const uint32_t phi = (uint32_t)mini((int32_t)thi, (int32_t)(plo + TILE_CHUNK));
+    const bool one = (uint32_t)__builtin_amdgcn_readfirstlane((int)(phi - plo)) <= 64u;
     Dbg g;
     if (bnd) {
-      if (ncell == 1u) nbr_chunk<true, false>(D, lutWG, c0, ncell, s, chunk, lane, W0, g);
-      else nbr_chunk<true, true>(D, lutWG, c0, ncell, s, chunk, lane, W0, g);
+      if (ncell == 1u) {
+        if (one) nbr_chunk<true, false, 1>(D, lutWG, c0, ncell, s, plo, phi, lane, W0, g);
+        else nbr_chunk<true, false, NBR_PPL>(D, lutWG, c0, ncell, s, plo, phi, lane, W0, g);
+      } else {
+        if (one) nbr_chunk<true, true, 1>(D, lutWG, c0, ncell, s, plo, phi, lane, W0, g);
+        else nbr_chunk<true, true, NBR_PPL>(D, lutWG, c0, ncell, s, plo, phi, lane, W0, g);
+      }
     } else {
-      nbr_chunk<false, true>(D, lutWG, c0, ncell, s, chunk, lane, W0, g);
+      nbr_chunk<false, true, NBR_PPL>(D, lutWG, c0, ncell, s, plo, phi, lane, W0, g);
     }
     flush_dbg(D, s, g, 0);
   }
