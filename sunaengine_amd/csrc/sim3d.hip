@@ -50,6 +50,10 @@ constexpr int32_t S_TERM = 8388608;
 constexpr int32_t MARGIN = 32768;
 constexpr uint32_t INACTIVE = 0xffffffffu;
 constexpr uint32_t LUT3_WORDS = OFF_MAT + NMAT3 * MAT_WORDS;
+constexpr uint32_t LUT_N3 = 2049;          // W and G bins
+constexpr uint32_t MIXED = 0xfffffffeu;    // a wave/block spanning scenes
+constexpr int HPART3 = 9;                  // 8 hash words + truncation count
+constexpr uint32_t CHUNK3 = 64;            // particles per neighbour-sweep item
 
 struct DevError3 {
   std::string msg;
@@ -108,6 +112,11 @@ struct Dev3 {
   int4* aux;     // last dp.xyz, rho (for read-back of the full record)
   uint32_t* nbr;
   uint32_t* nbrN;
+  uint32_t* drop;            // candidates dropped by truncation, per particle
+  uint32_t* hpart;           // per-block partial sums [block][HPART3]
+  const uint32_t* cellScene; // global cell -> scene, INACTIVE if its scene is inactive
+  uint2* dlist;              // this substep's occupied cell chunks {cell, chunk}
+  uint32_t* listCount;       // [1] entries of dlist (cleared by predict)
   const int32_t* luts;
   const Scene3P* sp;
   Scene3D* sd;
@@ -169,22 +178,35 @@ __device__ __forceinline__ bool ball_term(const Dev3& D, int3 pi, int32_t& w, in
   return true;
 }
 
+// Wrapping u32 sum over the 64 lanes (every lane must be active).
+__device__ __forceinline__ uint32_t wave_sum3(uint32_t v) {
+#pragma unroll
+  for (int off = 32; off >= 1; off >>= 1) v += (uint32_t)__shfl_xor((int)v, off);
+  return v;
+}
+// Debug counters: summed per wave, one atomic per nonzero counter when the
+// wave is uniform in scene.  Must be called with every lane active.
 __device__ __forceinline__ void flush_dbg3(const Dev3& D, uint32_t s, const Dbg& g, uint32_t trunc) {
   uint32_t v[5] = {g.mul, g.div, g.div0, g.add, trunc};
   const uint32_t slot[5] = {0, 1, 2, 5, 6};
+  const uint32_t s0 = (uint32_t)__builtin_amdgcn_readfirstlane((int)s);
+  const bool uniform = __all(s == s0);
 #pragma unroll
-  for (int k = 0; k < 5; ++k)
-    if (__any(v[k] != 0) && s != INACTIVE && v[k]) atomicAdd(&D.sd[s].dbg[slot[k]], v[k]);
+  for (int k = 0; k < 5; ++k) {
+    uint32_t x = v[k];
+    if (!__any(x != 0)) continue;
+    if (uniform) {
+      x = wave_sum3(x);
+      if ((threadIdx.x & 63u) == 0u && s0 != INACTIVE) atomicAdd(&D.sd[s0].dbg[slot[k]], x);
+    } else if (s != INACTIVE && x) {
+      atomicAdd(&D.sd[s].dbg[slot[k]], x);
+    }
+  }
 }
 
 // ------------------------------------------------------------------ predict
-__global__ __launch_bounds__(256) void k3_predict_count(Dev3 D) {
-  uint32_t p = blockIdx.x * 256u + threadIdx.x;
-  if (p >= D.N) return;
-  uint32_t s = D.sidOf[p];
-  if (s == INACTIVE) return;
+__device__ __forceinline__ void predict3(const Dev3& D, uint32_t p, uint32_t s, Dbg& g) {
   const Scene3P& P = D.sp[s];
-  Dbg g;
   int4 a = D.ps[p];
   int4 v4 = D.vel[p];
   uint32_t mi = (uint32_t)a.w;
@@ -202,6 +224,13 @@ __global__ __launch_bounds__(256) void k3_predict_count(Dev3 D) {
   D.pr0[p] = make_int4(pr.x, pr.y, pr.z, 0);
   D.cell0[p] = c;
   D.rank0[p] = atomicAdd(&D.cellCount[c], 1u);
+}
+__global__ __launch_bounds__(256) void k3_predict_count(Dev3 D) {
+  uint32_t p = blockIdx.x * 256u + threadIdx.x;
+  if (blockIdx.x == 0 && threadIdx.x == 0) D.listCount[0] = 0u;  // rebuilt after the scan
+  uint32_t s = p < D.N ? D.sidOf[p] : INACTIVE;
+  Dbg g;
+  if (s != INACTIVE) predict3(D, p, s, g);
   flush_dbg3(D, s, g, 0);
 }
 
@@ -277,48 +306,168 @@ __device__ __forceinline__ void dens_finish(const Dev3& D, const Scene3P& P, uin
   D.aux[p].w = A.rho;
 }
 
-// buildNbr (3d/sim3d.wgsl:341-390) fused with iteration 0's density
-__global__ __launch_bounds__(256) void k3_nbr_density(Dev3 D) {
-  uint32_t p = blockIdx.x * 256u + threadIdx.x;
-  uint32_t s = p < D.N ? D.sidOf[p] : INACTIVE;
-  Dbg g;
-  uint32_t dropped = 0;
-  if (s != INACTIVE) {
+// ============================================================ per-substep work
+// After the scan: one item {cell, chunk} per CHUNK3 particles of every
+// occupied cell of an active scene, appended with wave-aggregated atomics.
+// Items are independent, so list order (left open by the atomics) affects
+// no result; chunking bounds each item's work however densely a cell packs.
+__device__ __forceinline__ void append_chunks3(uint32_t n, uint32_t* counter, uint2* list, uint32_t key) {
+  const uint32_t lane = threadIdx.x & 63u;
+  uint32_t incl = n;
+#pragma unroll
+  for (uint32_t off = 1; off < 64u; off <<= 1) {
+    const uint32_t v = (uint32_t)__shfl_up((int)incl, off);
+    if (lane >= off) incl += v;
+  }
+  const uint32_t total = (uint32_t)__shfl((int)incl, 63);
+  if (total == 0u) return;
+  uint32_t base = 0;
+  if (lane == 0u) base = atomicAdd(counter, total);
+  base = (uint32_t)__shfl((int)base, 0) + incl - n;
+  for (uint32_t k = 0; k < n; ++k) list[base + k] = make_uint2(key, k);
+}
+
+// Called with every lane of every wave active.
+__global__ __launch_bounds__(256) void k3_build_lists(Dev3 D) {
+  const uint32_t t = blockIdx.x * 256u + threadIdx.x;
+  uint32_t nch = 0;
+  if (t < D.C && D.cellScene[t] != INACTIVE) nch = (D.cellStart[t + 1] - D.cellStart[t] + CHUNK3 - 1u) / CHUNK3;
+  append_chunks3(nch, &D.listCount[0], D.dlist, t);
+}
+
+// XCD x (= blockIdx.x % 8) walks the x-th eighth of a device list.
+struct ListRange3 {
+  uint32_t lo, hi, first, stride;
+};
+__device__ __forceinline__ ListRange3 xcd_range3(uint32_t n, uint32_t unitsPerBlock, uint32_t unit) {
+  const uint32_t xcd = blockIdx.x & 7u, nb = gridDim.x >> 3, lb = blockIdx.x >> 3;
+  ListRange3 r;
+  r.lo = (uint32_t)(((uint64_t)n * xcd) >> 3);
+  r.hi = (uint32_t)(((uint64_t)n * (xcd + 1u)) >> 3);
+  r.first = r.lo + lb * unitsPerBlock + unit;
+  r.stride = nb * unitsPerBlock;
+  return r;
+}
+__device__ __forceinline__ int32_t wave_min3(int32_t v) {
+#pragma unroll
+  for (int off = 32; off >= 1; off >>= 1) v = mini(v, __shfl_xor(v, off));
+  return v;
+}
+__device__ __forceinline__ int32_t wave_max3(int32_t v) {
+#pragma unroll
+  for (int off = 32; off >= 1; off >>= 1) v = maxi(v, __shfl_xor(v, off));
+  return v;
+}
+
+// buildNbr (3d/sim3d.wgsl:341-390) fused with iteration 0's density.  One
+// wave per chunk of up to 64 particles of one cell, one particle per lane.
+// The chunk's candidates are the cell's 27-cell stencil: nine (dz, dy) rows,
+// each a contiguous sorted range over x in cx-1..cx+1, visited in exactly the
+// canonical order (dz, dy, dx, ascending id).  They are loaded 64 at a time,
+// one per lane, and broadcast with v_readlane.  A candidate is accepted only
+// with |d| < H on every axis; one outside the chunk's bounding box grown by
+// H - 1 is therefore rejected by every lane (positions lie inside the world,
+// so no difference wraps), and each block is reduced to a ballot of the
+// candidates inside that box.  The kernel tables (W, G) sit in LDS.
+__global__ __launch_bounds__(256) void k3_nbr_tile(Dev3 D) {
+  __shared__ int32_t lutW[LUT_N3], lutG[LUT_N3];
+  __shared__ int32_t rhoW[NMAT3];
+  for (uint32_t i = threadIdx.x; i < LUT_N3; i += 256u) {
+    lutW[i] = D.luts[OFF_W + i];
+    lutG[i] = D.luts[OFF_G + i];
+  }
+  if (threadIdx.x < NMAT3) rhoW[threadIdx.x] = mat_rhoW(D, threadIdx.x);
+  __syncthreads();
+  const uint32_t lane = threadIdx.x & 63u;
+  const uint32_t nwork = (uint32_t)__builtin_amdgcn_readfirstlane((int)D.listCount[0]);
+  const ListRange3 R = xcd_range3(nwork, 4u, threadIdx.x >> 6);
+  for (uint32_t w = R.first; w < R.hi; w += R.stride) {
+    const uint2 item = D.dlist[w];
+    const uint32_t c = __builtin_amdgcn_readfirstlane(item.x);
+    const uint32_t chunk = __builtin_amdgcn_readfirstlane(item.y);
+    const uint32_t s = __builtin_amdgcn_readfirstlane(D.cellScene[c]);
     const Scene3P& P = D.sp[s];
-    int4 pr4 = D.prA[p];
-    int3 pi = make_int3(pr4.x, pr4.y, pr4.z);
-    uint32_t lc = D.cellY[p] - P.cellBase;
-    int32_t cx = (int32_t)(lc % P.gx), cy = (int32_t)((lc / P.gx) % P.gy), cz = (int32_t)(lc / (P.gx * P.gy));
-    int32_t xlo = maxi(cx - 1, 0), xhi = mini(cx + 1, (int32_t)P.gx - 1);
-    DensAcc A{wmul(D.luts[OFF_W], wadd(256, mat_rhoW(D, D.matY[p]))) >> 8, 0, 0, 0, 0};
-    uint32_t cnt = 0;
-    uint32_t* out = D.nbr + p;
+    const int32_t sh = P.posShift;
+    const uint32_t plo = (uint32_t)((int32_t)D.cellStart[c] + sh) + chunk * CHUNK3;
+    const uint32_t phi = (uint32_t)mini((int32_t)((int32_t)D.cellStart[c + 1] + sh), (int32_t)(plo + CHUNK3));
+    const uint32_t lc = c - P.cellBase;
+    const int32_t cx = (int32_t)(lc % P.gx), cy = (int32_t)((lc / P.gx) % P.gy), cz = (int32_t)(lc / (P.gx * P.gy));
+    const int32_t xlo = maxi(cx - 1, 0), xhi = mini(cx + 1, (int32_t)P.gx - 1);
+    const uint32_t p = plo + lane;
+    const bool valid = p < phi;
+    Dbg g;
+    int3 pi = make_int3(0, 0, 0);
+    uint32_t mi = 0;
+    if (valid) {
+      const int4 pr4 = D.prA[p];
+      pi = make_int3(pr4.x, pr4.y, pr4.z);
+      mi = D.matY[p];
+    }
+    DensAcc A{wmul(lutW[0], wadd(256, rhoW[mi])) >> 8, 0, 0, 0, 0};
+    uint32_t cnt = 0, dropped = 0;
+    const int32_t bx0 = wave_min3(valid ? pi.x : I32_MAX) - (H - 1), bx1 = wave_max3(valid ? pi.x : I32_MIN) + (H - 1);
+    const int32_t by0 = wave_min3(valid ? pi.y : I32_MAX) - (H - 1), by1 = wave_max3(valid ? pi.y : I32_MIN) + (H - 1);
+    const int32_t bz0 = wave_min3(valid ? pi.z : I32_MAX) - (H - 1), bz1 = wave_max3(valid ? pi.z : I32_MIN) + (H - 1);
     for (int32_t dz = -1; dz <= 1; ++dz) {
-      int32_t gz = cz + dz;
+      const int32_t gz = cz + dz;
       if (gz < 0 || gz >= (int32_t)P.gz) continue;
       for (int32_t dy = -1; dy <= 1; ++dy) {
-        int32_t gy = cy + dy;
+        const int32_t gy = cy + dy;
         if (gy < 0 || gy >= (int32_t)P.gy) continue;
-        uint32_t rowc = P.cellBase + ((uint32_t)gz * P.gy + (uint32_t)gy) * P.gx;
-        uint32_t lo = (uint32_t)((int32_t)D.cellStart[rowc + (uint32_t)xlo] + P.posShift);
-        uint32_t hi = (uint32_t)((int32_t)D.cellStart[rowc + (uint32_t)xhi + 1u] + P.posShift);
-        for (uint32_t q = lo; q < hi; ++q) {
-          if (q == p) continue;
-          int4 pq = D.prA[q];
-          int3 d = make_int3(wsub(pq.x, pi.x), wsub(pq.y, pi.y), wsub(pq.z, pi.z));
-          if (fp_abs(d.x, g) >= H || fp_abs(d.y, g) >= H || fp_abs(d.z, g) >= H) continue;
-          int32_t r2q = r2q_of(d);
-          if (r2q >= H2Q) continue;
-          if (cnt >= MAXNBR) { dropped++; continue; }
-          out[(size_t)(cnt++) * D.NS] = q;
-          dens_term(D, A, d, (uint32_t)r2q >> 7, D.matY[q], g);
+        const uint32_t rowc = P.cellBase + ((uint32_t)gz * P.gy + (uint32_t)gy) * P.gx;
+        const uint32_t lo = (uint32_t)((int32_t)D.cellStart[rowc + (uint32_t)xlo] + sh);
+        const uint32_t hi = (uint32_t)((int32_t)D.cellStart[rowc + (uint32_t)xhi + 1u] + sh);
+        for (uint32_t q0 = lo; q0 < hi; q0 += 64u) {
+          const uint32_t qn = (uint32_t)mini((int32_t)(hi - q0), 64);
+          int4 cp = make_int4(0, 0, 0, 0);
+          uint32_t cm = 0;
+          if (lane < qn) {
+            cp = D.prA[q0 + lane];
+            cm = D.matY[q0 + lane];
+          }
+          // wait for this block's candidates here, so that the candidate
+          // loop below never waits on the neighbour-list stores it issues
+          __builtin_amdgcn_s_waitcnt(0x0F70);  // vmcnt(0)
+          const bool want = lane < qn && cp.x >= bx0 && cp.x <= bx1 && cp.y >= by0 && cp.y <= by1 &&
+                            cp.z >= bz0 && cp.z <= bz1;
+          uint64_t todo = __ballot(want);
+          while (todo) {
+            const uint32_t j = (uint32_t)__builtin_ctzll(todo);
+            todo &= todo - 1ull;
+            const int32_t px = __builtin_amdgcn_readlane(cp.x, j);
+            const int32_t py = __builtin_amdgcn_readlane(cp.y, j);
+            const int32_t pz = __builtin_amdgcn_readlane(cp.z, j);
+            const uint32_t mj = (uint32_t)__builtin_amdgcn_readlane((int)cm, j);
+            const uint32_t q = q0 + j;
+            if (!valid || q == p) continue;
+            const int3 d = make_int3(wsub(px, pi.x), wsub(py, pi.y), wsub(pz, pi.z));
+            if (fp_abs(d.x, g) >= H || fp_abs(d.y, g) >= H || fp_abs(d.z, g) >= H) continue;
+            const int32_t r2q = r2q_of(d);
+            if (r2q >= H2Q) continue;
+            if (cnt >= MAXNBR) {
+              dropped++;
+              continue;
+            }
+            D.nbr[(size_t)(cnt++) * D.NS + p] = q;
+            const uint32_t ui = (uint32_t)r2q >> 7;
+            A.rho = wadd(A.rho, wmul(lutW[ui], wadd(256, rhoW[mj])) >> 8);
+            const int32_t gt = lutG[ui];
+            const int32_t gx = fp_mul_s(gt, d.x, 16, g), gy2 = fp_mul_s(gt, d.y, 16, g), gz2 = fp_mul_s(gt, d.z, 16, g);
+            A.sx = wadd(A.sx, gx);
+            A.sy = wadd(A.sy, gy2);
+            A.sz = wadd(A.sz, gz2);
+            A.S = wadd(wadd(wadd(A.S, s_term(gx, g)), s_term(gy2, g)), s_term(gz2, g));
+          }
         }
       }
     }
-    D.nbrN[p] = cnt;
-    dens_finish(D, P, p, pi, A, D.prA, g);
+    if (valid) {
+      D.nbrN[p] = cnt;
+      D.drop[p] = dropped;
+      dens_finish(D, P, p, pi, A, D.prA, g);
+    }
+    flush_dbg3(D, s, g, 0);
   }
-  flush_dbg3(D, s, g, dropped);
 }
 
 __global__ __launch_bounds__(256) void k3_density(Dev3 D, int4* pr) {
@@ -415,29 +564,62 @@ __global__ __launch_bounds__(256) void k3_solve_apply(Dev3 D, const int4* pr, in
   }
   flush_dbg3(D, s, g, 0);
   if (!last) return;
-  uint32_t s0 = __shfl(s, 0);
-  bool uniform = __all(s == s0);
-  if (uniform) {
-    if (s0 == INACTIVE) return;
+  // Block partial (8 hash words + truncation count): a block lying wholly in
+  // one scene stores its sums in hpart[block] for k3_hash_finish; a block
+  // straddling scenes adds per lane with atomics.  All wrapping u32 sums.
+  uint32_t h9[HPART3];
 #pragma unroll
-    for (int l = 0; l < 8; ++l) {
-      uint32_t v = hv[l];
-      for (int off = 32; off >= 1; off >>= 1) v += __shfl_xor(v, off);
-      hv[l] = v;
+  for (int l = 0; l < 8; ++l) h9[l] = hv[l];
+  h9[8] = s != INACTIVE ? D.drop[p] : 0u;
+  __shared__ uint32_t wpart[4][HPART3];
+  __shared__ uint32_t wscene[4];
+  const uint32_t wv = threadIdx.x >> 6, lane = threadIdx.x & 63u;
+  const uint32_t s0 = (uint32_t)__builtin_amdgcn_readfirstlane((int)s);
+  const bool wuni = __all(s == s0);
+#pragma unroll
+  for (int l = 0; l < HPART3; ++l) {
+    const uint32_t v = wave_sum3(h9[l]);
+    if (lane == 0u) wpart[wv][l] = v;
+  }
+  if (lane == 0u) wscene[wv] = wuni ? s0 : MIXED;
+  __syncthreads();
+  const uint32_t bs = wscene[0];
+  const bool buni = bs != MIXED && wscene[1] == bs && wscene[2] == bs && wscene[3] == bs;
+  if (buni) {
+    if (bs != INACTIVE && threadIdx.x < (uint32_t)HPART3) {
+      const uint32_t t = threadIdx.x;
+      D.hpart[(size_t)blockIdx.x * HPART3 + t] = wpart[0][t] + wpart[1][t] + wpart[2][t] + wpart[3][t];
     }
-    if ((threadIdx.x & 63u) == 0u)
-      for (int l = 0; l < 8; ++l) atomicAdd(&D.sd[s0].acc[l], hv[l]);
   } else if (s != INACTIVE) {
-    for (int l = 0; l < 8; ++l) atomicAdd(&D.sd[s].acc[l], hv[l]);
+#pragma unroll
+    for (int l = 0; l < 8; ++l) atomicAdd(&D.sd[s].acc[l], h9[l]);
+    if (h9[8]) atomicAdd(&D.sd[s].dbg[6], h9[8]);
   }
 }
 
-__global__ void k3_hash_finish(Dev3 D, const uint8_t* active) {
-  uint32_t s = blockIdx.x * 64u + threadIdx.x;
-  if (s >= D.S || !active[s]) return;
+// One wave per scene: fold the block partials of the blocks lying wholly in
+// the scene's slot range into the atomically summed remainder, then the
+// chain step, the H ring append and the substep counter.
+__global__ __launch_bounds__(64) void k3_hash_finish(Dev3 D, const uint8_t* active) {
+  const uint32_t s = blockIdx.x;
+  if (!active[s]) return;
   Scene3D& SD = D.sd[s];
+  const Scene3P& P = D.sp[s];
+  const uint32_t lane = threadIdx.x;
+  const uint32_t b0 = (P.partBase + 255u) / 256u, b1 = (P.partBase + P.n) / 256u;
+  uint32_t sum[HPART3];
+#pragma unroll
+  for (int l = 0; l < HPART3; ++l) sum[l] = 0u;
+  for (uint32_t b = b0 + lane; b < b1; b += 64u) {
+#pragma unroll
+    for (int l = 0; l < HPART3; ++l) sum[l] += D.hpart[(size_t)b * HPART3 + l];
+  }
+#pragma unroll
+  for (int l = 0; l < HPART3; ++l) sum[l] = wave_sum3(sum[l]);
+  if (lane != 0u) return;
   uint32_t a[8];
-  for (int l = 0; l < 8; ++l) a[l] = SD.acc[l];
+  for (int l = 0; l < 8; ++l) a[l] = SD.acc[l] + sum[l];
+  SD.dbg[6] += sum[8];
   for (int j = 0; j < 8; ++j) SD.chain[j] = mm_step(SD.chain[j], a[j] ^ rotl32(a[(j + 1) & 7], 7));
   uint32_t* r = D.hring + ((size_t)s * D.ringCap + (SD.streamPos % D.ringCap)) * 8;
   for (int l = 0; l < 8; ++l) { r[l] = a[l]; SD.acc[l] = 0u; }
@@ -576,6 +758,7 @@ struct sn3_engine {
   uint32_t graphSubsteps = 0;
   bool loaded = false;
   bool slotted = false;
+  uint32_t nbrBlocks = 8;    // persistent grid of the neighbour sweep
   uint32_t maxEvents = 0;
   Event3* dEvents = nullptr;
   uint32_t eventCap = 0;
@@ -624,7 +807,8 @@ void enqueue_substep3(sn3_engine* e, hipEvent_t* marks) {
   hipLaunchKernelGGL(k3_scatter, dim3(gN), dim3(256), 0, st, D);
   hipLaunchKernelGGL(k3_canon_gather, dim3(gN), dim3(256), 0, st, D);
   if (marks) hipEventRecord(marks[1], st);
-  hipLaunchKernelGGL(k3_nbr_density, dim3(gN), dim3(256), 0, st, D);
+  hipLaunchKernelGGL(k3_build_lists, dim3(div_up3(std::max(D.C, 1u), 256)), dim3(256), 0, st, D);
+  hipLaunchKernelGGL(k3_nbr_tile, dim3(e->nbrBlocks), dim3(256), 0, st, D);
   if (marks) hipEventRecord(marks[2], st);
   int4* cur = D.prA;
   int4* nxt = D.prB;
@@ -636,7 +820,7 @@ void enqueue_substep3(sn3_engine* e, hipEvent_t* marks) {
     std::swap(cur, nxt);
   }
   if (marks) hipEventRecord(marks[3], st);
-  hipLaunchKernelGGL(k3_hash_finish, dim3(div_up3(e->S, 64)), dim3(64), 0, st, D, e->dActive);
+  hipLaunchKernelGGL(k3_hash_finish, dim3(e->S), dim3(64), 0, st, D, e->dActive);
   if (marks) hipEventRecord(marks[4], st);
 }
 int4* last_pred3(sn3_engine* e) { return (e->opts.iters % 2 == 1) ? e->D.prB : e->D.prA; }
@@ -653,6 +837,13 @@ void upload_layout3(sn3_engine* e) {
     }
   }
   HIPCHECK3(hipMemcpy((void*)e->D.sidOf, sid.data(), 4 * (size_t)e->N, hipMemcpyHostToDevice));
+  std::vector<uint32_t> cs(std::max(e->C, 1u), INACTIVE);
+  for (uint32_t s = 0; s < e->S; ++s) {
+    if (!e->active[s]) continue;
+    const Scene3P& P = e->sp[s];
+    for (uint32_t k = 0; k < P.cellTotal; ++k) cs[P.cellBase + k] = s;
+  }
+  HIPCHECK3(hipMemcpy((void*)e->D.cellScene, cs.data(), 4 * (size_t)e->C, hipMemcpyHostToDevice));
   HIPCHECK3(hipMemcpy((void*)e->D.sp, e->sp.data(), sizeof(Scene3P) * e->S, hipMemcpyHostToDevice));
   HIPCHECK3(hipMemcpy(e->dActive, e->active.data(), e->S, hipMemcpyHostToDevice));
 }
@@ -721,6 +912,20 @@ int sn3_create(const sn_opts* opts, const int32_t* luts3, uint32_t lut_words, sn
     D.nbr = dalloc3<uint32_t>(e, (size_t)NC * MAXNBR);
     D.NS = NC;
     D.nbrN = dalloc3<uint32_t>(e, NC);
+    D.drop = dalloc3<uint32_t>(e, NC);
+    D.hpart = dalloc3<uint32_t>(e, (size_t)div_up3(NC, 256) * HPART3);
+    D.cellScene = dalloc3<uint32_t>(e, CC);
+    // at most one item per cell plus one per full chunk
+    D.dlist = dalloc3<uint2>(e, (size_t)CC + NC / CHUNK3 + 64);
+    D.listCount = dalloc3<uint32_t>(e, 8);
+    {
+      hipDeviceProp_t prop;
+      HIPCHECK3(hipGetDeviceProperties(&prop, (int)e->opts.device));
+      int bN = 0;
+      HIPCHECK3(hipOccupancyMaxActiveBlocksPerMultiprocessor(&bN, reinterpret_cast<const void*>(k3_nbr_tile), 256, 0));
+      const uint32_t cus = (uint32_t)std::max(prop.multiProcessorCount, 1);
+      e->nbrBlocks = std::max(8u, (cus * (uint32_t)std::max(bN, 1)) & ~7u);
+    }
     D.hitFlag = dalloc3<uint32_t>(e, NC);
     D.hitScan = dalloc3<uint32_t>(e, (size_t)NC + 1);
     int32_t* l = dalloc3<int32_t>(e, LUT3_WORDS);
