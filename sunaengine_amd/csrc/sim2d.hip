@@ -247,6 +247,17 @@ __device__ __forceinline__ int32_t lut_idx(int2 d, Dbg& g) {
     return lut_index(d, g);
   }
 }
+// lut_idx for a bounded scene when |d| < 2^19 on both axes (a candidate of a
+// chunk of one cell no wider than 2^18, inside the chunk's reach box, is
+// within 2^18 + REACH): then ax, ay < 2^15, the
+// squared radius fits 32 bits, and ax > HS already gives r2 >= (HS + 1)^2 >
+// H2S, so the radius test alone decides.
+__device__ __forceinline__ int32_t lut_idx_near(int2 d) {
+  const uint32_t ax = ((uint32_t)abs(d.x) + 8u) >> 4, ay = ((uint32_t)abs(d.y) + 8u) >> 4;
+  const uint32_t r2 = __umul24(ax, ax) + __umul24(ay, ay);
+  return r2 >= (uint32_t)H2S ? -1 : (int32_t)(r2 >> 15);
+}
+static_assert((uint64_t)(HS + 1) * (HS + 1) >= (uint64_t)H2S, "ax > HS implies r2 >= H2S");
 __device__ __forceinline__ int32_t grad_q16(int32_t gt, int32_t d, Dbg& g) { return fp_mul_s(gt, d, 26, g); }
 __device__ __forceinline__ int2 wall_clamp(const SceneP& P, int2 p) {
   return make_int2(clampi(p.x, WALL, P.domW - WALL), clampi(p.y, WALL, P.domH - WALL));
@@ -512,6 +523,7 @@ __device__ __forceinline__ ListRange xcd_range(uint32_t n, uint32_t unitsPerBloc
 // pair geometry; WIN is false for one-cell tiles, whose candidate union is
 // exactly every particle's own 3x3 stencil, so no per-lane column window
 // applies; PPL is the particles per lane (1 when the chunk fits one wave).
+// WIN false is used only for one-cell tiles of cells no wider than 2^18.
 // In a bounded scene a candidate is accepted only with |dx|, |dy| <= REACH
 // (lut_idx rejects (|d| + 8) >> 4 > HS), so a candidate outside the chunk's
 // bounding box grown by REACH is rejected by every lane and has no other
@@ -610,7 +622,8 @@ __device__ __forceinline__ void nbr_chunk(const Dev& D, const int2* lutWG, uint3
             if (!live[k] || q == pk[k]) continue;
             if (WIN && (qc < cxk[k] - 1 || qc > cxk[k] + 1)) continue;
             const int2 d = pdelta<BND>(pik[k], make_int2(px, py), g);
-            const int32_t idx = lut_idx<BND>(d, g);
+            // WIN false: one cell of <= 2^18 ticks, so |d| < 2^18 + REACH < 2^19
+            const int32_t idx = (BND && !WIN) ? lut_idx_near(d) : lut_idx<BND>(d, g);
             if (idx < 0) continue;
             bits[k] |= qi & (INF_SEAL | INF_CONGEAL);
             if (cnt[k] >= MAXNBR) {
@@ -667,7 +680,9 @@ __global__ __launch_bounds__(256) void k_nbr_tile(Dev D) {
     const bool one = (uint32_t)__builtin_amdgcn_readfirstlane((int)(phi - plo)) <= 64u;
     Dbg g;
     if (bnd) {
-      if (ncell == 1u) {
+      // one-cell tiles of cells no wider than 4 wu: every particle's own
+      // stencil is the whole candidate union (no window) and |d| < 2^19
+      if (ncell == 1u && D.sp[s].cellShift <= 18u) {
         if (one) nbr_chunk<true, false, 1>(D, lutWG, c0, ncell, s, plo, phi, lane, W0, g);
         else nbr_chunk<true, false, NBR_PPL>(D, lutWG, c0, ncell, s, plo, phi, lane, W0, g);
       } else {

@@ -440,9 +440,14 @@ __global__ __launch_bounds__(256) void k3_nbr_tile(Dev3 D) {
             const uint32_t mj = (uint32_t)__builtin_amdgcn_readlane((int)cm, j);
             const uint32_t q = q0 + j;
             if (!valid || q == p) continue;
+            // Every lane's particle lies in this cell and the candidate inside
+            // the reach box, so |d| < 2H - 1 on each axis: the quantised squared
+            // radius cannot overflow, and it is >= H2Q whenever some |d| >= H
+            // ((|d| >> 8)^2 >= 512^2), so the radius test alone decides, as the
+            // reference's per-axis test plus radius test do (no |d| can be
+            // INT_MIN for in-world positions, so fp_abs counts nothing).
             const int3 d = make_int3(wsub(px, pi.x), wsub(py, pi.y), wsub(pz, pi.z));
-            if (fp_abs(d.x, g) >= H || fp_abs(d.y, g) >= H || fp_abs(d.z, g) >= H) continue;
-            const int32_t r2q = r2q_of(d);
+            const int32_t r2q = __mul24(d.x >> 8, d.x >> 8) + __mul24(d.y >> 8, d.y >> 8) + __mul24(d.z >> 8, d.z >> 8);
             if (r2q >= H2Q) continue;
             if (cnt >= MAXNBR) {
               dropped++;

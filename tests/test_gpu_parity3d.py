@@ -89,3 +89,19 @@ def test_slotted_3d_world():
         assert got.shape == ref.shape
         assert first_mismatch(got, ref) is None
     e.destroy()
+
+
+def test_large_block_first_substeps():
+    """A 64^3 block (262,144 particles, 8x rest density, cells packed far
+    past one wave) in a 48 x 40 x 48 grid: the tiled neighbour sweep at scale
+    against the oracle on every substep."""
+    from oracle.oracle import set_threads
+    from sunaengine_amd import scenes3d
+    set_threads(16)
+    s = scenes3d.block_scene(side=64, gx=48, gy=40, gz=48, seed=0x3D64)
+    print(f"3d block {s.n} x3", flush=True)
+    o, e, ref, got = _run(s, 3, 3)
+    assert first_mismatch(got, ref) is None
+    assert np.array_equal(e.debug_counters(), o.debug_counters())
+    assert np.array_equal(e.read_state(), o.read())
+    e.destroy()
