@@ -339,6 +339,7 @@ __global__ __launch_bounds__(256) void k_predict_count(Dev D) {
   if (blockIdx.x == 0 && threadIdx.x < 5u) D.listCount[threadIdx.x] = 0u;  // rebuilt after the scan
   uint32_t s = p < D.N ? D.sidOf[p] : INACTIVE;
   Dbg g;
+  uint32_t cell = INACTIVE;
   if (s != INACTIVE) {
     const SceneP& P = D.sp[s];
     int4 a = D.ps[p];
@@ -360,10 +361,27 @@ __global__ __launch_bounds__(256) void k_predict_count(Dev D) {
       }
     }
     CC c = cell_coord(P, pr);
-    uint32_t cell = P.cellBase + (uint32_t)c.y * P.gridW + (uint32_t)c.x;
+    cell = P.cellBase + (uint32_t)c.y * P.gridW + (uint32_t)c.x;
     D.pred0[p] = pr;
     D.cell0[p] = cell;
-    D.rank0[p] = atomicAdd(&D.cellCount[cell], 1u);
+  }
+  // Arrival ranks, one atomic per run of equal cells in the wave (the old
+  // order is cell-sorted, so most of a wave shares a few cells): the run's
+  // first lane reserves the run's length, every lane takes base + offset.
+  // Any assignment of distinct ranks within a cell is valid (the canonical
+  // gather re-ranks by slot id).
+  {
+    const uint32_t lane = threadIdx.x & 63u;
+    const uint32_t prev = (uint32_t)__shfl_up((int)cell, 1);
+    const uint64_t starts = __ballot(lane == 0u || prev != cell);
+    const uint64_t upto = starts & (lane == 63u ? ~0ull : ((2ull << lane) - 1ull));
+    const uint32_t lead = 63u - (uint32_t)__builtin_clzll(upto);
+    const uint64_t after = starts & ~(lane == 63u ? ~0ull : ((2ull << lane) - 1ull));
+    const uint32_t next = after ? (uint32_t)__builtin_ctzll(after) : 64u;
+    uint32_t base = 0;
+    if (lane == lead && cell != INACTIVE) base = atomicAdd(&D.cellCount[cell], next - lead);
+    base = (uint32_t)__shfl((int)base, (int)lead);
+    if (cell != INACTIVE) D.rank0[p] = base + (lane - lead);
   }
   flush_dbg(D, s, g, 0);
 }
@@ -446,7 +464,7 @@ __device__ __forceinline__ void solveA_finish(const Dev& D, uint32_t p, SolveAAc
 // atomics; every item is processed independently of the others, so list
 // order (which the atomics leave open) affects no result.
 constexpr int NBR_PPL = 2;             // particles per lane in the neighbour sweep
-constexpr uint32_t CELL_CHUNK = 64;
+constexpr uint32_t CELL_CHUNK = 32;
 constexpr uint32_t TILE_CHUNK = 128;   // 64 lanes x NBR_PPL
 
 __device__ __forceinline__ void append_chunks(uint32_t n, uint32_t* counter, uint2* list, uint32_t key) {
@@ -712,47 +730,97 @@ __device__ __forceinline__ uint32_t nbr_at(const Dev& D, const uint32_t* nb, uin
 }
 
 // ================================================================ solveA (k>0)
-template <bool BND>
-__device__ __forceinline__ void solveA_at(const Dev& D, uint32_t p, Dbg& g) {
-  const int2 pi = D.pred[p];
-  const uint32_t cnt = D.nbrN[p] & 0x1fffffffu;
-  const uint32_t* nb = D.nbr + p;
-  SolveAAcc acc{D.lut4[0].x, 0, 0, 0};
-  for (uint32_t k = 0; k < cnt; k += NBR_UNROLL) {
-    uint32_t q[NBR_UNROLL];
-    int2 d[NBR_UNROLL];
-    int32_t idx[NBR_UNROLL];
-    int4 L[NBR_UNROLL];
-#pragma unroll
-    for (uint32_t u = 0; u < NBR_UNROLL; ++u) q[u] = nbr_at(D, nb, k + u, cnt, p);
-#pragma unroll
-    for (uint32_t u = 0; u < NBR_UNROLL; ++u) {
-      d[u] = pdelta<BND>(pi, D.pred[q[u] & ~NBR_BND], g);
-      idx[u] = (k + u < cnt) ? lut_idx<BND>(d[u], g) : -1;
-      L[u] = D.lut4[maxi(idx[u], 0)];
-    }
-#pragma unroll
-    for (uint32_t u = 0; u < NBR_UNROLL; ++u)
-      if (idx[u] >= 0) solveA_term(acc, d[u], L[u], g);
-  }
-  solveA_finish(D, p, acc, MAT(D, (uint32_t)D.ptY[p].x)[M_EPS], g);
-}
-
 // True on every lane when the wave's active lanes all belong to bounded scenes.
 __device__ __forceinline__ bool wave_bounded(const Dev& D, uint32_t s) {
   return __all(s == INACTIVE || D.sp[s].bounded != 0u);
 }
 
-__global__ __launch_bounds__(256) void k_solveA(Dev D) {
-  const uint32_t p = xcd_block() * 256u + threadIdx.x;
-  const uint32_t s = p < D.N ? D.sidOf[p] : INACTIVE;
-  Dbg g;
-  const bool bnd = wave_bounded(D, s);
-  if (s != INACTIVE && (D.info[p] & INF_FLUID)) {
-    if (bnd) solveA_at<true>(D, p, g);
-    else solveA_at<false>(D, p, g);
+// solveA over the cell chunks of all four colour lists: one workgroup per
+// chunk, each particle's neighbour list split over L lanes as in k_solveB
+// (every solveA sum is a wrapping u32 sum; only lane 0 of a particle starts
+// from the self term W(0)), with (W, G) in LDS.
+template <bool BND>
+__device__ __forceinline__ void solveA_part(const Dev& D, uint32_t p, uint32_t sub, uint32_t L, const int2* lutWG,
+                                            SolveAAcc& acc, Dbg& g) {
+  const int2 pi = D.pred[p];
+  const uint32_t cnt = D.nbrN[p] & 0x1fffffffu;
+  const uint32_t* nb = D.nbr + p;
+  for (uint32_t k = sub; k < cnt; k += 2u * L) {
+    const uint32_t k1 = k + L;
+    const uint32_t q0 = nb[(size_t)k * D.NS] & ~NBR_BND;
+    const uint32_t q1 = k1 < cnt ? nb[(size_t)k1 * D.NS] & ~NBR_BND : p;
+    const int2 d0 = pdelta<BND>(pi, D.pred[q0], g), d1 = pdelta<BND>(pi, D.pred[q1], g);
+    const int32_t i0 = lut_idx<BND>(d0, g);
+    const int32_t i1 = k1 < cnt ? lut_idx<BND>(d1, g) : -1;
+    if (i0 >= 0) {
+      const int2 T = lutWG[i0];
+      solveA_term(acc, d0, make_int4(T.x, T.y, 0, 0), g);
+    }
+    if (i1 >= 0) {
+      const int2 T = lutWG[i1];
+      solveA_term(acc, d1, make_int4(T.x, T.y, 0, 0), g);
+    }
   }
-  flush_dbg(D, s, g, 0);
+}
+
+// The particles of cell-chunk item {cell, chunk}: scene, first sorted slot, count.
+__device__ __forceinline__ void chunk_range(const Dev& D, uint2 item, uint32_t& s, uint32_t& lo, uint32_t& m) {
+  const uint32_t c = item.x;
+  s = D.cellScene[c];
+  const int32_t sh = D.sp[s].posShift;
+  const uint32_t clo = (uint32_t)((int32_t)D.cellStart[c] + sh);
+  const uint32_t chi = (uint32_t)((int32_t)D.cellStart[c + 1] + sh);
+  lo = clo + item.y * CELL_CHUNK;
+  m = (uint32_t)mini((int32_t)(chi - lo), (int32_t)CELL_CHUNK);
+}
+
+// Item w of the four colour lists taken end to end.
+__device__ __forceinline__ uint2 colour_item(const Dev& D, uint32_t w, const uint32_t n[4]) {
+  uint32_t c = 0;
+  while (c < 3u && w >= n[c]) w -= n[c++];
+  return D.dlist[c][w];
+}
+
+__global__ __launch_bounds__(256) void k_solveA(Dev D) {
+  __shared__ int2 lutWG[TN];
+  for (uint32_t i = threadIdx.x; i < TN; i += 256u) {
+    const int4 L = D.lut4[i];
+    lutWG[i] = make_int2(L.x, L.y);
+  }
+  __syncthreads();
+  uint32_t n[4];
+#pragma unroll
+  for (int c = 0; c < 4; ++c) n[c] = (uint32_t)__builtin_amdgcn_readfirstlane((int)D.listCount[c]);
+  const ListRange R = xcd_range(n[0] + n[1] + n[2] + n[3], 1u, 0u);
+  const int32_t W0 = lutWG[0].x;
+  for (uint32_t w = R.first; w < R.hi; w += R.stride) {
+    uint2 item = colour_item(D, w, n);
+    item.x = __builtin_amdgcn_readfirstlane(item.x);
+    item.y = __builtin_amdgcn_readfirstlane(item.y);
+    uint32_t s, lo, m;
+    chunk_range(D, item, s, lo, m);
+    s = __builtin_amdgcn_readfirstlane(s);
+    m = __builtin_amdgcn_readfirstlane(m);
+    Dbg g;
+    uint32_t L = 16;
+    while (L > 1u && L * m > 256u) L >>= 1;
+    const uint32_t pl = threadIdx.x / L, sub = threadIdx.x & (L - 1u);
+    const uint32_t p = lo + pl;
+    const bool act = pl < m && (D.info[p] & INF_FLUID);
+    SolveAAcc acc{sub == 0u ? W0 : 0, 0, 0, 0};
+    if (act) {
+      if (D.sp[s].bounded) solveA_part<true>(D, p, sub, L, lutWG, acc, g);
+      else solveA_part<false>(D, p, sub, L, lutWG, acc, g);
+    }
+    for (uint32_t off = 1; off < L; off <<= 1) {
+      acc.rho = wadd(acc.rho, __shfl_xor(acc.rho, (int)off));
+      acc.S = wadd(acc.S, __shfl_xor(acc.S, (int)off));
+      acc.sgx = wadd(acc.sgx, __shfl_xor(acc.sgx, (int)off));
+      acc.sgy = wadd(acc.sgy, __shfl_xor(acc.sgy, (int)off));
+    }
+    if (act && sub == 0u) solveA_finish(D, p, acc, MAT(D, (uint32_t)D.ptY[p].x)[M_EPS], g);
+    flush_dbg(D, s, g, 0);
+  }
 }
 
 // ================================================================ colour pass
@@ -800,15 +868,6 @@ __device__ __forceinline__ int2 solveB_part(const Dev& D, uint32_t p, uint32_t s
   return make_int2(dpx, dpy);
 }
 
-__device__ __forceinline__ void chunk_range(const Dev& D, uint2 item, uint32_t& s, uint32_t& lo, uint32_t& m) {
-  const uint32_t c = item.x;
-  s = D.cellScene[c];
-  const int32_t sh = D.sp[s].posShift;
-  const uint32_t clo = (uint32_t)((int32_t)D.cellStart[c] + sh);
-  const uint32_t chi = (uint32_t)((int32_t)D.cellStart[c + 1] + sh);
-  lo = clo + item.y * CELL_CHUNK;
-  m = (uint32_t)mini((int32_t)(chi - lo), (int32_t)CELL_CHUNK);
-}
 
 __global__ __launch_bounds__(256) void k_solveB(Dev D, uint32_t colour) {
   __shared__ int2 lutGS[TN];
@@ -1466,7 +1525,7 @@ void enqueue_substep(sn_engine* e) {
   for (uint32_t it = 0; it < iters; ++it) {
     if (it > 0) {
       ProfScope ps(e, PC_SOLVEA);
-      hipLaunchKernelGGL(k_solveA, dim3(gN), dim3(256), 0, st, D);
+      hipLaunchKernelGGL(k_solveA, dim3(e->colourBlocks), dim3(256), 0, st, D);
     }
     for (uint32_t c = 0; c < 4; ++c) {
       {
