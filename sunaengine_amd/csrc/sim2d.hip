@@ -730,37 +730,48 @@ __device__ __forceinline__ uint32_t nbr_at(const Dev& D, const uint32_t* nb, uin
 }
 
 // ================================================================ solveA (k>0)
+
+template <bool BND>
+__device__ __forceinline__ void solveA_at(const Dev& D, uint32_t p, Dbg& g) {
+  const int2 pi = D.pred[p];
+  const uint32_t cnt = D.nbrN[p] & 0x1fffffffu;
+  const uint32_t* nb = D.nbr + p;
+  SolveAAcc acc{D.lut4[0].x, 0, 0, 0};
+  for (uint32_t k = 0; k < cnt; k += NBR_UNROLL) {
+    uint32_t q[NBR_UNROLL];
+    int2 d[NBR_UNROLL];
+    int32_t idx[NBR_UNROLL];
+    int4 L[NBR_UNROLL];
+#pragma unroll
+    for (uint32_t u = 0; u < NBR_UNROLL; ++u) q[u] = nbr_at(D, nb, k + u, cnt, p);
+#pragma unroll
+    for (uint32_t u = 0; u < NBR_UNROLL; ++u) {
+      d[u] = pdelta<BND>(pi, D.pred[q[u] & ~NBR_BND], g);
+      idx[u] = (k + u < cnt) ? lut_idx<BND>(d[u], g) : -1;
+      L[u] = D.lut4[maxi(idx[u], 0)];
+    }
+#pragma unroll
+    for (uint32_t u = 0; u < NBR_UNROLL; ++u)
+      if (idx[u] >= 0) solveA_term(acc, d[u], L[u], g);
+  }
+  solveA_finish(D, p, acc, MAT(D, (uint32_t)D.ptY[p].x)[M_EPS], g);
+}
+
 // True on every lane when the wave's active lanes all belong to bounded scenes.
 __device__ __forceinline__ bool wave_bounded(const Dev& D, uint32_t s) {
   return __all(s == INACTIVE || D.sp[s].bounded != 0u);
 }
 
-// solveA over the cell chunks of all four colour lists: one workgroup per
-// chunk, each particle's neighbour list split over L lanes as in k_solveB
-// (every solveA sum is a wrapping u32 sum; only lane 0 of a particle starts
-// from the self term W(0)), with (W, G) in LDS.
-template <bool BND>
-__device__ __forceinline__ void solveA_part(const Dev& D, uint32_t p, uint32_t sub, uint32_t L, const int2* lutWG,
-                                            SolveAAcc& acc, Dbg& g) {
-  const int2 pi = D.pred[p];
-  const uint32_t cnt = D.nbrN[p] & 0x1fffffffu;
-  const uint32_t* nb = D.nbr + p;
-  for (uint32_t k = sub; k < cnt; k += 2u * L) {
-    const uint32_t k1 = k + L;
-    const uint32_t q0 = nb[(size_t)k * D.NS] & ~NBR_BND;
-    const uint32_t q1 = k1 < cnt ? nb[(size_t)k1 * D.NS] & ~NBR_BND : p;
-    const int2 d0 = pdelta<BND>(pi, D.pred[q0], g), d1 = pdelta<BND>(pi, D.pred[q1], g);
-    const int32_t i0 = lut_idx<BND>(d0, g);
-    const int32_t i1 = k1 < cnt ? lut_idx<BND>(d1, g) : -1;
-    if (i0 >= 0) {
-      const int2 T = lutWG[i0];
-      solveA_term(acc, d0, make_int4(T.x, T.y, 0, 0), g);
-    }
-    if (i1 >= 0) {
-      const int2 T = lutWG[i1];
-      solveA_term(acc, d1, make_int4(T.x, T.y, 0, 0), g);
-    }
+__global__ __launch_bounds__(256) void k_solveA(Dev D) {
+  const uint32_t p = xcd_block() * 256u + threadIdx.x;
+  const uint32_t s = p < D.N ? D.sidOf[p] : INACTIVE;
+  Dbg g;
+  const bool bnd = wave_bounded(D, s);
+  if (s != INACTIVE && (D.info[p] & INF_FLUID)) {
+    if (bnd) solveA_at<true>(D, p, g);
+    else solveA_at<false>(D, p, g);
   }
+  flush_dbg(D, s, g, 0);
 }
 
 // The particles of cell-chunk item {cell, chunk}: scene, first sorted slot, count.
@@ -774,54 +785,6 @@ __device__ __forceinline__ void chunk_range(const Dev& D, uint2 item, uint32_t& 
   m = (uint32_t)mini((int32_t)(chi - lo), (int32_t)CELL_CHUNK);
 }
 
-// Item w of the four colour lists taken end to end.
-__device__ __forceinline__ uint2 colour_item(const Dev& D, uint32_t w, const uint32_t n[4]) {
-  uint32_t c = 0;
-  while (c < 3u && w >= n[c]) w -= n[c++];
-  return D.dlist[c][w];
-}
-
-__global__ __launch_bounds__(256) void k_solveA(Dev D) {
-  __shared__ int2 lutWG[TN];
-  for (uint32_t i = threadIdx.x; i < TN; i += 256u) {
-    const int4 L = D.lut4[i];
-    lutWG[i] = make_int2(L.x, L.y);
-  }
-  __syncthreads();
-  uint32_t n[4];
-#pragma unroll
-  for (int c = 0; c < 4; ++c) n[c] = (uint32_t)__builtin_amdgcn_readfirstlane((int)D.listCount[c]);
-  const ListRange R = xcd_range(n[0] + n[1] + n[2] + n[3], 1u, 0u);
-  const int32_t W0 = lutWG[0].x;
-  for (uint32_t w = R.first; w < R.hi; w += R.stride) {
-    uint2 item = colour_item(D, w, n);
-    item.x = __builtin_amdgcn_readfirstlane(item.x);
-    item.y = __builtin_amdgcn_readfirstlane(item.y);
-    uint32_t s, lo, m;
-    chunk_range(D, item, s, lo, m);
-    s = __builtin_amdgcn_readfirstlane(s);
-    m = __builtin_amdgcn_readfirstlane(m);
-    Dbg g;
-    uint32_t L = 16;
-    while (L > 1u && L * m > 256u) L >>= 1;
-    const uint32_t pl = threadIdx.x / L, sub = threadIdx.x & (L - 1u);
-    const uint32_t p = lo + pl;
-    const bool act = pl < m && (D.info[p] & INF_FLUID);
-    SolveAAcc acc{sub == 0u ? W0 : 0, 0, 0, 0};
-    if (act) {
-      if (D.sp[s].bounded) solveA_part<true>(D, p, sub, L, lutWG, acc, g);
-      else solveA_part<false>(D, p, sub, L, lutWG, acc, g);
-    }
-    for (uint32_t off = 1; off < L; off <<= 1) {
-      acc.rho = wadd(acc.rho, __shfl_xor(acc.rho, (int)off));
-      acc.S = wadd(acc.S, __shfl_xor(acc.S, (int)off));
-      acc.sgx = wadd(acc.sgx, __shfl_xor(acc.sgx, (int)off));
-      acc.sgy = wadd(acc.sgy, __shfl_xor(acc.sgy, (int)off));
-    }
-    if (act && sub == 0u) solveA_finish(D, p, acc, MAT(D, (uint32_t)D.ptY[p].x)[M_EPS], g);
-    flush_dbg(D, s, g, 0);
-  }
-}
 
 // ================================================================ colour pass
 // solveB_c then applyDp_c (src/sim.wgsl:1337-1474) for one colour, as two
@@ -1525,7 +1488,7 @@ void enqueue_substep(sn_engine* e) {
   for (uint32_t it = 0; it < iters; ++it) {
     if (it > 0) {
       ProfScope ps(e, PC_SOLVEA);
-      hipLaunchKernelGGL(k_solveA, dim3(e->colourBlocks), dim3(256), 0, st, D);
+      hipLaunchKernelGGL(k_solveA, dim3(gN), dim3(256), 0, st, D);
     }
     for (uint32_t c = 0; c < 4; ++c) {
       {
