@@ -454,6 +454,21 @@ __device__ __forceinline__ void solveA_finish(const Dev& D, uint32_t p, SolveAAc
   D.lam[p] = clampi(fp_neg_sat(fp_divshift(Cq, wadd(a.S, eps), 25, g), g), -LAM_MAX, LAM_MAX);
 }
 
+// ============================================================ neighbour loops
+// Pair loops walk a particle's list NBR_UNROLL entries at a time with every
+// load of the group issued before any term is formed, so each lane keeps
+// several gathers in flight.  Entries past the count read as the particle
+// itself (a zero offset, no saturation) and are skipped; every pair sum is a
+// wrapping u32 sum, so grouping does not change it.
+constexpr uint32_t NBR_UNROLL = 4;
+constexpr uint32_t XNC_UNROLL = 2;
+static_assert(MAXNBR % NBR_UNROLL == 0 && MAXNBR % XNC_UNROLL == 0, "list slots k + u stay below MAXNBR");
+__device__ __forceinline__ uint32_t nbr_at(const Dev& D, const uint32_t* nb, uint32_t k, uint32_t cnt,
+                                           uint32_t self) {
+  uint32_t e = nb[(size_t)k * D.NS];
+  return k < cnt ? e : self;
+}
+
 // ============================================================ per-substep work
 // After the scan: work lists of this substep.  [0..3]: every occupied cell
 // of colour c, one item {cell, chunk} per CELL_CHUNK of its particles; [4]:
@@ -561,7 +576,7 @@ __device__ __forceinline__ int32_t wave_max(int32_t v) {
 }
 
 template <bool BND, bool WIN, int PPL>
-__device__ __forceinline__ void nbr_chunk(const Dev& D, const int2* lutWG, uint32_t c0, uint32_t ncell, uint32_t s,
+__device__ __forceinline__ void nbr_chunk(const Dev& D, uint32_t c0, uint32_t ncell, uint32_t s,
                                           uint32_t plo, uint32_t phi, uint32_t lane, int32_t W0, Dbg& g) {
   const SceneP& P = D.sp[s];
   const int32_t sh = P.posShift;
@@ -650,10 +665,7 @@ __device__ __forceinline__ void nbr_chunk(const Dev& D, const int2* lutWG, uint3
             }
             D.nbr[(size_t)cnt[k] * D.NS + pk[k]] = q | ((qi & INF_FLUID) ? 0u : NBR_BND);
             cnt[k]++;
-            if (inf[k] & INF_FLUID) {
-              const int2 L = lutWG[idx];
-              solveA_term(acc[k], d, make_int4(L.x, L.y, 0, 0), g);
-            }
+            if (inf[k] & INF_FLUID) solveA_term(acc[k], d, D.lut4[idx], g);
           }
         }
       }
@@ -670,17 +682,137 @@ __device__ __forceinline__ void nbr_chunk(const Dev& D, const int2* lutWG, uint3
   }
 }
 
-__global__ __launch_bounds__(256) void k_nbr_tile(Dev D) {
-  __shared__ int2 lutWG[TN];
-  for (uint32_t i = threadIdx.x; i < TN; i += 256u) {
-    const int4 L = D.lut4[i];
-    lutWG[i] = make_int2(L.x, L.y);
+// The one-wave form of the sweep (one particle per lane) with the lists
+// staged in LDS.  The candidate loop only tests and records: an accepted
+// candidate is stored as 16 bits, (row << 14) | (boundary << 13) | offset in
+// its row range, at list[k][lane].  Afterwards the lists are written out
+// transposed, one coalesced 256-byte store per k, and iteration 0 of solveA
+// runs over them with every lane busy, instead of inside the candidate loop
+// where only the few accepting lanes of each candidate would run it.
+// Returns false, having done nothing, when a row range is too long for the
+// 13-bit offset; the caller then takes nbr_chunk.
+constexpr uint32_t ROW_OFF_MAX = 1u << 13;
+
+template <bool BND, bool WIN>
+__device__ __forceinline__ bool nbr_chunk_staged(const Dev& D, uint16_t* lst, uint32_t c0,
+                                                 uint32_t ncell, uint32_t s, uint32_t plo, uint32_t phi,
+                                                 uint32_t lane, int32_t W0, Dbg& g) {
+  const SceneP& P = D.sp[s];
+  const int32_t sh = P.posShift;
+  const uint32_t gridW = P.gridW, gridH = P.gridH, cellBase = P.cellBase;
+  const uint32_t lc0 = c0 - cellBase;
+  const int32_t cy = (int32_t)(lc0 / gridW), cx0 = (int32_t)(lc0 % gridW);
+  const int32_t xlo = maxi(cx0 - 1, 0), xhi = mini(cx0 + (int32_t)ncell, (int32_t)gridW - 1);
+  const uint32_t myRow = cellBase + (uint32_t)cy * gridW;
+  uint32_t rlo[3], rhi[3];
+#pragma unroll
+  for (int r = 0; r < 3; ++r) {
+    const int32_t gy = cy + r - 1;
+    rlo[r] = rhi[r] = 0;
+    if (gy < 0 || gy >= (int32_t)gridH) continue;
+    const uint32_t rowc = cellBase + (uint32_t)gy * gridW;
+    rlo[r] = (uint32_t)((int32_t)D.cellStart[rowc + (uint32_t)xlo] + sh);
+    rhi[r] = (uint32_t)((int32_t)D.cellStart[rowc + (uint32_t)xhi + 1u] + sh);
+    if (rhi[r] - rlo[r] > ROW_OFF_MAX) return false;
   }
-  __syncthreads();
+  const uint32_t p = plo + lane;
+  const bool valid = p < phi;
+  const uint32_t inf = valid ? D.info[p] : 0u;
+  const int2 pi = valid ? D.pred[p] : make_int2(0, 0);
+  const int32_t cxk = valid ? (int32_t)(D.cellY[p] - myRow) : 0;
+  const bool live = valid && !(inf & INF_DEAD);
+  uint32_t cnt = 0, bits = 0, dropped = 0;
+  int32_t bx0 = 0, bx1 = 0, by0 = 0, by1 = 0;
+  if constexpr (BND) {
+    bx0 = wave_min(live ? pi.x : I32_MAX) - REACH;
+    bx1 = wave_max(live ? pi.x : I32_MIN) + REACH;
+    by0 = wave_min(live ? pi.y : I32_MAX) - REACH;
+    by1 = wave_max(live ? pi.y : I32_MIN) + REACH;
+  }
+#pragma unroll
+  for (uint32_t r = 0; r < 3; ++r) {
+    const uint32_t rowc = cellBase + (uint32_t)(cy + (int32_t)r - 1) * gridW;
+    for (uint32_t q0 = rlo[r]; q0 < rhi[r]; q0 += 64u) {
+      const uint32_t qn = (uint32_t)mini((int32_t)(rhi[r] - q0), 64);
+      int2 cp = make_int2(0, 0);
+      uint32_t ci = 0;
+      int32_t cc = 0;
+      if (lane < qn) {
+        cp = D.pred[q0 + lane];
+        ci = D.info[q0 + lane];
+        cc = (int32_t)(D.cellY[q0 + lane] - rowc);
+      }
+      bool want = lane < qn && !(ci & INF_DEAD);   // nobody sees a corpse
+      if constexpr (BND) want = want && cp.x >= bx0 && cp.x <= bx1 && cp.y >= by0 && cp.y <= by1;
+      uint64_t todo = __ballot(want);
+      while (todo) {
+        const uint32_t j = (uint32_t)__builtin_ctzll(todo);
+        todo &= todo - 1ull;
+        const int32_t px = __builtin_amdgcn_readlane(cp.x, j);
+        const int32_t py = __builtin_amdgcn_readlane(cp.y, j);
+        const uint32_t qi = (uint32_t)__builtin_amdgcn_readlane((int)ci, j);
+        const int32_t qc = WIN ? __builtin_amdgcn_readlane(cc, j) : 0;
+        const uint32_t off = q0 - rlo[r] + j;
+        if (!live || q0 + j == p) continue;
+        if (WIN && (qc < cxk - 1 || qc > cxk + 1)) continue;
+        const int2 d = pdelta<BND>(pi, make_int2(px, py), g);
+        const int32_t idx = (BND && !WIN) ? lut_idx_near(d) : lut_idx<BND>(d, g);
+        if (idx < 0) continue;
+        bits |= qi & (INF_SEAL | INF_CONGEAL);
+        if (cnt >= MAXNBR) {
+          dropped++;
+          continue;
+        }
+        lst[cnt * 64u + lane] = (uint16_t)((r << 14) | ((qi & INF_FLUID) ? 0u : (1u << 13)) | off);
+        cnt++;
+      }
+    }
+  }
+  // write-out (coalesced per k) and iteration 0 of solveA over the lists,
+  // NBR_UNROLL entries at a time with their gathers issued together
+  SolveAAcc acc{W0, 0, 0, 0};
+  const uint32_t kmax = (uint32_t)wave_max((int32_t)cnt);
+  const bool fl = live && (inf & INF_FLUID);
+  for (uint32_t k = 0; k < kmax; k += NBR_UNROLL) {
+    uint32_t q[NBR_UNROLL];
+    int2 d[NBR_UNROLL];
+    int32_t idx[NBR_UNROLL];
+    int4 L[NBR_UNROLL];
+#pragma unroll
+    for (uint32_t u = 0; u < NBR_UNROLL; ++u) {
+      q[u] = p;
+      if (k + u < cnt) {
+        const uint32_t e = lst[(k + u) * 64u + lane];
+        const uint32_t r = e >> 14;
+        q[u] = (r == 0u ? rlo[0] : (r == 1u ? rlo[1] : rlo[2])) + (e & (ROW_OFF_MAX - 1u));
+        D.nbr[(size_t)(k + u) * D.NS + p] = q[u] | ((e & (1u << 13)) ? NBR_BND : 0u);
+      }
+    }
+#pragma unroll
+    for (uint32_t u = 0; u < NBR_UNROLL; ++u) {
+      d[u] = pdelta<BND>(pi, valid ? D.pred[q[u]] : pi, g);
+      idx[u] = (fl && k + u < cnt) ? ((BND && !WIN) ? lut_idx_near(d[u]) : lut_idx<BND>(d[u], g)) : -1;
+      L[u] = D.lut4[maxi(idx[u], 0)];
+    }
+#pragma unroll
+    for (uint32_t u = 0; u < NBR_UNROLL; ++u)
+      if (idx[u] >= 0) solveA_term(acc, d[u], L[u], g);
+  }
+  if (valid) {
+    D.nbrN[p] = live ? (cnt | ((bits & INF_SEAL) ? NBR_SEAL_BIT : 0u) | ((bits & INF_CONGEAL) ? NBR_CONGEAL_BIT : 0u))
+                     : 0u;
+    D.drop[p] = dropped;
+    if (inf & INF_FLUID) solveA_finish(D, p, acc, MAT(D, (uint32_t)D.ptY[p].x)[M_EPS], g);
+  }
+  return true;
+}
+
+__global__ __launch_bounds__(256) void k_nbr_tile(Dev D) {
+  __shared__ uint16_t lists[4][MAXNBR * 64];
   const uint32_t lane = threadIdx.x & 63u;
   const uint32_t nwork = (uint32_t)__builtin_amdgcn_readfirstlane((int)D.listCount[4]);
   const ListRange R = xcd_range(nwork, 4u, threadIdx.x >> 6);
-  const int32_t W0 = lutWG[0].x;
+  const int32_t W0 = D.lut4[0].x;
   for (uint32_t w = R.first; w < R.hi; w += R.stride) {
     const uint2 item = D.dlist[4][w];
     const uint32_t tix = __builtin_amdgcn_readfirstlane(item.x);
@@ -700,33 +832,19 @@ __global__ __launch_bounds__(256) void k_nbr_tile(Dev D) {
     if (bnd) {
       // one-cell tiles of cells no wider than 4 wu: every particle's own
       // stencil is the whole candidate union (no window) and |d| < 2^19
+      uint16_t* lst = lists[threadIdx.x >> 6];
       if (ncell == 1u && D.sp[s].cellShift <= 18u) {
-        if (one) nbr_chunk<true, false, 1>(D, lutWG, c0, ncell, s, plo, phi, lane, W0, g);
-        else nbr_chunk<true, false, NBR_PPL>(D, lutWG, c0, ncell, s, plo, phi, lane, W0, g);
+        if (!(one && nbr_chunk_staged<true, false>(D, lst, c0, ncell, s, plo, phi, lane, W0, g)))
+          nbr_chunk<true, false, NBR_PPL>(D, c0, ncell, s, plo, phi, lane, W0, g);
       } else {
-        if (one) nbr_chunk<true, true, 1>(D, lutWG, c0, ncell, s, plo, phi, lane, W0, g);
-        else nbr_chunk<true, true, NBR_PPL>(D, lutWG, c0, ncell, s, plo, phi, lane, W0, g);
+        if (!(one && nbr_chunk_staged<true, true>(D, lst, c0, ncell, s, plo, phi, lane, W0, g)))
+          nbr_chunk<true, true, NBR_PPL>(D, c0, ncell, s, plo, phi, lane, W0, g);
       }
     } else {
-      nbr_chunk<false, true, NBR_PPL>(D, lutWG, c0, ncell, s, plo, phi, lane, W0, g);
+      nbr_chunk<false, true, NBR_PPL>(D, c0, ncell, s, plo, phi, lane, W0, g);
     }
     flush_dbg(D, s, g, 0);
   }
-}
-
-// ============================================================ neighbour loops
-// Pair loops walk a particle's list NBR_UNROLL entries at a time with every
-// load of the group issued before any term is formed, so each lane keeps
-// several gathers in flight.  Entries past the count read as the particle
-// itself (a zero offset, no saturation) and are skipped; every pair sum is a
-// wrapping u32 sum, so grouping does not change it.
-constexpr uint32_t NBR_UNROLL = 4;
-constexpr uint32_t XNC_UNROLL = 2;
-static_assert(MAXNBR % NBR_UNROLL == 0 && MAXNBR % XNC_UNROLL == 0, "list slots k + u stay below MAXNBR");
-__device__ __forceinline__ uint32_t nbr_at(const Dev& D, const uint32_t* nb, uint32_t k, uint32_t cnt,
-                                           uint32_t self) {
-  uint32_t e = nb[(size_t)k * D.NS];
-  return k < cnt ? e : self;
 }
 
 // ================================================================ solveA (k>0)
