@@ -216,21 +216,18 @@ class WaterDemo {
     };
   }
   restore(snap) {
-    this._loadPayload(snap);
+    let acc = null;
     if (snap.derived) {
-      // reload with the captured carried acceleration (the only scratch a
-      // substep reads from its predecessor: Derived.stAcc, words 8..9)
-      const n = snap.scene.n, acc = new Int32Array(n * 2);
+      // the carried surface-tension acceleration (Derived words 8..9) is the
+      // only scratch a substep reads from its predecessor
+      const n = snap.scene.n;
+      acc = new Int32Array(n * 2);
       for (let i = 0; i < n; i++) { acc[2 * i] = snap.derived[i * 16 + 8]; acc[2 * i + 1] = snap.derived[i * 16 + 9]; }
-      const im = eng.buildState(this.engine._scene, this.engine.maxParticles, this.engine.maxCells);
-      im.state = Int32Array.from(snap.state);
-      im.stAcc = acc;
-      require('./sunaengine').Engine.prototype; // keep module loaded
-      this.engine._reloadImage(im);
     }
+    this._loadPayload(snap, acc);
     this.substep = snap.substep || 0;
   }
-  _loadPayload(p) {
+  _loadPayload(p, stAcc) {
     const s = p.scene || (p.initial && p.initial.scene);
     const state = p.state || (p.initial && p.initial.state);
     if (!s || !state) throw new Error('[demo_boot] payload has no scene/state');
@@ -243,7 +240,8 @@ class WaterDemo {
       pos[2 * i] = state[o]; pos[2 * i + 1] = state[o + 1]; vel[2 * i] = state[o + 2]; vel[2 * i + 1] = state[o + 3];
       matId[i] = su[o + 4]; flags[i] = su[o + 5] & eng.FLAG_MASK; temp[i] = state[o + 6]; fuel[i] = state[o + 7];
     }
-    this.engine.loadScene({ n, nFluid: s.nFluid, pos, vel, matId, flags, temp, fuel, domW: s.domW, domH: s.domH, cellShift: s.cellShift });
+    this.engine.loadScene({ n, nFluid: s.nFluid, pos, vel, matId, flags, temp, fuel, domW: s.domW, domH: s.domH,
+      cellShift: s.cellShift }, stAcc);
     this._schedule.clear(); this._pushes.clear(); this._rec = null; this.substep = 0;
   }
   async startRecording() {

@@ -242,8 +242,14 @@ class Engine {
     this.n = 0; this.nFluid = 0; this._scene = null; this.substepCount = 0;
   }
   get matCount() { return mats().rows.length; }
-  loadScene(scene) {
+  // stAcc (optional): n*2 words of carried surface-tension acceleration
+  // (Derived words 8..9), for an exact restore.
+  loadScene(scene, stAcc) {
     const im = buildState(scene, this.maxParticles, this.maxCells);
+    if (stAcc) {
+      assert(stAcc.length === im.n * 2, 'stAcc must hold n*2 words');
+      im.stAcc = Int32Array.from(stAcc);
+    }
     native().loadScenes(this._h, [im]);
     this.n = im.n; this.nFluid = im.nFluid; this._scene = scene;
     this.domW = im.domW; this.domH = im.domH; this.cellShift = im.cellShift;

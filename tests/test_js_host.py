@@ -1,0 +1,49 @@
+"""The Node.js host (js/): CPU-side checks run here; the GPU run is marked gpu."""
+import json
+import os
+import shutil
+import subprocess
+import sys
+
+import pytest
+
+HERE = os.path.dirname(os.path.abspath(__file__))
+ROOT = os.path.dirname(HERE)
+ADDON = os.path.join(ROOT, "js", "build", "sunaengine_napi.node")
+MATS = os.path.join(ROOT, "js", "build", "mats.json")
+
+needs_node = pytest.mark.skipif(shutil.which("node") is None, reason="node is not installed")
+
+
+def _expected_scenes():
+    sys.path.insert(0, ROOT)
+    from sunaengine_amd import scenes, tables
+    out = {}
+    for name, kw in (("dambreak_v1", {}), ("overdense_v1", {}), ("pool_v1", {"w": 60, "h": 20})):
+        s = scenes.make_scene(name, **kw)
+        im = scenes.load_image(s)
+        acc = tables.digest_words(im.state.view("uint32").reshape(-1), 8, 8)
+        out[name] = {"n": int(im.n), "nFluid": int(im.nFluid), "digest": tables.hex8(acc)}
+    return out
+
+
+@needs_node
+def test_node_host_cpu_side():
+    if not os.path.exists(MATS):
+        pytest.skip("js/build/mats.json not built (python js/build_addon.py)")
+    arg = json.dumps({"scenes": _expected_scenes()})
+    r = subprocess.run(["node", os.path.join(HERE, "js", "host_cpu.js"), arg], capture_output=True, text=True,
+                       timeout=120)
+    assert r.returncode == 0 and r.stdout.startswith("OK"), r.stdout + r.stderr
+
+
+@pytest.mark.gpu
+@needs_node
+def test_node_host_on_gpu():
+    assert os.path.exists(ADDON), "N-API addon not built"
+    with open(os.path.join(HERE, "golden", "dambreak_v1.json")) as f:
+        g = json.load(f)
+    r = subprocess.run(["node", os.path.join(HERE, "js", "host_gpu.js"), json.dumps(g["checkpoints"])],
+                       capture_output=True, text=True, timeout=300)
+    print(r.stdout)
+    assert r.returncode == 0 and "OK node host" in r.stdout, r.stdout + r.stderr
