@@ -83,6 +83,19 @@ int main() {
     EXPECT(oref::fp_mul_s(a, b, s) == want, "oracle fp_mul_s(%d,%d,%u)", a, b, s);
     EXPECT(sn::fp_mul_s(a, b, s, g) == want, "device fp_mul_s(%d,%d,%u)", a, b, s);
   }
+  // the saturation count of fp_mul_s (dbg[0]) matches the exact quotient
+  for (int it = 0; it < 200000; ++it) {
+    int32_t a = rnd(), b = rnd();
+    unsigned s = 1 + (unsigned)(rng() % 31);
+    i128 v = (i128)a * b;
+    i128 q = v >> s;                                         // floor
+    i128 rem = v - (q << s), half = (i128)1 << (s - 1);
+    if (rem > half || (rem == half && (q & 1))) q += 1;     // round half to even
+    bool sat = q > INT32_MAX || q < INT32_MIN;
+    sn::Dbg g1;
+    sn::fp_mul_s(a, b, s, g1);
+    EXPECT(g1.mul == (sat ? 1u : 0u), "saturation count fp_mul_s(%d,%d,%u)", a, b, s);
+  }
   // exhaustive small operands, all shifts
   for (int32_t a = -300; a <= 300; ++a)
     for (int32_t b = -40; b <= 40; ++b)
