@@ -11,7 +11,6 @@
 // held in registers and flushed once per kernel; the counters are u32 sums,
 // so the totals do not depend on execution order.
 #pragma once
-#include <math.h>
 #include <stdint.h>
 
 #if defined(__HIPCC__) || defined(__HIP__)
@@ -55,27 +54,7 @@ SN_HD int32_t shr_rne_sat64(int64_t v, unsigned s, Dbg& g) {
   return ovf ? sat_of((int32_t)(v >> 32)) : (int32_t)q;
 }
 
-// RNE(a * b / 2^s) saturated to i32.  When |a * b| < 2^53 the product is
-// exact in IEEE double, scaling by 2^-s is exact, and rint() rounds half to
-// even: the result equals the integer form, in a few full-rate FP64
-// instructions instead of 64-bit integer multiplies and shifts.  Larger
-// products take the integer form.
-SN_HD double pow2_neg(unsigned s) {
-  union {
-    uint64_t u;
-    double d;
-  } v;
-  v.u = (uint64_t)(1023u - s) << 52;   // 2^-s, s in 1..31
-  return v.d;
-}
 SN_HD int32_t fp_mul_s(int32_t a, int32_t b, unsigned s, Dbg& g) {
-  const double p = (double)a * (double)b;
-  if (__builtin_expect(fabs(p) < 9007199254740992.0, 1)) {   // 2^53
-    const double q = rint(p * pow2_neg(s));
-    const bool ovf = !(q >= -2147483648.0 && q <= 2147483647.0);
-    g.mul += ovf ? 1u : 0u;
-    return ovf ? (p < 0.0 ? I32_MIN : I32_MAX) : (int32_t)q;
-  }
   return shr_rne_sat64((int64_t)a * (int64_t)b, s, g);
 }
 
