@@ -5,17 +5,20 @@ schedule; its result is the per-substep H_set ‖ H_slot stream.  Jobs arrive
 over time (a Poisson stream in the benchmark), wait in a queue, and are
 admitted into a free partition of a resident world as soon as one that fits
 them is free; admission is by device memory, each partition being a fixed
-slice of particle slots and grid cells.  The 2D and 3D worlds live on their own
-HIP streams, so a round enqueues both before waiting and their work overlaps
-on the device.  Every round advances each world's running jobs by the same
-number of substeps, at most `chunk` and never past any job's end, so a job
-never runs a substep it did not ask for; a finished job's stream is read back
-and its partition retired.  A scene's hashes depend only on its own slots, so
-a job's stream is the same alone and in any mix.
+slice of particle slots and grid cells.  The 2D and 3D worlds each have a
+worker thread and their own HIP stream, so the two advance independently and
+their work overlaps on the device.  A worker advances all running jobs of
+its world by rounds of `chunk` substeps (fewer when every running job needs
+fewer); a job that ends inside a round simply runs on to the round's end,
+and its result is the first `substeps` entries of its stream, which the
+extra substeps do not touch.  A finished job's stream is read back and its
+partition retired.  A scene's hashes depend only on its own slots, so a
+job's stream is the same alone and in any mix.
 """
 from __future__ import annotations
 
 import math
+import threading
 import time
 from collections import deque
 from dataclasses import dataclass, field
@@ -153,7 +156,7 @@ class Server:
     """Two resident slotted worlds (2D, 3D) and the admission / round loop."""
 
     def __init__(self, classes2d: Sequence[SlotClass] = DEFAULT_2D, classes3d: Sequence[SlotClass] = DEFAULT_3D,
-                 iters2d: int = 4, iters3d: int = 3, chunk: int = 32, stream_cap: int = 8192):
+                 iters2d: int = 4, iters3d: int = 3, chunk: int = 64, stream_cap: int = 8192):
         self.chunk = chunk
         self.caps = {}
         self.engines = {}
@@ -211,63 +214,72 @@ class Server:
         """Serve `jobs` (arrival times in seconds from the start).  With
         realtime=False every job is treated as already arrived."""
         for j in jobs:
-            if j.substeps > self.stream_cap:
+            if j.substeps + self.chunk > self.stream_cap:
                 raise ValueError(f"job {j.jid}: {j.substeps} substeps exceed the H ring ({self.stream_cap})")
             if not self.fits_anywhere(j):
                 raise ValueError(f"job {j.jid}: no partition can hold it")
-        pending = deque(sorted(jobs, key=lambda j: j.arrival))
-        queue: List[Job] = []
         fps = {j.jid: _footprint(j) for j in jobs}
+        pending = {k: deque(sorted((j for j in jobs if j.kind == k), key=lambda j: j.arrival))
+                   for k in self.engines}
         t0 = clock()
-        rounds = 0
-        while pending or queue or any(x is not None for v in self.busy.values() for x in v):
-            now = clock() - t0
-            while pending and (not realtime or pending[0].arrival <= now):
-                queue.append(pending.popleft())
-            changed = set()
-            for job in list(queue):
-                s = self._find_slot(job, fps[job.jid])
-                if s is None:
-                    continue
-                eng = self.engines[job.kind]
-                eng.load_slot(s, job.scene)
-                self.busy[job.kind][s] = job
-                job.slot, job.admitted, job.done = s, clock() - t0, 0
-                queue.remove(job)
-                changed.add(job.kind)
-            for kind in changed:
-                self._set_events(kind)
-            running = {k: [j for j in v if j is not None] for k, v in self.busy.items()}
-            if not any(running.values()):
-                if pending:
-                    time.sleep(max(0.0, min(pending[0].arrival - (clock() - t0), 0.005)))
-                continue
-            steps = {}
-            for kind, js in running.items():
-                if js:
-                    k = min(self.chunk, min(j.substeps - j.done for j in js))
-                    self.engines[kind].step(k)          # enqueue; both worlds then run together
-                    steps[kind] = k
-            for kind in steps:
-                self.engines[kind].sync()
-            rounds += 1
-            for kind, k in steps.items():
-                retired = False
-                for j in running[kind]:
-                    j.done += k
-                    if j.done == j.substeps:
-                        eng = self.engines[kind]
-                        j.stream = eng.hash_stream(j.slot) if kind == "2d" else \
-                            eng.hash_stream(j.slot, max_substeps=j.substeps)
-                        j.finished = clock() - t0
-                        self.busy[kind][j.slot] = None
-                        retired = True
-                        if on_finish:
-                            on_finish(j)
-                if retired:
-                    self.engines[kind].set_active(self._mask(kind))
-        wall = clock() - t0
-        return {"wall_s": wall, "rounds": rounds}
+        rounds = {k: 0 for k in self.engines}
+        errors = []
+
+        def worker(kind: str):
+            eng = self.engines[kind]
+            pend = pending[kind]
+            queue: List[Job] = []
+            try:
+                while pend or queue or any(x is not None for x in self.busy[kind]):
+                    now = clock() - t0
+                    while pend and (not realtime or pend[0].arrival <= now):
+                        queue.append(pend.popleft())
+                    admitted = False
+                    for job in list(queue):
+                        sl = self._find_slot(job, fps[job.jid])
+                        if sl is None:
+                            continue
+                        eng.load_slot(sl, job.scene)
+                        self.busy[kind][sl] = job
+                        job.slot, job.admitted, job.done = sl, clock() - t0, 0
+                        queue.remove(job)
+                        admitted = True
+                    if admitted:
+                        self._set_events(kind)
+                    running = [j for j in self.busy[kind] if j is not None]
+                    if not running:
+                        if pend:
+                            time.sleep(max(0.0, min(pend[0].arrival - (clock() - t0), 0.005)))
+                        continue
+                    k = min(self.chunk, max(j.substeps - j.done for j in running))
+                    eng.step(k)
+                    eng.sync()
+                    rounds[kind] += 1
+                    retired = False
+                    for j in running:
+                        j.done += k
+                        if j.done >= j.substeps:
+                            st = eng.hash_stream(j.slot) if kind == "2d" else \
+                                eng.hash_stream(j.slot, max_substeps=j.done)
+                            j.stream = st[: j.substeps]
+                            j.finished = clock() - t0
+                            self.busy[kind][j.slot] = None
+                            retired = True
+                            if on_finish:
+                                on_finish(j)
+                    if retired:
+                        eng.set_active(self._mask(kind))
+            except BaseException as ex:   # surfaced to the caller below
+                errors.append(ex)
+
+        threads = [threading.Thread(target=worker, args=(k,), daemon=True) for k in self.engines]
+        for t in threads:
+            t.start()
+        for t in threads:
+            t.join()
+        if errors:
+            raise errors[0]
+        return {"wall_s": clock() - t0, "rounds": sum(rounds.values()), "rounds_by_world": rounds}
 
 
 def summarize(jobs: Sequence[Job], wall: float) -> dict:
