@@ -339,13 +339,27 @@ def run_scene1m(args):
     dom = max(per_launch, key=lambda k: per_launch[k]["ms_per_substep"])
     d = per_launch[dom]
     achieved_all = by["total"] / (ms_per_step * 1e-3) / 1e9
+    # HBM bytes per launch of the dominant kernel as rocprofv3 counted them
+    # (FETCH_SIZE and WRITE_SIZE runs of this workload, each calibrated by a
+    # copy of known size), and its VALU counters, when that profile is present
+    traffic, counters = None, None
+    pj = os.path.join(ROOT, "profiles", "substep2d_1m_pmc.json")
+    if args.side == 1024 and os.path.exists(pj):
+        with open(pj) as f:
+            pm = json.load(f)["kernels"].get(KERNEL_OF.get(dom))
+        if pm:
+            traffic = int(pm["hbm_bytes"])
+            counters = {"valu_instructions_per_launch": pm["valu_insts"], "valu_busy_pct": round(pm["valu_busy_pct"], 1),
+                        "valu_lane_util_pct": round(pm["valu_lane_util_pct"], 1),
+                        "source": "rocprofv3 --pmc runs of this workload, profiles/substep2d_1m_pmc.md"}
     roofline = {
         "bound": "hbm",
         "achieved": d["achieved_GBs"],
         "peak": HBM_PEAK_GBS,
         "unit": "GB/s",
         "frac": round(d["achieved_GBs"] / HBM_PEAK_GBS, 6),
-        "traffic": None,
+        "traffic": traffic,
+        "kernel_counters": counters,
         "kernel": d["kernel"],
         "kernel_avg_launch_us": d["avg_launch_us"],
         "kernel_algorithmic_bytes_per_launch": d["algorithmic_bytes_per_launch"],

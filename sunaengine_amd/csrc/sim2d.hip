@@ -745,25 +745,30 @@ __device__ __forceinline__ bool nbr_chunk_staged(const Dev& D, uint16_t* lst, ui
       bool want = lane < qn && !(ci & INF_DEAD);   // nobody sees a corpse
       if constexpr (BND) want = want && cp.x >= bx0 && cp.x <= bx1 && cp.y >= by0 && cp.y <= by1;
       uint64_t todo = __ballot(want);
+      // the candidates' flag bits as wave masks: read per candidate with
+      // scalar shifts instead of a vector lane read
+      const uint64_t mBnd = __ballot(lane < qn && !(ci & INF_FLUID));
+      const uint64_t mSeal = __ballot(lane < qn && (ci & INF_SEAL));
+      const uint64_t mCong = __ballot(lane < qn && (ci & INF_CONGEAL));
+      const uint32_t selfj = p - q0;                 // lane index of the particle itself, if in this block
       while (todo) {
         const uint32_t j = (uint32_t)__builtin_ctzll(todo);
         todo &= todo - 1ull;
         const int32_t px = __builtin_amdgcn_readlane(cp.x, j);
         const int32_t py = __builtin_amdgcn_readlane(cp.y, j);
-        const uint32_t qi = (uint32_t)__builtin_amdgcn_readlane((int)ci, j);
         const int32_t qc = WIN ? __builtin_amdgcn_readlane(cc, j) : 0;
-        const uint32_t off = q0 - rlo[r] + j;
-        if (!live || q0 + j == p) continue;
-        if (WIN && (qc < cxk - 1 || qc > cxk + 1)) continue;
         const int2 d = pdelta<BND>(pi, make_int2(px, py), g);
         const int32_t idx = (BND && !WIN) ? lut_idx_near(d) : lut_idx<BND>(d, g);
-        if (idx < 0) continue;
-        bits |= qi & (INF_SEAL | INF_CONGEAL);
+        bool acc = live && j != selfj && idx >= 0;
+        if (WIN) acc = acc && qc >= cxk - 1 && qc <= cxk + 1;
+        if (!acc) continue;
+        bits |= (((mSeal >> j) & 1ull) ? INF_SEAL : 0u) | (((mCong >> j) & 1ull) ? INF_CONGEAL : 0u);
         if (cnt >= MAXNBR) {
           dropped++;
           continue;
         }
-        lst[cnt * 64u + lane] = (uint16_t)((r << 14) | ((qi & INF_FLUID) ? 0u : (1u << 13)) | off);
+        const uint32_t off = q0 - rlo[r] + j;
+        lst[cnt * 64u + lane] = (uint16_t)((r << 14) | (((mBnd >> j) & 1ull) ? (1u << 13) : 0u) | off);
         cnt++;
       }
     }
@@ -2095,8 +2100,15 @@ int sn_step(sn_engine* e, uint32_t count) {
     const uint32_t G = 8;
     if (!e->graph) {
       hipGraph_t g;
-      HIPCHECK(hipStreamBeginCapture(e->stream, hipStreamCaptureModeThreadLocal));
-      for (uint32_t k = 0; k < G; ++k) enqueue_substep(e);
+      HIPCHECK(hipStreamBeginCapture(e->stream, hipStreamCaptureModeRelaxed));
+      try {
+        for (uint32_t k = 0; k < G; ++k) enqueue_substep(e);
+      } catch (const DevError&) {
+        hipGraph_t broken = nullptr;
+        (void)hipStreamEndCapture(e->stream, &broken);   // leave the stream out of capture
+        if (broken) (void)hipGraphDestroy(broken);
+        throw;
+      }
       HIPCHECK(hipStreamEndCapture(e->stream, &g));
       HIPCHECK(hipGraphInstantiate(&e->graph, g, nullptr, nullptr, 0));
       HIPCHECK(hipGraphDestroy(g));

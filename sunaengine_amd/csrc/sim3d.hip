@@ -1171,8 +1171,15 @@ int sn3_step(sn3_engine* e, uint32_t count) {
     const uint32_t G = 8;
     if (!e->graph) {
       hipGraph_t g;
-      HIPCHECK3(hipStreamBeginCapture(e->stream, hipStreamCaptureModeThreadLocal));
-      for (uint32_t k = 0; k < G; ++k) enqueue_substep3(e, nullptr);
+      HIPCHECK3(hipStreamBeginCapture(e->stream, hipStreamCaptureModeRelaxed));
+      try {
+        for (uint32_t k = 0; k < G; ++k) enqueue_substep3(e, nullptr);
+      } catch (const DevError3&) {
+        hipGraph_t broken = nullptr;
+        (void)hipStreamEndCapture(e->stream, &broken);   // leave the stream out of capture
+        if (broken) (void)hipGraphDestroy(broken);
+        throw;
+      }
       HIPCHECK3(hipStreamEndCapture(e->stream, &g));
       HIPCHECK3(hipGraphInstantiate(&e->graph, g, nullptr, nullptr, 0));
       HIPCHECK3(hipGraphDestroy(g));
