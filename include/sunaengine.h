@@ -78,6 +78,10 @@ int sn_step(sn_engine* e, uint32_t count);
 /* Wait for enqueued work; returns SN_E_UNSUPPORTED if a scene left the
  * supported material paths (the run is then void and must be reloaded). */
 int sn_sync(sn_engine* e);
+/* Non-blocking: 0 when all enqueued work has finished, 1 while some is still
+ * running (no counterpart in the reference; for host loops that keep several
+ * worlds busy from one thread). */
+int sn_query(sn_engine* e);
 
 /* digest() (src/engine.js:4453-4466) of scene `scene`: acc[0..3] = H_set,
  * acc[4..7] = H_slot, computed from the current state. */
@@ -178,6 +182,7 @@ int sn3_set_inputs(sn3_engine* e, int32_t stir_amp, int32_t stir_phase, uint32_t
                    int32_t ball_reach);
 int sn3_step(sn3_engine* e, uint32_t count);
 int sn3_sync(sn3_engine* e);
+int sn3_query(sn3_engine* e);   /* as sn_query */
 int sn3_read_hash(sn3_engine* e, uint32_t scene, uint32_t acc_out[8]);
 int sn3_read_stream(sn3_engine* e, uint32_t scene, uint32_t* out, size_t max_substeps, size_t* n_out);
 int sn3_reset_stream(sn3_engine* e);

@@ -2139,6 +2139,14 @@ int sn_sync(sn_engine* e) {
   }
 }
 
+int sn_query(sn_engine* e) {
+  if (!e) return SN_E_ARG;
+  hipError_t q = hipStreamQuery(e->stream);
+  if (q == hipSuccess) return SN_OK;
+  if (q == hipErrorNotReady) return 1;
+  return fail(e, SN_E_DEVICE, std::string("hipStreamQuery: ") + hipGetErrorString(q));
+}
+
 int sn_profile_step(sn_engine* e, uint32_t count, float out_ms[SN_PROF_SLOTS], uint32_t out_calls[SN_PROF_SLOTS]) {
   if (!e || !out_ms || !out_calls) return SN_E_ARG;
   if (!e->loaded) return fail(e, SN_E_STATE, "[aether] step() called before loadScene()");

@@ -1198,6 +1198,14 @@ int sn3_step(sn3_engine* e, uint32_t count) {
   return SN_OK;
 }
 
+int sn3_query(sn3_engine* e) {
+  if (!e) return SN_E_ARG;
+  hipError_t q = hipStreamQuery(e->stream);
+  if (q == hipSuccess) return SN_OK;
+  if (q == hipErrorNotReady) return 1;
+  return fail3(e, SN_E_DEVICE, std::string("hipStreamQuery: ") + hipGetErrorString(q));
+}
+
 int sn3_sync(sn3_engine* e) {
   if (!e) return SN_E_ARG;
   try {

@@ -138,6 +138,14 @@ class Engine:
         self._check(native.lib().sn_step(self._h, int(count)))
         return self.substep_count()
 
+    def busy(self) -> bool:
+        """True while enqueued work is still running (does not wait)."""
+        rc = native.lib().sn_query(self._h)
+        if rc == 1:
+            return True
+        self._check(rc)
+        return False
+
     def sync(self):
         self._check(native.lib().sn_sync(self._h))
 

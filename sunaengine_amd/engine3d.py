@@ -80,6 +80,14 @@ class Engine3D:
     def step(self, count: int = 1):
         self._check(native.lib().sn3_step(self._h, int(count)))
 
+    def busy(self) -> bool:
+        """True while enqueued work is still running (does not wait)."""
+        rc = native.lib().sn3_query(self._h)
+        if rc == 1:
+            return True
+        self._check(rc)
+        return False
+
     def sync(self):
         self._check(native.lib().sn3_sync(self._h))
 

@@ -21,7 +21,7 @@ EXPORTS = [
     "sn_sync", "sn_read_hash", "sn_read_hash_stream", "sn_reset_stream", "sn_read_chain",
     "sn_read_state", "sn_read_derived", "sn_debug_counters", "sn_substep_count",
     "sn_apply_impulse", "sn_spawn_fluid", "sn_set_events", "sn_set_active", "sn_profile_step",
-    "sn_reserve_slots", "sn_load_slot",
+    "sn_reserve_slots", "sn_load_slot", "sn_query", "sn3_query",
     "sn3_create", "sn3_destroy", "sn3_last_error", "sn3_load", "sn3_set_inputs", "sn3_step",
     "sn3_sync", "sn3_read_hash", "sn3_read_stream", "sn3_reset_stream", "sn3_read_chain",
     "sn3_read_state", "sn3_debug_counters", "sn3_fnv_digest", "sn3_set_events", "sn3_set_active",
@@ -106,6 +106,8 @@ def lib():
         "sn_set_events": (C.c_int, [P, P, u32]),
         "sn_set_active": (C.c_int, [P, P, u32]),
         "sn_reserve_slots": (C.c_int, [P, u32, P, P]),
+        "sn_query": (C.c_int, [P]),
+        "sn3_query": (C.c_int, [P]),
         "sn_load_slot": (C.c_int, [P, u32, P]),
         "sn_profile_step": (C.c_int, [P, u32, P, P]),
         "sn3_create": (C.c_int, [C.POINTER(sn_opts), P, u32, C.POINTER(P)]),

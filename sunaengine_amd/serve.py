@@ -5,10 +5,11 @@ schedule; its result is the per-substep H_set ‖ H_slot stream.  Jobs arrive
 over time (a Poisson stream in the benchmark), wait in a queue, and are
 admitted into a free partition of a resident world as soon as one that fits
 them is free; admission is by device memory, each partition being a fixed
-slice of particle slots and grid cells.  The 2D and 3D worlds each have a
-worker thread and their own HIP stream, so the two advance independently and
-their work overlaps on the device.  A worker advances all running jobs of
-its world by rounds of `chunk` substeps (fewer when every running job needs
+slice of particle slots and grid cells.  The 2D and 3D worlds each have their
+own HIP stream, so the two advance independently and their work overlaps on
+the device; one host thread enqueues each world's next round as soon as a
+non-blocking query shows its last one done.  A world advances all running
+jobs by rounds of `chunk` substeps (fewer when every running job needs
 fewer); a job that ends inside a round simply runs on to the round's end,
 and its result is the first `substeps` entries of its stream, which the
 extra substeps do not touch.  A finished job's stream is read back and its
@@ -18,7 +19,6 @@ job's stream is the same alone and in any mix.
 from __future__ import annotations
 
 import math
-import threading
 import time
 from collections import deque
 from dataclasses import dataclass, field
@@ -210,9 +210,15 @@ class Server:
         return [1 if j is not None else 0 for j in self.busy[kind]]
 
     def run(self, jobs: Sequence[Job], clock: Callable[[], float] = time.perf_counter,
-            on_finish: Optional[Callable[[Job], None]] = None, realtime: bool = True) -> dict:
+            on_finish: Optional[Callable[[Job], None]] = None, realtime: bool = True,
+            poll_s: float = 0.0002) -> dict:
         """Serve `jobs` (arrival times in seconds from the start).  With
-        realtime=False every job is treated as already arrived."""
+        realtime=False every job is treated as already arrived.
+
+        One host thread drives both worlds: each world has at most one round
+        in flight on its own stream, and the loop only enqueues, polls
+        (non-blocking) and reads back, so the 2D and 3D rounds overlap on the
+        device without concurrent host calls into the runtime."""
         for j in jobs:
             if j.substeps + self.chunk > self.stream_cap:
                 raise ValueError(f"job {j.jid}: {j.substeps} substeps exceed the H ring ({self.stream_cap})")
@@ -221,64 +227,76 @@ class Server:
         fps = {j.jid: _footprint(j) for j in jobs}
         pending = {k: deque(sorted((j for j in jobs if j.kind == k), key=lambda j: j.arrival))
                    for k in self.engines}
-        t0 = clock()
+        queue = {k: [] for k in self.engines}
+        inflight = {k: 0 for k in self.engines}      # substeps of the round on the device, 0 = idle
         rounds = {k: 0 for k in self.engines}
-        errors = []
+        t0 = clock()
 
-        def worker(kind: str):
+        def finish_round(kind: str):
             eng = self.engines[kind]
-            pend = pending[kind]
-            queue: List[Job] = []
-            try:
-                while pend or queue or any(x is not None for x in self.busy[kind]):
-                    now = clock() - t0
-                    while pend and (not realtime or pend[0].arrival <= now):
-                        queue.append(pend.popleft())
-                    admitted = False
-                    for job in list(queue):
-                        sl = self._find_slot(job, fps[job.jid])
-                        if sl is None:
-                            continue
-                        eng.load_slot(sl, job.scene)
-                        self.busy[kind][sl] = job
-                        job.slot, job.admitted, job.done = sl, clock() - t0, 0
-                        queue.remove(job)
-                        admitted = True
-                    if admitted:
-                        self._set_events(kind)
-                    running = [j for j in self.busy[kind] if j is not None]
-                    if not running:
-                        if pend:
-                            time.sleep(max(0.0, min(pend[0].arrival - (clock() - t0), 0.005)))
-                        continue
-                    k = min(self.chunk, max(j.substeps - j.done for j in running))
-                    eng.step(k)
-                    eng.sync()
-                    rounds[kind] += 1
-                    retired = False
-                    for j in running:
-                        j.done += k
-                        if j.done >= j.substeps:
-                            st = eng.hash_stream(j.slot) if kind == "2d" else \
-                                eng.hash_stream(j.slot, max_substeps=j.done)
-                            j.stream = st[: j.substeps]
-                            j.finished = clock() - t0
-                            self.busy[kind][j.slot] = None
-                            retired = True
-                            if on_finish:
-                                on_finish(j)
-                    if retired:
-                        eng.set_active(self._mask(kind))
-            except BaseException as ex:   # surfaced to the caller below
-                errors.append(ex)
+            eng.sync()
+            k = inflight[kind]
+            inflight[kind] = 0
+            rounds[kind] += 1
+            retired = False
+            for j in self.busy[kind]:
+                if j is None:
+                    continue
+                j.done += k
+                if j.done >= j.substeps:
+                    st = eng.hash_stream(j.slot) if kind == "2d" else \
+                        eng.hash_stream(j.slot, max_substeps=j.done)
+                    j.stream = st[: j.substeps]
+                    j.finished = clock() - t0
+                    self.busy[kind][j.slot] = None
+                    retired = True
+                    if on_finish:
+                        on_finish(j)
+            if retired:
+                eng.set_active(self._mask(kind))
 
-        threads = [threading.Thread(target=worker, args=(k,), daemon=True) for k in self.engines]
-        for t in threads:
-            t.start()
-        for t in threads:
-            t.join()
-        if errors:
-            raise errors[0]
+        def admit_and_launch(kind: str):
+            eng = self.engines[kind]
+            admitted = False
+            for job in list(queue[kind]):
+                sl = self._find_slot(job, fps[job.jid])
+                if sl is None:
+                    continue
+                eng.load_slot(sl, job.scene)
+                self.busy[kind][sl] = job
+                job.slot, job.admitted, job.done = sl, clock() - t0, 0
+                queue[kind].remove(job)
+                admitted = True
+            if admitted:
+                self._set_events(kind)
+            running = [j for j in self.busy[kind] if j is not None]
+            if running:
+                k = min(self.chunk, max(j.substeps - j.done for j in running))
+                eng.step(k)                 # enqueued; finish_round reads it back
+                inflight[kind] = k
+
+        while True:
+            now = clock() - t0
+            for kind in self.engines:
+                pend = pending[kind]
+                while pend and (not realtime or pend[0].arrival <= now):
+                    queue[kind].append(pend.popleft())
+            progressed = False
+            for kind, eng in self.engines.items():
+                if inflight[kind]:
+                    if eng.busy():
+                        continue
+                    finish_round(kind)
+                    progressed = True
+                admit_and_launch(kind)
+                progressed = progressed or bool(inflight[kind])
+            if not any(inflight.values()) and not any(queue.values()):
+                if not any(pending.values()):
+                    break
+                nxt = min(p[0].arrival for p in pending.values() if p)
+                time.sleep(max(0.0, min(nxt - (clock() - t0), 0.005)))
+            elif not progressed:
+                time.sleep(poll_s)
         return {"wall_s": clock() - t0, "rounds": sum(rounds.values()), "rounds_by_world": rounds}
 
 
