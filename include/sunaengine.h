@@ -44,6 +44,9 @@ typedef struct sn_opts {
   uint32_t iters;         /* density iterations per substep, 1..16 (ITERS = 4) */
   uint32_t stream_cap;    /* per-scene H ring capacity in substeps (0 = 4096) */
   uint32_t device;        /* HIP device ordinal */
+  uint32_t no_debug;      /* 1 = createEngine({debug: false}) (src/engine.js:2791,2884-2894):
+                             debug counters stay zero, and sweeps skip work that
+                             only feeds them; hashes and state are unchanged */
 } sn_opts;
 
 /* One scene as loadScene writes it (src/engine.js:3888-3937): n records of 8

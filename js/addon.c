@@ -71,14 +71,18 @@ static napi_value num(napi_env env, double x) {
   return v;
 }
 
-/* create(maxParticles, maxCells, maxScenes, iters, streamCap, device, luts:Int32Array, mats:Int32Array) */
+/* create(maxParticles, maxCells, maxScenes, iters, streamCap, device, luts:Int32Array, mats:Int32Array
+ *        [, debug = 1]) */
 static napi_value js_create(napi_env env, napi_callback_info info) {
-  napi_value a[8];
-  if (get_args(env, info, 8, a) < 8) {
+  napi_value a[9];
+  const int argc = get_args(env, info, 9, a);
+  if (argc < 8) {
     napi_throw_error(env, NULL, "create: 8 arguments expected");
     return NULL;
   }
   sn_opts o;
+  memset(&o, 0, sizeof(o));
+  o.no_debug = (argc > 8 && u32(env, a[8]) == 0) ? 1u : 0u;
   o.max_particles = u32(env, a[0]);
   o.max_cells = u32(env, a[1]);
   o.max_scenes = u32(env, a[2]);

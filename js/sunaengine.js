@@ -238,7 +238,7 @@ class Engine {
     this.iters = opts.iters;
     const img = Int32Array.from(mats().image);
     this._h = native().create(opts.maxParticles, opts.maxCells, opts.maxScenes || 1, opts.iters,
-      opts.streamCap || 4096, opts.device || 0, loadLuts(), img);
+      opts.streamCap || 4096, opts.device || 0, loadLuts(), img, opts.debug === false ? 0 : 1);
     this.n = 0; this.nFluid = 0; this._scene = null; this.substepCount = 0;
   }
   get matCount() { return mats().rows.length; }
@@ -329,7 +329,7 @@ async function createEngine(opts) {
   const iters = opts.iters !== undefined ? opts.iters : ITERS;
   assert(Number.isInteger(iters) && iters >= 1 && iters <= 16, `iters = ${iters} must be an integer in 1..16`);
   return new Engine({ maxParticles: opts.maxParticles || 20000, maxCells: opts.maxCells || 65536, iters,
-    maxScenes: 1, streamCap: opts.streamCap || 4096, device: opts.device || 0 });
+    maxScenes: 1, streamCap: opts.streamCap || 4096, device: opts.device || 0, debug: opts.debug !== false });
 }
 
 module.exports = {

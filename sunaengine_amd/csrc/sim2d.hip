@@ -184,9 +184,11 @@ struct Dev {
   uint32_t* hpart;        // per-block partial sums [block][HPART_WORDS]
   const uint32_t* cellScene;  // global cell -> scene, INACTIVE if its scene is inactive
   const uint2* tileAll;   // every neighbour tile of the active scenes {first cell, ncell | scene << 8}
-  uint2* dlist[5];        // this substep's occupied cells per colour [0..3], occupied tiles [4]
-  uint32_t* listCount;    // [5] entries of dlist (cleared by predict)
+  uint2* dlist[6];        // this substep's occupied cells per colour [0..3], occupied tiles [4],
+                          // chunks of heavy cells for the canonical rank [5]
+  uint32_t* listCount;    // [6] entries of dlist (cleared by predict)
   uint32_t nTiles;
+  uint32_t dbgOn;         // 0: the reference's debug:false build, counters stay zero
   const int4* lut4;       // {W, G, C, S} per LUT bin
   const int32_t* lutA;    // adhesion table
   const int32_t* mats;
@@ -302,6 +304,7 @@ __device__ __forceinline__ uint32_t wave_sum(uint32_t v) {
 // is uniform in scene (the counters are order-free u32 sums).  Must be called
 // with every lane of the wave active.
 __device__ __forceinline__ void flush_dbg(const Dev& D, uint32_t s, const Dbg& g, uint32_t trunc) {
+  if (!D.dbgOn) return;
   uint32_t v[5] = {g.mul, g.div, g.div0, g.add, trunc};
   const uint32_t slot[5] = {0, 1, 2, 5, 6};
   const uint32_t s0 = (uint32_t)__builtin_amdgcn_readfirstlane((int)s);
@@ -336,7 +339,7 @@ __device__ __forceinline__ uint32_t xcd_block() {
 // which replaces the reference's second atomic pass in `scatter`.
 __global__ __launch_bounds__(256) void k_predict_count(Dev D) {
   uint32_t p = xcd_block() * 256u + threadIdx.x;
-  if (blockIdx.x == 0 && threadIdx.x < 5u) D.listCount[threadIdx.x] = 0u;  // rebuilt after the scan
+  if (blockIdx.x == 0 && threadIdx.x < 6u) D.listCount[threadIdx.x] = 0u;  // rebuilt after the scan
   uint32_t s = p < D.N ? D.sidOf[p] : INACTIVE;
   Dbg g;
   uint32_t cell = INACTIVE;
@@ -406,20 +409,18 @@ __global__ __launch_bounds__(256) void k_scatter(Dev D) {
 // canonicalize (src/sim.wgsl:1155-1168): rank = #{ids in my cell below mine};
 // the particle's new position is cellStart + rank.  The whole persistent
 // record moves there, together with pred and the cell.
-__global__ __launch_bounds__(256) void k_canon_gather(Dev D) {
-  uint32_t t = xcd_block() * 256u + threadIdx.x;
-  if (t >= D.N) return;
-  uint32_t s = D.sidOf[t];
-  if (s == INACTIVE) return;
-  const SceneP& P = D.sp[s];
-  uint32_t me = D.bucket[t];
-  uint32_t myid = D.bucketId[t];
-  uint32_t c = D.cell0[me];
-  uint32_t lo = (uint32_t)((int32_t)D.cellStart[c] + P.posShift);
-  uint32_t hi = (uint32_t)((int32_t)D.cellStart[c + 1] + P.posShift);
-  uint32_t rank = 0;
-  for (uint32_t u = lo; u < hi; ++u) rank += (D.bucketId[u] < myid) ? 1u : 0u;
-  uint32_t q = lo + rank;
+//
+// A cell of up to CANON_HEAVY particles is ranked by each thread walking the
+// cell.  Walls and collisions can pack thousands into one cell, where that
+// walk is quadratic on a handful of waves; such cells are listed per substep
+// (dlist[5], CANON_ITEM particles per item) and k_canon_heavy ranks them with
+// the cell's ids streamed through LDS, spread over the whole chip.
+constexpr uint32_t CANON_HEAVY = 256;
+constexpr uint32_t CANON_ITEM = 256;
+constexpr uint32_t CANON_TILE = 2048;
+
+__device__ __forceinline__ void canon_place(const Dev& D, const SceneP& P, uint32_t me, uint32_t myid, uint32_t c,
+                                            uint32_t q) {
   int4 pt = D.pt[me];
   D.psY[q] = D.ps[me];
   D.ptY[q] = pt;
@@ -432,6 +433,58 @@ __global__ __launch_bounds__(256) void k_canon_gather(Dev D) {
   D.info[q] = ((fl & FLAG_DEAD) ? INF_DEAD : 0u) | (fluid ? INF_FLUID : 0u) |
               ((fluid && (fl & FLAG_SEAL)) ? INF_SEAL : 0u) |
               ((fluid && (fl & FLAG_CONGEAL)) ? INF_CONGEAL : 0u);
+}
+
+__global__ __launch_bounds__(256) void k_canon_gather(Dev D) {
+  uint32_t t = xcd_block() * 256u + threadIdx.x;
+  if (t >= D.N) return;
+  uint32_t s = D.sidOf[t];
+  if (s == INACTIVE) return;
+  const SceneP& P = D.sp[s];
+  uint32_t me = D.bucket[t];
+  uint32_t myid = D.bucketId[t];
+  uint32_t c = D.cell0[me];
+  uint32_t lo = (uint32_t)((int32_t)D.cellStart[c] + P.posShift);
+  uint32_t hi = (uint32_t)((int32_t)D.cellStart[c + 1] + P.posShift);
+  if (hi - lo > CANON_HEAVY) return;   // k_canon_heavy
+  uint32_t rank = 0;
+  for (uint32_t u = lo; u < hi; ++u) rank += (D.bucketId[u] < myid) ? 1u : 0u;
+  canon_place(D, P, me, myid, c, lo + rank);
+}
+
+// Persistent: block b takes items b, b + grid, ...; an item is CANON_ITEM
+// consecutive bucket positions of one heavy cell.  Ids are distinct within a
+// scene and below 2^32 - 1, so the all-ones padding of a short tile never
+// counts as smaller.
+__global__ __launch_bounds__(256) void k_canon_heavy(Dev D) {
+  __shared__ uint4 ids[CANON_TILE / 4];
+  const uint32_t n = (uint32_t)__builtin_amdgcn_readfirstlane((int)D.listCount[5]);
+  for (uint32_t w = blockIdx.x; w < n; w += gridDim.x) {
+    const uint2 item = D.dlist[5][w];
+    const uint32_t c = (uint32_t)__builtin_amdgcn_readfirstlane((int)item.x);
+    const uint32_t chunk = (uint32_t)__builtin_amdgcn_readfirstlane((int)item.y);
+    const uint32_t s = D.cellScene[c];
+    const SceneP& P = D.sp[s];
+    const uint32_t lo = (uint32_t)((int32_t)D.cellStart[c] + P.posShift);
+    const uint32_t hi = (uint32_t)((int32_t)D.cellStart[c + 1] + P.posShift);
+    const uint32_t t = lo + chunk * CANON_ITEM + threadIdx.x;
+    const bool valid = t < hi;
+    const uint32_t myid = valid ? D.bucketId[t] : 0u;
+    uint32_t rank = 0;
+    for (uint32_t base = lo; base < hi; base += CANON_TILE) {
+      __syncthreads();   // the previous tile is no longer read
+      uint32_t* flat = reinterpret_cast<uint32_t*>(ids);
+      for (uint32_t i = threadIdx.x; i < CANON_TILE; i += 256u)
+        flat[i] = base + i < hi ? D.bucketId[base + i] : 0xffffffffu;
+      __syncthreads();
+      const uint32_t nq = (min(CANON_TILE, hi - base) + 3u) / 4u;
+      for (uint32_t i = 0; i < nq; ++i) {
+        const uint4 v = ids[i];
+        rank += (v.x < myid ? 1u : 0u) + (v.y < myid ? 1u : 0u) + (v.z < myid ? 1u : 0u) + (v.w < myid ? 1u : 0u);
+      }
+    }
+    if (valid) canon_place(D, P, D.bucket[t], myid, c, lo + rank);
+  }
 }
 
 // ===================================================== buildNbr + solveA(it 0)
@@ -524,6 +577,14 @@ __global__ __launch_bounds__(256) void k_build_lists(Dev D) {
       nch = (cnt + TILE_CHUNK - 1u) / TILE_CHUNK;
     }
     append_chunks(nch, &D.listCount[4], D.dlist[4], t);
+  }
+  {
+    uint32_t nch = 0;
+    if (t < D.C && D.cellScene[t] != INACTIVE) {
+      const uint32_t cnt = D.cellStart[t + 1] - D.cellStart[t];
+      if (cnt > CANON_HEAVY) nch = (cnt + CANON_ITEM - 1u) / CANON_ITEM;
+    }
+    append_chunks(nch, &D.listCount[5], D.dlist[5], t);
   }
 }
 
@@ -642,6 +703,14 @@ __device__ __forceinline__ void nbr_chunk(const Dev& D, uint32_t c0, uint32_t nc
         bool want = lane < qn && !(ci & INF_DEAD);   // nobody sees a corpse
         if constexpr (BND) want = want && cp.x >= bx0 && cp.x <= bx1 && cp.y >= by0 && cp.y <= by1;
         uint64_t todo = __ballot(want);
+        if (!D.dbgOn) {
+          // every list of the wave full: without debug counters only a
+          // candidate carrying SEAL or CONGEAL can still change a result
+          bool full = true;
+#pragma unroll
+          for (int k = 0; k < PPL; ++k) full = full && (!live[k] || cnt[k] >= MAXNBR);
+          if (__all(full)) todo &= __ballot(lane < qn && (ci & (INF_SEAL | INF_CONGEAL)));
+        }
         while (todo) {
           const uint32_t j = (uint32_t)__builtin_ctzll(todo);
           todo &= todo - 1ull;
@@ -676,7 +745,7 @@ __device__ __forceinline__ void nbr_chunk(const Dev& D, uint32_t c0, uint32_t nc
       D.nbrN[pk[k]] = live[k] ? (cnt[k] | ((bits[k] & INF_SEAL) ? NBR_SEAL_BIT : 0u) |
                                  ((bits[k] & INF_CONGEAL) ? NBR_CONGEAL_BIT : 0u))
                               : 0u;
-      D.drop[pk[k]] = dropped[k];
+      D.drop[pk[k]] = D.dbgOn ? dropped[k] : 0u;
       if (inf[k] & INF_FLUID) solveA_finish(D, pk[k], acc[k], MAT(D, (uint32_t)D.ptY[pk[k]].x)[M_EPS], g);
     }
   }
@@ -751,6 +820,7 @@ __device__ __forceinline__ bool nbr_chunk_staged(const Dev& D, uint16_t* lst, ui
       const uint64_t mSeal = __ballot(lane < qn && (ci & INF_SEAL));
       const uint64_t mCong = __ballot(lane < qn && (ci & INF_CONGEAL));
       const uint32_t selfj = p - q0;                 // lane index of the particle itself, if in this block
+      if (!D.dbgOn && __all(!live || cnt >= MAXNBR)) todo &= mSeal | mCong;   // as in nbr_chunk
       while (todo) {
         const uint32_t j = (uint32_t)__builtin_ctzll(todo);
         todo &= todo - 1ull;
@@ -806,7 +876,7 @@ __device__ __forceinline__ bool nbr_chunk_staged(const Dev& D, uint16_t* lst, ui
   if (valid) {
     D.nbrN[p] = live ? (cnt | ((bits & INF_SEAL) ? NBR_SEAL_BIT : 0u) | ((bits & INF_CONGEAL) ? NBR_CONGEAL_BIT : 0u))
                      : 0u;
-    D.drop[p] = dropped;
+    D.drop[p] = D.dbgOn ? dropped : 0u;
     if (inf & INF_FLUID) solveA_finish(D, p, acc, MAT(D, (uint32_t)D.ptY[p].x)[M_EPS], g);
   }
   return true;
@@ -1486,7 +1556,7 @@ struct sn_engine {
   uint8_t* dActive = nullptr;
   std::vector<uint2> tiles;          // every neighbour tile of the active scenes
   uint32_t tileG = 1;
-  uint32_t nbrBlocks = 0, colourBlocks = 0, applyBlocks = 0;  // persistent grid sizes
+  uint32_t nbrBlocks = 0, colourBlocks = 0, applyBlocks = 0, canonBlocks = 0;  // persistent grid sizes
   uint32_t S = 0, N = 0, C = 0;
   uint32_t maxEventsPerSubstep = 0;
   Event* dEvents = nullptr;
@@ -1596,12 +1666,13 @@ void enqueue_substep(sn_engine* e) {
     hipLaunchKernelGGL(k_scatter, dim3(gN), dim3(256), 0, st, D);
   }
   {
-    ProfScope ps(e, PC_CANON);
-    hipLaunchKernelGGL(k_canon_gather, dim3(gN), dim3(256), 0, st, D);
-  }
-  {
     ProfScope ps(e, PC_LISTS);
     hipLaunchKernelGGL(k_build_lists, dim3(div_up(std::max(std::max(D.C, D.nTiles), 1u), 256)), dim3(256), 0, st, D);
+  }
+  {
+    ProfScope ps(e, PC_CANON);
+    hipLaunchKernelGGL(k_canon_gather, dim3(gN), dim3(256), 0, st, D);
+    hipLaunchKernelGGL(k_canon_heavy, dim3(e->canonBlocks), dim3(256), 0, st, D);
   }
   {
     ProfScope ps(e, PC_NBR);
@@ -1720,6 +1791,7 @@ int sn_create(const sn_opts* opts, const int32_t* luts, const int32_t* mats, sn_
     HIPCHECK(hipStreamCreateWithFlags(&e->stream, hipStreamNonBlocking));
     const uint32_t NC = e->opts.max_particles, CC = e->opts.max_cells, SC = e->opts.max_scenes;
     Dev& D = e->D;
+    D.dbgOn = e->opts.no_debug ? 0u : 1u;
     D.ps = dalloc<int4>(e, NC);
     D.pt = dalloc<int4>(e, NC);
     D.id = dalloc<uint32_t>(e, NC);
@@ -1777,7 +1849,7 @@ int sn_create(const sn_opts* opts, const int32_t* luts, const int32_t* mats, sn_
     D.cellScene = dalloc<uint32_t>(e, CC);
     D.tileAll = dalloc<uint2>(e, CC);
     // a list holds at most one item per cell (tile) plus one per full chunk
-    for (uint32_t c = 0; c < 5; ++c) D.dlist[c] = dalloc<uint2>(e, (size_t)CC + NC / CELL_CHUNK + 64);
+    for (uint32_t c = 0; c < 6; ++c) D.dlist[c] = dalloc<uint2>(e, (size_t)CC + NC / CELL_CHUNK + 64);
     D.listCount = dalloc<uint32_t>(e, 8);
     // persistent grids: as many workgroups as can be resident, a multiple of 8 (XCDs)
     {
@@ -1791,6 +1863,7 @@ int sn_create(const sn_opts* opts, const int32_t* luts, const int32_t* mats, sn_
       e->nbrBlocks = std::max(8u, (cus * (uint32_t)std::max(bN, 1)) & ~7u);
       e->colourBlocks = std::max(8u, (cus * (uint32_t)std::max(bC, 1)) & ~7u);
       e->applyBlocks = std::max(8u, (cus * (uint32_t)std::max(bA, 1)) & ~7u);
+      e->canonBlocks = 2u * cus;
     }
     HIPCHECK(hipMemset(D.cellCount, 0, sizeof(uint32_t) * CC));
   } catch (const DevError& d) {

@@ -116,7 +116,9 @@ struct Dev3 {
   uint32_t* hpart;           // per-block partial sums [block][HPART3]
   const uint32_t* cellScene; // global cell -> scene, INACTIVE if its scene is inactive
   uint2* dlist;              // this substep's occupied cell chunks {cell, chunk}
-  uint32_t* listCount;       // [1] entries of dlist (cleared by predict)
+  uint2* dheavy;             // chunks of heavy cells for the canonical rank {cell, chunk}
+  uint32_t* listCount;       // [0] entries of dlist, [1] of dheavy (cleared by predict)
+  uint32_t dbgOn;            // 0: the reference's debug:false build, counters stay zero
   const int32_t* luts;
   const Scene3P* sp;
   Scene3D* sd;
@@ -187,6 +189,7 @@ __device__ __forceinline__ uint32_t wave_sum3(uint32_t v) {
 // Debug counters: summed per wave, one atomic per nonzero counter when the
 // wave is uniform in scene.  Must be called with every lane active.
 __device__ __forceinline__ void flush_dbg3(const Dev3& D, uint32_t s, const Dbg& g, uint32_t trunc) {
+  if (!D.dbgOn) return;
   uint32_t v[5] = {g.mul, g.div, g.div0, g.add, trunc};
   const uint32_t slot[5] = {0, 1, 2, 5, 6};
   const uint32_t s0 = (uint32_t)__builtin_amdgcn_readfirstlane((int)s);
@@ -227,7 +230,7 @@ __device__ __forceinline__ void predict3(const Dev3& D, uint32_t p, uint32_t s, 
 }
 __global__ __launch_bounds__(256) void k3_predict_count(Dev3 D) {
   uint32_t p = blockIdx.x * 256u + threadIdx.x;
-  if (blockIdx.x == 0 && threadIdx.x == 0) D.listCount[0] = 0u;  // rebuilt after the scan
+  if (blockIdx.x == 0 && threadIdx.x < 2u) D.listCount[threadIdx.x] = 0u;  // rebuilt after the scan
   uint32_t s = p < D.N ? D.sidOf[p] : INACTIVE;
   Dbg g;
   if (s != INACTIVE) predict3(D, p, s, g);
@@ -244,6 +247,23 @@ __global__ __launch_bounds__(256) void k3_scatter(Dev3 D) {
   D.bucketId[q] = D.id[p];
 }
 
+// canonicalize: rank = #{ids in my cell below mine}.  Cells of more than
+// CANON3_HEAVY particles (packed by walls or collisions) are left to
+// k3_canon_heavy, which streams the cell's ids through LDS over many
+// workgroups instead of a quadratic walk per thread.
+constexpr uint32_t CANON3_HEAVY = 256;
+constexpr uint32_t CANON3_ITEM = 256;
+constexpr uint32_t CANON3_TILE = 2048;
+
+__device__ __forceinline__ void canon_place3(const Dev3& D, uint32_t me, uint32_t myid, uint32_t c, uint32_t q) {
+  int4 a = D.ps[me];
+  D.psY[q] = a;
+  D.idY[q] = myid;
+  D.matY[q] = (uint32_t)a.w;
+  D.prA[q] = D.pr0[me];
+  D.cellY[q] = c;
+}
+
 __global__ __launch_bounds__(256) void k3_canon_gather(Dev3 D) {
   uint32_t t = blockIdx.x * 256u + threadIdx.x;
   if (t >= D.N) return;
@@ -253,15 +273,42 @@ __global__ __launch_bounds__(256) void k3_canon_gather(Dev3 D) {
   uint32_t me = D.bucket[t], myid = D.bucketId[t];
   uint32_t c = D.cell0[me];
   uint32_t lo = (uint32_t)((int32_t)D.cellStart[c] + sh), hi = (uint32_t)((int32_t)D.cellStart[c + 1] + sh);
+  if (hi - lo > CANON3_HEAVY) return;   // k3_canon_heavy
   uint32_t rank = 0;
   for (uint32_t u = lo; u < hi; ++u) rank += (D.bucketId[u] < myid) ? 1u : 0u;
-  uint32_t q = lo + rank;
-  int4 a = D.ps[me];
-  D.psY[q] = a;
-  D.idY[q] = myid;
-  D.matY[q] = (uint32_t)a.w;
-  D.prA[q] = D.pr0[me];
-  D.cellY[q] = c;
+  canon_place3(D, me, myid, c, lo + rank);
+}
+
+// Persistent: block b takes heavy items b, b + grid, ...  Ids are distinct
+// within a scene and below 2^32 - 1, so the all-ones padding never counts.
+__global__ __launch_bounds__(256) void k3_canon_heavy(Dev3 D) {
+  __shared__ uint4 ids[CANON3_TILE / 4];
+  const uint32_t n = (uint32_t)__builtin_amdgcn_readfirstlane((int)D.listCount[1]);
+  for (uint32_t w = blockIdx.x; w < n; w += gridDim.x) {
+    const uint2 item = D.dheavy[w];
+    const uint32_t c = (uint32_t)__builtin_amdgcn_readfirstlane((int)item.x);
+    const uint32_t chunk = (uint32_t)__builtin_amdgcn_readfirstlane((int)item.y);
+    const int32_t sh = D.sp[D.cellScene[c]].posShift;
+    const uint32_t lo = (uint32_t)((int32_t)D.cellStart[c] + sh);
+    const uint32_t hi = (uint32_t)((int32_t)D.cellStart[c + 1] + sh);
+    const uint32_t t = lo + chunk * CANON3_ITEM + threadIdx.x;
+    const bool valid = t < hi;
+    const uint32_t myid = valid ? D.bucketId[t] : 0u;
+    uint32_t rank = 0;
+    for (uint32_t base = lo; base < hi; base += CANON3_TILE) {
+      __syncthreads();   // the previous tile is no longer read
+      uint32_t* flat = reinterpret_cast<uint32_t*>(ids);
+      for (uint32_t i = threadIdx.x; i < CANON3_TILE; i += 256u)
+        flat[i] = base + i < hi ? D.bucketId[base + i] : 0xffffffffu;
+      __syncthreads();
+      const uint32_t nq = (min(CANON3_TILE, hi - base) + 3u) / 4u;
+      for (uint32_t i = 0; i < nq; ++i) {
+        const uint4 v = ids[i];
+        rank += (v.x < myid ? 1u : 0u) + (v.y < myid ? 1u : 0u) + (v.z < myid ? 1u : 0u) + (v.w < myid ? 1u : 0u);
+      }
+    }
+    if (valid) canon_place3(D, D.bucket[t], myid, c, lo + rank);
+  }
 }
 
 // ------------------------------------------------------------------ density
@@ -330,9 +377,14 @@ __device__ __forceinline__ void append_chunks3(uint32_t n, uint32_t* counter, ui
 // Called with every lane of every wave active.
 __global__ __launch_bounds__(256) void k3_build_lists(Dev3 D) {
   const uint32_t t = blockIdx.x * 256u + threadIdx.x;
-  uint32_t nch = 0;
-  if (t < D.C && D.cellScene[t] != INACTIVE) nch = (D.cellStart[t + 1] - D.cellStart[t] + CHUNK3 - 1u) / CHUNK3;
+  uint32_t nch = 0, nheavy = 0;
+  if (t < D.C && D.cellScene[t] != INACTIVE) {
+    const uint32_t cnt = D.cellStart[t + 1] - D.cellStart[t];
+    nch = (cnt + CHUNK3 - 1u) / CHUNK3;
+    if (cnt > CANON3_HEAVY) nheavy = (cnt + CANON3_ITEM - 1u) / CANON3_ITEM;
+  }
   append_chunks3(nch, &D.listCount[0], D.dlist, t);
+  append_chunks3(nheavy, &D.listCount[1], D.dheavy, t);
 }
 
 // XCD x (= blockIdx.x % 8) walks the x-th eighth of a device list.
@@ -418,6 +470,8 @@ __global__ __launch_bounds__(256) void k3_nbr_tile(Dev3 D) {
         const uint32_t lo = (uint32_t)((int32_t)D.cellStart[rowc + (uint32_t)xlo] + sh);
         const uint32_t hi = (uint32_t)((int32_t)D.cellStart[rowc + (uint32_t)xhi + 1u] + sh);
         for (uint32_t q0 = lo; q0 < hi; q0 += 64u) {
+          // without debug counters, a wave whose lists are all full is done
+          if (!D.dbgOn && __all(!valid || cnt >= MAXNBR)) break;
           const uint32_t qn = (uint32_t)mini((int32_t)(hi - q0), 64);
           int4 cp = make_int4(0, 0, 0, 0);
           uint32_t cm = 0;
@@ -468,7 +522,7 @@ __global__ __launch_bounds__(256) void k3_nbr_tile(Dev3 D) {
     }
     if (valid) {
       D.nbrN[p] = cnt;
-      D.drop[p] = dropped;
+      D.drop[p] = D.dbgOn ? dropped : 0u;
       dens_finish(D, P, p, pi, A, D.prA, g);
     }
     flush_dbg3(D, s, g, 0);
@@ -764,6 +818,7 @@ struct sn3_engine {
   bool loaded = false;
   bool slotted = false;
   uint32_t nbrBlocks = 8;    // persistent grid of the neighbour sweep
+  uint32_t canonBlocks = 8;  // persistent grid of the heavy-cell rank
   uint32_t maxEvents = 0;
   Event3* dEvents = nullptr;
   uint32_t eventCap = 0;
@@ -810,9 +865,10 @@ void enqueue_substep3(sn3_engine* e, hipEvent_t* marks) {
   hipLaunchKernelGGL(k3_predict_count, dim3(gN), dim3(256), 0, st, D);
   scan3(e, D.cellCount, D.C, D.cellStart, true);
   hipLaunchKernelGGL(k3_scatter, dim3(gN), dim3(256), 0, st, D);
-  hipLaunchKernelGGL(k3_canon_gather, dim3(gN), dim3(256), 0, st, D);
-  if (marks) hipEventRecord(marks[1], st);
   hipLaunchKernelGGL(k3_build_lists, dim3(div_up3(std::max(D.C, 1u), 256)), dim3(256), 0, st, D);
+  hipLaunchKernelGGL(k3_canon_gather, dim3(gN), dim3(256), 0, st, D);
+  hipLaunchKernelGGL(k3_canon_heavy, dim3(e->canonBlocks), dim3(256), 0, st, D);
+  if (marks) hipEventRecord(marks[1], st);
   hipLaunchKernelGGL(k3_nbr_tile, dim3(e->nbrBlocks), dim3(256), 0, st, D);
   if (marks) hipEventRecord(marks[2], st);
   int4* cur = D.prA;
@@ -895,6 +951,7 @@ int sn3_create(const sn_opts* opts, const int32_t* luts3, uint32_t lut_words, sn
     const uint32_t NC = e->opts.max_particles, CC = e->opts.max_cells, SC = e->opts.max_scenes;
     if (std::max(NC, CC) > 4096u * SCAN_BLOCK) return fail3(e, SN_E_CAPACITY, "[aether-3d] capacity above 16M");
     Dev3& D = e->D;
+    D.dbgOn = e->opts.no_debug ? 0u : 1u;
     D.ps = dalloc3<int4>(e, NC);
     D.vel = dalloc3<int4>(e, NC);
     D.id = dalloc3<uint32_t>(e, NC);
@@ -922,6 +979,7 @@ int sn3_create(const sn_opts* opts, const int32_t* luts3, uint32_t lut_words, sn
     D.cellScene = dalloc3<uint32_t>(e, CC);
     // at most one item per cell plus one per full chunk
     D.dlist = dalloc3<uint2>(e, (size_t)CC + NC / CHUNK3 + 64);
+    D.dheavy = dalloc3<uint2>(e, (size_t)NC / 128 + 64);   // a heavy cell of k > 256 has < 2k/256 items
     D.listCount = dalloc3<uint32_t>(e, 8);
     {
       hipDeviceProp_t prop;
@@ -930,6 +988,7 @@ int sn3_create(const sn_opts* opts, const int32_t* luts3, uint32_t lut_words, sn
       HIPCHECK3(hipOccupancyMaxActiveBlocksPerMultiprocessor(&bN, reinterpret_cast<const void*>(k3_nbr_tile), 256, 0));
       const uint32_t cus = (uint32_t)std::max(prop.multiProcessorCount, 1);
       e->nbrBlocks = std::max(8u, (cus * (uint32_t)std::max(bN, 1)) & ~7u);
+      e->canonBlocks = 2u * cus;
     }
     D.hitFlag = dalloc3<uint32_t>(e, NC);
     D.hitScan = dalloc3<uint32_t>(e, (size_t)NC + 1);

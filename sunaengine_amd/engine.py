@@ -39,7 +39,8 @@ class Engine:
     """One world of one or more independent 2D scenes resident on one GPU."""
 
     def __init__(self, max_particles: int = 20000, max_cells: int = 65536, iters: int = ITERS,
-                 max_scenes: int = 1, stream_cap: int = 4096, device: int = 0, mats=None, luts=None):
+                 max_scenes: int = 1, stream_cap: int = 4096, device: int = 0, mats=None, luts=None,
+                 debug: bool = True):
         if not (isinstance(iters, int) and 1 <= iters <= 16):
             raise sc.SceneError(f"iters = {iters} must be an integer in 1..16 (shipped default {ITERS})")
         self.mats_rows = mt.MATS if mats is None else mats
@@ -48,7 +49,9 @@ class Engine:
         self.max_particles, self.max_cells, self.iters = max_particles, max_cells, iters
         self.max_scenes = max_scenes
         L = native.lib()
-        o = native.sn_opts(max_particles, max_cells, max_scenes, iters, stream_cap, device)
+        # debug=False is the reference's stripped build (src/engine.js:2884-2894):
+        # debug_counters() reads zeros; hashes and state are unchanged
+        o = native.sn_opts(max_particles, max_cells, max_scenes, iters, stream_cap, device, 0 if debug else 1)
         h = C.c_void_p()
         rc = L.sn_create(C.byref(o), self._luts.ctypes.data, self._mats_img.ctypes.data, C.byref(h))
         self._h = h

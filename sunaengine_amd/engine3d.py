@@ -14,12 +14,13 @@ from . import tables3d
 
 class Engine3D:
     def __init__(self, max_particles: int, max_cells: int, iters: int = 3, max_scenes: int = 1,
-                 stream_cap: int = 4096, device: int = 0, ball_r: float = tables3d.BALL_R_DEFAULT):
+                 stream_cap: int = 4096, device: int = 0, ball_r: float = tables3d.BALL_R_DEFAULT,
+                 debug: bool = True):
         if not (isinstance(iters, int) and 1 <= iters <= 16):
             raise s3.Scene3Error("iters must be an integer in 1..16")
         self._luts = np.ascontiguousarray(tables3d.luts3d(ball_r), np.int32)
         L = native.lib()
-        o = native.sn_opts(max_particles, max_cells, max_scenes, iters, stream_cap, device)
+        o = native.sn_opts(max_particles, max_cells, max_scenes, iters, stream_cap, device, 0 if debug else 1)
         h = C.c_void_p()
         rc = L.sn3_create(C.byref(o), self._luts.ctypes.data, self._luts.size, C.byref(h))
         self._h = h

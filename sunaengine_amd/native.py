@@ -44,7 +44,7 @@ class SnError(RuntimeError):
 class sn_opts(C.Structure):
     _fields_ = [("max_particles", C.c_uint32), ("max_cells", C.c_uint32),
                 ("max_scenes", C.c_uint32), ("iters", C.c_uint32),
-                ("stream_cap", C.c_uint32), ("device", C.c_uint32)]
+                ("stream_cap", C.c_uint32), ("device", C.c_uint32), ("no_debug", C.c_uint32)]
 
 
 class sn_scene(C.Structure):

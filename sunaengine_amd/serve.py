@@ -156,7 +156,12 @@ class Server:
     """Two resident slotted worlds (2D, 3D) and the admission / round loop."""
 
     def __init__(self, classes2d: Sequence[SlotClass] = DEFAULT_2D, classes3d: Sequence[SlotClass] = DEFAULT_3D,
-                 iters2d: int = 4, iters3d: int = 3, chunk: int = 64, stream_cap: int = 8192):
+                 iters2d: int = 4, iters3d: int = 3, chunk: int = 64, stream_cap: int = 8192,
+                 debug: bool = False):
+        """`debug=False` (the default here) is the reference's stripped build:
+        the same hashes, no debug counters, and neighbour sweeps that skip
+        work only the counters need, which keeps cells that walls have
+        packed with thousands of particles cheap."""
         self.chunk = chunk
         self.caps = {}
         self.engines = {}
@@ -169,10 +174,10 @@ class Server:
                 continue
             if kind == "2d":
                 e = Engine(max_particles=sum(cp), max_cells=sum(cc), iters=iters, max_scenes=len(cp),
-                           stream_cap=stream_cap)
+                           stream_cap=stream_cap, debug=debug)
             else:
                 e = Engine3D(max_particles=sum(cp), max_cells=sum(cc), iters=iters, max_scenes=len(cp),
-                             stream_cap=stream_cap)
+                             stream_cap=stream_cap, debug=debug)
             e.reserve_slots(cp, cc)
             self.engines[kind] = e
             self.caps[kind] = list(zip(cp, cc))

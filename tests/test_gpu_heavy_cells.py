@@ -1,0 +1,101 @@
+"""GPU parity on cells packed with thousands of particles.
+
+Walls and collisions can pile thousands of particles into one grid cell.
+Such cells take their own paths: the canonical rank of a cell above 256
+particles runs in k_canon_heavy / k3_canon_heavy, and a 2D neighbour row
+range above 8192 slots takes the unstaged sweep.  These scenes start with
+3,600 / 10,000 particles (2D) or 4,096 (3D) inside one cell and keep cells of
+several hundred to thousands for every substep checked.
+
+The same scenes also pin the debug:false build (src/engine.js:2884-2894):
+identical hashes, chain and state, and debug counters that stay zero.  In
+that build a sweep stops testing candidates once every list of its wave is
+full, which these scenes reach on every substep.
+"""
+import numpy as np
+import pytest
+
+from tests.helpers import first_mismatch, oracle_world
+
+pytestmark = pytest.mark.gpu
+
+ONE = 65536
+
+
+def _packed2d(side, pitch_shift):
+    from sunaengine_amd import scenes
+    # x0 = 101 wu and the top row at 127 wu: the block starts inside the one
+    # cell [100, 104) x [124, 128) of 4 wu cells
+    return scenes.walled_block_scene(side, side, 256, 130, 0xFACE + side, pitch=ONE >> pitch_shift,
+                                     x0=101 * ONE, name=f"packed_{side * side}")
+
+
+def _gpu2d(scene, iters, steps, debug):
+    from sunaengine_amd import scenes
+    from sunaengine_amd.engine import Engine
+    im = scenes.load_image(scene)
+    e = Engine(max_particles=im.n, max_cells=im.cellTotal, iters=iters, debug=debug)
+    e.load_scene(scene)
+    e.step(steps)
+    return e
+
+
+@pytest.mark.parametrize("side,pitch_shift", [(60, 5), (100, 6)])
+def test_packed_cell_2d(side, pitch_shift):
+    s = _packed2d(side, pitch_shift)
+    print(f"{s.name} x12 (debug on and off)", flush=True)
+    o = oracle_world(s, 4)
+    ref = o.step(12, stream=True)
+    assert o.debug_counters()[6] > 0
+    for debug in (True, False):
+        e = _gpu2d(s, 4, 12, debug)
+        got = e.hash_stream()
+        bad = first_mismatch(got, ref)
+        assert bad is None, f"{s.name} debug={debug}: first differing substep {bad}"
+        assert np.array_equal(e.chain(), o.chain())
+        assert np.array_equal(e.read_state(), o.read_state())
+        if debug:
+            assert np.array_equal(e.debug_counters(), o.debug_counters())
+        else:
+            assert not e.debug_counters().any()
+        e.destroy()
+
+
+def test_debug_off_matches_on_dam_break():
+    """An ordinary scene: debug:false changes no hash, zeroes the counters."""
+    from sunaengine_amd import scenes
+    s = scenes.make_scene("overdense_v1")
+    print("overdense_v1 x60 debug off", flush=True)
+    o = oracle_world(s, 4)
+    ref = o.step(60, stream=True)
+    e = _gpu2d(s, 4, 60, False)
+    assert first_mismatch(e.hash_stream(), ref) is None
+    assert np.array_equal(e.read_state(), o.read_state())
+    assert not e.debug_counters().any()
+    e.destroy()
+
+
+def test_packed_cell_3d():
+    from oracle import OracleWorld3
+    from sunaengine_amd import scenes3d, tables3d
+    from sunaengine_amd.engine3d import Engine3D
+    # 16^3 at 1/16 wu from 2.5 wu: every particle inside cell (1, 1, 1) of 2 wu cells
+    s = scenes3d.block_scene(side=16, gx=8, gy=8, gz=8, pitch=ONE >> 4, origin_wu=2.5, seed=0x3EA7)
+    assert len({tuple(r) for r in (s.parts[:, :3] >> 17).tolist()}) == 1
+    print("3d packed 4096 x10 (debug on and off)", flush=True)
+    o = OracleWorld3(s, tables3d.luts3d(), 3)
+    ref = o.step(10, stream=True)
+    assert o.debug_counters()[6] > 0
+    for debug in (True, False):
+        e = Engine3D(max_particles=s.n, max_cells=s.cell_total, iters=3, debug=debug)
+        e.load([s])
+        e.step(10)
+        bad = first_mismatch(e.hash_stream(), ref)
+        assert bad is None, f"3d packed debug={debug}: first differing substep {bad}"
+        assert np.array_equal(e.chain(), o.chain())
+        assert np.array_equal(e.read_state()[:, :8], o.read()[:, :8])
+        if debug:
+            assert np.array_equal(e.debug_counters(), o.debug_counters())
+        else:
+            assert not e.debug_counters().any()
+        e.destroy()
