@@ -137,6 +137,8 @@ struct SceneD {
   uint32_t evCount;    // events of this scene in the schedule
   uint32_t evBase;     // first event of this scene in the schedule
   uint32_t fired;      // substep of the event group being applied (bookkeeping)
+  uint32_t flagged;    // nonzero once this substep's canonicalize met a SEAL or CONGEAL
+                       // particle; cleared at the end of every substep
 };
 
 struct Event {
@@ -181,6 +183,7 @@ struct Dev {
   uint32_t* nbr;
   uint32_t* nbrN;
   uint32_t* drop;         // candidates dropped by truncation, per particle
+  uint32_t* crank;        // partial canonical ranks of heavy-cell slots (zero between uses)
   uint32_t* hpart;        // per-block partial sums [block][HPART_WORDS]
   const uint32_t* cellScene;  // global cell -> scene, INACTIVE if its scene is inactive
   const uint2* tileAll;   // every neighbour tile of the active scenes {first cell, ncell | scene << 8}
@@ -418,9 +421,12 @@ __global__ __launch_bounds__(256) void k_scatter(Dev D) {
 constexpr uint32_t CANON_HEAVY = 256;
 constexpr uint32_t CANON_ITEM = 256;
 constexpr uint32_t CANON_TILE = 2048;
+constexpr uint32_t CANON_SPLIT = 16;      // id-range parts per heavy item, at most
+constexpr uint32_t CANON_PART_MIN = 512;  // ids per part, at least
 
-__device__ __forceinline__ void canon_place(const Dev& D, const SceneP& P, uint32_t me, uint32_t myid, uint32_t c,
+__device__ __forceinline__ void canon_place(const Dev& D, uint32_t s, uint32_t me, uint32_t myid, uint32_t c,
                                             uint32_t q) {
+  const SceneP& P = D.sp[s];
   int4 pt = D.pt[me];
   D.psY[q] = D.ps[me];
   D.ptY[q] = pt;
@@ -430,9 +436,11 @@ __device__ __forceinline__ void canon_place(const Dev& D, const SceneP& P, uint3
   D.cellY[q] = c;
   uint32_t fl = (uint32_t)pt.y;
   bool fluid = myid < P.nFluid;
-  D.info[q] = ((fl & FLAG_DEAD) ? INF_DEAD : 0u) | (fluid ? INF_FLUID : 0u) |
-              ((fluid && (fl & FLAG_SEAL)) ? INF_SEAL : 0u) |
-              ((fluid && (fl & FLAG_CONGEAL)) ? INF_CONGEAL : 0u);
+  const uint32_t inf = ((fl & FLAG_DEAD) ? INF_DEAD : 0u) | (fluid ? INF_FLUID : 0u) |
+                       ((fluid && (fl & FLAG_SEAL)) ? INF_SEAL : 0u) |
+                       ((fluid && (fl & FLAG_CONGEAL)) ? INF_CONGEAL : 0u);
+  D.info[q] = inf;
+  if (inf & (INF_SEAL | INF_CONGEAL)) atomicOr(&D.sd[s].flagged, 1u);
 }
 
 __global__ __launch_bounds__(256) void k_canon_gather(Dev D) {
@@ -446,44 +454,73 @@ __global__ __launch_bounds__(256) void k_canon_gather(Dev D) {
   uint32_t c = D.cell0[me];
   uint32_t lo = (uint32_t)((int32_t)D.cellStart[c] + P.posShift);
   uint32_t hi = (uint32_t)((int32_t)D.cellStart[c + 1] + P.posShift);
-  if (hi - lo > CANON_HEAVY) return;   // k_canon_heavy
+  if (hi - lo > CANON_HEAVY) return;   // k_canon_heavy_rank / _place
   uint32_t rank = 0;
   for (uint32_t u = lo; u < hi; ++u) rank += (D.bucketId[u] < myid) ? 1u : 0u;
-  canon_place(D, P, me, myid, c, lo + rank);
+  canon_place(D, s, me, myid, c, lo + rank);
 }
 
-// Persistent: block b takes items b, b + grid, ...; an item is CANON_ITEM
-// consecutive bucket positions of one heavy cell.  Ids are distinct within a
-// scene and below 2^32 - 1, so the all-ones padding of a short tile never
-// counts as smaller.
-__global__ __launch_bounds__(256) void k_canon_heavy(Dev D) {
+// Heavy cells, in two passes.  Rank: work item (i, part) counts, for the
+// CANON_ITEM slots of heavy item i, the ids below theirs in one of up to
+// CANON_SPLIT parts of the cell, streamed through LDS in tiles, and adds the
+// count to crank; a cell of k slots is thus spread over about
+// k / CANON_ITEM * min(16, k / 512) workgroups.  Place: moves each slot to
+// lo + its rank and clears crank for the next substep.  Ids are distinct
+// within a scene and below 2^32 - 1, so the all-ones padding never counts.
+struct HeavySpan {
+  uint32_t c, s, lo, hi, t;
+};
+__device__ __forceinline__ HeavySpan heavy_item(const Dev& D, uint32_t i) {
+  const uint2 item = D.dlist[5][i];
+  HeavySpan h;
+  h.c = (uint32_t)__builtin_amdgcn_readfirstlane((int)item.x);
+  const uint32_t chunk = (uint32_t)__builtin_amdgcn_readfirstlane((int)item.y);
+  h.s = D.cellScene[h.c];
+  const int32_t sh = D.sp[h.s].posShift;
+  h.lo = (uint32_t)((int32_t)D.cellStart[h.c] + sh);
+  h.hi = (uint32_t)((int32_t)D.cellStart[h.c + 1] + sh);
+  h.t = h.lo + chunk * CANON_ITEM + threadIdx.x;
+  return h;
+}
+
+__global__ __launch_bounds__(256) void k_canon_heavy_rank(Dev D) {
   __shared__ uint4 ids[CANON_TILE / 4];
-  const uint32_t n = (uint32_t)__builtin_amdgcn_readfirstlane((int)D.listCount[5]);
+  const uint32_t n = (uint32_t)__builtin_amdgcn_readfirstlane((int)D.listCount[5]) * CANON_SPLIT;
   for (uint32_t w = blockIdx.x; w < n; w += gridDim.x) {
-    const uint2 item = D.dlist[5][w];
-    const uint32_t c = (uint32_t)__builtin_amdgcn_readfirstlane((int)item.x);
-    const uint32_t chunk = (uint32_t)__builtin_amdgcn_readfirstlane((int)item.y);
-    const uint32_t s = D.cellScene[c];
-    const SceneP& P = D.sp[s];
-    const uint32_t lo = (uint32_t)((int32_t)D.cellStart[c] + P.posShift);
-    const uint32_t hi = (uint32_t)((int32_t)D.cellStart[c + 1] + P.posShift);
-    const uint32_t t = lo + chunk * CANON_ITEM + threadIdx.x;
-    const bool valid = t < hi;
-    const uint32_t myid = valid ? D.bucketId[t] : 0u;
+    const HeavySpan h = heavy_item(D, w / CANON_SPLIT);
+    const uint32_t part = w % CANON_SPLIT, k = h.hi - h.lo;
+    const uint32_t nparts = min(CANON_SPLIT, (k + CANON_PART_MIN - 1u) / CANON_PART_MIN);
+    if (part >= nparts) continue;   // uniform over the block
+    const uint32_t span = (k + nparts - 1u) / nparts;
+    const uint32_t ilo = h.lo + part * span, ihi = min(h.hi, ilo + span);
+    const bool valid = h.t < h.hi;
+    const uint32_t myid = valid ? D.bucketId[h.t] : 0u;
     uint32_t rank = 0;
-    for (uint32_t base = lo; base < hi; base += CANON_TILE) {
+    for (uint32_t base = ilo; base < ihi; base += CANON_TILE) {
       __syncthreads();   // the previous tile is no longer read
       uint32_t* flat = reinterpret_cast<uint32_t*>(ids);
       for (uint32_t i = threadIdx.x; i < CANON_TILE; i += 256u)
-        flat[i] = base + i < hi ? D.bucketId[base + i] : 0xffffffffu;
+        flat[i] = base + i < ihi ? D.bucketId[base + i] : 0xffffffffu;
       __syncthreads();
-      const uint32_t nq = (min(CANON_TILE, hi - base) + 3u) / 4u;
+      const uint32_t nq = (min(CANON_TILE, ihi - base) + 3u) / 4u;
+#pragma unroll 4
       for (uint32_t i = 0; i < nq; ++i) {
         const uint4 v = ids[i];
         rank += (v.x < myid ? 1u : 0u) + (v.y < myid ? 1u : 0u) + (v.z < myid ? 1u : 0u) + (v.w < myid ? 1u : 0u);
       }
     }
-    if (valid) canon_place(D, P, D.bucket[t], myid, c, lo + rank);
+    if (valid && rank) atomicAdd(&D.crank[h.t], rank);
+  }
+}
+
+__global__ __launch_bounds__(256) void k_canon_heavy_place(Dev D) {
+  const uint32_t n = (uint32_t)__builtin_amdgcn_readfirstlane((int)D.listCount[5]);
+  for (uint32_t w = blockIdx.x; w < n; w += gridDim.x) {
+    const HeavySpan h = heavy_item(D, w);
+    if (h.t >= h.hi) continue;
+    const uint32_t r = D.crank[h.t];
+    D.crank[h.t] = 0u;
+    canon_place(D, h.s, D.bucket[h.t], D.bucketId[h.t], h.c, h.lo + r);
   }
 }
 
@@ -646,6 +683,9 @@ __device__ __forceinline__ void nbr_chunk(const Dev& D, uint32_t c0, uint32_t nc
   const int32_t cy = (int32_t)(lc0 / gridW), cx0 = (int32_t)(lc0 % gridW);
   const int32_t xlo = maxi(cx0 - 1, 0), xhi = mini(cx0 + (int32_t)ncell, (int32_t)gridW - 1);
   const uint32_t myRow = cellBase + (uint32_t)cy * gridW;
+  // without debug counters, and with no SEAL/CONGEAL particle in the scene,
+  // a wave whose lists are all full has nothing left to find
+  const bool stopFull = !D.dbgOn && D.sd[s].flagged == 0u;
   for (uint32_t pbase = plo; pbase < phi; pbase += 64u * PPL) {
     uint32_t pk[PPL], inf[PPL], cnt[PPL], bits[PPL], dropped[PPL];
     int2 pik[PPL];
@@ -680,13 +720,23 @@ __device__ __forceinline__ void nbr_chunk(const Dev& D, uint32_t c0, uint32_t nc
       by0 = wave_min(mny) - REACH;
       by1 = wave_max(mxy) + REACH;
     }
-    for (int32_t dy = -1; dy <= 1; ++dy) {
+    bool done = false;
+    for (int32_t dy = -1; dy <= 1 && !done; ++dy) {
       const int32_t gy = cy + dy;
       if (gy < 0 || gy >= (int32_t)gridH) continue;
       const uint32_t rowc = cellBase + (uint32_t)gy * gridW;
       const uint32_t rlo = (uint32_t)((int32_t)D.cellStart[rowc + (uint32_t)xlo] + sh);
       const uint32_t rhi = (uint32_t)((int32_t)D.cellStart[rowc + (uint32_t)xhi + 1u] + sh);
       for (uint32_t q0 = rlo; q0 < rhi; q0 += 64u) {
+        if (stopFull) {
+          bool full = true;
+#pragma unroll
+          for (int k = 0; k < PPL; ++k) full = full && (!live[k] || cnt[k] >= MAXNBR);
+          if (__all(full)) {
+            done = true;
+            break;
+          }
+        }
         const uint32_t qn = (uint32_t)mini((int32_t)(rhi - q0), 64);
         const uint32_t ql = q0 + lane;
         int2 cp = make_int2(0, 0);
@@ -706,6 +756,7 @@ __device__ __forceinline__ void nbr_chunk(const Dev& D, uint32_t c0, uint32_t nc
         if (!D.dbgOn) {
           // every list of the wave full: without debug counters only a
           // candidate carrying SEAL or CONGEAL can still change a result
+          // (nbrN's seen bits); with none in the scene the sweep has stopped
           bool full = true;
 #pragma unroll
           for (int k = 0; k < PPL; ++k) full = full && (!live[k] || cnt[k] >= MAXNBR);
@@ -798,10 +849,17 @@ __device__ __forceinline__ bool nbr_chunk_staged(const Dev& D, uint16_t* lst, ui
     by0 = wave_min(live ? pi.y : I32_MAX) - REACH;
     by1 = wave_max(live ? pi.y : I32_MIN) + REACH;
   }
+  const bool stopFull = !D.dbgOn && D.sd[s].flagged == 0u;   // as in nbr_chunk
+  bool done = false;
 #pragma unroll
   for (uint32_t r = 0; r < 3; ++r) {
+    if (done) break;
     const uint32_t rowc = cellBase + (uint32_t)(cy + (int32_t)r - 1) * gridW;
     for (uint32_t q0 = rlo[r]; q0 < rhi[r]; q0 += 64u) {
+      if (stopFull && __all(!live || cnt >= MAXNBR)) {
+        done = true;
+        break;
+      }
       const uint32_t qn = (uint32_t)mini((int32_t)(rhi[r] - q0), 64);
       int2 cp = make_int2(0, 0);
       uint32_t ci = 0;
@@ -1429,6 +1487,7 @@ __global__ __launch_bounds__(64) void k_hash_finish(Dev D, const uint8_t* active
   uint32_t a[8];
   for (int l = 0; l < 8; ++l) a[l] = SD.acc[l] + sum[l];
   SD.dbg[6] += sum[8];
+  SD.flagged = 0u;
   for (int j = 0; j < 8; ++j) SD.chain[j] = mm_step(SD.chain[j], a[j] ^ rotl32(a[(j + 1) & 7], 7));
   uint32_t slot = SD.streamPos % D.ringCap;
   uint32_t* r = D.hring + ((size_t)s * D.ringCap + slot) * 8;
@@ -1672,7 +1731,8 @@ void enqueue_substep(sn_engine* e) {
   {
     ProfScope ps(e, PC_CANON);
     hipLaunchKernelGGL(k_canon_gather, dim3(gN), dim3(256), 0, st, D);
-    hipLaunchKernelGGL(k_canon_heavy, dim3(e->canonBlocks), dim3(256), 0, st, D);
+    hipLaunchKernelGGL(k_canon_heavy_rank, dim3(e->canonBlocks), dim3(256), 0, st, D);
+    hipLaunchKernelGGL(k_canon_heavy_place, dim3(e->canonBlocks), dim3(256), 0, st, D);
   }
   {
     ProfScope ps(e, PC_NBR);
@@ -1824,6 +1884,8 @@ int sn_create(const sn_opts* opts, const int32_t* luts, const int32_t* mats, sn_
     D.NS = NC;
     D.nbrN = dalloc<uint32_t>(e, NC);
     D.drop = dalloc<uint32_t>(e, NC);
+    D.crank = dalloc<uint32_t>(e, NC);
+    HIPCHECK(hipMemset(D.crank, 0, sizeof(uint32_t) * std::max(NC, 1u)));
     D.hpart = dalloc<uint32_t>(e, (size_t)div_up(NC, 256) * HPART_WORDS);
     D.hitFlag = dalloc<uint32_t>(e, NC);
     D.hitScan = dalloc<uint32_t>(e, (size_t)NC + 1);

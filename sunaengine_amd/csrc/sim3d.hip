@@ -113,6 +113,7 @@ struct Dev3 {
   uint32_t* nbr;
   uint32_t* nbrN;
   uint32_t* drop;            // candidates dropped by truncation, per particle
+  uint32_t* crank;           // partial canonical ranks of heavy-cell slots (zero between uses)
   uint32_t* hpart;           // per-block partial sums [block][HPART3]
   const uint32_t* cellScene; // global cell -> scene, INACTIVE if its scene is inactive
   uint2* dlist;              // this substep's occupied cell chunks {cell, chunk}
@@ -279,35 +280,68 @@ __global__ __launch_bounds__(256) void k3_canon_gather(Dev3 D) {
   canon_place3(D, me, myid, c, lo + rank);
 }
 
-// Persistent: block b takes heavy items b, b + grid, ...  Ids are distinct
-// within a scene and below 2^32 - 1, so the all-ones padding never counts.
-__global__ __launch_bounds__(256) void k3_canon_heavy(Dev3 D) {
+// Heavy cells, in two passes.  Rank: work item (i, part) counts, for the
+// CANON3_ITEM slots of heavy item i, the ids below theirs in one of up to
+// CANON3_SPLIT parts of the cell, streamed through LDS, and adds the count to
+// crank.  Place: moves each slot to lo + its rank and clears crank.  Ids are
+// distinct within a scene and below 2^32 - 1, so the all-ones padding never
+// counts.
+constexpr uint32_t CANON3_SPLIT = 16;
+constexpr uint32_t CANON3_PART_MIN = 512;
+
+struct HeavySpan3 {
+  uint32_t c, lo, hi, t;
+};
+__device__ __forceinline__ HeavySpan3 heavy_item3(const Dev3& D, uint32_t i) {
+  const uint2 item = D.dheavy[i];
+  HeavySpan3 h;
+  h.c = (uint32_t)__builtin_amdgcn_readfirstlane((int)item.x);
+  const uint32_t chunk = (uint32_t)__builtin_amdgcn_readfirstlane((int)item.y);
+  const int32_t sh = D.sp[D.cellScene[h.c]].posShift;
+  h.lo = (uint32_t)((int32_t)D.cellStart[h.c] + sh);
+  h.hi = (uint32_t)((int32_t)D.cellStart[h.c + 1] + sh);
+  h.t = h.lo + chunk * CANON3_ITEM + threadIdx.x;
+  return h;
+}
+
+__global__ __launch_bounds__(256) void k3_canon_heavy_rank(Dev3 D) {
   __shared__ uint4 ids[CANON3_TILE / 4];
-  const uint32_t n = (uint32_t)__builtin_amdgcn_readfirstlane((int)D.listCount[1]);
+  const uint32_t n = (uint32_t)__builtin_amdgcn_readfirstlane((int)D.listCount[1]) * CANON3_SPLIT;
   for (uint32_t w = blockIdx.x; w < n; w += gridDim.x) {
-    const uint2 item = D.dheavy[w];
-    const uint32_t c = (uint32_t)__builtin_amdgcn_readfirstlane((int)item.x);
-    const uint32_t chunk = (uint32_t)__builtin_amdgcn_readfirstlane((int)item.y);
-    const int32_t sh = D.sp[D.cellScene[c]].posShift;
-    const uint32_t lo = (uint32_t)((int32_t)D.cellStart[c] + sh);
-    const uint32_t hi = (uint32_t)((int32_t)D.cellStart[c + 1] + sh);
-    const uint32_t t = lo + chunk * CANON3_ITEM + threadIdx.x;
-    const bool valid = t < hi;
-    const uint32_t myid = valid ? D.bucketId[t] : 0u;
+    const HeavySpan3 h = heavy_item3(D, w / CANON3_SPLIT);
+    const uint32_t part = w % CANON3_SPLIT, k = h.hi - h.lo;
+    const uint32_t nparts = min(CANON3_SPLIT, (k + CANON3_PART_MIN - 1u) / CANON3_PART_MIN);
+    if (part >= nparts) continue;   // uniform over the block
+    const uint32_t span = (k + nparts - 1u) / nparts;
+    const uint32_t ilo = h.lo + part * span, ihi = min(h.hi, ilo + span);
+    const bool valid = h.t < h.hi;
+    const uint32_t myid = valid ? D.bucketId[h.t] : 0u;
     uint32_t rank = 0;
-    for (uint32_t base = lo; base < hi; base += CANON3_TILE) {
+    for (uint32_t base = ilo; base < ihi; base += CANON3_TILE) {
       __syncthreads();   // the previous tile is no longer read
       uint32_t* flat = reinterpret_cast<uint32_t*>(ids);
       for (uint32_t i = threadIdx.x; i < CANON3_TILE; i += 256u)
-        flat[i] = base + i < hi ? D.bucketId[base + i] : 0xffffffffu;
+        flat[i] = base + i < ihi ? D.bucketId[base + i] : 0xffffffffu;
       __syncthreads();
-      const uint32_t nq = (min(CANON3_TILE, hi - base) + 3u) / 4u;
+      const uint32_t nq = (min(CANON3_TILE, ihi - base) + 3u) / 4u;
+#pragma unroll 4
       for (uint32_t i = 0; i < nq; ++i) {
         const uint4 v = ids[i];
         rank += (v.x < myid ? 1u : 0u) + (v.y < myid ? 1u : 0u) + (v.z < myid ? 1u : 0u) + (v.w < myid ? 1u : 0u);
       }
     }
-    if (valid) canon_place3(D, D.bucket[t], myid, c, lo + rank);
+    if (valid && rank) atomicAdd(&D.crank[h.t], rank);
+  }
+}
+
+__global__ __launch_bounds__(256) void k3_canon_heavy_place(Dev3 D) {
+  const uint32_t n = (uint32_t)__builtin_amdgcn_readfirstlane((int)D.listCount[1]);
+  for (uint32_t w = blockIdx.x; w < n; w += gridDim.x) {
+    const HeavySpan3 h = heavy_item3(D, w);
+    if (h.t >= h.hi) continue;
+    const uint32_t r = D.crank[h.t];
+    D.crank[h.t] = 0u;
+    canon_place3(D, D.bucket[h.t], D.bucketId[h.t], h.c, h.lo + r);
   }
 }
 
@@ -867,7 +901,8 @@ void enqueue_substep3(sn3_engine* e, hipEvent_t* marks) {
   hipLaunchKernelGGL(k3_scatter, dim3(gN), dim3(256), 0, st, D);
   hipLaunchKernelGGL(k3_build_lists, dim3(div_up3(std::max(D.C, 1u), 256)), dim3(256), 0, st, D);
   hipLaunchKernelGGL(k3_canon_gather, dim3(gN), dim3(256), 0, st, D);
-  hipLaunchKernelGGL(k3_canon_heavy, dim3(e->canonBlocks), dim3(256), 0, st, D);
+  hipLaunchKernelGGL(k3_canon_heavy_rank, dim3(e->canonBlocks), dim3(256), 0, st, D);
+  hipLaunchKernelGGL(k3_canon_heavy_place, dim3(e->canonBlocks), dim3(256), 0, st, D);
   if (marks) hipEventRecord(marks[1], st);
   hipLaunchKernelGGL(k3_nbr_tile, dim3(e->nbrBlocks), dim3(256), 0, st, D);
   if (marks) hipEventRecord(marks[2], st);
@@ -975,6 +1010,8 @@ int sn3_create(const sn_opts* opts, const int32_t* luts3, uint32_t lut_words, sn
     D.NS = NC;
     D.nbrN = dalloc3<uint32_t>(e, NC);
     D.drop = dalloc3<uint32_t>(e, NC);
+    D.crank = dalloc3<uint32_t>(e, NC);
+    HIPCHECK3(hipMemset(D.crank, 0, sizeof(uint32_t) * std::max(NC, 1u)));
     D.hpart = dalloc3<uint32_t>(e, (size_t)div_up3(NC, 256) * HPART3);
     D.cellScene = dalloc3<uint32_t>(e, CC);
     // at most one item per cell plus one per full chunk
