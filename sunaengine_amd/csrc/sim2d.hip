@@ -184,6 +184,7 @@ struct Dev {
   uint32_t* nbrN;
   uint32_t* drop;         // candidates dropped by truncation, per particle
   uint32_t* crank;        // partial canonical ranks of heavy-cell slots (zero between uses)
+  const uint32_t* blockMap;  // per-particle kernels: logical block -> 256-slot block holding live slots
   uint32_t* hpart;        // per-block partial sums [block][HPART_WORDS]
   const uint32_t* cellScene;  // global cell -> scene, INACTIVE if its scene is inactive
   const uint2* tileAll;   // every neighbour tile of the active scenes {first cell, ncell | scene << 8}
@@ -336,12 +337,20 @@ __device__ __forceinline__ uint32_t xcd_block() {
   return (b & 7u) * (full >> 3) + (b >> 3);
 }
 
+// The 256-slot block a per-particle kernel's workgroup serves.  Those
+// kernels are launched over exactly the blocks that hold live slots of
+// active scenes (blockMap, rebuilt with the layout), so the idle capacity of
+// a slotted world costs no workgroups; XCD order is kept through xcd_block.
+__device__ __forceinline__ uint32_t slot_block(const Dev& D) {
+  return D.blockMap[xcd_block()];
+}
+
 // ===================================================================== predict
 // predict + gridCount (src/sim.wgsl:958-1016) over the old order; the cell
 // counter's atomic return value is the particle's arrival rank in its cell,
 // which replaces the reference's second atomic pass in `scatter`.
 __global__ __launch_bounds__(256) void k_predict_count(Dev D) {
-  uint32_t p = xcd_block() * 256u + threadIdx.x;
+  uint32_t p = slot_block(D) * 256u + threadIdx.x;
   if (blockIdx.x == 0 && threadIdx.x < 6u) D.listCount[threadIdx.x] = 0u;  // rebuilt after the scan
   uint32_t s = p < D.N ? D.sidOf[p] : INACTIVE;
   Dbg g;
@@ -397,7 +406,7 @@ __global__ __launch_bounds__(256) void k_predict_count(Dev D) {
 
 // ==================================================================== scatter
 __global__ __launch_bounds__(256) void k_scatter(Dev D) {
-  uint32_t p = xcd_block() * 256u + threadIdx.x;
+  uint32_t p = slot_block(D) * 256u + threadIdx.x;
   if (p >= D.N) return;
   uint32_t s = D.sidOf[p];
   if (s == INACTIVE) return;
@@ -444,7 +453,7 @@ __device__ __forceinline__ void canon_place(const Dev& D, uint32_t s, uint32_t m
 }
 
 __global__ __launch_bounds__(256) void k_canon_gather(Dev D) {
-  uint32_t t = xcd_block() * 256u + threadIdx.x;
+  uint32_t t = slot_block(D) * 256u + threadIdx.x;
   if (t >= D.N) return;
   uint32_t s = D.sidOf[t];
   if (s == INACTIVE) return;
@@ -1014,7 +1023,7 @@ __device__ __forceinline__ bool wave_bounded(const Dev& D, uint32_t s) {
 }
 
 __global__ __launch_bounds__(256) void k_solveA(Dev D) {
-  const uint32_t p = xcd_block() * 256u + threadIdx.x;
+  const uint32_t p = slot_block(D) * 256u + threadIdx.x;
   const uint32_t s = p < D.N ? D.sidOf[p] : INACTIVE;
   Dbg g;
   const bool bnd = wave_bounded(D, s);
@@ -1239,7 +1248,7 @@ __device__ __forceinline__ void xnc_fluid(const Dev& D, uint32_t p, Dbg& g) {
 }
 
 __global__ __launch_bounds__(256) void k_xnc(Dev D) {
-  uint32_t p = xcd_block() * 256u + threadIdx.x;
+  uint32_t p = slot_block(D) * 256u + threadIdx.x;
   uint32_t s = p < D.N ? D.sidOf[p] : INACTIVE;
   Dbg g;
   const bool bnd = wave_bounded(D, s);
@@ -1320,7 +1329,8 @@ __device__ __forceinline__ int2 st_sum(const Dev& D, uint32_t p, const int32_t* 
 // bond rows stay empty for bondless rows), then the particle's state hash
 // (src/hash/statehash.wgsl:83-159) reduced per scene.
 __global__ __launch_bounds__(256) void k_st_thermal_hash(Dev D) {
-  uint32_t p = xcd_block() * 256u + threadIdx.x;
+  const uint32_t blk = slot_block(D);
+  uint32_t p = blk * 256u + threadIdx.x;
   uint32_t s = p < D.N ? D.sidOf[p] : INACTIVE;
   Dbg g;
   uint32_t words[8];
@@ -1433,7 +1443,7 @@ __global__ __launch_bounds__(256) void k_st_thermal_hash(Dev D) {
   if (buni) {
     if (bs != INACTIVE && threadIdx.x < HPART_WORDS) {
       uint32_t t = threadIdx.x;
-      D.hpart[(size_t)xcd_block() * HPART_WORDS + t] = wpart[0][t] + wpart[1][t] + wpart[2][t] + wpart[3][t];
+      D.hpart[(size_t)blk * HPART_WORDS + t] = wpart[0][t] + wpart[1][t] + wpart[2][t] + wpart[3][t];
     }
   } else if (s != INACTIVE) {
 #pragma unroll
@@ -1504,7 +1514,7 @@ __global__ __launch_bounds__(64) void k_hash_finish(Dev D, const uint8_t* active
 // an exclusive scan over the flags ranks them; k_impulse_apply adds the
 // impulse to the first `limit` hits of each scene.
 __global__ __launch_bounds__(256) void k_impulse_mark(Dev D, uint32_t group) {
-  uint32_t p = blockIdx.x * 256u + threadIdx.x;
+  uint32_t p = slot_block(D) * 256u + threadIdx.x;
   if (p >= D.N) return;
   uint32_t s = D.sidOf[p];
   if (s == INACTIVE) return;
@@ -1531,7 +1541,7 @@ __global__ __launch_bounds__(256) void k_impulse_mark(Dev D, uint32_t group) {
 }
 
 __global__ __launch_bounds__(256) void k_impulse_apply(Dev D, uint32_t group) {
-  uint32_t p = blockIdx.x * 256u + threadIdx.x;
+  uint32_t p = slot_block(D) * 256u + threadIdx.x;
   if (p >= D.N) return;
   uint32_t s = D.sidOf[p];
   if (s == INACTIVE) return;
@@ -1616,6 +1626,8 @@ struct sn_engine {
   std::vector<uint2> tiles;          // every neighbour tile of the active scenes
   uint32_t tileG = 1;
   uint32_t nbrBlocks = 0, colourBlocks = 0, applyBlocks = 0, canonBlocks = 0;  // persistent grid sizes
+  uint32_t actBlocks = 1;            // entries of D.blockMap: the grid of the per-particle kernels
+  uint32_t* dBlockMap = nullptr;
   uint32_t S = 0, N = 0, C = 0;
   uint32_t maxEventsPerSubstep = 0;
   Event* dEvents = nullptr;
@@ -1698,14 +1710,13 @@ struct ProfScope {
 
 void enqueue_substep(sn_engine* e) {
   Dev& D = e->D;
-  const uint32_t gN = div_up(std::max(D.N, 1u), 256);
   hipStream_t st = e->stream;
   // scheduled stir events due at this substep (applied in arming order)
   for (uint32_t gi = 0; gi < e->maxEventsPerSubstep; ++gi) {
     ProfScope ps(e, PC_IMPULSE);
-    hipLaunchKernelGGL(k_impulse_mark, dim3(gN), dim3(256), 0, st, D, gi);
+    hipLaunchKernelGGL(k_impulse_mark, dim3(e->actBlocks), dim3(256), 0, st, D, gi);
     enqueue_scan(e, D.hitFlag, D.N, D.hitScan, false);
-    hipLaunchKernelGGL(k_impulse_apply, dim3(gN), dim3(256), 0, st, D, gi);
+    hipLaunchKernelGGL(k_impulse_apply, dim3(e->actBlocks), dim3(256), 0, st, D, gi);
   }
   if (e->maxEventsPerSubstep) {
     ProfScope ps(e, PC_IMPULSE);
@@ -1714,7 +1725,7 @@ void enqueue_substep(sn_engine* e) {
   }
   {
     ProfScope ps(e, PC_PREDICT);
-    hipLaunchKernelGGL(k_predict_count, dim3(gN), dim3(256), 0, st, D);
+    hipLaunchKernelGGL(k_predict_count, dim3(e->actBlocks), dim3(256), 0, st, D);
   }
   {
     ProfScope ps(e, PC_SCAN);
@@ -1722,7 +1733,7 @@ void enqueue_substep(sn_engine* e) {
   }
   {
     ProfScope ps(e, PC_SCATTER);
-    hipLaunchKernelGGL(k_scatter, dim3(gN), dim3(256), 0, st, D);
+    hipLaunchKernelGGL(k_scatter, dim3(e->actBlocks), dim3(256), 0, st, D);
   }
   {
     ProfScope ps(e, PC_LISTS);
@@ -1730,7 +1741,7 @@ void enqueue_substep(sn_engine* e) {
   }
   {
     ProfScope ps(e, PC_CANON);
-    hipLaunchKernelGGL(k_canon_gather, dim3(gN), dim3(256), 0, st, D);
+    hipLaunchKernelGGL(k_canon_gather, dim3(e->actBlocks), dim3(256), 0, st, D);
     hipLaunchKernelGGL(k_canon_heavy_rank, dim3(e->canonBlocks), dim3(256), 0, st, D);
     hipLaunchKernelGGL(k_canon_heavy_place, dim3(e->canonBlocks), dim3(256), 0, st, D);
   }
@@ -1742,7 +1753,7 @@ void enqueue_substep(sn_engine* e) {
   for (uint32_t it = 0; it < iters; ++it) {
     if (it > 0) {
       ProfScope ps(e, PC_SOLVEA);
-      hipLaunchKernelGGL(k_solveA, dim3(gN), dim3(256), 0, st, D);
+      hipLaunchKernelGGL(k_solveA, dim3(e->actBlocks), dim3(256), 0, st, D);
     }
     for (uint32_t c = 0; c < 4; ++c) {
       {
@@ -1755,11 +1766,11 @@ void enqueue_substep(sn_engine* e) {
   }
   {
     ProfScope ps(e, PC_XNC);
-    hipLaunchKernelGGL(k_xnc, dim3(gN), dim3(256), 0, st, D);
+    hipLaunchKernelGGL(k_xnc, dim3(e->actBlocks), dim3(256), 0, st, D);
   }
   {
     ProfScope ps(e, PC_STH);
-    hipLaunchKernelGGL(k_st_thermal_hash, dim3(gN), dim3(256), 0, st, D);
+    hipLaunchKernelGGL(k_st_thermal_hash, dim3(e->actBlocks), dim3(256), 0, st, D);
   }
   {
     ProfScope ps(e, PC_FINISH);
@@ -1793,6 +1804,18 @@ void upload_layout(sn_engine* e) {
       before += P.n;
     }
   }
+  // the 256-slot blocks holding live slots of active scenes, ascending
+  std::vector<uint32_t> blocks;
+  for (uint32_t s = 0; s < e->S; ++s) {
+    const SceneP& P = e->sp[s];
+    if (!e->active[s] || P.n == 0) continue;
+    for (uint32_t b = P.partBase / 256u; b <= (P.partBase + P.n - 1u) / 256u; ++b)
+      if (blocks.empty() || blocks.back() < b) blocks.push_back(b);
+  }
+  if (blocks.empty()) blocks.push_back(0u);   // one idle workgroup (it still clears the work-list counts)
+  e->actBlocks = (uint32_t)blocks.size();
+  HIPCHECK(hipMemcpyAsync(e->dBlockMap, blocks.data(), sizeof(uint32_t) * blocks.size(), hipMemcpyHostToDevice,
+                          e->stream));
   // inactive scenes keep no cells: their counts stay zero
   HIPCHECK(hipMemcpyAsync((void*)e->D.sidOf, sid.data(), sizeof(uint32_t) * e->N, hipMemcpyHostToDevice, e->stream));
   HIPCHECK(hipMemcpyAsync((void*)e->D.sp, e->sp.data(), sizeof(SceneP) * e->S, hipMemcpyHostToDevice, e->stream));
@@ -1885,6 +1908,12 @@ int sn_create(const sn_opts* opts, const int32_t* luts, const int32_t* mats, sn_
     D.nbrN = dalloc<uint32_t>(e, NC);
     D.drop = dalloc<uint32_t>(e, NC);
     D.crank = dalloc<uint32_t>(e, NC);
+    e->dBlockMap = dalloc<uint32_t>(e, div_up(std::max(NC, 1u), 256));
+    D.blockMap = e->dBlockMap;
+    {
+      const uint32_t zero = 0;   // a valid map of one block until the first layout upload
+      HIPCHECK(hipMemcpy(e->dBlockMap, &zero, sizeof(zero), hipMemcpyHostToDevice));
+    }
     HIPCHECK(hipMemset(D.crank, 0, sizeof(uint32_t) * std::max(NC, 1u)));
     D.hpart = dalloc<uint32_t>(e, (size_t)div_up(NC, 256) * HPART_WORDS);
     D.hitFlag = dalloc<uint32_t>(e, NC);
@@ -2488,10 +2517,9 @@ int sn_apply_impulse(sn_engine* e, uint32_t scene, int32_t cx, int32_t cy, int32
       tmp[s].evCount = (s == scene) ? 1u : 0u;
     }
     HIPCHECK(hipMemcpy(e->D.sd, tmp.data(), sizeof(SceneD) * e->S, hipMemcpyHostToDevice));
-    uint32_t gN = div_up(std::max(D.N, 1u), 256);
-    hipLaunchKernelGGL(k_impulse_mark, dim3(gN), dim3(256), 0, e->stream, D, 0u);
+    hipLaunchKernelGGL(k_impulse_mark, dim3(e->actBlocks), dim3(256), 0, e->stream, D, 0u);
     enqueue_scan(e, D.hitFlag, D.N, D.hitScan, false);
-    hipLaunchKernelGGL(k_impulse_apply, dim3(gN), dim3(256), 0, e->stream, D, 0u);
+    hipLaunchKernelGGL(k_impulse_apply, dim3(e->actBlocks), dim3(256), 0, e->stream, D, 0u);
     HIPCHECK(hipStreamSynchronize(e->stream));
     const SceneP& P = e->sp[scene];
     uint32_t total = 0;

@@ -114,6 +114,7 @@ struct Dev3 {
   uint32_t* nbrN;
   uint32_t* drop;            // candidates dropped by truncation, per particle
   uint32_t* crank;           // partial canonical ranks of heavy-cell slots (zero between uses)
+  const uint32_t* blockMap;  // per-particle kernels: workgroup -> 256-slot block holding live slots
   uint32_t* hpart;           // per-block partial sums [block][HPART3]
   const uint32_t* cellScene; // global cell -> scene, INACTIVE if its scene is inactive
   uint2* dlist;              // this substep's occupied cell chunks {cell, chunk}
@@ -209,6 +210,13 @@ __device__ __forceinline__ void flush_dbg3(const Dev3& D, uint32_t s, const Dbg&
 }
 
 // ------------------------------------------------------------------ predict
+// The 256-slot block a per-particle kernel's workgroup serves: those kernels
+// run over exactly the blocks holding live slots of active scenes (blockMap,
+// rebuilt with the layout), so a slotted world's idle capacity costs nothing.
+__device__ __forceinline__ uint32_t slot_block3(const Dev3& D) {
+  return D.blockMap[blockIdx.x];
+}
+
 __device__ __forceinline__ void predict3(const Dev3& D, uint32_t p, uint32_t s, Dbg& g) {
   const Scene3P& P = D.sp[s];
   int4 a = D.ps[p];
@@ -230,7 +238,7 @@ __device__ __forceinline__ void predict3(const Dev3& D, uint32_t p, uint32_t s, 
   D.rank0[p] = atomicAdd(&D.cellCount[c], 1u);
 }
 __global__ __launch_bounds__(256) void k3_predict_count(Dev3 D) {
-  uint32_t p = blockIdx.x * 256u + threadIdx.x;
+  uint32_t p = slot_block3(D) * 256u + threadIdx.x;
   if (blockIdx.x == 0 && threadIdx.x < 2u) D.listCount[threadIdx.x] = 0u;  // rebuilt after the scan
   uint32_t s = p < D.N ? D.sidOf[p] : INACTIVE;
   Dbg g;
@@ -239,7 +247,7 @@ __global__ __launch_bounds__(256) void k3_predict_count(Dev3 D) {
 }
 
 __global__ __launch_bounds__(256) void k3_scatter(Dev3 D) {
-  uint32_t p = blockIdx.x * 256u + threadIdx.x;
+  uint32_t p = slot_block3(D) * 256u + threadIdx.x;
   if (p >= D.N) return;
   uint32_t s = D.sidOf[p];
   if (s == INACTIVE) return;
@@ -266,7 +274,7 @@ __device__ __forceinline__ void canon_place3(const Dev3& D, uint32_t me, uint32_
 }
 
 __global__ __launch_bounds__(256) void k3_canon_gather(Dev3 D) {
-  uint32_t t = blockIdx.x * 256u + threadIdx.x;
+  uint32_t t = slot_block3(D) * 256u + threadIdx.x;
   if (t >= D.N) return;
   uint32_t s = D.sidOf[t];
   if (s == INACTIVE) return;
@@ -564,7 +572,7 @@ __global__ __launch_bounds__(256) void k3_nbr_tile(Dev3 D) {
 }
 
 __global__ __launch_bounds__(256) void k3_density(Dev3 D, int4* pr) {
-  uint32_t p = blockIdx.x * 256u + threadIdx.x;
+  uint32_t p = slot_block3(D) * 256u + threadIdx.x;
   uint32_t s = p < D.N ? D.sidOf[p] : INACTIVE;
   Dbg g;
   if (s != INACTIVE) {
@@ -590,7 +598,8 @@ __global__ __launch_bounds__(256) void k3_density(Dev3 D, int4* pr) {
 // solveDp + applyDp (3d/sim3d.wgsl:476-537): reads `pr`, writes `out`
 // (the other buffer).  On the last iteration: integrate and hash.
 __global__ __launch_bounds__(256) void k3_solve_apply(Dev3 D, const int4* pr, int4* out, uint32_t last) {
-  uint32_t p = blockIdx.x * 256u + threadIdx.x;
+  const uint32_t blk = slot_block3(D);
+  uint32_t p = blk * 256u + threadIdx.x;
   uint32_t s = p < D.N ? D.sidOf[p] : INACTIVE;
   Dbg g;
   uint32_t hv[8] = {0, 0, 0, 0, 0, 0, 0, 0};
@@ -681,7 +690,7 @@ __global__ __launch_bounds__(256) void k3_solve_apply(Dev3 D, const int4* pr, in
   if (buni) {
     if (bs != INACTIVE && threadIdx.x < (uint32_t)HPART3) {
       const uint32_t t = threadIdx.x;
-      D.hpart[(size_t)blockIdx.x * HPART3 + t] = wpart[0][t] + wpart[1][t] + wpart[2][t] + wpart[3][t];
+      D.hpart[(size_t)blk * HPART3 + t] = wpart[0][t] + wpart[1][t] + wpart[2][t] + wpart[3][t];
     }
   } else if (s != INACTIVE) {
 #pragma unroll
@@ -783,7 +792,7 @@ __global__ void k3_fnv_stripes(const int32_t* rec16, uint32_t n, uint32_t* words
 // slots in ascending order within r of (cx, cy, cz), IEEE double test, at
 // most `limit`, get (ix, iy, iz) added to v, clamped to +-65536.
 __global__ __launch_bounds__(256) void k3_impulse_mark(Dev3 D, uint32_t group) {
-  uint32_t p = blockIdx.x * 256u + threadIdx.x;
+  uint32_t p = slot_block3(D) * 256u + threadIdx.x;
   if (p >= D.N) return;
   uint32_t s = D.sidOf[p];
   if (s == INACTIVE) return;
@@ -804,7 +813,7 @@ __global__ __launch_bounds__(256) void k3_impulse_mark(Dev3 D, uint32_t group) {
   D.hitFlag[P.partBase + id] = hit;
 }
 __global__ __launch_bounds__(256) void k3_impulse_apply(Dev3 D, uint32_t group) {
-  uint32_t p = blockIdx.x * 256u + threadIdx.x;
+  uint32_t p = slot_block3(D) * 256u + threadIdx.x;
   if (p >= D.N) return;
   uint32_t s = D.sidOf[p];
   if (s == INACTIVE) return;
@@ -853,6 +862,8 @@ struct sn3_engine {
   bool slotted = false;
   uint32_t nbrBlocks = 8;    // persistent grid of the neighbour sweep
   uint32_t canonBlocks = 8;  // persistent grid of the heavy-cell rank
+  uint32_t actBlocks = 1;    // entries of D.blockMap: the grid of the per-particle kernels
+  uint32_t* dBlockMap = nullptr;
   uint32_t maxEvents = 0;
   Event3* dEvents = nullptr;
   uint32_t eventCap = 0;
@@ -887,20 +898,19 @@ void scan3(sn3_engine* e, uint32_t* count, uint32_t n, uint32_t* out, bool clear
 void enqueue_substep3(sn3_engine* e, hipEvent_t* marks) {
   Dev3& D = e->D;
   hipStream_t st = e->stream;
-  const uint32_t gN = div_up3(std::max(D.N, 1u), 256);
   for (uint32_t gi = 0; gi < e->maxEvents; ++gi) {
-    hipLaunchKernelGGL(k3_impulse_mark, dim3(gN), dim3(256), 0, st, D, gi);
+    hipLaunchKernelGGL(k3_impulse_mark, dim3(e->actBlocks), dim3(256), 0, st, D, gi);
     scan3(e, D.hitFlag, D.N, D.hitScan, false);
-    hipLaunchKernelGGL(k3_impulse_apply, dim3(gN), dim3(256), 0, st, D, gi);
+    hipLaunchKernelGGL(k3_impulse_apply, dim3(e->actBlocks), dim3(256), 0, st, D, gi);
   }
   if (e->maxEvents)
     hipLaunchKernelGGL(k3_impulse_advance, dim3(div_up3(e->S, 64)), dim3(64), 0, st, D, e->dActive, e->maxEvents);
   if (marks) hipEventRecord(marks[0], st);
-  hipLaunchKernelGGL(k3_predict_count, dim3(gN), dim3(256), 0, st, D);
+  hipLaunchKernelGGL(k3_predict_count, dim3(e->actBlocks), dim3(256), 0, st, D);
   scan3(e, D.cellCount, D.C, D.cellStart, true);
-  hipLaunchKernelGGL(k3_scatter, dim3(gN), dim3(256), 0, st, D);
+  hipLaunchKernelGGL(k3_scatter, dim3(e->actBlocks), dim3(256), 0, st, D);
   hipLaunchKernelGGL(k3_build_lists, dim3(div_up3(std::max(D.C, 1u), 256)), dim3(256), 0, st, D);
-  hipLaunchKernelGGL(k3_canon_gather, dim3(gN), dim3(256), 0, st, D);
+  hipLaunchKernelGGL(k3_canon_gather, dim3(e->actBlocks), dim3(256), 0, st, D);
   hipLaunchKernelGGL(k3_canon_heavy_rank, dim3(e->canonBlocks), dim3(256), 0, st, D);
   hipLaunchKernelGGL(k3_canon_heavy_place, dim3(e->canonBlocks), dim3(256), 0, st, D);
   if (marks) hipEventRecord(marks[1], st);
@@ -910,8 +920,8 @@ void enqueue_substep3(sn3_engine* e, hipEvent_t* marks) {
   int4* nxt = D.prB;
   const uint32_t iters = e->opts.iters;
   for (uint32_t it = 0; it < iters; ++it) {
-    if (it > 0) hipLaunchKernelGGL(k3_density, dim3(gN), dim3(256), 0, st, D, cur);
-    hipLaunchKernelGGL(k3_solve_apply, dim3(gN), dim3(256), 0, st, D, (const int4*)cur, nxt,
+    if (it > 0) hipLaunchKernelGGL(k3_density, dim3(e->actBlocks), dim3(256), 0, st, D, cur);
+    hipLaunchKernelGGL(k3_solve_apply, dim3(e->actBlocks), dim3(256), 0, st, D, (const int4*)cur, nxt,
                        (uint32_t)(it + 1 == iters));
     std::swap(cur, nxt);
   }
@@ -933,6 +943,17 @@ void upload_layout3(sn3_engine* e) {
     }
   }
   HIPCHECK3(hipMemcpy((void*)e->D.sidOf, sid.data(), 4 * (size_t)e->N, hipMemcpyHostToDevice));
+  // the 256-slot blocks holding live slots of active scenes, ascending
+  std::vector<uint32_t> blocks;
+  for (uint32_t s = 0; s < e->S; ++s) {
+    const Scene3P& P = e->sp[s];
+    if (!e->active[s] || P.n == 0) continue;
+    for (uint32_t b = P.partBase / 256u; b <= (P.partBase + P.n - 1u) / 256u; ++b)
+      if (blocks.empty() || blocks.back() < b) blocks.push_back(b);
+  }
+  if (blocks.empty()) blocks.push_back(0u);   // one idle workgroup (it still clears the work-list counts)
+  e->actBlocks = (uint32_t)blocks.size();
+  HIPCHECK3(hipMemcpy(e->dBlockMap, blocks.data(), 4 * blocks.size(), hipMemcpyHostToDevice));
   std::vector<uint32_t> cs(std::max(e->C, 1u), INACTIVE);
   for (uint32_t s = 0; s < e->S; ++s) {
     if (!e->active[s]) continue;
@@ -1011,6 +1032,12 @@ int sn3_create(const sn_opts* opts, const int32_t* luts3, uint32_t lut_words, sn
     D.nbrN = dalloc3<uint32_t>(e, NC);
     D.drop = dalloc3<uint32_t>(e, NC);
     D.crank = dalloc3<uint32_t>(e, NC);
+    e->dBlockMap = dalloc3<uint32_t>(e, div_up3(std::max(NC, 1u), 256));
+    D.blockMap = e->dBlockMap;
+    {
+      const uint32_t zero = 0;   // a valid map of one block until the first layout upload
+      HIPCHECK3(hipMemcpy(e->dBlockMap, &zero, sizeof(zero), hipMemcpyHostToDevice));
+    }
     HIPCHECK3(hipMemset(D.crank, 0, sizeof(uint32_t) * std::max(NC, 1u)));
     D.hpart = dalloc3<uint32_t>(e, (size_t)div_up3(NC, 256) * HPART3);
     D.cellScene = dalloc3<uint32_t>(e, CC);
