@@ -5,11 +5,10 @@ schedule; its result is the per-substep H_set ‖ H_slot stream.  Jobs arrive
 over time (a Poisson stream in the benchmark), wait in a queue, and are
 admitted into a free partition of a resident world as soon as one that fits
 them is free; admission is by device memory, each partition being a fixed
-slice of particle slots and grid cells.  The 2D and 3D worlds each have their
-own HIP stream, so the two advance independently and their work overlaps on
-the device; one host thread enqueues each world's next round as soon as a
-non-blocking query shows its last one done.  A world advances all running
-jobs by rounds of `chunk` substeps (fewer when every running job needs
+slice of particle slots and grid cells.  The 2D and 3D worlds take turns on
+the device, one round at a time; one host thread enqueues a world's round,
+polls it without blocking, and admits and retires jobs in between.  A world
+advances all running jobs by rounds of `chunk` substeps (fewer when every running job needs
 fewer); a job that ends inside a round simply runs on to the round's end,
 and its result is the first `substeps` entries of its stream, which the
 extra substeps do not touch.  A finished job's stream is read back and its
@@ -220,10 +219,12 @@ class Server:
         """Serve `jobs` (arrival times in seconds from the start).  With
         realtime=False every job is treated as already arrived.
 
-        One host thread drives both worlds: each world has at most one round
-        in flight on its own stream, and the loop only enqueues, polls
-        (non-blocking) and reads back, so the 2D and 3D rounds overlap on the
-        device without concurrent host calls into the runtime."""
+        One host thread drives both worlds, one round on the device at a
+        time, alternating between the worlds that have running jobs; it
+        enqueues a round, polls it (non-blocking), reads back and admits.  The
+        worlds' persistent kernels each size their grids to the whole chip, so
+        rounds of two worlds run side by side delay each other several times
+        over; run one after the other, each gets the chip."""
         for j in jobs:
             if j.substeps + self.chunk > self.stream_cap:
                 raise ValueError(f"job {j.jid}: {j.substeps} substeps exceed the H ring ({self.stream_cap})")
@@ -280,28 +281,35 @@ class Server:
                 eng.step(k)                 # enqueued; finish_round reads it back
                 inflight[kind] = k
 
+        order = list(self.engines)
+        turn = 0              # the world whose round is tried first next time
+        current = None        # the world with a round on the device
         while True:
             now = clock() - t0
             for kind in self.engines:
                 pend = pending[kind]
                 while pend and (not realtime or pend[0].arrival <= now):
                     queue[kind].append(pend.popleft())
-            progressed = False
-            for kind, eng in self.engines.items():
-                if inflight[kind]:
-                    if eng.busy():
-                        continue
-                    finish_round(kind)
-                    progressed = True
+            if current is not None:
+                if self.engines[current].busy():
+                    time.sleep(poll_s)
+                    continue
+                finish_round(current)
+                current = None
+            for i in range(len(order)):
+                kind = order[(turn + i) % len(order)]
                 admit_and_launch(kind)
-                progressed = progressed or bool(inflight[kind])
-            if not any(inflight.values()) and not any(queue.values()):
+                if inflight[kind]:
+                    current = kind
+                    turn = (order.index(kind) + 1) % len(order)
+                    break
+            if current is None:
+                if any(queue.values()):   # every partition is free here, and each job fits one
+                    raise RuntimeError("queued jobs fit no free partition of an idle world")
                 if not any(pending.values()):
                     break
                 nxt = min(p[0].arrival for p in pending.values() if p)
                 time.sleep(max(0.0, min(nxt - (clock() - t0), 0.005)))
-            elif not progressed:
-                time.sleep(poll_s)
         return {"wall_s": clock() - t0, "rounds": sum(rounds.values()), "rounds_by_world": rounds}
 
 
