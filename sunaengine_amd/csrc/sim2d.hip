@@ -682,6 +682,77 @@ __device__ __forceinline__ int32_t wave_max(int32_t v) {
   return v;
 }
 
+// The sweep's tail for one particle (lane L of the wave) whose list is not
+// yet full while the rest of the wave is: the remaining candidates, from row
+// dy0 at slot q0s on, are tested one per lane, 64 at a time, and the
+// accepted ones taken in canonical order (ballot rank) until the list is
+// full.  Its iteration-0 solveA terms are summed over the lanes and added to
+// lane L's sums (wrapping u32 sums, so the grouping changes nothing).  Used
+// only without debug counters and without SEAL/CONGEAL particles in the
+// scene, where a full list ends a particle's sweep; then a cell that walls
+// have packed with thousands of particles costs a few lanes' worth of work
+// per chunk instead of one broadcast per candidate.
+constexpr uint32_t TAIL_LANES = 8;   // at most this many unfilled lists switch to the tail
+
+template <bool BND, bool WIN>
+__device__ __forceinline__ void nbr_tail(const Dev& D, const SceneP& P, int32_t cy, int32_t xlo, int32_t xhi,
+                                         int32_t dy0, uint32_t q0s, uint32_t lane, uint32_t L, int2 pi,
+                                         uint32_t pk, uint32_t inf, int32_t cxk, uint32_t& cnt, SolveAAcc& acc,
+                                         Dbg& g) {
+  const int32_t sh = P.posShift;
+  const int2 pL = make_int2(__builtin_amdgcn_readlane(pi.x, L), __builtin_amdgcn_readlane(pi.y, L));
+  const uint32_t pkL = (uint32_t)__builtin_amdgcn_readlane((int)pk, L);
+  const bool fluidL = ((uint32_t)__builtin_amdgcn_readlane((int)inf, L) & INF_FLUID) != 0u;
+  const int32_t cxL = WIN ? __builtin_amdgcn_readlane(cxk, L) : 0;
+  uint32_t c = (uint32_t)__builtin_amdgcn_readlane((int)cnt, L);
+  SolveAAcc part{0, 0, 0, 0};
+  for (int32_t dy = dy0; dy <= 1 && c < MAXNBR; ++dy) {
+    const int32_t gy = cy + dy;
+    if (gy < 0 || gy >= (int32_t)P.gridH) continue;
+    const uint32_t rowc = P.cellBase + (uint32_t)gy * P.gridW;
+    const uint32_t rlo = (uint32_t)((int32_t)D.cellStart[rowc + (uint32_t)xlo] + sh);
+    const uint32_t rhi = (uint32_t)((int32_t)D.cellStart[rowc + (uint32_t)xhi + 1u] + sh);
+    for (uint32_t q0 = dy == dy0 ? q0s : rlo; q0 < rhi && c < MAXNBR; q0 += 64u) {
+      const uint32_t q = q0 + lane;
+      bool ok = q < rhi;
+      int2 cp = make_int2(0, 0);
+      uint32_t ci = 0;
+      int32_t cc = 0;
+      if (ok) {
+        cp = D.pred[q];
+        ci = D.info[q];
+        if (WIN) cc = (int32_t)(D.cellY[q] - rowc);
+      }
+      ok = ok && !(ci & INF_DEAD) && q != pkL;
+      if (WIN) ok = ok && cc >= cxL - 1 && cc <= cxL + 1;
+      // WIN false: one cell of <= 2^18 ticks, so |d| < 2^18 + REACH < 2^19
+      // for every candidate of its rows that can be accepted
+      const int2 d = pdelta<BND>(pL, cp, g);
+      const int32_t idx = ok ? ((BND && !WIN) ? lut_idx_near(d) : lut_idx<BND>(d, g)) : -1;
+      ok = ok && idx >= 0;
+      const uint64_t m = __ballot(ok);
+      const uint32_t rank = __builtin_amdgcn_mbcnt_hi((uint32_t)(m >> 32), __builtin_amdgcn_mbcnt_lo((uint32_t)m, 0u));
+      const uint32_t room = MAXNBR - c;
+      if (ok && rank < room) {
+        D.nbr[(size_t)(c + rank) * D.NS + pkL] = q | ((ci & INF_FLUID) ? 0u : NBR_BND);
+        if (fluidL) solveA_term(part, d, D.lut4[idx], g);
+      }
+      c += min((uint32_t)__popcll(m), room);
+    }
+  }
+  part.rho = (int32_t)wave_sum((uint32_t)part.rho);
+  part.S = (int32_t)wave_sum((uint32_t)part.S);
+  part.sgx = (int32_t)wave_sum((uint32_t)part.sgx);
+  part.sgy = (int32_t)wave_sum((uint32_t)part.sgy);
+  if (lane == L) {
+    cnt = c;
+    acc.rho = wadd(acc.rho, part.rho);
+    acc.S = wadd(acc.S, part.S);
+    acc.sgx = wadd(acc.sgx, part.sgx);
+    acc.sgy = wadd(acc.sgy, part.sgy);
+  }
+}
+
 template <bool BND, bool WIN, int PPL>
 __device__ __forceinline__ void nbr_chunk(const Dev& D, uint32_t c0, uint32_t ncell, uint32_t s,
                                           uint32_t plo, uint32_t phi, uint32_t lane, int32_t W0, Dbg& g) {
@@ -738,10 +809,21 @@ __device__ __forceinline__ void nbr_chunk(const Dev& D, uint32_t c0, uint32_t nc
       const uint32_t rhi = (uint32_t)((int32_t)D.cellStart[rowc + (uint32_t)xhi + 1u] + sh);
       for (uint32_t q0 = rlo; q0 < rhi; q0 += 64u) {
         if (stopFull) {
-          bool full = true;
+          uint32_t open = 0;   // lists still filling, over the wave
 #pragma unroll
-          for (int k = 0; k < PPL; ++k) full = full && (!live[k] || cnt[k] >= MAXNBR);
-          if (__all(full)) {
+          for (int k = 0; k < PPL; ++k) open += (uint32_t)__popcll(__ballot(live[k] && cnt[k] < MAXNBR));
+          if (open <= TAIL_LANES) {
+            // few lists left: finish each alone over the remaining candidates
+#pragma unroll
+            for (int k = 0; k < PPL; ++k) {
+              uint64_t m = __ballot(live[k] && cnt[k] < MAXNBR);
+              while (m) {
+                const uint32_t L = (uint32_t)__builtin_ctzll(m);
+                m &= m - 1ull;
+                nbr_tail<BND, WIN>(D, P, cy, xlo, xhi, dy, q0, lane, L, pik[k], pk[k], inf[k], cxk[k], cnt[k],
+                                   acc[k], g);
+              }
+            }
             done = true;
             break;
           }

@@ -10,7 +10,8 @@ several hundred to thousands for every substep checked.
 The same scenes also pin the debug:false build (src/engine.js:2884-2894):
 identical hashes, chain and state, and debug counters that stay zero.  In
 that build a sweep stops testing candidates once every list of its wave is
-full, which these scenes reach on every substep.
+full, and finishes the last few unfilled lists of a wave one particle at a
+time; these scenes reach both on every substep.
 """
 import numpy as np
 import pytest
@@ -99,3 +100,36 @@ def test_packed_cell_3d():
         else:
             assert not e.debug_counters().any()
         e.destroy()
+
+
+def test_corner_pileup_debug_off_long_run():
+    """A 76,175-particle dam break in the 256 x 128 wu box piles about 26,000
+    particles into its floor corner cell within a few hundred substeps; most
+    chunks of that cell then hold a particle with fewer than 48 neighbours in
+    range.  Without debug counters such lists are finished by the per-particle
+    tail of the sweep.  600 substeps: the stripped build's stream equals the
+    instrumented build's (itself pinned to the oracle above and, here, over
+    the first 20 substeps)."""
+    from sunaengine_amd import scenes, serve
+    from sunaengine_amd.engine import Engine
+    job = serve.make_job(6, 0xC0A1, "2d", 76175, 600)
+    s = job.scene
+    im = scenes.load_image(s)
+    print("corner pile-up 76175 x600 (debug on vs off) + oracle x20", flush=True)
+    o = oracle_world(s, 4)
+    ref20 = o.step(20, stream=True)
+    streams = []
+    for debug in (True, False):
+        e = Engine(max_particles=im.n, max_cells=im.cellTotal, iters=4, debug=debug)
+        e.load_scene(s)
+        e.step(600)
+        streams.append((e.hash_stream(), e.chain(), e.read_state()))
+        e.destroy()
+    on, off = streams
+    assert first_mismatch(on[0][:20], ref20) is None
+    bad = first_mismatch(off[0], on[0])
+    assert bad is None, f"debug off differs from debug on at substep {bad}"
+    assert np.array_equal(off[1], on[1]) and np.array_equal(off[2], on[2])
+    st = off[2]
+    cells = (st[:, 1].astype(np.int64) >> 18) * 64 + (st[:, 0].astype(np.int64) >> 18)
+    assert np.unique(cells, return_counts=True)[1].max() > 8192   # the pile-up formed
