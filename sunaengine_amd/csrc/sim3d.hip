@@ -453,6 +453,82 @@ __device__ __forceinline__ int32_t wave_max3(int32_t v) {
   return v;
 }
 
+// The sweep's tail for one particle (lane L) whose list is still filling
+// while the rest of the wave is full: the remaining candidates, from row
+// (dz0, dy0) at slot q0s on, are tested one per lane, 64 at a time, and the
+// accepted ones taken in canonical order (ballot rank) until the list is
+// full; the density terms are summed over the lanes and added to lane L's
+// (wrapping u32 sums).  Only without debug counters, where a full list ends
+// a particle's sweep.  Every candidate lies in the 27-cell stencil, so
+// |d| < 2 cells = 2^18 per axis: the quantised squared radius cannot
+// overflow and decides alone, as in the main sweep.
+constexpr uint32_t TAIL3_LANES = 8;
+
+__device__ __forceinline__ void nbr_tail3(const Dev3& D, const Scene3P& P, int32_t cy, int32_t cz, int32_t xlo,
+                                          int32_t xhi, int32_t dz0, int32_t dy0, uint32_t q0s, uint32_t lane,
+                                          uint32_t L, int3 pi, uint32_t p, uint32_t& cnt, DensAcc& A,
+                                          const int32_t* lutW, const int32_t* lutG, const int32_t* rhoW, Dbg& g) {
+  const int32_t sh = P.posShift;
+  const int3 pL = make_int3(__builtin_amdgcn_readlane(pi.x, L), __builtin_amdgcn_readlane(pi.y, L),
+                            __builtin_amdgcn_readlane(pi.z, L));
+  const uint32_t pLs = (uint32_t)__builtin_amdgcn_readlane((int)p, L);
+  uint32_t c = (uint32_t)__builtin_amdgcn_readlane((int)cnt, L);
+  DensAcc part{0, 0, 0, 0, 0};
+  for (int32_t dz = dz0; dz <= 1 && c < MAXNBR; ++dz) {
+    const int32_t gz = cz + dz;
+    if (gz < 0 || gz >= (int32_t)P.gz) continue;
+    for (int32_t dy = (dz == dz0 ? dy0 : -1); dy <= 1 && c < MAXNBR; ++dy) {
+      const int32_t gy = cy + dy;
+      if (gy < 0 || gy >= (int32_t)P.gy) continue;
+      const uint32_t rowc = P.cellBase + ((uint32_t)gz * P.gy + (uint32_t)gy) * P.gx;
+      const uint32_t lo = (uint32_t)((int32_t)D.cellStart[rowc + (uint32_t)xlo] + sh);
+      const uint32_t hi = (uint32_t)((int32_t)D.cellStart[rowc + (uint32_t)xhi + 1u] + sh);
+      const bool first = dz == dz0 && dy == dy0;
+      for (uint32_t q0 = first ? q0s : lo; q0 < hi && c < MAXNBR; q0 += 64u) {
+        const uint32_t q = q0 + lane;
+        bool ok = q < hi && q != pLs;
+        int4 cp = make_int4(0, 0, 0, 0);
+        uint32_t mj = 0;
+        if (q < hi) {
+          cp = D.prA[q];
+          mj = D.matY[q];
+        }
+        const int3 d = make_int3(wsub(cp.x, pL.x), wsub(cp.y, pL.y), wsub(cp.z, pL.z));
+        const int32_t r2q = __mul24(d.x >> 8, d.x >> 8) + __mul24(d.y >> 8, d.y >> 8) + __mul24(d.z >> 8, d.z >> 8);
+        ok = ok && r2q < H2Q;
+        const uint64_t m = __ballot(ok);
+        const uint32_t rank = __builtin_amdgcn_mbcnt_hi((uint32_t)(m >> 32), __builtin_amdgcn_mbcnt_lo((uint32_t)m, 0u));
+        const uint32_t room = MAXNBR - c;
+        if (ok && rank < room) {
+          D.nbr[(size_t)(c + rank) * D.NS + pLs] = q;
+          const uint32_t ui = (uint32_t)r2q >> 7;
+          part.rho = wadd(part.rho, wmul(lutW[ui], wadd(256, rhoW[mj])) >> 8);
+          const int32_t gt = lutG[ui];
+          const int32_t gx = fp_mul_s(gt, d.x, 16, g), gy2 = fp_mul_s(gt, d.y, 16, g), gz2 = fp_mul_s(gt, d.z, 16, g);
+          part.sx = wadd(part.sx, gx);
+          part.sy = wadd(part.sy, gy2);
+          part.sz = wadd(part.sz, gz2);
+          part.S = wadd(wadd(wadd(part.S, s_term(gx, g)), s_term(gy2, g)), s_term(gz2, g));
+        }
+        c += min((uint32_t)__popcll(m), room);
+      }
+    }
+  }
+  part.rho = (int32_t)wave_sum3((uint32_t)part.rho);
+  part.S = (int32_t)wave_sum3((uint32_t)part.S);
+  part.sx = (int32_t)wave_sum3((uint32_t)part.sx);
+  part.sy = (int32_t)wave_sum3((uint32_t)part.sy);
+  part.sz = (int32_t)wave_sum3((uint32_t)part.sz);
+  if (lane == L) {
+    cnt = c;
+    A.rho = wadd(A.rho, part.rho);
+    A.S = wadd(A.S, part.S);
+    A.sx = wadd(A.sx, part.sx);
+    A.sy = wadd(A.sy, part.sy);
+    A.sz = wadd(A.sz, part.sz);
+  }
+}
+
 // buildNbr (3d/sim3d.wgsl:341-390) fused with iteration 0's density.  One
 // wave per chunk of up to 64 particles of one cell, one particle per lane.
 // The chunk's candidates are the cell's 27-cell stencil: nine (dz, dy) rows,
@@ -502,18 +578,31 @@ __global__ __launch_bounds__(256) void k3_nbr_tile(Dev3 D) {
     const int32_t bx0 = wave_min3(valid ? pi.x : I32_MAX) - (H - 1), bx1 = wave_max3(valid ? pi.x : I32_MIN) + (H - 1);
     const int32_t by0 = wave_min3(valid ? pi.y : I32_MAX) - (H - 1), by1 = wave_max3(valid ? pi.y : I32_MIN) + (H - 1);
     const int32_t bz0 = wave_min3(valid ? pi.z : I32_MAX) - (H - 1), bz1 = wave_max3(valid ? pi.z : I32_MIN) + (H - 1);
-    for (int32_t dz = -1; dz <= 1; ++dz) {
+    bool done = false;
+    for (int32_t dz = -1; dz <= 1 && !done; ++dz) {
       const int32_t gz = cz + dz;
       if (gz < 0 || gz >= (int32_t)P.gz) continue;
-      for (int32_t dy = -1; dy <= 1; ++dy) {
+      for (int32_t dy = -1; dy <= 1 && !done; ++dy) {
         const int32_t gy = cy + dy;
         if (gy < 0 || gy >= (int32_t)P.gy) continue;
         const uint32_t rowc = P.cellBase + ((uint32_t)gz * P.gy + (uint32_t)gy) * P.gx;
         const uint32_t lo = (uint32_t)((int32_t)D.cellStart[rowc + (uint32_t)xlo] + sh);
         const uint32_t hi = (uint32_t)((int32_t)D.cellStart[rowc + (uint32_t)xhi + 1u] + sh);
         for (uint32_t q0 = lo; q0 < hi; q0 += 64u) {
-          // without debug counters, a wave whose lists are all full is done
-          if (!D.dbgOn && __all(!valid || cnt >= MAXNBR)) break;
+          if (!D.dbgOn) {
+            // without debug counters a full list ends a particle's sweep: a
+            // wave with only a few lists still filling finishes each alone
+            uint64_t open = __ballot(valid && cnt < MAXNBR);
+            if ((uint32_t)__popcll(open) <= TAIL3_LANES) {
+              while (open) {
+                const uint32_t L = (uint32_t)__builtin_ctzll(open);
+                open &= open - 1ull;
+                nbr_tail3(D, P, cy, cz, xlo, xhi, dz, dy, q0, lane, L, pi, p, cnt, A, lutW, lutG, rhoW, g);
+              }
+              done = true;
+              break;
+            }
+          }
           const uint32_t qn = (uint32_t)mini((int32_t)(hi - q0), 64);
           int4 cp = make_int4(0, 0, 0, 0);
           uint32_t cm = 0;
