@@ -110,6 +110,8 @@ def main():
     ap.add_argument("--cpu-substeps", type=int, default=8, help="CPU-baseline window in substeps")
     ap.add_argument("--cpu-threads", type=int, default=0, help="0 = all available host cores")
     ap.add_argument("--no-cpu", action="store_true")
+    ap.add_argument("--no-debug", action="store_true",
+                    help="run on the stripped build (createEngine({debug: false})): same hashes, no debug counters")
     ap.add_argument("--scenes", type=int, default=512, help="batched: scene count")
     ap.add_argument("--solo-sample", type=int, default=16, help="batched: scenes also run one at a time")
     ap.add_argument("--rates", default="10,50,200,500", help="serve: arrival rates in jobs/s")
@@ -163,7 +165,8 @@ def run_batched(args):
     C = sum(im.cellTotal for im in imgs)
     print(f"# {len(ss)} scenes: {F:,} fluid, {N:,} slots, {C:,} cells, "
           f"{sum(len(e) for e in evs)} stir events ({time.time() - t0:.1f}s)", file=sys.stderr, flush=True)
-    eng = Engine(max_particles=N, max_cells=C, iters=iters, max_scenes=len(ss), stream_cap=K + W + 8)
+    eng = Engine(max_particles=N, max_cells=C, iters=iters, max_scenes=len(ss), stream_cap=K + W + 8,
+                 debug=not args.no_debug)
     eng.load_scenes(ss)
     eng.set_events([(k,) + tuple(e) for k, es in enumerate(evs) for e in es])
     eng.step(W)
@@ -180,7 +183,8 @@ def run_batched(args):
     pick = sorted({order[int(round(i * (len(order) - 1) / max(1, m - 1)))] for i in range(m)})
     solo_work, solo_time, equal = 0, 0.0, True
     for k in pick:
-        e1 = Engine(max_particles=imgs[k].n, max_cells=imgs[k].cellTotal, iters=iters, stream_cap=K + W + 8)
+        e1 = Engine(max_particles=imgs[k].n, max_cells=imgs[k].cellTotal, iters=iters, stream_cap=K + W + 8,
+                    debug=not args.no_debug)
         e1.load_scene(ss[k])
         e1.set_events([(0,) + tuple(e) for e in evs[k]])
         e1.step(W)
@@ -207,6 +211,7 @@ def run_batched(args):
         "one_at_a_time_sample": f"{len(pick)} scenes spread over the size range, each alone in its own world",
         "batched_over_one_at_a_time": round(agg / solo, 3),
         "streams_equal_alone_and_batched": equal,
+        "debug_counters": not args.no_debug,
     }
     print(json.dumps(_line(agg, "particle-substeps/s", K, W, dt * 1e3 / K, cfg)), flush=True)
     eng.destroy()
@@ -261,7 +266,7 @@ def run_3d(args):
     side = 128 if args.side == 1024 else args.side
     s = scenes3d.block_scene(side=side)
     print(f"# 3D block n={s.n} cells={s.cell_total} ({time.time() - t0:.1f}s)", file=sys.stderr, flush=True)
-    e = Engine3D(max_particles=s.n, max_cells=s.cell_total, iters=iters,
+    e = Engine3D(max_particles=s.n, max_cells=s.cell_total, iters=iters, debug=not args.no_debug,
                  stream_cap=max(4096, args.steps + args.warmup + 8))
     e.load([s])
     e.step(args.warmup)
@@ -301,7 +306,8 @@ def run_scene1m(args):
     print(f"# scene built: n={N} nFluid={F} cells={C} ({time.time() - t0:.1f}s)",
           file=sys.stderr, flush=True)
 
-    eng = Engine(max_particles=N, max_cells=C, iters=iters, stream_cap=max(4096, args.steps + args.warmup + 64))
+    eng = Engine(max_particles=N, max_cells=C, iters=iters, stream_cap=max(4096, args.steps + args.warmup + 64),
+                 debug=not args.no_debug)
     eng.load_scene(scene)
     eng.step(args.warmup)
     eng.sync()
@@ -370,6 +376,26 @@ def run_scene1m(args):
         "per_kernel": per_launch,
     }
 
+    # ---- the other build (debug counters on <-> off) over the same substeps,
+    # graph-replayed; its per-substep stream must equal the timed one
+    other = Engine(max_particles=N, max_cells=C, iters=iters, stream_cap=max(4096, args.steps + args.warmup + 64),
+                   debug=args.no_debug)
+    other.load_scene(scene)
+    other.step(args.warmup)
+    other.sync()
+    other.reset_stream()
+    t3 = time.perf_counter()
+    other.step(args.steps)
+    other.sync()
+    w3 = time.perf_counter() - t3
+    other_build = {"debug_counters": bool(args.no_debug), "value": round(F * args.steps / w3, 1),
+                   "ms_per_step": round(w3 * 1e3 / args.steps, 4),
+                   "stream_equal": bool(np.array_equal(other.hash_stream(), hs)),
+                   "timing": "hipGraph-replayed substeps, no per-launch events"}
+    other.destroy()
+    print(f"# other build (debug={not args.no_debug} -> {args.no_debug}): {other_build['ms_per_step']:.3f} ms/substep, "
+          f"stream equal: {other_build['stream_equal']}", file=sys.stderr, flush=True)
+
     # ---- CPU baseline: the C++ restatement on this host's cores, same scene
     cpu_baseline = None
     if not args.no_cpu:
@@ -426,8 +452,10 @@ def run_scene1m(args):
             "n_fluid": F,
             "cells": C,
             "iters": iters,
+            "debug_counters": not args.no_debug,
             "dbg_truncation_events": int(dbg[6]),
             "dbg": [int(x) for x in dbg],
+            "other_build": other_build,
         },
         "roofline": roofline,
         "cpu_baseline": cpu_baseline,
