@@ -113,13 +113,10 @@ struct Dev3 {
   uint32_t* nbr;
   uint32_t* nbrN;
   uint32_t* drop;            // candidates dropped by truncation, per particle
-  uint32_t* crank;           // partial canonical ranks of heavy-cell slots (zero between uses)
-  const uint32_t* blockMap;  // per-particle kernels: workgroup -> 256-slot block holding live slots
   uint32_t* hpart;           // per-block partial sums [block][HPART3]
   const uint32_t* cellScene; // global cell -> scene, INACTIVE if its scene is inactive
   uint2* dlist;              // this substep's occupied cell chunks {cell, chunk}
-  uint2* dheavy;             // chunks of heavy cells for the canonical rank {cell, chunk}
-  uint32_t* listCount;       // [0] entries of dlist, [1] of dheavy (cleared by predict)
+  uint32_t* listCount;       // [1] entries of dlist (cleared by predict)
   uint32_t dbgOn;            // 0: the reference's debug:false build, counters stay zero
   const int32_t* luts;
   const Scene3P* sp;
@@ -210,13 +207,6 @@ __device__ __forceinline__ void flush_dbg3(const Dev3& D, uint32_t s, const Dbg&
 }
 
 // ------------------------------------------------------------------ predict
-// The 256-slot block a per-particle kernel's workgroup serves: those kernels
-// run over exactly the blocks holding live slots of active scenes (blockMap,
-// rebuilt with the layout), so a slotted world's idle capacity costs nothing.
-__device__ __forceinline__ uint32_t slot_block3(const Dev3& D) {
-  return D.blockMap[blockIdx.x];
-}
-
 __device__ __forceinline__ void predict3(const Dev3& D, uint32_t p, uint32_t s, Dbg& g) {
   const Scene3P& P = D.sp[s];
   int4 a = D.ps[p];
@@ -238,8 +228,8 @@ __device__ __forceinline__ void predict3(const Dev3& D, uint32_t p, uint32_t s, 
   D.rank0[p] = atomicAdd(&D.cellCount[c], 1u);
 }
 __global__ __launch_bounds__(256) void k3_predict_count(Dev3 D) {
-  uint32_t p = slot_block3(D) * 256u + threadIdx.x;
-  if (blockIdx.x == 0 && threadIdx.x < 2u) D.listCount[threadIdx.x] = 0u;  // rebuilt after the scan
+  uint32_t p = blockIdx.x * 256u + threadIdx.x;
+  if (blockIdx.x == 0 && threadIdx.x == 0) D.listCount[0] = 0u;  // rebuilt after the scan
   uint32_t s = p < D.N ? D.sidOf[p] : INACTIVE;
   Dbg g;
   if (s != INACTIVE) predict3(D, p, s, g);
@@ -247,7 +237,7 @@ __global__ __launch_bounds__(256) void k3_predict_count(Dev3 D) {
 }
 
 __global__ __launch_bounds__(256) void k3_scatter(Dev3 D) {
-  uint32_t p = slot_block3(D) * 256u + threadIdx.x;
+  uint32_t p = blockIdx.x * 256u + threadIdx.x;
   if (p >= D.N) return;
   uint32_t s = D.sidOf[p];
   if (s == INACTIVE) return;
@@ -256,25 +246,8 @@ __global__ __launch_bounds__(256) void k3_scatter(Dev3 D) {
   D.bucketId[q] = D.id[p];
 }
 
-// canonicalize: rank = #{ids in my cell below mine}.  Cells of more than
-// CANON3_HEAVY particles (packed by walls or collisions) are left to
-// k3_canon_heavy, which streams the cell's ids through LDS over many
-// workgroups instead of a quadratic walk per thread.
-constexpr uint32_t CANON3_HEAVY = 256;
-constexpr uint32_t CANON3_ITEM = 256;
-constexpr uint32_t CANON3_TILE = 2048;
-
-__device__ __forceinline__ void canon_place3(const Dev3& D, uint32_t me, uint32_t myid, uint32_t c, uint32_t q) {
-  int4 a = D.ps[me];
-  D.psY[q] = a;
-  D.idY[q] = myid;
-  D.matY[q] = (uint32_t)a.w;
-  D.prA[q] = D.pr0[me];
-  D.cellY[q] = c;
-}
-
 __global__ __launch_bounds__(256) void k3_canon_gather(Dev3 D) {
-  uint32_t t = slot_block3(D) * 256u + threadIdx.x;
+  uint32_t t = blockIdx.x * 256u + threadIdx.x;
   if (t >= D.N) return;
   uint32_t s = D.sidOf[t];
   if (s == INACTIVE) return;
@@ -282,75 +255,15 @@ __global__ __launch_bounds__(256) void k3_canon_gather(Dev3 D) {
   uint32_t me = D.bucket[t], myid = D.bucketId[t];
   uint32_t c = D.cell0[me];
   uint32_t lo = (uint32_t)((int32_t)D.cellStart[c] + sh), hi = (uint32_t)((int32_t)D.cellStart[c + 1] + sh);
-  if (hi - lo > CANON3_HEAVY) return;   // k3_canon_heavy
   uint32_t rank = 0;
   for (uint32_t u = lo; u < hi; ++u) rank += (D.bucketId[u] < myid) ? 1u : 0u;
-  canon_place3(D, me, myid, c, lo + rank);
-}
-
-// Heavy cells, in two passes.  Rank: work item (i, part) counts, for the
-// CANON3_ITEM slots of heavy item i, the ids below theirs in one of up to
-// CANON3_SPLIT parts of the cell, streamed through LDS, and adds the count to
-// crank.  Place: moves each slot to lo + its rank and clears crank.  Ids are
-// distinct within a scene and below 2^32 - 1, so the all-ones padding never
-// counts.
-constexpr uint32_t CANON3_SPLIT = 16;
-constexpr uint32_t CANON3_PART_MIN = 512;
-
-struct HeavySpan3 {
-  uint32_t c, lo, hi, t;
-};
-__device__ __forceinline__ HeavySpan3 heavy_item3(const Dev3& D, uint32_t i) {
-  const uint2 item = D.dheavy[i];
-  HeavySpan3 h;
-  h.c = (uint32_t)__builtin_amdgcn_readfirstlane((int)item.x);
-  const uint32_t chunk = (uint32_t)__builtin_amdgcn_readfirstlane((int)item.y);
-  const int32_t sh = D.sp[D.cellScene[h.c]].posShift;
-  h.lo = (uint32_t)((int32_t)D.cellStart[h.c] + sh);
-  h.hi = (uint32_t)((int32_t)D.cellStart[h.c + 1] + sh);
-  h.t = h.lo + chunk * CANON3_ITEM + threadIdx.x;
-  return h;
-}
-
-__global__ __launch_bounds__(256) void k3_canon_heavy_rank(Dev3 D) {
-  __shared__ uint4 ids[CANON3_TILE / 4];
-  const uint32_t n = (uint32_t)__builtin_amdgcn_readfirstlane((int)D.listCount[1]) * CANON3_SPLIT;
-  for (uint32_t w = blockIdx.x; w < n; w += gridDim.x) {
-    const HeavySpan3 h = heavy_item3(D, w / CANON3_SPLIT);
-    const uint32_t part = w % CANON3_SPLIT, k = h.hi - h.lo;
-    const uint32_t nparts = min(CANON3_SPLIT, (k + CANON3_PART_MIN - 1u) / CANON3_PART_MIN);
-    if (part >= nparts) continue;   // uniform over the block
-    const uint32_t span = (k + nparts - 1u) / nparts;
-    const uint32_t ilo = h.lo + part * span, ihi = min(h.hi, ilo + span);
-    const bool valid = h.t < h.hi;
-    const uint32_t myid = valid ? D.bucketId[h.t] : 0u;
-    uint32_t rank = 0;
-    for (uint32_t base = ilo; base < ihi; base += CANON3_TILE) {
-      __syncthreads();   // the previous tile is no longer read
-      uint32_t* flat = reinterpret_cast<uint32_t*>(ids);
-      for (uint32_t i = threadIdx.x; i < CANON3_TILE; i += 256u)
-        flat[i] = base + i < ihi ? D.bucketId[base + i] : 0xffffffffu;
-      __syncthreads();
-      const uint32_t nq = (min(CANON3_TILE, ihi - base) + 3u) / 4u;
-#pragma unroll 4
-      for (uint32_t i = 0; i < nq; ++i) {
-        const uint4 v = ids[i];
-        rank += (v.x < myid ? 1u : 0u) + (v.y < myid ? 1u : 0u) + (v.z < myid ? 1u : 0u) + (v.w < myid ? 1u : 0u);
-      }
-    }
-    if (valid && rank) atomicAdd(&D.crank[h.t], rank);
-  }
-}
-
-__global__ __launch_bounds__(256) void k3_canon_heavy_place(Dev3 D) {
-  const uint32_t n = (uint32_t)__builtin_amdgcn_readfirstlane((int)D.listCount[1]);
-  for (uint32_t w = blockIdx.x; w < n; w += gridDim.x) {
-    const HeavySpan3 h = heavy_item3(D, w);
-    if (h.t >= h.hi) continue;
-    const uint32_t r = D.crank[h.t];
-    D.crank[h.t] = 0u;
-    canon_place3(D, D.bucket[h.t], D.bucketId[h.t], h.c, h.lo + r);
-  }
+  uint32_t q = lo + rank;
+  int4 a = D.ps[me];
+  D.psY[q] = a;
+  D.idY[q] = myid;
+  D.matY[q] = (uint32_t)a.w;
+  D.prA[q] = D.pr0[me];
+  D.cellY[q] = c;
 }
 
 // ------------------------------------------------------------------ density
@@ -419,14 +332,9 @@ __device__ __forceinline__ void append_chunks3(uint32_t n, uint32_t* counter, ui
 // Called with every lane of every wave active.
 __global__ __launch_bounds__(256) void k3_build_lists(Dev3 D) {
   const uint32_t t = blockIdx.x * 256u + threadIdx.x;
-  uint32_t nch = 0, nheavy = 0;
-  if (t < D.C && D.cellScene[t] != INACTIVE) {
-    const uint32_t cnt = D.cellStart[t + 1] - D.cellStart[t];
-    nch = (cnt + CHUNK3 - 1u) / CHUNK3;
-    if (cnt > CANON3_HEAVY) nheavy = (cnt + CANON3_ITEM - 1u) / CANON3_ITEM;
-  }
+  uint32_t nch = 0;
+  if (t < D.C && D.cellScene[t] != INACTIVE) nch = (D.cellStart[t + 1] - D.cellStart[t] + CHUNK3 - 1u) / CHUNK3;
   append_chunks3(nch, &D.listCount[0], D.dlist, t);
-  append_chunks3(nheavy, &D.listCount[1], D.dheavy, t);
 }
 
 // XCD x (= blockIdx.x % 8) walks the x-th eighth of a device list.
@@ -451,82 +359,6 @@ __device__ __forceinline__ int32_t wave_max3(int32_t v) {
 #pragma unroll
   for (int off = 32; off >= 1; off >>= 1) v = maxi(v, __shfl_xor(v, off));
   return v;
-}
-
-// The sweep's tail for one particle (lane L) whose list is still filling
-// while the rest of the wave is full: the remaining candidates, from row
-// (dz0, dy0) at slot q0s on, are tested one per lane, 64 at a time, and the
-// accepted ones taken in canonical order (ballot rank) until the list is
-// full; the density terms are summed over the lanes and added to lane L's
-// (wrapping u32 sums).  Only without debug counters, where a full list ends
-// a particle's sweep.  Every candidate lies in the 27-cell stencil, so
-// |d| < 2 cells = 2^18 per axis: the quantised squared radius cannot
-// overflow and decides alone, as in the main sweep.
-constexpr uint32_t TAIL3_LANES = 8;
-
-__device__ __forceinline__ void nbr_tail3(const Dev3& D, const Scene3P& P, int32_t cy, int32_t cz, int32_t xlo,
-                                          int32_t xhi, int32_t dz0, int32_t dy0, uint32_t q0s, uint32_t lane,
-                                          uint32_t L, int3 pi, uint32_t p, uint32_t& cnt, DensAcc& A,
-                                          const int32_t* lutW, const int32_t* lutG, const int32_t* rhoW, Dbg& g) {
-  const int32_t sh = P.posShift;
-  const int3 pL = make_int3(__builtin_amdgcn_readlane(pi.x, L), __builtin_amdgcn_readlane(pi.y, L),
-                            __builtin_amdgcn_readlane(pi.z, L));
-  const uint32_t pLs = (uint32_t)__builtin_amdgcn_readlane((int)p, L);
-  uint32_t c = (uint32_t)__builtin_amdgcn_readlane((int)cnt, L);
-  DensAcc part{0, 0, 0, 0, 0};
-  for (int32_t dz = dz0; dz <= 1 && c < MAXNBR; ++dz) {
-    const int32_t gz = cz + dz;
-    if (gz < 0 || gz >= (int32_t)P.gz) continue;
-    for (int32_t dy = (dz == dz0 ? dy0 : -1); dy <= 1 && c < MAXNBR; ++dy) {
-      const int32_t gy = cy + dy;
-      if (gy < 0 || gy >= (int32_t)P.gy) continue;
-      const uint32_t rowc = P.cellBase + ((uint32_t)gz * P.gy + (uint32_t)gy) * P.gx;
-      const uint32_t lo = (uint32_t)((int32_t)D.cellStart[rowc + (uint32_t)xlo] + sh);
-      const uint32_t hi = (uint32_t)((int32_t)D.cellStart[rowc + (uint32_t)xhi + 1u] + sh);
-      const bool first = dz == dz0 && dy == dy0;
-      for (uint32_t q0 = first ? q0s : lo; q0 < hi && c < MAXNBR; q0 += 64u) {
-        const uint32_t q = q0 + lane;
-        bool ok = q < hi && q != pLs;
-        int4 cp = make_int4(0, 0, 0, 0);
-        uint32_t mj = 0;
-        if (q < hi) {
-          cp = D.prA[q];
-          mj = D.matY[q];
-        }
-        const int3 d = make_int3(wsub(cp.x, pL.x), wsub(cp.y, pL.y), wsub(cp.z, pL.z));
-        const int32_t r2q = __mul24(d.x >> 8, d.x >> 8) + __mul24(d.y >> 8, d.y >> 8) + __mul24(d.z >> 8, d.z >> 8);
-        ok = ok && r2q < H2Q;
-        const uint64_t m = __ballot(ok);
-        const uint32_t rank = __builtin_amdgcn_mbcnt_hi((uint32_t)(m >> 32), __builtin_amdgcn_mbcnt_lo((uint32_t)m, 0u));
-        const uint32_t room = MAXNBR - c;
-        if (ok && rank < room) {
-          D.nbr[(size_t)(c + rank) * D.NS + pLs] = q;
-          const uint32_t ui = (uint32_t)r2q >> 7;
-          part.rho = wadd(part.rho, wmul(lutW[ui], wadd(256, rhoW[mj])) >> 8);
-          const int32_t gt = lutG[ui];
-          const int32_t gx = fp_mul_s(gt, d.x, 16, g), gy2 = fp_mul_s(gt, d.y, 16, g), gz2 = fp_mul_s(gt, d.z, 16, g);
-          part.sx = wadd(part.sx, gx);
-          part.sy = wadd(part.sy, gy2);
-          part.sz = wadd(part.sz, gz2);
-          part.S = wadd(wadd(wadd(part.S, s_term(gx, g)), s_term(gy2, g)), s_term(gz2, g));
-        }
-        c += min((uint32_t)__popcll(m), room);
-      }
-    }
-  }
-  part.rho = (int32_t)wave_sum3((uint32_t)part.rho);
-  part.S = (int32_t)wave_sum3((uint32_t)part.S);
-  part.sx = (int32_t)wave_sum3((uint32_t)part.sx);
-  part.sy = (int32_t)wave_sum3((uint32_t)part.sy);
-  part.sz = (int32_t)wave_sum3((uint32_t)part.sz);
-  if (lane == L) {
-    cnt = c;
-    A.rho = wadd(A.rho, part.rho);
-    A.S = wadd(A.S, part.S);
-    A.sx = wadd(A.sx, part.sx);
-    A.sy = wadd(A.sy, part.sy);
-    A.sz = wadd(A.sz, part.sz);
-  }
 }
 
 // buildNbr (3d/sim3d.wgsl:341-390) fused with iteration 0's density.  One
@@ -578,31 +410,18 @@ __global__ __launch_bounds__(256) void k3_nbr_tile(Dev3 D) {
     const int32_t bx0 = wave_min3(valid ? pi.x : I32_MAX) - (H - 1), bx1 = wave_max3(valid ? pi.x : I32_MIN) + (H - 1);
     const int32_t by0 = wave_min3(valid ? pi.y : I32_MAX) - (H - 1), by1 = wave_max3(valid ? pi.y : I32_MIN) + (H - 1);
     const int32_t bz0 = wave_min3(valid ? pi.z : I32_MAX) - (H - 1), bz1 = wave_max3(valid ? pi.z : I32_MIN) + (H - 1);
-    bool done = false;
-    for (int32_t dz = -1; dz <= 1 && !done; ++dz) {
+    for (int32_t dz = -1; dz <= 1; ++dz) {
       const int32_t gz = cz + dz;
       if (gz < 0 || gz >= (int32_t)P.gz) continue;
-      for (int32_t dy = -1; dy <= 1 && !done; ++dy) {
+      for (int32_t dy = -1; dy <= 1; ++dy) {
         const int32_t gy = cy + dy;
         if (gy < 0 || gy >= (int32_t)P.gy) continue;
         const uint32_t rowc = P.cellBase + ((uint32_t)gz * P.gy + (uint32_t)gy) * P.gx;
         const uint32_t lo = (uint32_t)((int32_t)D.cellStart[rowc + (uint32_t)xlo] + sh);
         const uint32_t hi = (uint32_t)((int32_t)D.cellStart[rowc + (uint32_t)xhi + 1u] + sh);
         for (uint32_t q0 = lo; q0 < hi; q0 += 64u) {
-          if (!D.dbgOn) {
-            // without debug counters a full list ends a particle's sweep: a
-            // wave with only a few lists still filling finishes each alone
-            uint64_t open = __ballot(valid && cnt < MAXNBR);
-            if ((uint32_t)__popcll(open) <= TAIL3_LANES) {
-              while (open) {
-                const uint32_t L = (uint32_t)__builtin_ctzll(open);
-                open &= open - 1ull;
-                nbr_tail3(D, P, cy, cz, xlo, xhi, dz, dy, q0, lane, L, pi, p, cnt, A, lutW, lutG, rhoW, g);
-              }
-              done = true;
-              break;
-            }
-          }
+          // without debug counters, a wave whose lists are all full is done
+          if (!D.dbgOn && __all(!valid || cnt >= MAXNBR)) break;
           const uint32_t qn = (uint32_t)mini((int32_t)(hi - q0), 64);
           int4 cp = make_int4(0, 0, 0, 0);
           uint32_t cm = 0;
@@ -661,7 +480,7 @@ __global__ __launch_bounds__(256) void k3_nbr_tile(Dev3 D) {
 }
 
 __global__ __launch_bounds__(256) void k3_density(Dev3 D, int4* pr) {
-  uint32_t p = slot_block3(D) * 256u + threadIdx.x;
+  uint32_t p = blockIdx.x * 256u + threadIdx.x;
   uint32_t s = p < D.N ? D.sidOf[p] : INACTIVE;
   Dbg g;
   if (s != INACTIVE) {
@@ -687,8 +506,7 @@ __global__ __launch_bounds__(256) void k3_density(Dev3 D, int4* pr) {
 // solveDp + applyDp (3d/sim3d.wgsl:476-537): reads `pr`, writes `out`
 // (the other buffer).  On the last iteration: integrate and hash.
 __global__ __launch_bounds__(256) void k3_solve_apply(Dev3 D, const int4* pr, int4* out, uint32_t last) {
-  const uint32_t blk = slot_block3(D);
-  uint32_t p = blk * 256u + threadIdx.x;
+  uint32_t p = blockIdx.x * 256u + threadIdx.x;
   uint32_t s = p < D.N ? D.sidOf[p] : INACTIVE;
   Dbg g;
   uint32_t hv[8] = {0, 0, 0, 0, 0, 0, 0, 0};
@@ -779,7 +597,7 @@ __global__ __launch_bounds__(256) void k3_solve_apply(Dev3 D, const int4* pr, in
   if (buni) {
     if (bs != INACTIVE && threadIdx.x < (uint32_t)HPART3) {
       const uint32_t t = threadIdx.x;
-      D.hpart[(size_t)blk * HPART3 + t] = wpart[0][t] + wpart[1][t] + wpart[2][t] + wpart[3][t];
+      D.hpart[(size_t)blockIdx.x * HPART3 + t] = wpart[0][t] + wpart[1][t] + wpart[2][t] + wpart[3][t];
     }
   } else if (s != INACTIVE) {
 #pragma unroll
@@ -881,7 +699,7 @@ __global__ void k3_fnv_stripes(const int32_t* rec16, uint32_t n, uint32_t* words
 // slots in ascending order within r of (cx, cy, cz), IEEE double test, at
 // most `limit`, get (ix, iy, iz) added to v, clamped to +-65536.
 __global__ __launch_bounds__(256) void k3_impulse_mark(Dev3 D, uint32_t group) {
-  uint32_t p = slot_block3(D) * 256u + threadIdx.x;
+  uint32_t p = blockIdx.x * 256u + threadIdx.x;
   if (p >= D.N) return;
   uint32_t s = D.sidOf[p];
   if (s == INACTIVE) return;
@@ -902,7 +720,7 @@ __global__ __launch_bounds__(256) void k3_impulse_mark(Dev3 D, uint32_t group) {
   D.hitFlag[P.partBase + id] = hit;
 }
 __global__ __launch_bounds__(256) void k3_impulse_apply(Dev3 D, uint32_t group) {
-  uint32_t p = slot_block3(D) * 256u + threadIdx.x;
+  uint32_t p = blockIdx.x * 256u + threadIdx.x;
   if (p >= D.N) return;
   uint32_t s = D.sidOf[p];
   if (s == INACTIVE) return;
@@ -950,9 +768,6 @@ struct sn3_engine {
   bool loaded = false;
   bool slotted = false;
   uint32_t nbrBlocks = 8;    // persistent grid of the neighbour sweep
-  uint32_t canonBlocks = 8;  // persistent grid of the heavy-cell rank
-  uint32_t actBlocks = 1;    // entries of D.blockMap: the grid of the per-particle kernels
-  uint32_t* dBlockMap = nullptr;
   uint32_t maxEvents = 0;
   Event3* dEvents = nullptr;
   uint32_t eventCap = 0;
@@ -987,30 +802,29 @@ void scan3(sn3_engine* e, uint32_t* count, uint32_t n, uint32_t* out, bool clear
 void enqueue_substep3(sn3_engine* e, hipEvent_t* marks) {
   Dev3& D = e->D;
   hipStream_t st = e->stream;
+  const uint32_t gN = div_up3(std::max(D.N, 1u), 256);
   for (uint32_t gi = 0; gi < e->maxEvents; ++gi) {
-    hipLaunchKernelGGL(k3_impulse_mark, dim3(e->actBlocks), dim3(256), 0, st, D, gi);
+    hipLaunchKernelGGL(k3_impulse_mark, dim3(gN), dim3(256), 0, st, D, gi);
     scan3(e, D.hitFlag, D.N, D.hitScan, false);
-    hipLaunchKernelGGL(k3_impulse_apply, dim3(e->actBlocks), dim3(256), 0, st, D, gi);
+    hipLaunchKernelGGL(k3_impulse_apply, dim3(gN), dim3(256), 0, st, D, gi);
   }
   if (e->maxEvents)
     hipLaunchKernelGGL(k3_impulse_advance, dim3(div_up3(e->S, 64)), dim3(64), 0, st, D, e->dActive, e->maxEvents);
   if (marks) hipEventRecord(marks[0], st);
-  hipLaunchKernelGGL(k3_predict_count, dim3(e->actBlocks), dim3(256), 0, st, D);
+  hipLaunchKernelGGL(k3_predict_count, dim3(gN), dim3(256), 0, st, D);
   scan3(e, D.cellCount, D.C, D.cellStart, true);
-  hipLaunchKernelGGL(k3_scatter, dim3(e->actBlocks), dim3(256), 0, st, D);
-  hipLaunchKernelGGL(k3_build_lists, dim3(div_up3(std::max(D.C, 1u), 256)), dim3(256), 0, st, D);
-  hipLaunchKernelGGL(k3_canon_gather, dim3(e->actBlocks), dim3(256), 0, st, D);
-  hipLaunchKernelGGL(k3_canon_heavy_rank, dim3(e->canonBlocks), dim3(256), 0, st, D);
-  hipLaunchKernelGGL(k3_canon_heavy_place, dim3(e->canonBlocks), dim3(256), 0, st, D);
+  hipLaunchKernelGGL(k3_scatter, dim3(gN), dim3(256), 0, st, D);
+  hipLaunchKernelGGL(k3_canon_gather, dim3(gN), dim3(256), 0, st, D);
   if (marks) hipEventRecord(marks[1], st);
+  hipLaunchKernelGGL(k3_build_lists, dim3(div_up3(std::max(D.C, 1u), 256)), dim3(256), 0, st, D);
   hipLaunchKernelGGL(k3_nbr_tile, dim3(e->nbrBlocks), dim3(256), 0, st, D);
   if (marks) hipEventRecord(marks[2], st);
   int4* cur = D.prA;
   int4* nxt = D.prB;
   const uint32_t iters = e->opts.iters;
   for (uint32_t it = 0; it < iters; ++it) {
-    if (it > 0) hipLaunchKernelGGL(k3_density, dim3(e->actBlocks), dim3(256), 0, st, D, cur);
-    hipLaunchKernelGGL(k3_solve_apply, dim3(e->actBlocks), dim3(256), 0, st, D, (const int4*)cur, nxt,
+    if (it > 0) hipLaunchKernelGGL(k3_density, dim3(gN), dim3(256), 0, st, D, cur);
+    hipLaunchKernelGGL(k3_solve_apply, dim3(gN), dim3(256), 0, st, D, (const int4*)cur, nxt,
                        (uint32_t)(it + 1 == iters));
     std::swap(cur, nxt);
   }
@@ -1032,17 +846,6 @@ void upload_layout3(sn3_engine* e) {
     }
   }
   HIPCHECK3(hipMemcpy((void*)e->D.sidOf, sid.data(), 4 * (size_t)e->N, hipMemcpyHostToDevice));
-  // the 256-slot blocks holding live slots of active scenes, ascending
-  std::vector<uint32_t> blocks;
-  for (uint32_t s = 0; s < e->S; ++s) {
-    const Scene3P& P = e->sp[s];
-    if (!e->active[s] || P.n == 0) continue;
-    for (uint32_t b = P.partBase / 256u; b <= (P.partBase + P.n - 1u) / 256u; ++b)
-      if (blocks.empty() || blocks.back() < b) blocks.push_back(b);
-  }
-  if (blocks.empty()) blocks.push_back(0u);   // one idle workgroup (it still clears the work-list counts)
-  e->actBlocks = (uint32_t)blocks.size();
-  HIPCHECK3(hipMemcpy(e->dBlockMap, blocks.data(), 4 * blocks.size(), hipMemcpyHostToDevice));
   std::vector<uint32_t> cs(std::max(e->C, 1u), INACTIVE);
   for (uint32_t s = 0; s < e->S; ++s) {
     if (!e->active[s]) continue;
@@ -1120,19 +923,10 @@ int sn3_create(const sn_opts* opts, const int32_t* luts3, uint32_t lut_words, sn
     D.NS = NC;
     D.nbrN = dalloc3<uint32_t>(e, NC);
     D.drop = dalloc3<uint32_t>(e, NC);
-    D.crank = dalloc3<uint32_t>(e, NC);
-    e->dBlockMap = dalloc3<uint32_t>(e, div_up3(std::max(NC, 1u), 256));
-    D.blockMap = e->dBlockMap;
-    {
-      const uint32_t zero = 0;   // a valid map of one block until the first layout upload
-      HIPCHECK3(hipMemcpy(e->dBlockMap, &zero, sizeof(zero), hipMemcpyHostToDevice));
-    }
-    HIPCHECK3(hipMemset(D.crank, 0, sizeof(uint32_t) * std::max(NC, 1u)));
     D.hpart = dalloc3<uint32_t>(e, (size_t)div_up3(NC, 256) * HPART3);
     D.cellScene = dalloc3<uint32_t>(e, CC);
     // at most one item per cell plus one per full chunk
     D.dlist = dalloc3<uint2>(e, (size_t)CC + NC / CHUNK3 + 64);
-    D.dheavy = dalloc3<uint2>(e, (size_t)NC / 128 + 64);   // a heavy cell of k > 256 has < 2k/256 items
     D.listCount = dalloc3<uint32_t>(e, 8);
     {
       hipDeviceProp_t prop;
@@ -1141,7 +935,6 @@ int sn3_create(const sn_opts* opts, const int32_t* luts3, uint32_t lut_words, sn
       HIPCHECK3(hipOccupancyMaxActiveBlocksPerMultiprocessor(&bN, reinterpret_cast<const void*>(k3_nbr_tile), 256, 0));
       const uint32_t cus = (uint32_t)std::max(prop.multiProcessorCount, 1);
       e->nbrBlocks = std::max(8u, (cus * (uint32_t)std::max(bN, 1)) & ~7u);
-      e->canonBlocks = 2u * cus;
     }
     D.hitFlag = dalloc3<uint32_t>(e, NC);
     D.hitScan = dalloc3<uint32_t>(e, (size_t)NC + 1);

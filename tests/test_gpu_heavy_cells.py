@@ -1,17 +1,18 @@
 """GPU parity on cells packed with thousands of particles.
 
 Walls and collisions can pile thousands of particles into one grid cell.
-Such cells take their own paths: the canonical rank of a cell above 256
-particles runs in k_canon_heavy / k3_canon_heavy, and a 2D neighbour row
-range above 8192 slots takes the unstaged sweep.  These scenes start with
+Such cells take their own paths in 2D: the canonical rank of a cell above
+256 particles runs in k_canon_heavy_rank / _place, and a neighbour row range
+above 8192 slots takes the unstaged sweep.  In 3D they run the ordinary
+kernels, whose per-cell work then grows with the cell.  These scenes start with
 3,600 / 10,000 particles (2D) or 4,096 (3D) inside one cell and keep cells of
 several hundred to thousands for every substep checked.
 
 The same scenes also pin the debug:false build (src/engine.js:2884-2894):
 identical hashes, chain and state, and debug counters that stay zero.  In
 that build a sweep stops testing candidates once every list of its wave is
-full, and finishes the last few unfilled lists of a wave one particle at a
-time; these scenes reach both on every substep.
+full, and (2D) finishes the last few unfilled lists of a wave one particle
+at a time; these scenes reach both on every substep.
 """
 import numpy as np
 import pytest
