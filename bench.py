@@ -100,7 +100,8 @@ def cpu_info():
 def main():
     ap = argparse.ArgumentParser()
     ap.add_argument("--gpus", type=int, default=1)
-    ap.add_argument("--steps", type=int, default=100)
+    # the 1M configuration runs 500 substeps: 10 warm-up + 490 timed
+    ap.add_argument("--steps", type=int, default=490)
     ap.add_argument("--warmup", type=int, default=10)
     ap.add_argument("--workload", default="scene1m", choices=("scene1m", "batched", "serve", "3d"),
                     help="scene1m: the 1M-particle 2D scene (default); batched: 512 uneven scenes, "
@@ -447,7 +448,8 @@ def run_scene1m(args):
             "workload": (f"2D single scene: {F:,} water particles ({args.side}x{args.side} block at 0.5 wu, "
                          f"seeded jitter) + {N - F:,} boundary, walled box, {img.gridW}x{img.gridH} cells, "
                          f"{iters} density iterations, XSPH+cohesion on, thermal on (T uniform in [0,100) deg); "
-                         "unit = one fluid particle-substep"),
+                         f"substeps {args.warmup + 1}..{args.warmup + args.steps} from load timed (the dam break "
+                         "collapses and packs its floor cells as it runs); unit = one fluid particle-substep"),
             "n_particles": N,
             "n_fluid": F,
             "cells": C,

@@ -188,9 +188,10 @@ struct Dev {
   uint32_t* hpart;        // per-block partial sums [block][HPART_WORDS]
   const uint32_t* cellScene;  // global cell -> scene, INACTIVE if its scene is inactive
   const uint2* tileAll;   // every neighbour tile of the active scenes {first cell, ncell | scene << 8}
-  uint2* dlist[6];        // this substep's occupied cells per colour [0..3], occupied tiles [4],
-                          // chunks of heavy cells for the canonical rank [5]
-  uint32_t* listCount;    // [6] entries of dlist (cleared by predict)
+  uint2* dlist[7];        // this substep's occupied cells per colour [0..3], occupied tiles [4],
+                          // chunks of heavy cells for the canonical rank [5], tiles with
+                          // heavy candidate rows [6]
+  uint32_t* listCount;    // [7] entries of dlist, [7] next item of dlist[6] (cleared by predict)
   uint32_t nTiles;
   uint32_t dbgOn;         // 0: the reference's debug:false build, counters stay zero
   const int4* lut4;       // {W, G, C, S} per LUT bin
@@ -351,7 +352,7 @@ __device__ __forceinline__ uint32_t slot_block(const Dev& D) {
 // which replaces the reference's second atomic pass in `scatter`.
 __global__ __launch_bounds__(256) void k_predict_count(Dev D) {
   uint32_t p = slot_block(D) * 256u + threadIdx.x;
-  if (blockIdx.x == 0 && threadIdx.x < 6u) D.listCount[threadIdx.x] = 0u;  // rebuilt after the scan
+  if (blockIdx.x == 0 && threadIdx.x < 8u) D.listCount[threadIdx.x] = 0u;  // rebuilt after the scan
   uint32_t s = p < D.N ? D.sidOf[p] : INACTIVE;
   Dbg g;
   uint32_t cell = INACTIVE;
@@ -580,6 +581,11 @@ __device__ __forceinline__ uint32_t nbr_at(const Dev& D, const uint32_t* nb, uin
 constexpr int NBR_PPL = 2;             // particles per lane in the neighbour sweep
 constexpr uint32_t CELL_CHUNK = 32;
 constexpr uint32_t TILE_CHUNK = 128;   // 64 lanes x NBR_PPL
+// A tile whose candidate rows hold more than this many slots (a pile-up in
+// or next to it; 576 at 4x rest density) goes to dlist[6], whose chunks every
+// wave of the sweep takes first, one at a time from a shared counter, so that
+// such work spreads over the whole chip instead of one XCD's list range.
+constexpr uint32_t NBR_HEAVY_CAND = 4096;
 
 __device__ __forceinline__ void append_chunks(uint32_t n, uint32_t* counter, uint2* list, uint32_t key) {
   const uint32_t lane = threadIdx.x & 63u;
@@ -617,12 +623,28 @@ __global__ __launch_bounds__(256) void k_build_lists(Dev D) {
   }
   {
     uint32_t nch = 0;
+    bool heavy = false;
     if (t < D.nTiles) {
       const uint2 it = D.tileAll[t];
-      const uint32_t cnt = D.cellStart[it.x + (it.y & 0xffu)] - D.cellStart[it.x];
+      const uint32_t c0 = it.x, ncell = it.y & 0xffu;
+      const uint32_t cnt = D.cellStart[c0 + ncell] - D.cellStart[c0];
       nch = (cnt + TILE_CHUNK - 1u) / TILE_CHUNK;
+      if (nch) {
+        // the tile's candidates: its three rows over x in cx0-1 .. cx0+ncell
+        const SceneP& P = D.sp[it.y >> 8];
+        const uint32_t lc0 = c0 - P.cellBase;
+        const int32_t cy = (int32_t)(lc0 / P.gridW), cx0 = (int32_t)(lc0 % P.gridW);
+        const int32_t xlo = maxi(cx0 - 1, 0), xhi = mini(cx0 + (int32_t)ncell, (int32_t)P.gridW - 1);
+        uint32_t cand = 0;
+        for (int32_t gy = maxi(cy - 1, 0); gy <= mini(cy + 1, (int32_t)P.gridH - 1); ++gy) {
+          const uint32_t rowc = P.cellBase + (uint32_t)gy * P.gridW;
+          cand += D.cellStart[rowc + (uint32_t)xhi + 1u] - D.cellStart[rowc + (uint32_t)xlo];
+        }
+        heavy = cand > NBR_HEAVY_CAND;
+      }
     }
-    append_chunks(nch, &D.listCount[4], D.dlist[4], t);
+    append_chunks(heavy ? 0u : nch, &D.listCount[4], D.dlist[4], t);
+    append_chunks(heavy ? nch : 0u, &D.listCount[6], D.dlist[6], t);
   }
   {
     uint32_t nch = 0;
@@ -692,7 +714,10 @@ __device__ __forceinline__ int32_t wave_max(int32_t v) {
 // scene, where a full list ends a particle's sweep; then a cell that walls
 // have packed with thousands of particles costs a few lanes' worth of work
 // per chunk instead of one broadcast per candidate.
-constexpr uint32_t TAIL_LANES = 8;   // at most this many unfilled lists switch to the tail
+// The tail costs about 25 instructions per 64 candidates per open list, the
+// broadcast sweep about 20 per candidate for the whole wave, so up to about
+// 50 open lists the tail is cheaper; 32 keeps a margin.
+constexpr uint32_t TAIL_LANES = 32;   // at most this many unfilled lists switch to the tail
 
 template <bool BND, bool WIN>
 __device__ __forceinline__ void nbr_tail(const Dev& D, const SceneP& P, int32_t cy, int32_t xlo, int32_t xhi,
@@ -1031,44 +1056,54 @@ __device__ __forceinline__ bool nbr_chunk_staged(const Dev& D, uint16_t* lst, ui
   return true;
 }
 
+__device__ __forceinline__ void nbr_item(const Dev& D, uint2 item, uint16_t* lst, uint32_t lane, int32_t W0) {
+  const uint32_t tix = __builtin_amdgcn_readfirstlane(item.x);
+  const uint32_t chunk = __builtin_amdgcn_readfirstlane(item.y);
+  const uint2 tile = D.tileAll[tix];
+  const uint32_t c0 = __builtin_amdgcn_readfirstlane(tile.x);
+  const uint32_t yy = __builtin_amdgcn_readfirstlane(tile.y);
+  const uint32_t ncell = yy & 0xffu, s = yy >> 8;
+  const bool bnd = D.sp[s].bounded != 0u;
+  const int32_t sh = D.sp[s].posShift;
+  const uint32_t tlo = (uint32_t)((int32_t)D.cellStart[c0] + sh);
+  const uint32_t thi = (uint32_t)((int32_t)D.cellStart[c0 + ncell] + sh);
+  const uint32_t plo = tlo + chunk * TILE_CHUNK;
+  const uint32_t phi = (uint32_t)mini((int32_t)thi, (int32_t)(plo + TILE_CHUNK));
+  const bool one = (uint32_t)__builtin_amdgcn_readfirstlane((int)(phi - plo)) <= 64u;
+  Dbg g;
+  if (bnd) {
+    // one-cell tiles of cells no wider than 4 wu: every particle's own
+    // stencil is the whole candidate union (no window) and |d| < 2^19
+    if (ncell == 1u && D.sp[s].cellShift <= 18u) {
+      if (!(one && nbr_chunk_staged<true, false>(D, lst, c0, ncell, s, plo, phi, lane, W0, g)))
+        nbr_chunk<true, false, NBR_PPL>(D, c0, ncell, s, plo, phi, lane, W0, g);
+    } else {
+      if (!(one && nbr_chunk_staged<true, true>(D, lst, c0, ncell, s, plo, phi, lane, W0, g)))
+        nbr_chunk<true, true, NBR_PPL>(D, c0, ncell, s, plo, phi, lane, W0, g);
+    }
+  } else {
+    nbr_chunk<false, true, NBR_PPL>(D, c0, ncell, s, plo, phi, lane, W0, g);
+  }
+  flush_dbg(D, s, g, 0);
+}
+
 __global__ __launch_bounds__(256) void k_nbr_tile(Dev D) {
   __shared__ uint16_t lists[4][MAXNBR * 64];
   const uint32_t lane = threadIdx.x & 63u;
+  uint16_t* lst = lists[threadIdx.x >> 6];
+  const int32_t W0 = D.lut4[0].x;
+  // heavy tiles first, each chunk taken by the next free wave anywhere
+  const uint32_t nheavy = (uint32_t)__builtin_amdgcn_readfirstlane((int)D.listCount[6]);
+  while (nheavy) {
+    uint32_t w = 0;
+    if (lane == 0u) w = atomicAdd(&D.listCount[7], 1u);
+    w = (uint32_t)__builtin_amdgcn_readfirstlane((int)w);
+    if (w >= nheavy) break;
+    nbr_item(D, D.dlist[6][w], lst, lane, W0);
+  }
   const uint32_t nwork = (uint32_t)__builtin_amdgcn_readfirstlane((int)D.listCount[4]);
   const ListRange R = xcd_range(nwork, 4u, threadIdx.x >> 6);
-  const int32_t W0 = D.lut4[0].x;
-  for (uint32_t w = R.first; w < R.hi; w += R.stride) {
-    const uint2 item = D.dlist[4][w];
-    const uint32_t tix = __builtin_amdgcn_readfirstlane(item.x);
-    const uint32_t chunk = __builtin_amdgcn_readfirstlane(item.y);
-    const uint2 tile = D.tileAll[tix];
-    const uint32_t c0 = __builtin_amdgcn_readfirstlane(tile.x);
-    const uint32_t yy = __builtin_amdgcn_readfirstlane(tile.y);
-    const uint32_t ncell = yy & 0xffu, s = yy >> 8;
-    const bool bnd = D.sp[s].bounded != 0u;
-    const int32_t sh = D.sp[s].posShift;
-    const uint32_t tlo = (uint32_t)((int32_t)D.cellStart[c0] + sh);
-    const uint32_t thi = (uint32_t)((int32_t)D.cellStart[c0 + ncell] + sh);
-    const uint32_t plo = tlo + chunk * TILE_CHUNK;
-    const uint32_t phi = (uint32_t)mini((int32_t)thi, (int32_t)(plo + TILE_CHUNK));
-    const bool one = (uint32_t)__builtin_amdgcn_readfirstlane((int)(phi - plo)) <= 64u;
-    Dbg g;
-    if (bnd) {
-      // one-cell tiles of cells no wider than 4 wu: every particle's own
-      // stencil is the whole candidate union (no window) and |d| < 2^19
-      uint16_t* lst = lists[threadIdx.x >> 6];
-      if (ncell == 1u && D.sp[s].cellShift <= 18u) {
-        if (!(one && nbr_chunk_staged<true, false>(D, lst, c0, ncell, s, plo, phi, lane, W0, g)))
-          nbr_chunk<true, false, NBR_PPL>(D, c0, ncell, s, plo, phi, lane, W0, g);
-      } else {
-        if (!(one && nbr_chunk_staged<true, true>(D, lst, c0, ncell, s, plo, phi, lane, W0, g)))
-          nbr_chunk<true, true, NBR_PPL>(D, c0, ncell, s, plo, phi, lane, W0, g);
-      }
-    } else {
-      nbr_chunk<false, true, NBR_PPL>(D, c0, ncell, s, plo, phi, lane, W0, g);
-    }
-    flush_dbg(D, s, g, 0);
-  }
+  for (uint32_t w = R.first; w < R.hi; w += R.stride) nbr_item(D, D.dlist[4][w], lst, lane, W0);
 }
 
 // ================================================================ solveA (k>0)
@@ -2022,7 +2057,7 @@ int sn_create(const sn_opts* opts, const int32_t* luts, const int32_t* mats, sn_
     D.cellScene = dalloc<uint32_t>(e, CC);
     D.tileAll = dalloc<uint2>(e, CC);
     // a list holds at most one item per cell (tile) plus one per full chunk
-    for (uint32_t c = 0; c < 6; ++c) D.dlist[c] = dalloc<uint2>(e, (size_t)CC + NC / CELL_CHUNK + 64);
+    for (uint32_t c = 0; c < 7; ++c) D.dlist[c] = dalloc<uint2>(e, (size_t)CC + NC / CELL_CHUNK + 64);
     D.listCount = dalloc<uint32_t>(e, 8);
     // persistent grids: as many workgroups as can be resident, a multiple of 8 (XCDs)
     {
