@@ -111,6 +111,8 @@ def main():
     ap.add_argument("--cpu-substeps", type=int, default=8, help="CPU-baseline window in substeps")
     ap.add_argument("--cpu-threads", type=int, default=0, help="0 = all available host cores")
     ap.add_argument("--no-cpu", action="store_true")
+    ap.add_argument("--no-other", action="store_true",
+                    help="1M run: skip timing the other build (profiler runs of one build)")
     ap.add_argument("--no-debug", action="store_true",
                     help="run on the stripped build (createEngine({debug: false})): same hashes, no debug counters")
     ap.add_argument("--scenes", type=int, default=512, help="batched: scene count")
@@ -379,23 +381,25 @@ def run_scene1m(args):
 
     # ---- the other build (debug counters on <-> off) over the same substeps,
     # graph-replayed; its per-substep stream must equal the timed one
-    other = Engine(max_particles=N, max_cells=C, iters=iters, stream_cap=max(4096, args.steps + args.warmup + 64),
-                   debug=args.no_debug)
-    other.load_scene(scene)
-    other.step(args.warmup)
-    other.sync()
-    other.reset_stream()
-    t3 = time.perf_counter()
-    other.step(args.steps)
-    other.sync()
-    w3 = time.perf_counter() - t3
-    other_build = {"debug_counters": bool(args.no_debug), "value": round(F * args.steps / w3, 1),
-                   "ms_per_step": round(w3 * 1e3 / args.steps, 4),
-                   "stream_equal": bool(np.array_equal(other.hash_stream(), hs)),
-                   "timing": "hipGraph-replayed substeps, no per-launch events"}
-    other.destroy()
-    print(f"# other build (debug={not args.no_debug} -> {args.no_debug}): {other_build['ms_per_step']:.3f} ms/substep, "
-          f"stream equal: {other_build['stream_equal']}", file=sys.stderr, flush=True)
+    other_build = None
+    if not args.no_other:
+        other = Engine(max_particles=N, max_cells=C, iters=iters, stream_cap=max(4096, args.steps + args.warmup + 64),
+                       debug=args.no_debug)
+        other.load_scene(scene)
+        other.step(args.warmup)
+        other.sync()
+        other.reset_stream()
+        t3 = time.perf_counter()
+        other.step(args.steps)
+        other.sync()
+        w3 = time.perf_counter() - t3
+        other_build = {"debug_counters": bool(args.no_debug), "value": round(F * args.steps / w3, 1),
+                       "ms_per_step": round(w3 * 1e3 / args.steps, 4),
+                       "stream_equal": bool(np.array_equal(other.hash_stream(), hs)),
+                       "timing": "hipGraph-replayed substeps, no per-launch events"}
+        other.destroy()
+        print(f"# other build (debug={not args.no_debug} -> {args.no_debug}): {other_build['ms_per_step']:.3f} "
+              f"ms/substep, stream equal: {other_build['stream_equal']}", file=sys.stderr, flush=True)
 
     # ---- CPU baseline: the C++ restatement on this host's cores, same scene
     cpu_baseline = None
