@@ -52,7 +52,8 @@ def algorithmic_bytes(F: int, N: int, C: int, k: int) -> dict:
 
 # kernel behind each profiled class (the HIP symbol rocprofv3 reports)
 KERNEL_OF = {"predict": "k_predict_count", "scan": "k_scan_reduce+k_scan_top+k_scan_down",
-             "scatter": "k_scatter", "canon_gather": "k_canon_gather", "nbr_solveA": "k_nbr_tile",
+             "scatter": "k_scatter", "canon_gather": "k_canon_gather+k_canon_heavy_rank+k_canon_heavy_place",
+             "nbr_solveA": "k_nbr_tile",
              "solveA": "k_solveA", "solveB": "k_solveB", "apply": "k_apply", "xnc": "k_xnc",
              "st_thermal_hash": "k_st_thermal_hash", "hash_finish": "k_hash_finish",
              "impulse": "k_impulse_*", "lists": "k_build_lists"}
