@@ -578,9 +578,9 @@ __device__ __forceinline__ uint32_t nbr_at(const Dev& D, const uint32_t* nb, uin
 // waves instead of stalling one.  Items are appended with wave-aggregated
 // atomics; every item is processed independently of the others, so list
 // order (which the atomics leave open) affects no result.
-constexpr int NBR_PPL = 2;             // particles per lane in the neighbour sweep
+constexpr int NBR_PPL = 1;             // particles per lane in the neighbour sweep
 constexpr uint32_t CELL_CHUNK = 32;
-constexpr uint32_t TILE_CHUNK = 128;   // 64 lanes x NBR_PPL
+constexpr uint32_t TILE_CHUNK = 64;    // 64 lanes x NBR_PPL
 // A tile whose candidate rows hold more than this many slots (a pile-up in
 // or next to it; 576 at 4x rest density) goes to dlist[6], whose chunks every
 // wave of the sweep takes first, one at a time from a shared counter, so that
