@@ -869,14 +869,39 @@ __device__ __forceinline__ void nbr_chunk(const Dev& D, uint32_t c0, uint32_t nc
         bool want = lane < qn && !(ci & INF_DEAD);   // nobody sees a corpse
         if constexpr (BND) want = want && cp.x >= bx0 && cp.x <= bx1 && cp.y >= by0 && cp.y <= by1;
         uint64_t todo = __ballot(want);
-        if (!D.dbgOn) {
-          // every list of the wave full: without debug counters only a
-          // candidate carrying SEAL or CONGEAL can still change a result
-          // (nbrN's seen bits); with none in the scene the sweep has stopped
-          bool full = true;
+        bool full = true;
 #pragma unroll
-          for (int k = 0; k < PPL; ++k) full = full && (!live[k] || cnt[k] >= MAXNBR);
-          if (__all(full)) todo &= __ballot(lane < qn && (ci & (INF_SEAL | INF_CONGEAL)));
+        for (int k = 0; k < PPL; ++k) full = full && (!live[k] || cnt[k] >= MAXNBR);
+        if (__all(full)) {
+          // every list of the wave is full: a remaining candidate only adds
+          // to `dropped` (debug counters) and to nbrN's SEAL / CONGEAL seen
+          // bits, so without debug counters only candidates carrying those
+          // bits are left (with none in the scene the sweep has stopped)
+          if (!D.dbgOn) todo &= __ballot(lane < qn && (ci & (INF_SEAL | INF_CONGEAL)));
+          if constexpr (BND) {
+            // branch-free count; the bounded pair geometry counts no event,
+            // and `live` gates every lane whose inputs are not a real pair
+            const uint64_t mS = __ballot(lane < qn && (ci & INF_SEAL));
+            const uint64_t mC = __ballot(lane < qn && (ci & INF_CONGEAL));
+            while (todo) {
+              const uint32_t j = (uint32_t)__builtin_ctzll(todo);
+              todo &= todo - 1ull;
+              const int2 cj = make_int2(__builtin_amdgcn_readlane(cp.x, j), __builtin_amdgcn_readlane(cp.y, j));
+              const int32_t qc = WIN ? __builtin_amdgcn_readlane(cc, j) : 0;
+              const uint32_t q = q0 + j;
+              const uint32_t fb = (((mS >> j) & 1ull) ? INF_SEAL : 0u) | (((mC >> j) & 1ull) ? INF_CONGEAL : 0u);
+#pragma unroll
+              for (int k = 0; k < PPL; ++k) {
+                const int2 d = pdelta<true>(pik[k], cj, g);
+                const int32_t idx = WIN ? lut_idx<true>(d, g) : lut_idx_near(d);
+                bool a = live[k] && q != pk[k] && idx >= 0;
+                if (WIN) a = a && qc >= cxk[k] - 1 && qc <= cxk[k] + 1;
+                dropped[k] += a ? 1u : 0u;
+                bits[k] |= a ? fb : 0u;
+              }
+            }
+            continue;
+          }
         }
         while (todo) {
           const uint32_t j = (uint32_t)__builtin_ctzll(todo);
@@ -994,7 +1019,23 @@ __device__ __forceinline__ bool nbr_chunk_staged(const Dev& D, uint16_t* lst, ui
       const uint64_t mSeal = __ballot(lane < qn && (ci & INF_SEAL));
       const uint64_t mCong = __ballot(lane < qn && (ci & INF_CONGEAL));
       const uint32_t selfj = p - q0;                 // lane index of the particle itself, if in this block
-      if (!D.dbgOn && __all(!live || cnt >= MAXNBR)) todo &= mSeal | mCong;   // as in nbr_chunk
+      if (__all(!live || cnt >= MAXNBR)) {
+        // every list full: count and flag only, branch-free (as in nbr_chunk)
+        if (!D.dbgOn) todo &= mSeal | mCong;
+        while (todo) {
+          const uint32_t j = (uint32_t)__builtin_ctzll(todo);
+          todo &= todo - 1ull;
+          const int2 cj = make_int2(__builtin_amdgcn_readlane(cp.x, j), __builtin_amdgcn_readlane(cp.y, j));
+          const int32_t qc = WIN ? __builtin_amdgcn_readlane(cc, j) : 0;
+          const int2 d = pdelta<BND>(pi, cj, g);
+          const int32_t idx = (BND && !WIN) ? lut_idx_near(d) : lut_idx<BND>(d, g);
+          bool a = live && j != selfj && idx >= 0;
+          if (WIN) a = a && qc >= cxk - 1 && qc <= cxk + 1;
+          dropped += a ? 1u : 0u;
+          bits |= a ? ((((mSeal >> j) & 1ull) ? INF_SEAL : 0u) | (((mCong >> j) & 1ull) ? INF_CONGEAL : 0u)) : 0u;
+        }
+        continue;
+      }
       while (todo) {
         const uint32_t j = (uint32_t)__builtin_ctzll(todo);
         todo &= todo - 1ull;
