@@ -184,6 +184,7 @@ struct Dev {
   uint32_t* nbrN;
   uint32_t* drop;         // candidates dropped by truncation, per particle
   uint32_t* crank;        // partial canonical ranks of heavy-cell slots (zero between uses)
+  uint32_t* hperm;        // packed one-cell tiles: their slots in spatial (Morton bin) order
   const uint32_t* blockMap;  // per-particle kernels: logical block -> 256-slot block holding live slots
   uint32_t* hpart;        // per-block partial sums [block][HPART_WORDS]
   const uint32_t* cellScene;  // global cell -> scene, INACTIVE if its scene is inactive
@@ -780,7 +781,8 @@ __device__ __forceinline__ void nbr_tail(const Dev& D, const SceneP& P, int32_t 
 
 template <bool BND, bool WIN, int PPL>
 __device__ __forceinline__ void nbr_chunk(const Dev& D, uint32_t c0, uint32_t ncell, uint32_t s,
-                                          uint32_t plo, uint32_t phi, uint32_t lane, int32_t W0, Dbg& g) {
+                                          uint32_t plo, uint32_t phi, uint32_t lane, int32_t W0, Dbg& g,
+                                          const uint32_t* perm) {
   const SceneP& P = D.sp[s];
   const int32_t sh = P.posShift;
   const uint32_t gridW = P.gridW, gridH = P.gridH, cellBase = P.cellBase;
@@ -795,13 +797,15 @@ __device__ __forceinline__ void nbr_chunk(const Dev& D, uint32_t c0, uint32_t nc
     uint32_t pk[PPL], inf[PPL], cnt[PPL], bits[PPL], dropped[PPL];
     int2 pik[PPL];
     int32_t cxk[PPL];
-    bool live[PPL];
+    bool live[PPL], vld[PPL];
     SolveAAcc acc[PPL];
     int32_t mnx = I32_MAX, mny = I32_MAX, mxx = I32_MIN, mxy = I32_MIN;
 #pragma unroll
     for (int k = 0; k < PPL; ++k) {
-      pk[k] = pbase + (uint32_t)k * 64u + lane;
-      const bool valid = pk[k] < phi;
+      const uint32_t li = pbase + (uint32_t)k * 64u + lane;
+      const bool valid = li < phi;
+      vld[k] = valid;
+      pk[k] = (perm && valid) ? perm[li] : li;
       inf[k] = valid ? D.info[pk[k]] : 0u;
       pik[k] = valid ? D.pred[pk[k]] : make_int2(0, 0);
       cxk[k] = valid ? (int32_t)(D.cellY[pk[k]] - myRow) : 0;
@@ -933,7 +937,7 @@ __device__ __forceinline__ void nbr_chunk(const Dev& D, uint32_t c0, uint32_t nc
     }
 #pragma unroll
     for (int k = 0; k < PPL; ++k) {
-      if (pk[k] >= phi) continue;
+      if (!vld[k]) continue;
       D.nbrN[pk[k]] = live[k] ? (cnt[k] | ((bits[k] & INF_SEAL) ? NBR_SEAL_BIT : 0u) |
                                  ((bits[k] & INF_CONGEAL) ? NBR_CONGEAL_BIT : 0u))
                               : 0u;
@@ -957,7 +961,7 @@ constexpr uint32_t ROW_OFF_MAX = 1u << 13;
 template <bool BND, bool WIN>
 __device__ __forceinline__ bool nbr_chunk_staged(const Dev& D, uint16_t* lst, uint32_t c0,
                                                  uint32_t ncell, uint32_t s, uint32_t plo, uint32_t phi,
-                                                 uint32_t lane, int32_t W0, Dbg& g) {
+                                                 uint32_t lane, int32_t W0, Dbg& g, const uint32_t* perm) {
   const SceneP& P = D.sp[s];
   const int32_t sh = P.posShift;
   const uint32_t gridW = P.gridW, gridH = P.gridH, cellBase = P.cellBase;
@@ -976,8 +980,8 @@ __device__ __forceinline__ bool nbr_chunk_staged(const Dev& D, uint16_t* lst, ui
     rhi[r] = (uint32_t)((int32_t)D.cellStart[rowc + (uint32_t)xhi + 1u] + sh);
     if (rhi[r] - rlo[r] > ROW_OFF_MAX) return false;
   }
-  const uint32_t p = plo + lane;
-  const bool valid = p < phi;
+  const bool valid = plo + lane < phi;
+  const uint32_t p = (perm && valid) ? perm[plo + lane] : plo + lane;
   const uint32_t inf = valid ? D.info[p] : 0u;
   const int2 pi = valid ? D.pred[p] : make_int2(0, 0);
   const int32_t cxk = valid ? (int32_t)(D.cellY[p] - myRow) : 0;
@@ -1097,7 +1101,60 @@ __device__ __forceinline__ bool nbr_chunk_staged(const Dev& D, uint16_t* lst, ui
   return true;
 }
 
-__device__ __forceinline__ void nbr_item(const Dev& D, uint2 item, uint16_t* lst, uint32_t lane, int32_t W0) {
+// Spatial chunks for packed cells.  Which particles share a wave of the
+// sweep changes no result (each builds its own list in its own canonical
+// order, and every sum is order-free), so a one-cell tile of the heavy list
+// with more than one chunk is chunked by position: k_heavy_order sorts its
+// slots into 8 x 8 Morton bins of the cell (a counting sort in LDS, one
+// workgroup per tile, bin order within a bin left to the LDS atomics) and
+// writes them to hperm.  A chunk of 64 neighbouring particles then has a
+// tight reach box, so whole blocks of far candidates drop out of its ballot.
+__device__ __forceinline__ uint32_t morton_bin(const SceneP& P, uint32_t c0, int2 pr) {
+  const uint32_t lc0 = c0 - P.cellBase;
+  const int32_t x0 = (int32_t)((lc0 % P.gridW) << P.cellShift), y0 = (int32_t)((lc0 / P.gridW) << P.cellShift);
+  const int32_t bs = maxi((int32_t)P.cellShift - 3, 0);
+  const uint32_t bx = (uint32_t)clampi((pr.x - x0) >> bs, 0, 7), by = (uint32_t)clampi((pr.y - y0) >> bs, 0, 7);
+  return (bx & 1u) | ((by & 1u) << 1) | ((bx & 2u) << 1) | ((by & 2u) << 2) | ((bx & 4u) << 2) | ((by & 4u) << 3);
+}
+
+__global__ __launch_bounds__(256) void k_heavy_order(Dev D) {
+  __shared__ uint32_t bins[64];
+  const uint32_t n = (uint32_t)__builtin_amdgcn_readfirstlane((int)D.listCount[6]);
+  for (uint32_t w = blockIdx.x; w < n; w += gridDim.x) {
+    const uint2 item = D.dlist[6][w];
+    const uint32_t chunk = (uint32_t)__builtin_amdgcn_readfirstlane((int)item.y);
+    if (chunk != 0u) continue;   // one workgroup per tile
+    const uint2 tile = D.tileAll[(uint32_t)__builtin_amdgcn_readfirstlane((int)item.x)];
+    const uint32_t c0 = (uint32_t)__builtin_amdgcn_readfirstlane((int)tile.x);
+    const uint32_t yy = (uint32_t)__builtin_amdgcn_readfirstlane((int)tile.y);
+    if ((yy & 0xffu) != 1u) continue;
+    const SceneP& P = D.sp[yy >> 8];
+    const uint32_t lo = (uint32_t)((int32_t)D.cellStart[c0] + P.posShift);
+    const uint32_t hi = (uint32_t)((int32_t)D.cellStart[c0 + 1] + P.posShift);
+    if (hi - lo <= TILE_CHUNK) continue;
+    __syncthreads();   // the previous tile's bins are no longer read
+    if (threadIdx.x < 64u) bins[threadIdx.x] = 0u;
+    __syncthreads();
+    for (uint32_t i = lo + threadIdx.x; i < hi; i += 256u) atomicAdd(&bins[morton_bin(P, c0, D.pred[i])], 1u);
+    __syncthreads();
+    if (threadIdx.x < 64u) {   // exclusive scan of the 64 bin counts, one wave
+      const uint32_t v = bins[threadIdx.x];
+      uint32_t incl = v;
+#pragma unroll
+      for (uint32_t off = 1; off < 64u; off <<= 1) {
+        const uint32_t u = (uint32_t)__shfl_up((int)incl, off);
+        if (threadIdx.x >= off) incl += u;
+      }
+      bins[threadIdx.x] = incl - v;
+    }
+    __syncthreads();
+    for (uint32_t i = lo + threadIdx.x; i < hi; i += 256u)
+      D.hperm[lo + atomicAdd(&bins[morton_bin(P, c0, D.pred[i])], 1u)] = i;
+  }
+}
+
+__device__ __forceinline__ void nbr_item(const Dev& D, uint2 item, uint16_t* lst, uint32_t lane, int32_t W0,
+                                         bool heavy) {
   const uint32_t tix = __builtin_amdgcn_readfirstlane(item.x);
   const uint32_t chunk = __builtin_amdgcn_readfirstlane(item.y);
   const uint2 tile = D.tileAll[tix];
@@ -1111,19 +1168,22 @@ __device__ __forceinline__ void nbr_item(const Dev& D, uint2 item, uint16_t* lst
   const uint32_t plo = tlo + chunk * TILE_CHUNK;
   const uint32_t phi = (uint32_t)mini((int32_t)thi, (int32_t)(plo + TILE_CHUNK));
   const bool one = (uint32_t)__builtin_amdgcn_readfirstlane((int)(phi - plo)) <= 64u;
+  // a packed one-cell tile of the heavy list takes its chunks in the spatial
+  // order k_heavy_order built, so each wave's particles lie close together
+  const uint32_t* perm = (heavy && ncell == 1u && thi - tlo > TILE_CHUNK) ? D.hperm : nullptr;
   Dbg g;
   if (bnd) {
     // one-cell tiles of cells no wider than 4 wu: every particle's own
     // stencil is the whole candidate union (no window) and |d| < 2^19
     if (ncell == 1u && D.sp[s].cellShift <= 18u) {
-      if (!(one && nbr_chunk_staged<true, false>(D, lst, c0, ncell, s, plo, phi, lane, W0, g)))
-        nbr_chunk<true, false, NBR_PPL>(D, c0, ncell, s, plo, phi, lane, W0, g);
+      if (!(one && nbr_chunk_staged<true, false>(D, lst, c0, ncell, s, plo, phi, lane, W0, g, perm)))
+        nbr_chunk<true, false, NBR_PPL>(D, c0, ncell, s, plo, phi, lane, W0, g, perm);
     } else {
-      if (!(one && nbr_chunk_staged<true, true>(D, lst, c0, ncell, s, plo, phi, lane, W0, g)))
-        nbr_chunk<true, true, NBR_PPL>(D, c0, ncell, s, plo, phi, lane, W0, g);
+      if (!(one && nbr_chunk_staged<true, true>(D, lst, c0, ncell, s, plo, phi, lane, W0, g, perm)))
+        nbr_chunk<true, true, NBR_PPL>(D, c0, ncell, s, plo, phi, lane, W0, g, perm);
     }
   } else {
-    nbr_chunk<false, true, NBR_PPL>(D, c0, ncell, s, plo, phi, lane, W0, g);
+    nbr_chunk<false, true, NBR_PPL>(D, c0, ncell, s, plo, phi, lane, W0, g, perm);
   }
   flush_dbg(D, s, g, 0);
 }
@@ -1140,11 +1200,11 @@ __global__ __launch_bounds__(256) void k_nbr_tile(Dev D) {
     if (lane == 0u) w = atomicAdd(&D.listCount[7], 1u);
     w = (uint32_t)__builtin_amdgcn_readfirstlane((int)w);
     if (w >= nheavy) break;
-    nbr_item(D, D.dlist[6][w], lst, lane, W0);
+    nbr_item(D, D.dlist[6][w], lst, lane, W0, true);
   }
   const uint32_t nwork = (uint32_t)__builtin_amdgcn_readfirstlane((int)D.listCount[4]);
   const ListRange R = xcd_range(nwork, 4u, threadIdx.x >> 6);
-  for (uint32_t w = R.first; w < R.hi; w += R.stride) nbr_item(D, D.dlist[4][w], lst, lane, W0);
+  for (uint32_t w = R.first; w < R.hi; w += R.stride) nbr_item(D, D.dlist[4][w], lst, lane, W0, false);
 }
 
 // ================================================================ solveA (k>0)
@@ -1905,6 +1965,7 @@ void enqueue_substep(sn_engine* e) {
   }
   {
     ProfScope ps(e, PC_NBR);
+    hipLaunchKernelGGL(k_heavy_order, dim3(e->canonBlocks), dim3(256), 0, st, D);
     hipLaunchKernelGGL(k_nbr_tile, dim3(e->nbrBlocks), dim3(256), 0, st, D);
   }
   const uint32_t iters = e->opts.iters;
@@ -2066,6 +2127,7 @@ int sn_create(const sn_opts* opts, const int32_t* luts, const int32_t* mats, sn_
     D.nbrN = dalloc<uint32_t>(e, NC);
     D.drop = dalloc<uint32_t>(e, NC);
     D.crank = dalloc<uint32_t>(e, NC);
+    D.hperm = dalloc<uint32_t>(e, NC);
     e->dBlockMap = dalloc<uint32_t>(e, div_up(std::max(NC, 1u), 256));
     D.blockMap = e->dBlockMap;
     {
