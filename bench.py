@@ -50,6 +50,9 @@ def algorithmic_bytes(F: int, N: int, C: int, k: int) -> dict:
     return ph
 
 
+DOMINANT = "nbr_solveA"     # the neighbour sweep: the kernel class timed inside the timed region
+PROFILE_SUBSTEPS = 20       # follow-on substeps profiled with every kernel class timed
+
 # kernel behind each profiled class (the HIP symbol rocprofv3 reports)
 KERNEL_OF = {"predict": "k_predict_count", "scan": "k_scan_reduce+k_scan_top+k_scan_down",
              "scatter": "k_scatter", "canon_gather": "k_canon_gather+k_canon_heavy_rank+k_canon_heavy_place",
@@ -316,12 +319,14 @@ def run_scene1m(args):
     eng.step(args.warmup)
     eng.sync()
     eng.reset_stream()
-    # ---- timed region: K substeps, inputs resident in HBM.  Launched eagerly
-    # with a HIP event pair around every launch on the engine's stream, so the
-    # same region yields each kernel's average launch duration.
+    # ---- timed region: K substeps, inputs resident in HBM, launched eagerly
+    # with a HIP event pair around every launch of the dominant kernel (the
+    # neighbour sweep) on the engine's stream, so the region yields that
+    # kernel's average launch duration without timing events between the
+    # other launches.
     eng.sync()
     t1 = time.perf_counter()
-    prof = eng.profile_step(args.steps)
+    prof = eng.profile_step(args.steps, classes=(DOMINANT,))
     eng.sync()
     t2 = time.perf_counter()
     wall = t2 - t1
@@ -332,22 +337,32 @@ def run_scene1m(args):
     dbg = eng.debug_counters()
     print(f"# timed {args.steps} substeps: {ms_per_step:.3f} ms/substep, {value:.4e} ps/s",
           file=sys.stderr, flush=True)
+    # every kernel class, over the next PROFILE_SUBSTEPS substeps of the same run
+    prof_all = eng.profile_step(PROFILE_SUBSTEPS)
 
     by = algorithmic_bytes(F, N, C, iters)
     cls_bytes = class_bytes(F, N, C, iters)
     assert sum(cls_bytes.values()) == by["total"]
-    per_launch = {}
-    for k, v in prof.items():
-        if k == "total" or v["calls"] == 0:
-            continue
-        avg_ms = v["ms"] / v["calls"]
-        b = cls_bytes.get(k, 0) * args.steps / v["calls"]
-        per_launch[k] = {"kernel": KERNEL_OF.get(k, k), "launches_per_substep": v["calls"] / args.steps,
-                         "avg_launch_us": round(avg_ms * 1e3, 2), "ms_per_substep": round(v["ms"] / args.steps, 4),
-                         "algorithmic_bytes_per_launch": int(b),
-                         "achieved_GBs": round(b / (avg_ms * 1e-3) / 1e9, 2) if avg_ms > 0 else 0.0}
-    dom = max(per_launch, key=lambda k: per_launch[k]["ms_per_substep"])
-    d = per_launch[dom]
+
+    def launch_stats(pr, steps):
+        out = {}
+        for k, v in pr.items():
+            if k == "total" or v["calls"] == 0:
+                continue
+            avg_ms = v["ms"] / v["calls"]
+            b = cls_bytes.get(k, 0) * steps / v["calls"]
+            out[k] = {"kernel": KERNEL_OF.get(k, k), "launches_per_substep": v["calls"] / steps,
+                      "avg_launch_us": round(avg_ms * 1e3, 2), "ms_per_substep": round(v["ms"] / steps, 4),
+                      "algorithmic_bytes_per_launch": int(b),
+                      "achieved_GBs": round(b / (avg_ms * 1e-3) / 1e9, 2) if avg_ms > 0 else 0.0}
+        return out
+
+    timed = launch_stats(prof, args.steps)
+    per_launch = launch_stats(prof_all, PROFILE_SUBSTEPS)
+    if max(per_launch, key=lambda k: per_launch[k]["ms_per_substep"]) != DOMINANT:
+        print("# note: the timed kernel is not the largest in the follow-on profile", file=sys.stderr, flush=True)
+    dom = DOMINANT
+    d = timed[dom]
     achieved_all = by["total"] / (ms_per_step * 1e-3) / 1e9
     # HBM bytes per launch of the dominant kernel as rocprofv3 counted them
     # (FETCH_SIZE and WRITE_SIZE runs of this workload, each calibrated by a
@@ -378,6 +393,8 @@ def run_scene1m(args):
                           "achieved_GBs": round(achieved_all, 2),
                           "frac": round(achieved_all / HBM_PEAK_GBS, 6)},
         "per_kernel": per_launch,
+        "per_kernel_window": (f"all kernel classes, event-timed over substeps {args.warmup + args.steps + 1}.."
+                              f"{args.warmup + args.steps + PROFILE_SUBSTEPS} (after the timed region)"),
     }
 
     # ---- the other build (debug counters on <-> off) over the same substeps,

@@ -148,17 +148,18 @@ int sn_load_slot(sn_engine* e, uint32_t slot, const sn_scene* scene);
 
 /* Profiling form of sn_step (no counterpart in the reference, which has no
  * device timers): enqueue `count` substeps eagerly with a HIP event pair
- * around every launch on the engine's stream, wait, and return per kernel
- * class the summed device ms and the launch count.  Slot 0 is the whole
- * span (first launch start to last launch end; calls = substeps); slots
- * 1.. are: [1] predict + cell count, [2] cell scan, [3] scatter,
+ * around every launch of the kernel classes selected by `class_mask` (bit
+ * c = slot c + 1; ~0u = all) on the engine's stream, wait, and return per
+ * class the summed device ms and the launch count.  Slot 0 is the span from
+ * the first timed launch's start to the last one's end (calls = substeps);
+ * slots 1.. are: [1] predict + cell count, [2] cell scan, [3] scatter,
  * [4] canonical gather, [5] neighbour build + iteration-0 density,
  * [6] density iterations 1.., [7] solveB per colour, [8] xsph + normals +
  * conduct, [9] surface tension + thermal + hash, [10] chain fold, [11] stir
  * events, [12] work-list build, [13] applyDp per colour (finalize on the
  * last iteration). */
 #define SN_PROF_SLOTS 16
-int sn_profile_step(sn_engine* e, uint32_t count, float out_ms[SN_PROF_SLOTS],
+int sn_profile_step(sn_engine* e, uint32_t count, uint32_t class_mask, float out_ms[SN_PROF_SLOTS],
                     uint32_t out_calls[SN_PROF_SLOTS]);
 
 /* ===================================================================== 3D

@@ -1860,6 +1860,7 @@ struct sn_engine {
   std::vector<int> evClass;
   uint32_t evUsed = 0;
   bool profiling = false;
+  uint32_t profMask = ~0u;        // kernel classes timed while profiling (bit = ProfClass)
 };
 
 namespace {
@@ -1908,11 +1909,12 @@ struct ProfScope {
   sn_engine* e;
   int cls;
   ProfScope(sn_engine* e_, int cls_) : e(e_), cls(cls_) {
-    if (e->profiling) record();
+    if (on()) record();
   }
   ~ProfScope() {
-    if (e->profiling) record();
+    if (on()) record();
   }
+  bool on() const { return e->profiling && ((e->profMask >> cls) & 1u); }
   void record() {
     if (e->evUsed >= e->evPool.size()) {
       hipEvent_t ev;
@@ -2531,7 +2533,8 @@ int sn_query(sn_engine* e) {
   return fail(e, SN_E_DEVICE, std::string("hipStreamQuery: ") + hipGetErrorString(q));
 }
 
-int sn_profile_step(sn_engine* e, uint32_t count, float out_ms[SN_PROF_SLOTS], uint32_t out_calls[SN_PROF_SLOTS]) {
+int sn_profile_step(sn_engine* e, uint32_t count, uint32_t class_mask, float out_ms[SN_PROF_SLOTS],
+                    uint32_t out_calls[SN_PROF_SLOTS]) {
   if (!e || !out_ms || !out_calls) return SN_E_ARG;
   if (!e->loaded) return fail(e, SN_E_STATE, "[aether] step() called before loadScene()");
   try {
@@ -2542,6 +2545,7 @@ int sn_profile_step(sn_engine* e, uint32_t count, float out_ms[SN_PROF_SLOTS], u
     }
     if (count == 0) return SN_OK;
     e->profiling = true;
+    e->profMask = class_mask;
     e->evUsed = 0;
     try {
       for (uint32_t k = 0; k < count; ++k) enqueue_substep(e);
@@ -2551,6 +2555,11 @@ int sn_profile_step(sn_engine* e, uint32_t count, float out_ms[SN_PROF_SLOTS], u
     }
     e->profiling = false;
     const uint32_t used = e->evUsed;
+    if (used < 2) {   // no launch of the selected classes
+      HIPCHECK(hipStreamSynchronize(e->stream));
+      out_calls[0] = count;
+      return check_fault(e);
+    }
     HIPCHECK(hipEventSynchronize(e->evPool[used - 1]));
     for (uint32_t i = 0; i + 1 < used; i += 2) {
       float t;

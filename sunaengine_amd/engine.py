@@ -261,15 +261,21 @@ class Engine:
     PROFILE_KEYS = ("total", "predict", "scan", "scatter", "canon_gather", "nbr_solveA", "solveA",
                     "solveB", "xnc", "st_thermal_hash", "hash_finish", "impulse", "lists", "apply")
 
-    def profile_step(self, count: int = 1) -> dict:
-        """Run `count` substeps eagerly with an event pair around every launch.
+    def profile_step(self, count: int = 1, classes: Optional[Iterable[str]] = None) -> dict:
+        """Run `count` substeps eagerly with an event pair around every launch
+        of the kernel classes named in `classes` (all when None).
 
         Returns {class: {"ms": summed device ms, "calls": launches}}; "total"
-        is the span from the first launch's start to the last one's end.
+        is the span from the first timed launch's start to the last one's end.
         """
+        mask = 0xFFFFFFFF
+        if classes is not None:
+            mask = 0
+            for c in classes:
+                mask |= 1 << (self.PROFILE_KEYS.index(c) - 1)
         ms = np.zeros(16, np.float32)
         calls = np.zeros(16, np.uint32)
-        self._check(native.lib().sn_profile_step(self._h, int(count), ms.ctypes.data, calls.ctypes.data))
+        self._check(native.lib().sn_profile_step(self._h, int(count), mask, ms.ctypes.data, calls.ctypes.data))
         return {k: {"ms": float(ms[i]), "calls": int(calls[i])} for i, k in enumerate(self.PROFILE_KEYS)}
 
 

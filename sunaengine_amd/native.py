@@ -109,7 +109,7 @@ def lib():
         "sn_query": (C.c_int, [P]),
         "sn3_query": (C.c_int, [P]),
         "sn_load_slot": (C.c_int, [P, u32, P]),
-        "sn_profile_step": (C.c_int, [P, u32, P, P]),
+        "sn_profile_step": (C.c_int, [P, u32, u32, P, P]),
         "sn3_create": (C.c_int, [C.POINTER(sn_opts), P, u32, C.POINTER(P)]),
         "sn3_destroy": (None, [P]),
         "sn3_last_error": (C.c_char_p, [P]),
