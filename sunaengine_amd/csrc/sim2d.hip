@@ -705,6 +705,22 @@ __device__ __forceinline__ int32_t wave_max(int32_t v) {
   return v;
 }
 
+// The part of a stencil row (row gy, cells xa..xb) that a reach box can
+// reach.  A position lies in its cell's bounds except in the grid's edge
+// cells, where cell_coord clamps, so only interior rows and cells are cut:
+// a row whose y-range misses the box, and cells at either end whose x-range
+// misses it.  What remains is one contiguous sorted range, still visited in
+// canonical order; a candidate cut here lies outside the box, so every lane
+// would reject it.  False when nothing of the row remains.
+__device__ __forceinline__ bool row_span(const SceneP& P, int32_t gy, int32_t& xa, int32_t& xb, int32_t bx0,
+                                         int32_t bx1, int32_t by0, int32_t by1) {
+  const int32_t cs = (int32_t)P.cellShift, gw = (int32_t)P.gridW, gh = (int32_t)P.gridH;
+  if (gy >= 1 && gy <= gh - 2 && (((gy + 1) << cs) <= by0 || (gy << cs) > by1)) return false;
+  while (xa <= xb && xa >= 1 && xa <= gw - 2 && ((xa + 1) << cs) <= bx0) ++xa;
+  while (xb >= xa && xb >= 1 && xb <= gw - 2 && (xb << cs) > bx1) --xb;
+  return xa <= xb;
+}
+
 // The sweep's tail for one particle (lane L of the wave) whose list is not
 // yet full while the rest of the wave is: the remaining candidates, from row
 // dy0 at slot q0s on, are tested one per lane, 64 at a time, and the
@@ -735,10 +751,13 @@ __device__ __forceinline__ void nbr_tail(const Dev& D, const SceneP& P, int32_t 
   for (int32_t dy = dy0; dy <= 1 && c < MAXNBR; ++dy) {
     const int32_t gy = cy + dy;
     if (gy < 0 || gy >= (int32_t)P.gridH) continue;
+    int32_t xa = xlo, xb = xhi;
+    // the particle's own reach box (bounded scenes; positions within +-2^28)
+    if (BND && !row_span(P, gy, xa, xb, pL.x - REACH, pL.x + REACH, pL.y - REACH, pL.y + REACH)) continue;
     const uint32_t rowc = P.cellBase + (uint32_t)gy * P.gridW;
-    const uint32_t rlo = (uint32_t)((int32_t)D.cellStart[rowc + (uint32_t)xlo] + sh);
-    const uint32_t rhi = (uint32_t)((int32_t)D.cellStart[rowc + (uint32_t)xhi + 1u] + sh);
-    for (uint32_t q0 = dy == dy0 ? q0s : rlo; q0 < rhi && c < MAXNBR; q0 += 64u) {
+    const uint32_t rlo = (uint32_t)((int32_t)D.cellStart[rowc + (uint32_t)xa] + sh);
+    const uint32_t rhi = (uint32_t)((int32_t)D.cellStart[rowc + (uint32_t)xb + 1u] + sh);
+    for (uint32_t q0 = dy == dy0 ? max(q0s, rlo) : rlo; q0 < rhi && c < MAXNBR; q0 += 64u) {
       const uint32_t q = q0 + lane;
       bool ok = q < rhi;
       int2 cp = make_int2(0, 0);
@@ -833,9 +852,11 @@ __device__ __forceinline__ void nbr_chunk(const Dev& D, uint32_t c0, uint32_t nc
     for (int32_t dy = -1; dy <= 1 && !done; ++dy) {
       const int32_t gy = cy + dy;
       if (gy < 0 || gy >= (int32_t)gridH) continue;
+      int32_t xa = xlo, xb = xhi;
+      if (BND && !row_span(P, gy, xa, xb, bx0, bx1, by0, by1)) continue;
       const uint32_t rowc = cellBase + (uint32_t)gy * gridW;
-      const uint32_t rlo = (uint32_t)((int32_t)D.cellStart[rowc + (uint32_t)xlo] + sh);
-      const uint32_t rhi = (uint32_t)((int32_t)D.cellStart[rowc + (uint32_t)xhi + 1u] + sh);
+      const uint32_t rlo = (uint32_t)((int32_t)D.cellStart[rowc + (uint32_t)xa] + sh);
+      const uint32_t rhi = (uint32_t)((int32_t)D.cellStart[rowc + (uint32_t)xb + 1u] + sh);
       for (uint32_t q0 = rlo; q0 < rhi; q0 += 64u) {
         if (stopFull) {
           uint32_t open = 0;   // lists still filling, over the wave
@@ -969,17 +990,6 @@ __device__ __forceinline__ bool nbr_chunk_staged(const Dev& D, uint16_t* lst, ui
   const int32_t cy = (int32_t)(lc0 / gridW), cx0 = (int32_t)(lc0 % gridW);
   const int32_t xlo = maxi(cx0 - 1, 0), xhi = mini(cx0 + (int32_t)ncell, (int32_t)gridW - 1);
   const uint32_t myRow = cellBase + (uint32_t)cy * gridW;
-  uint32_t rlo[3], rhi[3];
-#pragma unroll
-  for (int r = 0; r < 3; ++r) {
-    const int32_t gy = cy + r - 1;
-    rlo[r] = rhi[r] = 0;
-    if (gy < 0 || gy >= (int32_t)gridH) continue;
-    const uint32_t rowc = cellBase + (uint32_t)gy * gridW;
-    rlo[r] = (uint32_t)((int32_t)D.cellStart[rowc + (uint32_t)xlo] + sh);
-    rhi[r] = (uint32_t)((int32_t)D.cellStart[rowc + (uint32_t)xhi + 1u] + sh);
-    if (rhi[r] - rlo[r] > ROW_OFF_MAX) return false;
-  }
   const bool valid = plo + lane < phi;
   const uint32_t p = (perm && valid) ? perm[plo + lane] : plo + lane;
   const uint32_t inf = valid ? D.info[p] : 0u;
@@ -993,6 +1003,20 @@ __device__ __forceinline__ bool nbr_chunk_staged(const Dev& D, uint16_t* lst, ui
     bx1 = wave_max(live ? pi.x : I32_MIN) + REACH;
     by0 = wave_min(live ? pi.y : I32_MAX) - REACH;
     by1 = wave_max(live ? pi.y : I32_MIN) + REACH;
+  }
+  // each row's candidates, cut to the reach box (row_span)
+  uint32_t rlo[3], rhi[3];
+#pragma unroll
+  for (int r = 0; r < 3; ++r) {
+    const int32_t gy = cy + r - 1;
+    rlo[r] = rhi[r] = 0;
+    if (gy < 0 || gy >= (int32_t)gridH) continue;
+    int32_t xa = xlo, xb = xhi;
+    if (BND && !row_span(P, gy, xa, xb, bx0, bx1, by0, by1)) continue;
+    const uint32_t rowc = cellBase + (uint32_t)gy * gridW;
+    rlo[r] = (uint32_t)((int32_t)D.cellStart[rowc + (uint32_t)xa] + sh);
+    rhi[r] = (uint32_t)((int32_t)D.cellStart[rowc + (uint32_t)xb + 1u] + sh);
+    if (rhi[r] - rlo[r] > ROW_OFF_MAX) return false;
   }
   const bool stopFull = !D.dbgOn && D.sd[s].flagged == 0u;   // as in nbr_chunk
   bool done = false;
