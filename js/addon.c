@@ -292,6 +292,246 @@ static napi_value js_set_active(napi_env env, napi_callback_info info) {
   return NULL;
 }
 
+/* ---- slotted worlds (continuous batching) and the non-blocking query ---- */
+
+/* reserveSlots(h, capParticles:Uint32Array, capCells:Uint32Array) */
+static napi_value js_reserve_slots(napi_env env, napi_callback_info info) {
+  napi_value a[3];
+  get_args(env, info, 3, a);
+  sn_engine* e = handle(env, a[0]);
+  size_t n = 0, m = 0;
+  const uint32_t* cp = (const uint32_t*)typed_data(env, a[1], &n);
+  const uint32_t* cc = (const uint32_t*)typed_data(env, a[2], &m);
+  if (n != m) {
+    napi_throw_error(env, NULL, "reserveSlots: capacity arrays differ in length");
+    return NULL;
+  }
+  int rc = sn_reserve_slots(e, (uint32_t)n, cp, cc);
+  if (rc) return throw_sn(env, e, rc);
+  return NULL;
+}
+
+/* loadSlot(h, slot, {n, nFluid, state:Int32Array, domW, domH, cellShift}) */
+static napi_value js_load_slot(napi_env env, napi_callback_info info) {
+  napi_value a[3], v;
+  get_args(env, info, 3, a);
+  sn_engine* e = handle(env, a[0]);
+  sn_scene sc;
+  memset(&sc, 0, sizeof(sc));
+  napi_get_named_property(env, a[2], "n", &v);
+  sc.n = u32(env, v);
+  napi_get_named_property(env, a[2], "nFluid", &v);
+  sc.n_fluid = u32(env, v);
+  napi_get_named_property(env, a[2], "state", &v);
+  sc.state8 = (const int32_t*)typed_data(env, v, NULL);
+  napi_get_named_property(env, a[2], "domW", &v);
+  sc.dom_w = i32(env, v);
+  napi_get_named_property(env, a[2], "domH", &v);
+  sc.dom_h = i32(env, v);
+  napi_get_named_property(env, a[2], "cellShift", &v);
+  sc.cell_shift = u32(env, v);
+  int rc = sn_load_slot(e, u32(env, a[1]), &sc);
+  if (rc) return throw_sn(env, e, rc);
+  return NULL;
+}
+
+/* query(h) -> true while enqueued work is still running */
+static napi_value js_query(napi_env env, napi_callback_info info) {
+  napi_value a[1], r;
+  get_args(env, info, 1, a);
+  sn_engine* e = handle(env, a[0]);
+  int rc = sn_query(e);
+  if (rc < 0) return throw_sn(env, e, rc);
+  napi_get_boolean(env, rc == 1, &r);
+  return r;
+}
+
+/* ---- the 3D solver ---- */
+static sn3_engine* handle3(napi_env env, napi_value v) {
+  void* p = NULL;
+  napi_get_value_external(env, v, &p);
+  return (sn3_engine*)p;
+}
+static napi_value throw_sn3(napi_env env, sn3_engine* e, int rc) {
+  const char* m = e ? sn3_last_error(e) : "sn3 call failed";
+  char buf[512];
+  snprintf(buf, sizeof buf, "%s (code %d)", (m && *m) ? m : "sn3 call failed", rc);
+  napi_throw_error(env, NULL, buf);
+  return NULL;
+}
+static int scene3_of(napi_env env, napi_value o, sn3_scene* sc) {
+  napi_value v;
+  napi_get_named_property(env, o, "n", &v);
+  sc->n = u32(env, v);
+  napi_get_named_property(env, o, "parts", &v);
+  sc->parts16 = (const int32_t*)typed_data(env, v, NULL);
+  napi_get_named_property(env, o, "gx", &v);
+  sc->gx = u32(env, v);
+  napi_get_named_property(env, o, "gy", &v);
+  sc->gy = u32(env, v);
+  napi_get_named_property(env, o, "gz", &v);
+  sc->gz = u32(env, v);
+  return 0;
+}
+
+/* create3(maxParticles, maxCells, maxScenes, iters, streamCap, device, luts3:Int32Array [, debug = 1]) */
+static napi_value js_create3(napi_env env, napi_callback_info info) {
+  napi_value a[8];
+  const int argc = get_args(env, info, 8, a);
+  if (argc < 7) {
+    napi_throw_error(env, NULL, "create3: 7 arguments expected");
+    return NULL;
+  }
+  sn_opts o;
+  memset(&o, 0, sizeof(o));
+  o.max_particles = u32(env, a[0]);
+  o.max_cells = u32(env, a[1]);
+  o.max_scenes = u32(env, a[2]);
+  o.iters = u32(env, a[3]);
+  o.stream_cap = u32(env, a[4]);
+  o.device = u32(env, a[5]);
+  o.no_debug = (argc > 7 && u32(env, a[7]) == 0) ? 1u : 0u;
+  size_t words = 0;
+  const int32_t* luts = (const int32_t*)typed_data(env, a[6], &words);
+  sn3_engine* e = NULL;
+  int rc = sn3_create(&o, luts, (uint32_t)words, &e);
+  if (rc) {
+    napi_value r = throw_sn3(env, e, rc);
+    if (e) sn3_destroy(e);
+    return r;
+  }
+  napi_value ext;
+  CALL(env, napi_create_external(env, e, NULL, NULL, &ext));
+  return ext;
+}
+static napi_value js_destroy3(napi_env env, napi_callback_info info) {
+  napi_value a[1];
+  get_args(env, info, 1, a);
+  sn3_destroy(handle3(env, a[0]));
+  return NULL;
+}
+/* load3(h, [{n, parts:Int32Array (n*16), gx, gy, gz}]) */
+static napi_value js_load3(napi_env env, napi_callback_info info) {
+  napi_value a[2];
+  get_args(env, info, 2, a);
+  sn3_engine* e = handle3(env, a[0]);
+  uint32_t count = 0;
+  napi_get_array_length(env, a[1], &count);
+  sn3_scene* sc = (sn3_scene*)calloc(count ? count : 1, sizeof(sn3_scene));
+  for (uint32_t k = 0; k < count; ++k) {
+    napi_value o;
+    napi_get_element(env, a[1], k, &o);
+    scene3_of(env, o, &sc[k]);
+  }
+  int rc = sn3_load(e, sc, count);
+  free(sc);
+  if (rc) return throw_sn3(env, e, rc);
+  return NULL;
+}
+/* reserveSlots3(h, capParticles:Uint32Array, capCells:Uint32Array) */
+static napi_value js_reserve_slots3(napi_env env, napi_callback_info info) {
+  napi_value a[3];
+  get_args(env, info, 3, a);
+  sn3_engine* e = handle3(env, a[0]);
+  size_t n = 0, m = 0;
+  const uint32_t* cp = (const uint32_t*)typed_data(env, a[1], &n);
+  const uint32_t* cc = (const uint32_t*)typed_data(env, a[2], &m);
+  if (n != m) {
+    napi_throw_error(env, NULL, "reserveSlots3: capacity arrays differ in length");
+    return NULL;
+  }
+  int rc = sn3_reserve_slots(e, (uint32_t)n, cp, cc);
+  if (rc) return throw_sn3(env, e, rc);
+  return NULL;
+}
+/* loadSlot3(h, slot, {n, parts, gx, gy, gz}) */
+static napi_value js_load_slot3(napi_env env, napi_callback_info info) {
+  napi_value a[3];
+  get_args(env, info, 3, a);
+  sn3_engine* e = handle3(env, a[0]);
+  sn3_scene sc;
+  scene3_of(env, a[2], &sc);
+  int rc = sn3_load_slot(e, u32(env, a[1]), &sc);
+  if (rc) return throw_sn3(env, e, rc);
+  return NULL;
+}
+/* setEvents3(h, Int32Array of 10-word events sorted by (scene, substep)) */
+static napi_value js_set_events3(napi_env env, napi_callback_info info) {
+  napi_value a[2];
+  get_args(env, info, 2, a);
+  sn3_engine* e = handle3(env, a[0]);
+  size_t n = 0;
+  const int32_t* w = (const int32_t*)typed_data(env, a[1], &n);
+  int rc = sn3_set_events(e, (const sn3_event*)w, (uint32_t)(n / 10));
+  if (rc) return throw_sn3(env, e, rc);
+  return NULL;
+}
+static napi_value js_set_active3(napi_env env, napi_callback_info info) {
+  napi_value a[2];
+  get_args(env, info, 2, a);
+  sn3_engine* e = handle3(env, a[0]);
+  size_t n = 0;
+  const uint8_t* m = (const uint8_t*)typed_data(env, a[1], &n);
+  int rc = sn3_set_active(e, m, (uint32_t)n);
+  if (rc) return throw_sn3(env, e, rc);
+  return NULL;
+}
+static napi_value js_step3(napi_env env, napi_callback_info info) {
+  napi_value a[2];
+  get_args(env, info, 2, a);
+  sn3_engine* e = handle3(env, a[0]);
+  int rc = sn3_step(e, u32(env, a[1]));
+  if (rc) return throw_sn3(env, e, rc);
+  return NULL;
+}
+static napi_value js_sync3(napi_env env, napi_callback_info info) {
+  napi_value a[1];
+  get_args(env, info, 1, a);
+  sn3_engine* e = handle3(env, a[0]);
+  int rc = sn3_sync(e);
+  if (rc) return throw_sn3(env, e, rc);
+  return NULL;
+}
+static napi_value js_query3(napi_env env, napi_callback_info info) {
+  napi_value a[1], r;
+  get_args(env, info, 1, a);
+  sn3_engine* e = handle3(env, a[0]);
+  int rc = sn3_query(e);
+  if (rc < 0) return throw_sn3(env, e, rc);
+  napi_get_boolean(env, rc == 1, &r);
+  return r;
+}
+/* readHashStream3(h, scene, maxSubsteps) -> Uint32Array (8 words per substep) */
+static napi_value js_read_stream3(napi_env env, napi_callback_info info) {
+  napi_value a[3];
+  get_args(env, info, 3, a);
+  sn3_engine* e = handle3(env, a[0]);
+  uint32_t scene = u32(env, a[1]), maxs = u32(env, a[2]);
+  uint32_t* tmp = (uint32_t*)malloc((size_t)(maxs ? maxs : 1) * 32);
+  size_t n = 0;
+  int rc = sn3_read_stream(e, scene, tmp, maxs, &n);
+  if (rc) {
+    free(tmp);
+    return throw_sn3(env, e, rc);
+  }
+  void* d = NULL;
+  napi_value arr = new_typed(env, napi_uint32_array, n * 8, 4, &d);
+  memcpy(d, tmp, n * 32);
+  free(tmp);
+  return arr;
+}
+/* readHash3(h, scene) -> Uint32Array(8) */
+static napi_value js_read_hash3(napi_env env, napi_callback_info info) {
+  napi_value a[2];
+  get_args(env, info, 2, a);
+  sn3_engine* e = handle3(env, a[0]);
+  void* d = NULL;
+  napi_value arr = new_typed(env, napi_uint32_array, 8, 4, &d);
+  int rc = sn3_read_hash(e, u32(env, a[1]), (uint32_t*)d);
+  if (rc) return throw_sn3(env, e, rc);
+  return arr;
+}
+
 static napi_value init(napi_env env, napi_value exports) {
   struct {
     const char* name;
@@ -303,6 +543,11 @@ static napi_value init(napi_env env, napi_value exports) {
       {"readHashStream", js_read_stream}, {"resetStream", js_reset_stream}, {"readState", js_read_state},
       {"readDerived", js_read_derived}, {"applyImpulse", js_impulse},  {"spawnFluid", js_spawn},
       {"setEvents", js_set_events},   {"setActive", js_set_active},
+      {"reserveSlots", js_reserve_slots}, {"loadSlot", js_load_slot}, {"query", js_query},
+      {"create3", js_create3},        {"destroy3", js_destroy3},       {"load3", js_load3},
+      {"reserveSlots3", js_reserve_slots3}, {"loadSlot3", js_load_slot3}, {"setEvents3", js_set_events3},
+      {"setActive3", js_set_active3}, {"step3", js_step3},             {"sync3", js_sync3},
+      {"query3", js_query3},          {"readHashStream3", js_read_stream3}, {"readHash3", js_read_hash3},
   };
   for (size_t k = 0; k < sizeof fns / sizeof fns[0]; ++k) {
     napi_value f;

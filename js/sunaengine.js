@@ -334,6 +334,7 @@ async function createEngine(opts) {
 
 module.exports = {
   createEngine, Engine, makeScene, buildState, digestWords, hex8, mmStep, fmix32, xorshift32, boundaryRing,
+  native, loadLuts, matsImage: () => Int32Array.from(mats().image),
   ONE, H, VMAX, MAXNBR, ITERS, PARTICLE_WORDS, DERIVED_WORDS, HASHED_WORDS: 8, MAT_WATER, TABLE_HASH,
   FLAG_MASK, AMBIENT,
 };

@@ -47,3 +47,28 @@ def test_node_host_on_gpu():
                        capture_output=True, text=True, timeout=300)
     print(r.stdout)
     assert r.returncode == 0 and "OK node host" in r.stdout, r.stdout + r.stderr
+
+
+@pytest.mark.gpu
+@needs_node
+def test_node_serving_mix_equals_alone():
+    """The JavaScript serving loop (js/serve.js): jobs in the mix = alone."""
+    assert os.path.exists(ADDON), "N-API addon not built"
+    r = subprocess.run(["node", os.path.join(HERE, "js", "serve_gpu.js")], capture_output=True, text=True,
+                       timeout=300)
+    print(r.stdout)
+    assert r.returncode == 0 and "OK js serving mix = alone" in r.stdout, r.stdout + r.stderr
+
+
+@needs_node
+def test_node_tables3d_match_python():
+    """js/tables3d.js builds the same 3D table image as tables3d.py."""
+    sys.path.insert(0, ROOT)
+    import numpy as np
+    from sunaengine_amd import tables3d
+    code = ("const t=require(%r);const a=t.buildLuts3d();"
+            "process.stdout.write(Buffer.from(a.buffer).toString('hex'))" % os.path.join(ROOT, "js", "tables3d.js"))
+    r = subprocess.run(["node", "-e", code], capture_output=True, text=True, timeout=120)
+    assert r.returncode == 0, r.stderr
+    got = np.frombuffer(bytes.fromhex(r.stdout), dtype=np.int32)
+    assert np.array_equal(got, tables3d.luts3d())
