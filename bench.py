@@ -124,6 +124,8 @@ def main():
     ap.add_argument("--rates", default="10,50,200,500", help="serve: arrival rates in jobs/s")
     ap.add_argument("--jobs", type=int, default=16, help="serve: jobs per arrival rate")
     ap.add_argument("--max-steps", type=int, default=5000, help="serve: upper bound of a job's substeps")
+    ap.add_argument("--host", default="python", choices=("python", "js"),
+                    help="serve: the host that runs the serving loop (js = node js/serve_bench.js over the addon)")
     args = ap.parse_args()
     if args.gpus != 1:
         print(json.dumps({"error": "this build runs on one GPU; --gpus must be 1",
@@ -231,6 +233,8 @@ def run_serve(args):
     admission by free partition memory, 2D and 3D worlds on their own streams,
     each job's per-substep H stream returned.  One sweep over arrival rates;
     `value` is the aggregate particle-substeps/s over the sweep."""
+    if args.host == "js":
+        return run_serve_js(args)
     from sunaengine_amd import serve
     rates = [float(r) for r in args.rates.split(",")]
     srv = serve.Server()
@@ -260,6 +264,29 @@ def run_serve(args):
     p99 = max(p["p99_latency_s"] for p in per)
     print(json.dumps(_line(value, "particle-substeps/s", 0, 0, 0.0,
                            dict(cfg, worst_p99_latency_s=p99))), flush=True)
+
+
+def run_serve_js(args):
+    """The same serving workload driven by the JavaScript host (js/serve.js
+    over the N-API addon): node runs the loop and reports one JSON object."""
+    import subprocess
+    r = subprocess.run(["node", os.path.join(ROOT, "js", "serve_bench.js"), args.rates, str(args.jobs),
+                        str(args.max_steps)], capture_output=True, text=True)
+    sys.stderr.write(r.stderr)
+    if r.returncode != 0:
+        raise SystemExit(f"node js/serve_bench.js failed ({r.returncode})")
+    out = json.loads(r.stdout.strip().splitlines()[-1])
+    per = out["rates"]
+    cfg = {
+        "workload": ("serving, JavaScript host (js/serve.js over the N-API addon): Poisson job stream, "
+                     "70% 2D / 30% 3D, particle counts log-uniform in [4096, 262144], substeps uniform in "
+                     f"[100, {args.max_steps}], stir schedules; continuous batching into slotted 2D and 3D worlds, "
+                     "admission by partition memory; unit = one particle-substep of a job"),
+        "rates": per,
+        "jobs_per_rate": args.jobs,
+        "worst_p99_latency_s": max(p["p99LatencyS"] for p in per),
+    }
+    print(json.dumps(_line(out["value"], "particle-substeps/s", 0, 0, 0.0, cfg)), flush=True)
 
 
 def run_3d(args):
