@@ -44,22 +44,14 @@ SN_HD int32_t mini(int32_t a, int32_t b) { return a < b ? a : b; }
 // splits on the (almost always false) saturation test.
 SN_HD int32_t sat_of(int32_t sign_src) { return (sign_src >> 31) ^ I32_MAX; }  // I32_MIN if < 0 else I32_MAX
 
-// RNE(v / 2^s) saturated to i32, s in 1..31, on the two 32-bit halves of v:
-// floor(v / 2^s) is (hi >> s : funnel(hi, lo, s)), the dropped bits decide
-// the half-to-even increment, and the result overflows exactly when the
-// incremented 64-bit value is not a sign-extended i32.
+// RNE(v / 2^s) saturated to i32, s in 1..31.  floor-shift of v biased by
+// (2^(s-1) - 1 + lsb(floor(v / 2^s))) is exactly round-half-to-even.
 SN_HD int32_t shr_rne_sat64(int64_t v, unsigned s, Dbg& g) {
-  const uint32_t lo = (uint32_t)v;
-  const int32_t hi = (int32_t)(v >> 32);
-  const uint32_t qlo = (lo >> s) | ((uint32_t)hi << (32u - s));
-  const int32_t qhi = hi >> s;
-  const uint32_t rem = lo & ((1u << s) - 1u), half = 1u << (s - 1u);
-  const uint32_t inc = (rem > half || (rem == half && (qlo & 1u))) ? 1u : 0u;
-  const uint32_t nlo = qlo + inc;
-  const int32_t nhi = qhi + ((inc && qlo == 0xffffffffu) ? 1 : 0);
-  const bool ovf = nhi != ((int32_t)nlo >> 31);
+  int64_t odd = (v >> s) & 1;
+  int64_t q = (v + ((int64_t)1 << (s - 1)) - 1 + odd) >> s;
+  const bool ovf = (int64_t)(int32_t)q != q;
   g.mul += ovf ? 1u : 0u;
-  return ovf ? sat_of(hi) : (int32_t)nlo;
+  return ovf ? sat_of((int32_t)(v >> 32)) : (int32_t)q;
 }
 
 SN_HD int32_t fp_mul_s(int32_t a, int32_t b, unsigned s, Dbg& g) {
