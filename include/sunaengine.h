@@ -47,6 +47,14 @@ typedef struct sn_opts {
   uint32_t no_debug;      /* 1 = createEngine({debug: false}) (src/engine.js:2791,2884-2894):
                              debug counters stay zero, and sweeps skip work that
                              only feeds them; hashes and state are unchanged */
+  uint32_t materials;     /* 0: the friction, containment and bond passes
+                             (src/sim.wgsl:1673-2235,2708-3097) switch on when a
+                             loaded scene has a fluid row with friction or bonds;
+                             a water-only world skips them (they are bit-inert for
+                             it) and reports SN_E_UNSUPPORTED if a transition later
+                             reaches such a row.  1: they run from creation.  With
+                             them on, a scene holds at most 131071 particles
+                             (MAX_PARTICLES_BONDED, src/engine.js:261) */
 } sn_opts;
 
 /* One scene as loadScene writes it (src/engine.js:3888-3937): n records of 8
@@ -100,6 +108,10 @@ int sn_read_chain(sn_engine* e, uint32_t scene, uint32_t chain_out[8]);
 int sn_read_state(sn_engine* e, uint32_t scene, int32_t* out, size_t words);
 /* readDerived() (src/engine.js:4290): n * 16 words in slot order. */
 int sn_read_derived(sn_engine* e, uint32_t scene, int32_t* out, size_t words);
+/* readBond() (src/engine.js:4295): n * 48 words in slot order, each a packed
+ * bond {partner slot:17, L0q:13, tombstone:1} (src/sim.wgsl:1590-1625);
+ * zero rows in a world that does not run the material passes. */
+int sn_read_bond(sn_engine* e, uint32_t scene, uint32_t* out, size_t words);
 /* debugCounters() (src/engine.js:4293): 16 words. */
 int sn_debug_counters(sn_engine* e, uint32_t scene, uint32_t out[16]);
 /* Substeps encoded since load for `scene` (the reference's `_substep`). */

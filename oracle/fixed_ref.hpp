@@ -5,6 +5,7 @@
 // Every function states the reference lines it follows.  WGSL i32 `+ - *`
 // wrap, so wrapping ops are spelled through uint32_t here.
 #pragma once
+#include <algorithm>
 #include <atomic>
 #include <cstdint>
 
@@ -22,6 +23,8 @@ inline int32_t wadd(int32_t a, int32_t b) { return (int32_t)((uint32_t)a + (uint
 inline int32_t wsub(int32_t a, int32_t b) { return (int32_t)((uint32_t)a - (uint32_t)b); }
 inline int32_t wmul(int32_t a, int32_t b) { return (int32_t)((uint32_t)a * (uint32_t)b); }
 inline int32_t clampi(int32_t x, int32_t lo, int32_t hi) { return x < lo ? lo : (x > hi ? hi : x); }
+// WGSL clamp: min(max(x, lo), hi), which differs from clampi when lo > hi.
+inline int32_t wclamp(int32_t x, int32_t lo, int32_t hi) { return std::min(std::max(x, lo), hi); }
 
 // i64_shr_rne_sat (src/fixed.wgsl:146-164): floor shift by s in 1..31, round
 // half to even on the exact remainder, saturate to i32 (dbg[0]) by the sign of v.

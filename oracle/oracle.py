@@ -53,6 +53,7 @@ def _lib():
                                        C.c_int32, C.c_uint32]
         L.or_spawn.restype = C.c_uint32
         L.or_spawn.argtypes = [P, C.c_uint32, i32p, i32p]
+        L.or_friction_probe.argtypes = [P, np.ctypeslib.ndpointer(np.int64, flags="C")]
         _LIB = L
     return _LIB
 
@@ -232,3 +233,12 @@ class OracleWorld3:
         a = np.zeros(8, np.uint32)
         _lib3().or3_fnv_digest(self._h, a)
         return a
+
+
+def friction_probe(world: "OracleWorld", n_fluid: int) -> dict:
+    """Run the next substep of `world` up to its first friction pass and sum
+    the friction dp over the fluid slots (the world is left mid-substep)."""
+    out = np.zeros(5, np.int64)
+    _lib().or_friction_probe(world._h, out)
+    return {"dp": (int(out[0]), int(out[1])), "pairs": int(out[2]), "boundary": int(out[3]),
+            "full": int(out[4])}

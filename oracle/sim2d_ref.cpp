@@ -205,7 +205,7 @@ inline int32_t bond_k(const int32_t* m, int32_t T) {
   if (m[M_BONDK] == 0) return 0;
   if (m[M_RAMPSIGN] == 0) return m[M_BONDK];
   int32_t span = fp_sub_sat(m[M_RAMPHI], m[M_RAMPLO]);
-  int32_t dT = clampi(fp_sub_sat(T, m[M_RAMPLO]), 0, span);
+  int32_t dT = wclamp(fp_sub_sat(T, m[M_RAMPLO]), 0, span);   // span < 0 possible for a hand-made row
   int32_t u = clampi(fp_mul_s(dT, m[M_RAMPINV], 16), 0, 65536);
   if (m[M_RAMPSIGN] < 0) u = 65536 - u;
   return fp_mul_s(m[M_BONDK], u, 16);
@@ -1170,6 +1170,35 @@ void or_debug_counters(or_world* w, uint32_t out[16]) {
   for (int j = 0; j < 16; ++j) out[j] = W.dbg[j].load();
 }
 uint32_t or_substep_index(or_world* w) { return reinterpret_cast<World*>(w)->substep; }
+
+// Test probe: the first iteration of the next substep up to and including
+// `friction`, then out = {sum dp.x, sum dp.y over fluid slots, neighbour
+// entries, boundary entries, full lists}.  The world is left mid-substep
+// (only for tests that discard it).
+void or_friction_probe(or_world* w, int64_t out[5]) {
+  World& W = *reinterpret_cast<World*>(w);
+  g_dbg = W.dbg;
+  const Particle* sin = W.state[W.parity].data();
+  predict(W, sin);
+  build_grid(W);
+  build_nbr(W, sin);
+  solve_a(W, sin);
+  for (uint32_t c = 0; c < 4; ++c) {
+    solve_b(W, sin, c);
+    apply_dp(W, sin, c);
+  }
+  friction(W, sin);
+  int64_t sx = 0, sy = 0, pairs = 0, bnd = 0, full = 0;
+  for (uint32_t i = 0; i < W.nFluid; ++i) {
+    sx += W.derived[i].dp.x;
+    sy += W.derived[i].dp.y;
+    const uint32_t cnt = nbr_count(W, i);
+    pairs += cnt;
+    full += cnt >= MAXNBR ? 1 : 0;
+    for (uint32_t k = 0; k < cnt; ++k) bnd += W.nbr[(size_t)i * MAXNBR + k] >= W.nFluid ? 1 : 0;
+  }
+  out[0] = sx; out[1] = sy; out[2] = pairs; out[3] = bnd; out[4] = full;
+}
 
 // applyImpulse on the exact current state (src/engine.js:6077-6098): fluid
 // slots in ascending order, JS-double circle test, up to `limit` hits, velocity

@@ -38,6 +38,9 @@ SN_HD int32_t wmul(int32_t a, int32_t b) { return (int32_t)((uint32_t)a * (uint3
 SN_HD int32_t clampi(int32_t x, int32_t lo, int32_t hi) { return x < lo ? lo : (x > hi ? hi : x); }
 SN_HD int32_t maxi(int32_t a, int32_t b) { return a > b ? a : b; }
 SN_HD int32_t mini(int32_t a, int32_t b) { return a < b ? a : b; }
+// WGSL clamp(x, lo, hi) = min(max(x, lo), hi), also when lo > hi (clampi
+// assumes lo <= hi, which holds wherever its bounds are constants).
+SN_HD int32_t wclamp(int32_t x, int32_t lo, int32_t hi) { return mini(maxi(x, lo), hi); }
 
 // Every operation below is branch-free: the saturated result is selected
 // and the debug counter advanced by the overflow predicate, so a wave never
@@ -131,6 +134,33 @@ SN_HD int32_t fp_divshift(int32_t num, int32_t den, unsigned k, Dbg& g) {
     return nn > 0 ? I32_MAX : I32_MIN;
   }
   return shr_rne_sat64((int64_t)nn * (int64_t)Rn, (unsigned)s, g);
+}
+
+// fp_div_s (src/fixed.wgsl:219-261): RNE(a * 2^s / b), s in 1..31, with the
+// reference's saturation and flag rules.  Its 64-step restoring division
+// yields the exact quotient and remainder of (|a| << s) / |b|; one 64-bit
+// integer division gives the same pair.  Only the containment passes call it.
+SN_HD int32_t fp_div_s(int32_t a, int32_t b, unsigned s, Dbg& g) {
+  if (b == 0) {
+    g.div0++;
+    if (a == 0) return 0;
+    return a > 0 ? I32_MAX : I32_MIN;
+  }
+  const bool neg = (a < 0) != (b < 0);
+  const uint64_t ua = a < 0 ? (uint64_t)(0u - (uint32_t)a) : (uint64_t)(uint32_t)a;
+  const uint64_t ub = b < 0 ? (uint64_t)(0u - (uint32_t)b) : (uint64_t)(uint32_t)b;
+  const uint64_t n = ua << s;
+  uint64_t q = n / ub;
+  const uint64_t rem = n - q * ub, other = ub - rem;
+  if (rem > other || (rem == other && (q & 1u))) q += 1;
+  if (q > 0xffffffffull) { g.div++; return neg ? I32_MIN : I32_MAX; }
+  const uint32_t q32 = (uint32_t)q;
+  if (neg) {
+    if (q32 > 2147483648u) { g.div++; return I32_MIN; }
+    return (int32_t)(0u - q32);
+  }
+  if (q32 > 2147483647u) { g.div++; return I32_MAX; }
+  return (int32_t)q32;
 }
 
 // isqrt64 (src/fixed.wgsl:318-332) for a 32-bit input: floor(sqrt(n)).

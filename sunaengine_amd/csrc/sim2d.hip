@@ -205,6 +205,13 @@ struct Dev {
   const Event* events;
   uint32_t* hitFlag;      // slot-ordered hit flags for stir events
   uint32_t* hitScan;
+  // worlds that run every material pass (gen = 1): bond rows in slot order,
+  // row (partBase + slot) of BOND_CAP words, and the sorted position of each
+  // slot this substep, so a bond word's partner slot finds its state
+  uint32_t* bond;
+  uint32_t* bondCand;     // bondForm1's candidate rows, same layout
+  uint32_t* posOf;        // partBase + slot -> sorted position (rebuilt by canon_place)
+  uint32_t gen;
   uint32_t N, C, S, ringCap;
   uint32_t nEvents;
   uint32_t NS;  // neighbour-list stride: nbr[k * NS + p]
@@ -452,6 +459,7 @@ __device__ __forceinline__ void canon_place(const Dev& D, uint32_t s, uint32_t m
                        ((fluid && (fl & FLAG_CONGEAL)) ? INF_CONGEAL : 0u);
   D.info[q] = inf;
   if (inf & (INF_SEAL | INF_CONGEAL)) atomicOr(&D.sd[s].flagged, 1u);
+  if (D.gen) D.posOf[P.partBase + myid] = q;
 }
 
 __global__ __launch_bounds__(256) void k_canon_gather(Dev D) {
@@ -1565,12 +1573,70 @@ __device__ __forceinline__ int2 st_sum(const Dev& D, uint32_t p, const int32_t* 
   return make_int2(ax, ay);
 }
 
+// ==================================================================== hash
+// One slot's terms of the state hash (src/hash/statehash.wgsl:83-159): four
+// salted MurmurHash3 lanes over the 8 state words; a fluid slot with bonds
+// folds in its bond count and the two order-free sums over its row (bn = 0
+// leaves the lanes as they are).  hv[0..3] are the H_set terms, hv[4..7] the
+// H_slot terms.
+__device__ __forceinline__ void hash_terms(const uint32_t words[8], uint32_t myid, uint32_t bn, uint32_t com,
+                                           uint32_t scom, uint32_t hv[8]) {
+  const uint32_t SEED[4] = {0x9e3779b1u, 0x85ebca77u, 0xc2b2ae3du, 0x27d4eb2fu};
+  const uint32_t SALT[4] = {0x00000000u, 0x51633e2du, 0x9c8f2a17u, 0xd0e5b943u};
+#pragma unroll
+  for (int l = 0; l < 4; ++l) {
+    uint32_t h = SEED[l];
+#pragma unroll
+    for (int w = 0; w < 8; ++w) h = mm_step(h, words[w] ^ SALT[l]);
+    uint32_t hs = h, hl = h;
+    if (bn != 0u) {
+      hs = mm_step(h, bn) ^ com;
+      hl = mm_step(h, bn) ^ scom;
+    }
+    hv[l] = fmix32(hs ^ 8u);
+    hv[4 + l] = fmix32(mm_step(hl, myid) ^ 9u);
+  }
+}
+
+// Block partial (8 hash words + truncation count): a block lying wholly in
+// one scene stores its sums in hpart[block], which k_hash_finish folds in;
+// a block straddling scenes adds per lane with atomics.  All are wrapping
+// u32 sums, so the result does not depend on the split.  Every thread of
+// the block must call it.
+__device__ __forceinline__ void hash_reduce(const Dev& D, uint32_t blk, uint32_t s,
+                                            const uint32_t hv[HPART_WORDS]) {
+  __shared__ uint32_t wpart[4][HPART_WORDS];
+  __shared__ uint32_t wscene[4];
+  const uint32_t wv = threadIdx.x >> 6, lane = threadIdx.x & 63u;
+  const uint32_t s0 = (uint32_t)__builtin_amdgcn_readfirstlane((int)s);
+  const bool wuni = __all(s == s0);
+#pragma unroll
+  for (int l = 0; l < HPART_WORDS; ++l) {
+    uint32_t v = wave_sum(hv[l]);
+    if (lane == 0u) wpart[wv][l] = v;
+  }
+  if (lane == 0u) wscene[wv] = wuni ? s0 : MIXED;
+  __syncthreads();
+  const uint32_t bs = wscene[0];
+  const bool buni = bs != MIXED && wscene[1] == bs && wscene[2] == bs && wscene[3] == bs;
+  if (buni) {
+    if (bs != INACTIVE && threadIdx.x < HPART_WORDS) {
+      uint32_t t = threadIdx.x;
+      D.hpart[(size_t)blk * HPART_WORDS + t] = wpart[0][t] + wpart[1][t] + wpart[2][t] + wpart[3][t];
+    }
+  } else if (s != INACTIVE) {
+#pragma unroll
+    for (int l = 0; l < 8; ++l) atomicAdd(&D.sd[s].acc[l], hv[l]);
+    if (hv[8]) atomicAdd(&D.sd[s].dbg[6], hv[8]);
+  }
+}
+
 // ==================================================== surfaceTension + thermal
 // src/sim.wgsl:2409-2494 (stAcc for the next substep), :2612-2707 (thermal,
 // elementwise), bondForm2's flag clear on formation substeps (:3086-3096;
 // bond rows stay empty for bondless rows), then the particle's state hash
 // (src/hash/statehash.wgsl:83-159) reduced per scene.
-__global__ __launch_bounds__(256) void k_st_thermal_hash(Dev D) {
+__global__ __launch_bounds__(256) void k_st_thermal_hash(Dev D, uint32_t gen) {
   const uint32_t blk = slot_block(D);
   uint32_t p = blk * 256u + threadIdx.x;
   uint32_t s = p < D.N ? D.sidOf[p] : INACTIVE;
@@ -1631,9 +1697,10 @@ __global__ __launch_bounds__(256) void k_st_thermal_hash(Dev D) {
       if (burning) fl |= FLAG_BURNING;
       if (Mn[M_PHANTOM] != 0) fl |= FLAG_DEAD;
       if (seal_class(Mn)) fl |= FLAG_SEAL;
-      if (!supported_row(Mn)) atomicOr(&D.sd[s].fault, FAULT_MATERIAL);
-      // ---- bondForm2's flag update on formation substeps
-      if (formation) {
+      if (!gen && !supported_row(Mn)) atomicOr(&D.sd[s].fault, FAULT_MATERIAL);
+      // ---- bondForm2's flag update on formation substeps (a world running
+      // the material passes does it in k_bond_form2)
+      if (formation && !gen) {
         fl &= ~FLAG_CONGEAL;
         if ((fl & FLAG_RIGID) == 0u && (fl & WELD_TAG_MASK) != P.weldHold) fl &= ~WELD_TAG_MASK;
       }
@@ -1646,52 +1713,528 @@ __global__ __launch_bounds__(256) void k_st_thermal_hash(Dev D) {
     words[4] = (uint32_t)b.x; words[5] = (uint32_t)b.y; words[6] = (uint32_t)b.z; words[7] = (uint32_t)b.w;
   }
   flush_dbg(D, s, g, 0);
+  if (gen) return;   // hashed by k_hash_gen once the material passes have run
   // ---- state hash: 4 salted lanes over the 8 words, H_set and H_slot terms
-  const uint32_t SEED[4] = {0x9e3779b1u, 0x85ebca77u, 0xc2b2ae3du, 0x27d4eb2fu};
-  const uint32_t SALT[4] = {0x00000000u, 0x51633e2du, 0x9c8f2a17u, 0xd0e5b943u};
   uint32_t hv[HPART_WORDS];
   if (s != INACTIVE) {
-#pragma unroll
-    for (int l = 0; l < 4; ++l) {
-      uint32_t h = SEED[l];
-#pragma unroll
-      for (int w = 0; w < 8; ++w) h = mm_step(h, words[w] ^ SALT[l]);
-      hv[l] = fmix32(h ^ 8u);
-      hv[4 + l] = fmix32(mm_step(h, myid) ^ 9u);
-    }
+    hash_terms(words, myid, 0u, 0u, 0u, hv);
     hv[8] = D.drop[p];
   } else {
 #pragma unroll
     for (int l = 0; l < HPART_WORDS; ++l) hv[l] = 0u;
   }
-  // Block partial (8 hash words + truncation count): a block lying wholly in
-  // one scene stores its sums in hpart[block], which k_hash_finish folds in;
-  // a block straddling scenes adds per lane with atomics.  All are wrapping
-  // u32 sums, so the result does not depend on the split.
-  __shared__ uint32_t wpart[4][HPART_WORDS];
-  __shared__ uint32_t wscene[4];
-  const uint32_t wv = threadIdx.x >> 6, lane = threadIdx.x & 63u;
-  const uint32_t s0 = (uint32_t)__builtin_amdgcn_readfirstlane((int)s);
-  const bool wuni = __all(s == s0);
-#pragma unroll
-  for (int l = 0; l < HPART_WORDS; ++l) {
-    uint32_t v = wave_sum(hv[l]);
-    if (lane == 0u) wpart[wv][l] = v;
+  hash_reduce(D, blk, s, hv);
+}
+
+// ========================================================= material passes
+// Worlds holding rows with friction or bonds (gen = 1) run the reference's
+// remaining passes: friction with the bond gather and applyFric in every
+// iteration (src/sim.wgsl:1673-1950), contain + containOverflow before
+// finalize (:1973-2235), containStress after thermal (:2742-2771), and on
+// formation substeps markCongealNbr + bondForm1 + bondForm2 (:2714-2730,
+// :2794-3097); the state hash then folds in the bond rows.  Rigid bodies
+// are refused at load, so FLAG_RIGID never occurs here.
+constexpr unsigned FPRE = 5, RPRE = 3, BPRE = 5;
+constexpr int32_t D2_FLOOR = 262144;
+constexpr int32_t FR_TERM = 16777216, BD_TERM = 16777216;
+constexpr uint32_t BOND_CAP = 48;
+constexpr int32_t BOND_R2 = 17623204, HARD_BREAK2 = 26214400, L0_MAX = 5120, L0_MIN = 512;
+constexpr uint32_t BOND_J_MASK = 0x0001ffffu, BOND_L0_SH = 17, BOND_L0_MASK = 0x3ffe0000u,
+                   BOND_TOMB = 0x40000000u;
+constexpr uint32_t BOND_SALT = 0x5bd1e995u, BOND_SALT2 = 0x2f5b3c17u;
+constexpr int32_t SEAL_SEG_MAX2 = 8820900, SEAL_SEG_MIN2 = 670761, SEAL_KEEP4 = 717, SEAL_REACH = 95027,
+                  SEAL_STRESS4 = 2560;
+constexpr uint32_t MAX_PARTICLES_BONDED = 131071;   // src/engine.js:261 (17-bit partner field)
+
+__device__ __forceinline__ bool seal_liquid(const int32_t* m) {
+  return m[M_BONDK] == 0 && m[M_FRIC] == 0 && m[M_GRAV] > 0 && m[M_PHANTOM] == 0;
+}
+// bond_k (src/sim.wgsl:1655-1663): stiffness at temperature T.
+__device__ __forceinline__ int32_t bond_k(const int32_t* m, int32_t T, Dbg& g) {
+  if (m[M_BONDK] == 0) return 0;
+  if (m[M_RAMPSIGN] == 0) return m[M_BONDK];
+  const int32_t span = fp_sub_sat(m[M_RAMPHI], m[M_RAMPLO], g);
+  const int32_t dT = wclamp(fp_sub_sat(T, m[M_RAMPLO], g), 0, span);
+  int32_t u = clampi(fp_mul_s(dT, m[M_RAMPINV], 16, g), 0, 65536);
+  if (m[M_RAMPSIGN] < 0) u = 65536 - u;
+  return fp_mul_s(m[M_BONDK], u, 16, g);
+}
+__device__ __forceinline__ uint32_t* bond_row(const Dev& D, uint32_t* base, const SceneP& P, uint32_t slot) {
+  return base + (size_t)(P.partBase + slot) * BOND_CAP;
+}
+__device__ __forceinline__ int32_t sq2(int32_t x, int32_t y) { return wadd(wmul(x, x), wmul(y, y)); }
+
+// friction (src/sim.wgsl:1673-1934) for fluid slot p: tangential Coulomb
+// correction against every neighbour, then the bond springs of p's row
+// (breaking ones are tombstoned in place).  Writes dp only.
+template <bool BND>
+__device__ void friction_at(const Dev& D, const SceneP& P, uint32_t p, Dbg& g) {
+  const int4 bi = D.ptY[p];
+  const int32_t* Mi = MAT(D, (uint32_t)bi.x);
+  const int32_t fi = Mi[M_FRIC];
+  const uint32_t flagsI = (uint32_t)bi.y;
+  if (fi == 0 && Mi[M_BONDK] == 0) {
+    D.dp[p] = make_int2(0, 0);
+    return;
   }
-  if (lane == 0u) wscene[wv] = wuni ? s0 : MIXED;
-  __syncthreads();
-  const uint32_t bs = wscene[0];
-  const bool buni = bs != MIXED && wscene[1] == bs && wscene[2] == bs && wscene[3] == bs;
-  if (buni) {
-    if (bs != INACTIVE && threadIdx.x < HPART_WORDS) {
-      uint32_t t = threadIdx.x;
-      D.hpart[(size_t)blk * HPART_WORDS + t] = wpart[0][t] + wpart[1][t] + wpart[2][t] + wpart[3][t];
+  const int2 pi = D.pred[p];
+  const int4 ai = D.psY[p];
+  const int2 dxi = make_int2(fp_sub_sat(pi.x, ai.x, g), fp_sub_sat(pi.y, ai.y, g));
+  int32_t fx = 0, fy = 0;
+  if (fi != 0) {
+    const uint32_t cnt = D.nbrN[p] & 0x1fffffffu;
+    for (uint32_t k = 0; k < cnt; ++k) {
+      const uint32_t e = D.nbr[(size_t)k * D.NS + p];
+      const uint32_t q = e & ~NBR_BND;
+      const int2 pj = D.pred[q];
+      const int2 d = pdelta<BND>(pi, pj, g);
+      const int32_t id = lut_idx<BND>(d, g);
+      if (id < 0) continue;
+      const int32_t d4x = fp_shr_rne(d.x, FPRE), d4y = fp_shr_rne(d.y, FPRE);
+      const int32_t d2 = sq2(d4x, d4y);
+      if (d2 < D2_FLOOR) continue;
+      int2 rel = dxi;
+      int32_t mu = fi;
+      if (!(e & NBR_BND)) {
+        const int4 aj = D.psY[q];
+        rel = make_int2(fp_sub_sat(dxi.x, fp_sub_sat(pj.x, aj.x, g), g),
+                        fp_sub_sat(dxi.y, fp_sub_sat(pj.y, aj.y, g), g));
+        mu = mat_mix(fi, MAT(D, (uint32_t)D.ptY[q].x)[M_FRIC]);
+      }
+      if (mu == 0) continue;
+      const int32_t r4x = fp_shr_rne(rel.x, RPRE), r4y = fp_shr_rne(rel.y, RPRE);
+      const int32_t dot = wadd(wmul(d4x, r4x), wmul(d4y, r4y));
+      const int32_t qq = fp_divshift(dot, d2, 16, g);
+      const int32_t tx = fp_sub_sat(r4x, fp_mul_s(d4x, qq, 16, g), g);
+      const int32_t ty = fp_sub_sat(r4y, fp_mul_s(d4y, qq, 16, g), g);
+      const int32_t wq = fp_shr_rne(D.lut4[id].x, 4);
+      const int32_t sx = fp_mul_s(fp_mul_s(tx, wq, 16 - ACC - RPRE, g), mu, 16, g);
+      const int32_t sy = fp_mul_s(fp_mul_s(ty, wq, 16 - ACC - RPRE, g), mu, 16, g);
+      fx = wsub(fx, clampi(sx, -FR_TERM, FR_TERM));
+      fy = wsub(fy, clampi(sy, -FR_TERM, FR_TERM));
     }
-  } else if (s != INACTIVE) {
-#pragma unroll
-    for (int l = 0; l < 8; ++l) atomicAdd(&D.sd[s].acc[l], hv[l]);
-    if (hv[8]) atomicAdd(&D.sd[s].dbg[6], hv[8]);
   }
+  int32_t bx = 0, by = 0;
+  if (Mi[M_BONDK] != 0 && (flagsI & FLAG_BURST) == 0u) {
+    const int32_t ki = bond_k(Mi, bi.z, g);
+    const int32_t li = D.lam[p];
+    uint32_t* row = bond_row(D, D.bond, P, D.idY[p]);
+    for (uint32_t k = 0; k < BOND_CAP; ++k) {
+      const uint32_t w = row[k];
+      if (w == 0u) break;                        // rows are dense
+      if (w & BOND_TOMB) continue;
+      const uint32_t q = D.posOf[P.partBase + (w & BOND_J_MASK)];
+      const int4 bj = D.ptY[q];
+      const int32_t* Mj = MAT(D, (uint32_t)bj.x);
+      if (((uint32_t)bj.y & FLAG_BURST) != 0u || Mj[M_BONDK] == 0) { row[k] = w | BOND_TOMB; continue; }
+      const int32_t L0 = (int32_t)((w >> BOND_L0_SH) & 0x1fffu);
+      const int32_t L02 = wmul(L0, L0);
+      const int2 d = pdelta<BND>(pi, D.pred[q], g);
+      const int32_t d4x = fp_shr_rne(d.x, BPRE), d4y = fp_shr_rne(d.y, BPRE);
+      if (abs(d4x) > L0_MAX || abs(d4y) > L0_MAX) { row[k] = w | BOND_TOMB; continue; }
+      const int32_t d2 = sq2(d4x, d4y);
+      if (d2 >= HARD_BREAK2) { row[k] = w | BOND_TOMB; continue; }
+      int32_t mb = mini(Mi[M_BONDBREAK], Mj[M_BONDBREAK]);
+      const int32_t mub = mini(Mi[M_BONDMU], Mj[M_BONDMU]);
+      if (mb != 0 && mub != 0) {
+        const int32_t lp = fp_neg_sat(fp_add_sat(li, D.lam[q], g), g);
+        const int32_t Pq = lp > 0 ? fp_shr_rne(lp, 4) : 0;
+        mb = wadd(wadd(65536, wsub(mb, 65536)), fp_mul_s(Pq, mub, 16, g));
+      }
+      if (mb != 0 && d2 > fp_mul_s(L02, mb, 16, g)) { row[k] = w | BOND_TOMB; continue; }
+      if (d2 < D2_FLOOR) continue;
+      const int32_t kp = mini(ki, bond_k(Mj, bj.z, g));
+      if (kp == 0) continue;
+      const int32_t C = fp_sub_sat(d2, L02, g);
+      const int32_t qq = fp_divshift(C, wmul(d2, 4), 16, g);
+      const int32_t qk = fp_mul_s(qq, kp, 16, g);
+      bx = wsub(bx, clampi(fp_mul_s(d4x, qk, 16 - BPRE - ACC, g), -BD_TERM, BD_TERM));
+      by = wsub(by, clampi(fp_mul_s(d4y, qk, 16 - BPRE - ACC, g), -BD_TERM, BD_TERM));
+    }
+  }
+  D.dp[p] = make_int2(fp_add_sat(fx, bx, g), fp_add_sat(fy, by, g));
+}
+
+__global__ __launch_bounds__(256) void k_friction(Dev D) {
+  const uint32_t p = slot_block(D) * 256u + threadIdx.x;
+  const uint32_t s = p < D.N ? D.sidOf[p] : INACTIVE;
+  Dbg g;
+  const bool bnd = wave_bounded(D, s);
+  if (s != INACTIVE && (D.info[p] & INF_FLUID)) {
+    if (bnd) friction_at<true>(D, D.sp[s], p, g);
+    else friction_at<false>(D, D.sp[s], p, g);
+  }
+  flush_dbg(D, s, g, 0);
+}
+
+// applyFric (src/sim.wgsl:1945-1950) = applyDp_at on every fluid slot.
+__global__ __launch_bounds__(256) void k_apply_fric(Dev D) {
+  const uint32_t p = slot_block(D) * 256u + threadIdx.x;
+  const uint32_t s = p < D.N ? D.sidOf[p] : INACTIVE;
+  Dbg g;
+  if (s != INACTIVE && (D.info[p] & INF_FLUID)) {
+    const int4 b = D.ptY[p];
+    if (((uint32_t)b.y & (FLAG_RIGID | FLAG_ANCHOR)) == 0u) {
+      const int32_t* M = MAT(D, (uint32_t)b.x);
+      const int32_t dm = M[M_DPMAX], dmN = fp_neg_sat(dm, g);
+      const int2 dp = D.dp[p];
+      const int2 pr = D.pred[p];
+      const int32_t ddx = clampi(fp_shr_rne(dp.x, ACC), dmN, dm);
+      const int32_t ddy = clampi(fp_shr_rne(dp.y, ACC), dmN, dm);
+      D.pred[p] = wall_clamp(D.sp[s], make_int2(fp_add_sat(pr.x, ddx, g), fp_add_sat(pr.y, ddy, g)));
+    }
+  }
+  flush_dbg(D, s, g, 0);
+}
+
+// ---- containment helpers (src/sim.wgsl:1994-1999, :2085-2135)
+__device__ __forceinline__ int32_t seal_cross_sign(int32_t ax, int32_t ay, int32_t bx, int32_t by) {
+  // sign of the wrapped 64-bit a.x*b.y - a.y*b.x
+  const uint64_t d = (uint64_t)((int64_t)ax * by) - (uint64_t)((int64_t)ay * bx);
+  if (d == 0) return 0;
+  return (d >> 63) ? -1 : 1;
+}
+// Push pr out to the keep-out radius of seal particle pj.
+__device__ __forceinline__ void seal_keep_push(int2 pj, int2& pr, Dbg& g) {
+  const int32_t dx4 = fp_shr_rne(fp_sub_sat(pr.x, pj.x, g), BPRE);
+  const int32_t dy4 = fp_shr_rne(fp_sub_sat(pr.y, pj.y, g), BPRE);
+  const int32_t d2 = sq2(dx4, dy4);
+  if (d2 > 0 && d2 < SEAL_KEEP4 * SEAL_KEEP4) {
+    const int32_t d4 = (int32_t)isqrt32((uint32_t)d2);
+    if (d4 > 0) {
+      const int32_t ux = fp_div_s(dx4, d4, 16, g), uy = fp_div_s(dy4, d4, 16, g);
+      const int32_t amt = (SEAL_KEEP4 - d4) << BPRE;
+      pr.x = fp_add_sat(pr.x, fp_mul_s(ux, amt, 16, g), g);
+      pr.y = fp_add_sat(pr.y, fp_mul_s(uy, amt, 16, g), g);
+    }
+  }
+}
+// Revert the move pos -> pr if it crosses the seal segment pj-pk.
+__device__ __forceinline__ void seal_segment_test(int2 pj, int2 pk, int2 pos, int2& pr, Dbg& g) {
+  const int32_t ex4 = fp_shr_rne(fp_sub_sat(pk.x, pj.x, g), BPRE);
+  const int32_t ey4 = fp_shr_rne(fp_sub_sat(pk.y, pj.y, g), BPRE);
+  const int32_t e2 = sq2(ex4, ey4);
+  if (e2 > SEAL_SEG_MAX2 || e2 < SEAL_SEG_MIN2) return;
+  if (mini(pj.x, pk.x) > maxi(pos.x, pr.x) || maxi(pj.x, pk.x) < mini(pos.x, pr.x) ||
+      mini(pj.y, pk.y) > maxi(pos.y, pr.y) || maxi(pj.y, pk.y) < mini(pos.y, pr.y))
+    return;
+  const int32_t ex = fp_sub_sat(pk.x, pj.x, g), ey = fp_sub_sat(pk.y, pj.y, g);
+  const int32_t s0 = seal_cross_sign(ex, ey, fp_sub_sat(pos.x, pj.x, g), fp_sub_sat(pos.y, pj.y, g));
+  const int32_t s1 = seal_cross_sign(ex, ey, fp_sub_sat(pr.x, pj.x, g), fp_sub_sat(pr.y, pj.y, g));
+  if (s0 == 0 || s0 == s1) return;
+  const int32_t mx = fp_sub_sat(pr.x, pos.x, g), my = fp_sub_sat(pr.y, pos.y, g);
+  const int32_t t0 = seal_cross_sign(mx, my, fp_sub_sat(pj.x, pos.x, g), fp_sub_sat(pj.y, pos.y, g));
+  const int32_t t1 = seal_cross_sign(mx, my, fp_sub_sat(pk.x, pos.x, g), fp_sub_sat(pk.y, pos.y, g));
+  if (t0 == t1 && t0 != 0) return;
+  pr = pos;
+}
+
+// contain (src/sim.wgsl:2056-2136) for a list below MAXNBR and
+// containOverflow (:2147-2235) for a full one, in one launch: both write only
+// the pred of a non-seal slot and read only the preds of seal-class slots,
+// which neither moves.
+__device__ void contain_at(const Dev& D, const SceneP& P, uint32_t p, Dbg& g) {
+  const uint32_t nn = D.nbrN[p];
+  if ((nn & NBR_SEAL_BIT) == 0u) return;
+  const int4 bi = D.ptY[p];
+  const int32_t* Mi = MAT(D, (uint32_t)bi.x);
+  if (seal_class(Mi) || Mi[M_PHANTOM] != 0) return;
+  if (((uint32_t)bi.y & (FLAG_ANCHOR | FLAG_RIGID)) != 0u) return;
+  const int4 ai = D.psY[p];
+  const int2 pos = make_int2(ai.x, ai.y);
+  int2 pr = D.pred[p];
+  const bool liquidI = seal_liquid(Mi);
+  const uint32_t cnt = nn & 0x1fffffffu;
+  if (cnt < MAXNBR) {
+    bool touched = false;
+    for (uint32_t k = 0; k < cnt; ++k) {
+      const uint32_t e = D.nbr[(size_t)k * D.NS + p];
+      if (e & NBR_BND) continue;
+      if (!seal_class(MAT(D, (uint32_t)D.ptY[e].x))) continue;
+      touched = true;
+      seal_keep_push(D.pred[e], pr, g);
+    }
+    if (touched) {
+      for (uint32_t k = 0; k < cnt; ++k) {
+        const uint32_t e = D.nbr[(size_t)k * D.NS + p];
+        if (e & NBR_BND) continue;
+        const int32_t* Mj = MAT(D, (uint32_t)D.ptY[e].x);
+        if (!seal_class(Mj)) continue;
+        const int2 pj = D.pred[e];
+        for (uint32_t k2 = k + 1; k2 < cnt; ++k2) {
+          const uint32_t e2 = D.nbr[(size_t)k2 * D.NS + p];
+          if (e2 & NBR_BND) continue;
+          const int32_t* Mk = MAT(D, (uint32_t)D.ptY[e2].x);
+          if (!seal_class(Mk)) continue;
+          if (liquidI && Mj[M_SEALMODE] != 0 && Mk[M_SEALMODE] != 0) continue;
+          seal_segment_test(pj, D.pred[e2], pos, pr, g);
+        }
+      }
+    }
+  } else {
+    // the seal-class slots of the 3x3 cells around pred, in canonical order,
+    // within SEAL_REACH of the swept box; the first 48 are kept
+    uint32_t seals[48];
+    uint32_t sealN = 0;
+    const int32_t loX = fp_sub_sat(mini(pos.x, pr.x), SEAL_REACH, g);
+    const int32_t hiX = fp_add_sat(maxi(pos.x, pr.x), SEAL_REACH, g);
+    const int32_t loY = fp_sub_sat(mini(pos.y, pr.y), SEAL_REACH, g);
+    const int32_t hiY = fp_add_sat(maxi(pos.y, pr.y), SEAL_REACH, g);
+    const CC cc = cell_coord(P, pr);
+    for (int32_t gy = cc.y - 1; gy <= cc.y + 1; ++gy) {
+      if (gy < 0 || gy >= (int32_t)P.gridH) continue;
+      for (int32_t gx = cc.x - 1; gx <= cc.x + 1; ++gx) {
+        if (gx < 0 || gx >= (int32_t)P.gridW) continue;
+        const uint32_t c = P.cellBase + (uint32_t)gy * P.gridW + (uint32_t)gx;
+        const uint32_t lo = (uint32_t)((int32_t)D.cellStart[c] + P.posShift);
+        const uint32_t hi = (uint32_t)((int32_t)D.cellStart[c + 1] + P.posShift);
+        for (uint32_t t = lo; t < hi; ++t) {
+          if (D.idY[t] >= P.nFluid) continue;
+          if (!seal_class(MAT(D, (uint32_t)D.ptY[t].x))) continue;
+          const int2 pj = D.pred[t];
+          if (pj.x < loX || pj.x > hiX || pj.y < loY || pj.y > hiY) continue;
+          if (sealN < 48u) seals[sealN++] = t;
+        }
+      }
+    }
+    for (uint32_t k = 0; k < sealN; ++k) seal_keep_push(D.pred[seals[k]], pr, g);
+    for (uint32_t k = 0; k < sealN; ++k) {
+      const int32_t* Mj = MAT(D, (uint32_t)D.ptY[seals[k]].x);
+      const int2 pj = D.pred[seals[k]];
+      for (uint32_t k2 = k + 1; k2 < sealN; ++k2) {
+        const int32_t* Mk = MAT(D, (uint32_t)D.ptY[seals[k2]].x);
+        if (liquidI && Mj[M_SEALMODE] != 0 && Mk[M_SEALMODE] != 0) continue;
+        seal_segment_test(pj, D.pred[seals[k2]], pos, pr, g);
+      }
+    }
+  }
+  D.pred[p] = pr;
+}
+
+// contain + containOverflow, then finalize (src/sim.wgsl:2245-2265) in a
+// second launch: finalize reads only the slot's own pred, but contain reads
+// seal preds that a fused finalize would not change either way; the split
+// keeps the finalize of k_apply's last colour pass as the one definition.
+__global__ __launch_bounds__(256) void k_contain(Dev D) {
+  const uint32_t p = slot_block(D) * 256u + threadIdx.x;
+  const uint32_t s = p < D.N ? D.sidOf[p] : INACTIVE;
+  Dbg g;
+  if (s != INACTIVE && (D.info[p] & INF_FLUID)) contain_at(D, D.sp[s], p, g);
+  flush_dbg(D, s, g, 0);
+}
+
+__global__ __launch_bounds__(256) void k_finalize(Dev D) {
+  const uint32_t p = slot_block(D) * 256u + threadIdx.x;
+  const uint32_t s = p < D.N ? D.sidOf[p] : INACTIVE;
+  Dbg g;
+  if (s != INACTIVE && (D.info[p] & INF_FLUID)) {
+    const SceneP& P = D.sp[s];
+    const int2 pr = D.pred[p];
+    const int4 a = D.psY[p];
+    const int32_t vmaxN = fp_neg_sat(P.vmax, g);
+    const int32_t vx = clampi(fp_sub_sat(pr.x, a.x, g), vmaxN, P.vmax);
+    const int32_t vy = clampi(fp_sub_sat(pr.y, a.y, g), vmaxN, P.vmax);
+    D.vtmp[p] = make_int2(vx, vy);
+    D.rinv[p] = clampi(fp_divshift(RHO0, maxi(D.rho[p], RHO_FLOOR_RINV), 16, g), 0, RINV_MAX);
+  }
+  flush_dbg(D, s, g, 0);
+}
+
+// containStress (src/sim.wgsl:2742-2771): a held seal particle crushed by
+// non-seal neighbours bursts.  Writes only its own flags word; reads the
+// positions and material ids of others, which no slot changes here.
+__global__ __launch_bounds__(256) void k_contain_stress(Dev D) {
+  const uint32_t p = slot_block(D) * 256u + threadIdx.x;
+  const uint32_t s = p < D.N ? D.sidOf[p] : INACTIVE;
+  Dbg g;
+  if (s != INACTIVE && (D.info[p] & INF_FLUID)) {
+    const int4 b = D.pt[p];
+    const int32_t* Mi = MAT(D, (uint32_t)b.x);
+    const uint32_t fl = (uint32_t)b.y;
+    if (seal_class(Mi) && Mi[M_BURSTP] > 0 && (fl & (FLAG_ANCHOR | FLAG_RIGID)) != 0u) {
+      const int4 a = D.ps[p];
+      const uint32_t cnt = D.nbrN[p] & 0x1fffffffu;
+      int32_t stress = 0;
+      for (uint32_t k = 0; k < cnt; ++k) {
+        const uint32_t e = D.nbr[(size_t)k * D.NS + p];
+        if (e & NBR_BND) continue;
+        const int32_t* Mj = MAT(D, (uint32_t)D.pt[e].x);
+        if (seal_class(Mj) || Mj[M_PHANTOM] != 0) continue;
+        const int4 aj = D.ps[e];
+        const int32_t dx4 = fp_shr_rne(fp_sub_sat(a.x, aj.x, g), BPRE);
+        const int32_t dy4 = fp_shr_rne(fp_sub_sat(a.y, aj.y, g), BPRE);
+        const int32_t d2 = sq2(dx4, dy4);
+        if (d2 >= SEAL_STRESS4 * SEAL_STRESS4) continue;
+        stress = wadd(stress, SEAL_STRESS4 - (int32_t)isqrt32((uint32_t)d2));
+      }
+      if (wmul(stress, 1 << BPRE) > Mi[M_BURSTP])
+        reinterpret_cast<int32_t*>(D.pt + p)[1] =
+            (int32_t)((fl & ~(FLAG_ANCHOR | FLAG_RIGID | WELD_TAG_MASK)) | FLAG_BURST);
+    }
+  }
+  flush_dbg(D, s, g, 0);
+}
+
+__device__ __forceinline__ bool formation_substep(const SceneD& SD) {
+  return ((SD.substep % BOND_PERIOD) == 0u) || SD.bondDirty;
+}
+
+// markCongealNbr (src/sim.wgsl:2714-2730) folded into bondForm1
+// (:2794-3027): the congeal-near bit is only read by bondForm1 of the same
+// slot, so it is computed where it is used.  Candidate rows: the live
+// bonds, then new partners in canonical neighbour order (weld tags,
+// congealing, or reform under the velocity and pressure gates).
+__global__ __launch_bounds__(256) void k_bond_form1(Dev D) {
+  const uint32_t p = slot_block(D) * 256u + threadIdx.x;
+  const uint32_t s = p < D.N ? D.sidOf[p] : INACTIVE;
+  if (s == INACTIVE || !(D.info[p] & INF_FLUID) || !formation_substep(D.sd[s])) return;
+  Dbg g;
+  uint32_t overflow = 0;
+  const SceneP& P = D.sp[s];
+  const uint32_t myid = D.id[p];
+  uint32_t* cand = bond_row(D, D.bondCand, P, myid);
+  const int4 a = D.ps[p], b = D.pt[p];
+  const uint32_t fl = (uint32_t)b.y, mi = (uint32_t)b.x;
+  const bool anchoredI = (fl & FLAG_ANCHOR) != 0u;
+  const int32_t* Mi = MAT(D, mi);
+  uint32_t cn = 0;
+  if (Mi[M_BONDK] != 0 && Mi[M_BONDGROUP] >= 0) {
+    const uint32_t* row = bond_row(D, D.bond, P, myid);
+    for (uint32_t k = 0; k < BOND_CAP; ++k) {
+      const uint32_t w = row[k];
+      if (w == 0u) break;
+      if (w & BOND_TOMB) continue;
+      cand[cn++] = w;   // a row holds at most BOND_CAP words, so cn < BOND_CAP here
+    }
+    const uint32_t tagI = fl & WELD_TAG_MASK;
+    const int32_t ki = bond_k(Mi, b.z, g);
+    const bool canReform = !anchoredI && Mi[M_BONDREFORM] != 0 && ki != 0;
+    const bool fuseI = fuse_row(Mi, mi);
+    const bool congealI = (fl & FLAG_CONGEAL) != 0u && fuseI;
+    const uint32_t cnt = D.nbrN[p] & 0x1fffffffu;
+    bool congealNear = false;
+    if (fuseI) {
+      for (uint32_t k = 0; k < cnt; ++k) {
+        const uint32_t e = D.nbr[(size_t)k * D.NS + p];
+        if (!(e & NBR_BND) && ((uint32_t)D.pt[e].y & FLAG_CONGEAL) != 0u) { congealNear = true; break; }
+      }
+    }
+    if (canReform || (!anchoredI && tagI != 0u) || congealI || congealNear) {
+      for (uint32_t k = 0; k < cnt; ++k) {
+        if (cn >= BOND_CAP) { overflow++; break; }
+        const uint32_t e = D.nbr[(size_t)k * D.NS + p];
+        if (e & NBR_BND) continue;                      // no bonds to boundary
+        const int4 bj = D.pt[e];
+        const uint32_t mj = (uint32_t)bj.x, fJ = (uint32_t)bj.y;
+        const int32_t* Mj = MAT(D, mj);
+        const uint32_t tagJ = fJ & WELD_TAG_MASK;
+        const bool weld = !anchoredI && tagI != 0u && tagI == tagJ;
+        const bool congealJ = (fJ & FLAG_CONGEAL) != 0u && fuse_row(Mj, mj);
+        const bool congeal = (congealI || congealJ) && !weld && mj == mi && (fJ & FLAG_RIGID) == 0u;
+        if (!congeal && Mj[M_BONDGROUP] != Mi[M_BONDGROUP]) continue;
+        if (!weld && !congeal) {
+          if (!canReform) continue;
+          if (bond_k(Mj, bj.z, g) == 0) continue;
+        }
+        const uint32_t j = D.id[e];
+        bool dup = false;
+        for (uint32_t q = 0; q < cn; ++q)
+          if ((cand[q] & BOND_J_MASK) == j) { dup = true; break; }
+        if (dup) continue;
+        const int4 aj = D.ps[e];
+        const int32_t d4x = fp_shr_rne(fp_sub_sat(a.x, aj.x, g), BPRE);
+        const int32_t d4y = fp_shr_rne(fp_sub_sat(a.y, aj.y, g), BPRE);
+        if (abs(d4x) > L0_MAX || abs(d4y) > L0_MAX) continue;
+        const int32_t d2 = sq2(d4x, d4y);
+        if (d2 < D2_FLOOR || d2 > BOND_R2) continue;
+        if (!weld && !congeal) {
+          const int32_t vl4 = fp_shr_rne(mini(Mi[M_BONDFORMV], Mj[M_BONDFORMV]), RPRE);
+          const int32_t r4x = fp_shr_rne(fp_sub_sat(a.z, aj.z, g), RPRE);
+          const int32_t r4y = fp_shr_rne(fp_sub_sat(a.w, aj.w, g), RPRE);
+          if (abs(r4x) > vl4 || abs(r4y) > vl4) continue;
+          if (sq2(r4x, r4y) > wmul(vl4, vl4)) continue;
+          const int32_t pl = mini(Mi[M_BONDFORMP], Mj[M_BONDFORMP]);
+          if (pl != 0 && (fp_neg_sat(D.lam[p], g) < pl || fp_neg_sat(D.lam[e], g) < pl)) continue;
+        }
+        const int32_t L0 = (int32_t)isqrt32((uint32_t)d2);
+        if (L0 < L0_MIN || L0 > L0_MAX) continue;
+        cand[cn++] = (((uint32_t)L0 & 0x1fffu) << BOND_L0_SH) | (j & BOND_J_MASK);
+      }
+    }
+  }
+  for (uint32_t k = cn; k < BOND_CAP; ++k) cand[k] = 0u;
+  if (D.dbgOn) {
+    Dbg z = g;
+    if (z.mul | z.div | z.div0 | z.add) {
+      if (z.mul) atomicAdd(&D.sd[s].dbg[0], z.mul);
+      if (z.div) atomicAdd(&D.sd[s].dbg[1], z.div);
+      if (z.div0) atomicAdd(&D.sd[s].dbg[2], z.div0);
+      if (z.add) atomicAdd(&D.sd[s].dbg[5], z.add);
+    }
+    if (overflow) atomicAdd(&D.sd[s].dbg[8], overflow);
+  }
+}
+
+// bondForm2 (src/sim.wgsl:3039-3097): keep the candidates whose partner
+// proposed the same rest length back, then the formation flag update.
+__global__ __launch_bounds__(256) void k_bond_form2(Dev D) {
+  const uint32_t p = slot_block(D) * 256u + threadIdx.x;
+  const uint32_t s = p < D.N ? D.sidOf[p] : INACTIVE;
+  if (s == INACTIVE || !(D.info[p] & INF_FLUID) || !formation_substep(D.sd[s])) return;
+  const SceneP& P = D.sp[s];
+  const uint32_t myid = D.id[p];
+  const uint32_t* cand = bond_row(D, D.bondCand, P, myid);
+  uint32_t* row = bond_row(D, D.bond, P, myid);
+  uint32_t n = 0;
+  for (uint32_t k = 0; k < BOND_CAP; ++k) {
+    const uint32_t w = cand[k];
+    if (w == 0u) break;
+    const uint32_t* other = bond_row(D, D.bondCand, P, w & BOND_J_MASK);
+    bool ok = false;
+    for (uint32_t q = 0; q < BOND_CAP; ++q) {
+      const uint32_t v = other[q];
+      if (v == 0u) break;
+      if ((v & BOND_J_MASK) == myid) { ok = (v & BOND_L0_MASK) == (w & BOND_L0_MASK); break; }
+    }
+    if (ok) row[n++] = w & ~BOND_TOMB;
+  }
+  for (uint32_t k = n; k < BOND_CAP; ++k) row[k] = 0u;
+  uint32_t f = (uint32_t)D.pt[p].y & ~FLAG_CONGEAL;
+  if ((f & FLAG_RIGID) == 0u && (f & WELD_TAG_MASK) != P.weldHold) f &= ~WELD_TAG_MASK;
+  reinterpret_cast<int32_t*>(D.pt + p)[1] = (int32_t)f;
+}
+
+// The state hash of a world running the material passes: as in
+// k_st_thermal_hash, with each fluid slot's bond row folded in.
+__global__ __launch_bounds__(256) void k_hash_gen(Dev D) {
+  const uint32_t blk = slot_block(D);
+  const uint32_t p = blk * 256u + threadIdx.x;
+  const uint32_t s = p < D.N ? D.sidOf[p] : INACTIVE;
+  uint32_t hv[HPART_WORDS];
+  if (s != INACTIVE) {
+    const SceneP& P = D.sp[s];
+    const int4 a = D.ps[p], b = D.pt[p];
+    const uint32_t myid = D.id[p];
+    const uint32_t words[8] = {(uint32_t)a.x, (uint32_t)a.y, (uint32_t)a.z, (uint32_t)a.w,
+                               (uint32_t)b.x, (uint32_t)b.y, (uint32_t)b.z, (uint32_t)b.w};
+    uint32_t bn = 0, com = 0, scom = 0;
+    if (myid < P.nFluid) {
+      const uint32_t* row = bond_row(D, D.bond, P, myid);
+      for (uint32_t k = 0; k < BOND_CAP; ++k) {
+        const uint32_t w = row[k];
+        if (w == 0u) break;
+        com += fmix32(((w >> 17) & 0x1fffu) ^ BOND_SALT);
+        scom += fmix32(w ^ BOND_SALT2);
+        bn++;
+      }
+    }
+    hash_terms(words, myid, bn, com, scom, hv);
+    hv[8] = D.drop[p];
+  } else {
+#pragma unroll
+    for (int l = 0; l < HPART_WORDS; ++l) hv[l] = 0u;
+  }
+  hash_reduce(D, blk, s, hv);
 }
 
 // Digest of the current state without advancing (digest(), src/engine.js:4453).
@@ -1879,6 +2422,7 @@ struct sn_engine {
   uint32_t graphSubsteps = 0;
   bool loaded = false;
   bool slotted = false;  // sn_reserve_slots layout: partitions keep their bases
+  bool gen = false;      // the material passes run (D.gen); never switched off again
   // profiling: an event pair around every launch, tagged with its class
   std::vector<hipEvent_t> evPool;
   std::vector<int> evClass;
@@ -1914,6 +2458,51 @@ void drop_graph(sn_engine* e) {
   }
 }
 
+// Switch a world to the full material pass set: bond and candidate rows for
+// every slot (2 x 192 B per slot) and the slot -> position map.  Once on, it
+// stays on; the next step re-records the graph.
+void enable_materials(sn_engine* e) {
+  if (e->gen) return;
+  const size_t NC = std::max(e->opts.max_particles, 1u);
+  Dev& D = e->D;
+  D.bond = dalloc<uint32_t>(e, NC * BOND_CAP);
+  D.bondCand = dalloc<uint32_t>(e, NC * BOND_CAP);
+  D.posOf = dalloc<uint32_t>(e, NC);
+  HIPCHECK(hipMemsetAsync(D.bond, 0, sizeof(uint32_t) * NC * BOND_CAP, e->stream));
+  HIPCHECK(hipMemsetAsync(D.bondCand, 0, sizeof(uint32_t) * NC * BOND_CAP, e->stream));
+  HIPCHECK(hipMemsetAsync(D.posOf, 0, sizeof(uint32_t) * NC, e->stream));
+  D.gen = 1u;
+  e->gen = true;
+  drop_graph(e);
+}
+
+// Empty the bond rows of slots [first, first + count) of the partition at partBase.
+void clear_bond_rows(sn_engine* e, uint32_t partBase, uint32_t first, uint32_t count) {
+  if (!e->gen || count == 0) return;
+  const size_t off = (size_t)(partBase + first) * BOND_CAP, len = (size_t)count * BOND_CAP;
+  HIPCHECK(hipMemsetAsync(e->D.bond + off, 0, sizeof(uint32_t) * len, e->stream));
+  HIPCHECK(hipMemsetAsync(e->D.bondCand + off, 0, sizeof(uint32_t) * len, e->stream));
+}
+
+// What a scene needs of the world: 1 if a fluid slot's row has friction or
+// bonds (the material passes must run), 0 if not; -1 with e->err set when
+// the scene cannot run here at all.
+int scene_needs(sn_engine* e, const std::vector<int32_t>& mats, const sn_scene& sc) {
+  bool need = false;
+  for (uint32_t i = 0; i < sc.n; ++i) {
+    const int32_t* w = sc.state8 + (size_t)i * 8;
+    if ((uint32_t)w[5] & FLAG_RIGID) {
+      fail(e, SN_E_UNSUPPORTED, "[aether] FLAG_RIGID: rigid bodies are not part of this build");
+      return -1;
+    }
+    if (i < sc.n_fluid) {
+      const int32_t* M = &mats[(uint32_t)w[4] * MAT_WORDS];
+      need = need || M[M_FRIC] != 0 || M[M_BONDK] != 0;
+    }
+  }
+  return need ? 1 : 0;
+}
+
 void enqueue_scan(sn_engine* e, uint32_t* count, uint32_t n, uint32_t* out, bool clear) {
   uint32_t nb = div_up(std::max(n, 1u), SCAN_BLOCK);
   hipLaunchKernelGGL(k_scan_reduce, dim3(nb), dim3(256), 0, e->stream, count, n, e->D.blockSums);
@@ -1925,7 +2514,7 @@ void enqueue_scan(sn_engine* e, uint32_t* count, uint32_t n, uint32_t* out, bool
 // Kernel classes timed by sn_profile_step (index into its outputs, + 1).
 enum ProfClass {
   PC_PREDICT = 0, PC_SCAN, PC_SCATTER, PC_CANON, PC_NBR, PC_SOLVEA, PC_COLOUR, PC_XNC, PC_STH, PC_FINISH,
-  PC_IMPULSE, PC_LISTS, PC_APPLY, PC_COUNT
+  PC_IMPULSE, PC_LISTS, PC_APPLY, PC_MATERIAL, PC_COUNT
 };
 
 // Record an event before (begin) / after (end) one launch while profiling.
@@ -1995,6 +2584,7 @@ void enqueue_substep(sn_engine* e) {
     hipLaunchKernelGGL(k_nbr_tile, dim3(e->nbrBlocks), dim3(256), 0, st, D);
   }
   const uint32_t iters = e->opts.iters;
+  const bool gen = D.gen != 0u;   // the material passes run (friction, containment, bonds)
   for (uint32_t it = 0; it < iters; ++it) {
     if (it > 0) {
       ProfScope ps(e, PC_SOLVEA);
@@ -2006,8 +2596,19 @@ void enqueue_substep(sn_engine* e) {
         hipLaunchKernelGGL(k_solveB, dim3(e->colourBlocks), dim3(256), 0, st, D, c);
       }
       ProfScope ps(e, PC_APPLY);
-      hipLaunchKernelGGL(k_apply, dim3(e->applyBlocks), dim3(256), 0, st, D, c, (uint32_t)(it + 1 == iters));
+      hipLaunchKernelGGL(k_apply, dim3(e->applyBlocks), dim3(256), 0, st, D, c,
+                         (uint32_t)(!gen && it + 1 == iters));
     }
+    if (gen) {
+      ProfScope ps(e, PC_MATERIAL);
+      hipLaunchKernelGGL(k_friction, dim3(e->actBlocks), dim3(256), 0, st, D);
+      hipLaunchKernelGGL(k_apply_fric, dim3(e->actBlocks), dim3(256), 0, st, D);
+    }
+  }
+  if (gen) {
+    ProfScope ps(e, PC_MATERIAL);
+    hipLaunchKernelGGL(k_contain, dim3(e->actBlocks), dim3(256), 0, st, D);
+    hipLaunchKernelGGL(k_finalize, dim3(e->actBlocks), dim3(256), 0, st, D);
   }
   {
     ProfScope ps(e, PC_XNC);
@@ -2015,7 +2616,14 @@ void enqueue_substep(sn_engine* e) {
   }
   {
     ProfScope ps(e, PC_STH);
-    hipLaunchKernelGGL(k_st_thermal_hash, dim3(e->actBlocks), dim3(256), 0, st, D);
+    hipLaunchKernelGGL(k_st_thermal_hash, dim3(e->actBlocks), dim3(256), 0, st, D, (uint32_t)gen);
+  }
+  if (gen) {
+    ProfScope ps(e, PC_MATERIAL);
+    hipLaunchKernelGGL(k_contain_stress, dim3(e->actBlocks), dim3(256), 0, st, D);
+    hipLaunchKernelGGL(k_bond_form1, dim3(e->actBlocks), dim3(256), 0, st, D);
+    hipLaunchKernelGGL(k_bond_form2, dim3(e->actBlocks), dim3(256), 0, st, D);
+    hipLaunchKernelGGL(k_hash_gen, dim3(e->actBlocks), dim3(256), 0, st, D);
   }
   {
     ProfScope ps(e, PC_FINISH);
@@ -2203,6 +2811,10 @@ int sn_create(const sn_opts* opts, const int32_t* luts, const int32_t* mats, sn_
       e->canonBlocks = 2u * cus;
     }
     HIPCHECK(hipMemset(D.cellCount, 0, sizeof(uint32_t) * CC));
+    if (e->opts.materials) {
+      enable_materials(e);
+      HIPCHECK(hipStreamSynchronize(e->stream));
+    }
   } catch (const DevError& d) {
     *out = e;
     return fail(e, SN_E_DEVICE, d.msg);
@@ -2285,18 +2897,21 @@ int sn_load_scenes(sn_engine* e, const sn_scene* scenes, uint32_t count) {
     HIPCHECK(hipSetDevice((int)e->opts.device));
     HIPCHECK(hipStreamSynchronize(e->stream));
     drop_graph(e);
-    // material check: every row present must be one this build runs
+    // rows with friction or bonds switch the world to the material passes
     std::vector<int32_t> mats(MAT_MAX * MAT_WORDS);
     HIPCHECK(hipMemcpy(mats.data(), e->D.mats, sizeof(int32_t) * mats.size(), hipMemcpyDeviceToHost));
-    for (uint32_t s = 0; s < count; ++s)
-      for (uint32_t i = 0; i < scenes[s].n_fluid; ++i) {
-        uint32_t m = (uint32_t)scenes[s].state8[(size_t)i * 8 + 4];
-        const int32_t* M = &mats[m * MAT_WORDS];
-        if (M[M_FRIC] != 0 || M[M_BONDK] != 0)
-          return fail(e, SN_E_UNSUPPORTED,
-                      "[aether] material row " + std::to_string(m) +
-                          " has friction or bonds; this build runs frictionless, bondless rows");
-      }
+    bool need = e->gen;
+    for (uint32_t s = 0; s < count; ++s) {
+      const int r = scene_needs(e, mats, scenes[s]);
+      if (r < 0) return SN_E_UNSUPPORTED;
+      need = need || r > 0;
+    }
+    if (need)
+      for (uint32_t s = 0; s < count; ++s)
+        if (scenes[s].n > MAX_PARTICLES_BONDED)
+          return fail(e, SN_E_CAPACITY, "[aether] scene.n = " + std::to_string(scenes[s].n) +
+                                            " exceeds MAX_PARTICLES_BONDED = 131071 (bond partner field)");
+    if (need) enable_materials(e);
     e->S = count;
     e->N = (uint32_t)totN;
     e->C = (uint32_t)totC;
@@ -2336,6 +2951,7 @@ int sn_load_scenes(sn_engine* e, const sn_scene* scenes, uint32_t count) {
     }
     HIPCHECK(hipMemcpyAsync(D.sd, sd.data(), sizeof(SceneD) * count, hipMemcpyHostToDevice, e->stream));
     HIPCHECK(hipMemsetAsync(D.cellCount, 0, sizeof(uint32_t) * std::max(e->C, 1u), e->stream));
+    for (uint32_t s = 0; s < count; ++s) clear_bond_rows(e, sp[s].partBase, 0, sp[s].cap);
     e->maxEventsPerSubstep = 0;
     // neighbour tile width from the initial occupancy of occupied cells
     {
@@ -2414,6 +3030,7 @@ int sn_reserve_slots(sn_engine* e, uint32_t count, const uint32_t* cap_particles
     for (auto& d : sd) std::memset(&d, 0, sizeof(d));
     HIPCHECK(hipMemcpyAsync(D.sd, sd.data(), sizeof(SceneD) * count, hipMemcpyHostToDevice, e->stream));
     HIPCHECK(hipMemsetAsync(D.cellCount, 0, sizeof(uint32_t) * std::max(e->C, 1u), e->stream));
+    for (uint32_t s = 0; s < count; ++s) clear_bond_rows(e, sp[s].partBase, 0, sp[s].cap);
     e->maxEventsPerSubstep = 0;
     e->tileG = 1;
     upload_layout(e);
@@ -2453,13 +3070,14 @@ int sn_load_slot(sn_engine* e, uint32_t slot, const sn_scene* sc) {
     HIPCHECK(hipStreamSynchronize(e->stream));
     std::vector<int32_t> mats(MAT_MAX * MAT_WORDS);
     HIPCHECK(hipMemcpy(mats.data(), e->D.mats, sizeof(int32_t) * mats.size(), hipMemcpyDeviceToHost));
-    for (uint32_t i = 0; i < sc->n_fluid; ++i) {
-      const int32_t* M = &mats[(uint32_t)sc->state8[(size_t)i * 8 + 4] * MAT_WORDS];
-      if (M[M_FRIC] != 0 || M[M_BONDK] != 0)
-        return fail(e, SN_E_UNSUPPORTED, "[aether] material row has friction or bonds; this build runs "
-                                         "frictionless, bondless rows");
-    }
+    const int r = scene_needs(e, mats, *sc);
+    if (r < 0) return SN_E_UNSUPPORTED;
+    if ((r > 0 || e->gen) && sc->n > MAX_PARTICLES_BONDED)
+      return fail(e, SN_E_CAPACITY, "[aether] scene.n = " + std::to_string(sc->n) +
+                                        " exceeds MAX_PARTICLES_BONDED = 131071 (bond partner field)");
+    if (r > 0) enable_materials(e);
     drop_graph(e);
+    clear_bond_rows(e, P.partBase, 0, P.cap);
     P.n = sc->n;
     P.nFluid = sc->n_fluid;
     P.gridW = gw;
@@ -2746,6 +3364,27 @@ int sn_read_derived(sn_engine* e, uint32_t scene, int32_t* out, size_t words) {
   }
 }
 
+int sn_read_bond(sn_engine* e, uint32_t scene, uint32_t* out, size_t words) {
+  if (!e || !out) return SN_E_ARG;
+  try {
+    SceneD sd;
+    int rc = read_sd(e, scene, sd);
+    if (rc) return rc;
+    const SceneP& P = e->sp[scene];
+    const size_t need = (size_t)P.n * BOND_CAP;
+    if (words < need) return fail(e, SN_E_ARG, "[aether] readBond buffer too small");
+    if (!e->gen) {
+      std::memset(out, 0, sizeof(uint32_t) * need);
+      return SN_OK;
+    }
+    HIPCHECK(hipMemcpy(out, e->D.bond + (size_t)P.partBase * BOND_CAP, sizeof(uint32_t) * need,
+                       hipMemcpyDeviceToHost));
+    return SN_OK;
+  } catch (const DevError& d) {
+    return fail(e, SN_E_DEVICE, d.msg);
+  }
+}
+
 int sn_apply_impulse(sn_engine* e, uint32_t scene, int32_t cx, int32_t cy, int32_t r, int32_t ix, int32_t iy,
                      uint32_t limit, uint32_t* hits) {
   if (!e || !hits) return SN_E_ARG;
@@ -2875,6 +3514,7 @@ int sn_spawn_fluid(sn_engine* e, const int32_t* fluid8, uint32_t k, const int32_
   if (nb != P.n - P.nFluid) return fail(e, SN_E_ARG, "[aether] spawnFluid: boundary block size mismatch");
   uint32_t newN = P.n + k;
   if (newN > e->opts.max_particles) return SN_OK;  // refused deterministically, n unchanged
+  if (e->gen && newN > MAX_PARTICLES_BONDED) return SN_OK;
   try {
     SceneD sd;
     int rc = read_sd(e, 0, sd);
@@ -2890,9 +3530,20 @@ int sn_spawn_fluid(sn_engine* e, const int32_t* fluid8, uint32_t k, const int32_
       acc[2 * i] = dv[(size_t)i * 16 + 8];
       acc[2 * i + 1] = dv[(size_t)i * 16 + 9];
     }
+    // fluid slots keep their ids, so their bond rows carry over unchanged;
+    // the new fluid slots start with empty rows (src/engine.js:5987-5992)
+    std::vector<uint32_t> rows;
+    if (e->gen && P.nFluid) {
+      rows.resize((size_t)P.nFluid * BOND_CAP);
+      HIPCHECK(hipMemcpy(rows.data(), e->D.bond + (size_t)P.partBase * BOND_CAP, sizeof(uint32_t) * rows.size(),
+                         hipMemcpyDeviceToHost));
+    }
     sn_scene sc{newN, P.nFluid + k, nst.data(), P.domW, P.domH, P.cellShift, acc.data()};
     rc = sn_load_scenes(e, &sc, 1);
     if (rc) return rc;
+    if (!rows.empty())
+      HIPCHECK(hipMemcpy(e->D.bond + (size_t)e->sp[0].partBase * BOND_CAP, rows.data(),
+                         sizeof(uint32_t) * rows.size(), hipMemcpyHostToDevice));
     // carry the engine-side counters across the reload; formation is dirty
     SceneD nd = sd;
     nd.bondDirty = 1;

@@ -40,7 +40,7 @@ class Engine:
 
     def __init__(self, max_particles: int = 20000, max_cells: int = 65536, iters: int = ITERS,
                  max_scenes: int = 1, stream_cap: int = 4096, device: int = 0, mats=None, luts=None,
-                 debug: bool = True):
+                 debug: bool = True, materials: bool = False):
         if not (isinstance(iters, int) and 1 <= iters <= 16):
             raise sc.SceneError(f"iters = {iters} must be an integer in 1..16 (shipped default {ITERS})")
         self.mats_rows = mt.MATS if mats is None else mats
@@ -51,7 +51,10 @@ class Engine:
         L = native.lib()
         # debug=False is the reference's stripped build (src/engine.js:2884-2894):
         # debug_counters() reads zeros; hashes and state are unchanged
-        o = native.sn_opts(max_particles, max_cells, max_scenes, iters, stream_cap, device, 0 if debug else 1)
+        # materials=True runs the friction, containment and bond passes from the
+        # start; otherwise they switch on when a loaded scene needs them
+        o = native.sn_opts(max_particles, max_cells, max_scenes, iters, stream_cap, device, 0 if debug else 1,
+                           1 if materials else 0)
         h = C.c_void_p()
         rc = L.sn_create(C.byref(o), self._luts.ctypes.data, self._mats_img.ctypes.data, C.byref(h))
         self._h = h
@@ -199,6 +202,13 @@ class Engine:
         self._check(native.lib().sn_read_derived(self._h, scene, out.ctypes.data, out.size))
         return out[:n]
 
+    def read_bond(self, scene: int = 0) -> np.ndarray:
+        """readBond (src/engine.js:4295): (n, 48) packed bond words in slot order."""
+        n = self.images[scene].n if scene < len(self.images) else 0
+        out = np.zeros((max(n, 1), 48), np.uint32)
+        self._check(native.lib().sn_read_bond(self._h, scene, out.ctypes.data, out.size))
+        return out[:n]
+
     def debug_counters(self, scene: int = 0) -> np.ndarray:
         a = np.zeros(16, np.uint32)
         self._check(native.lib().sn_debug_counters(self._h, scene, a.ctypes.data))
@@ -259,7 +269,8 @@ class Engine:
 
     # kernel classes of sn_profile_step, slot 0 = the whole span
     PROFILE_KEYS = ("total", "predict", "scan", "scatter", "canon_gather", "nbr_solveA", "solveA",
-                    "solveB", "xnc", "st_thermal_hash", "hash_finish", "impulse", "lists", "apply")
+                    "solveB", "xnc", "st_thermal_hash", "hash_finish", "impulse", "lists", "apply",
+                    "materials")
 
     def profile_step(self, count: int = 1, classes: Optional[Iterable[str]] = None) -> dict:
         """Run `count` substeps eagerly with an event pair around every launch

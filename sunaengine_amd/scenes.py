@@ -227,9 +227,10 @@ def load_image(scene: Scene, mats=None, max_particles: Optional[int] = None,
 
     Differences from the reference host, by design: the reference refuses
     maxParticles > 131071 unconditionally (a 17-bit bond partner field,
-    src/engine.js:3331-3334); here that cap applies only when a material in
-    the table can bond, because the partner field is only ever written for
-    bondable rows.
+    src/engine.js:3331-3334); here the native loader applies that cap per
+    scene, and only in a world that runs the material passes (a fluid row
+    with friction or bonds), because the partner field is only ever written
+    for bondable rows.
     """
     rows = mt.MATS if mats is None else mats
     mat_count = len(rows)

@@ -171,11 +171,17 @@ def _np_fmix(h):
     return h ^ (h >> np.uint64(16))
 
 
-def digest_words(words, stride: int, hashed: int):
+BOND_SALT, BOND_SALT2 = 0x5BD1E995, 0x2F5B3C17
+
+
+def digest_words(words, stride: int, hashed: int, bond=None, n_fluid: int = 0, bond_cap: int = 48):
     """8-lane digest of a word array laid out `stride` words per record.
 
-    The bondless case of ``digestState`` (src/hash/statehash.js:80-159):
-    lanes 0-3 are H_set, lanes 4-7 are H_slot.  Vectorised over records.
+    ``digestState`` (src/hash/statehash.js:80-159): lanes 0-3 are H_set,
+    lanes 4-7 are H_slot.  With `bond` (n x bond_cap words, slot order) the
+    rows of records below `n_fluid` are folded in: the bond count, the
+    order-free sum over the rest lengths (H_set) and over the whole words
+    (H_slot), up to the first zero word.  Vectorised over records.
     """
     u = np.asarray(words).view(np.uint32).astype(np.uint64).reshape(-1, stride)
     n_rec = u.shape[0]
@@ -186,13 +192,30 @@ def digest_words(words, stride: int, hashed: int):
         for w in range(hashed):
             h = _np_mm_step(h, u[:, w] ^ np.uint64(_SALT[lane]))
         lanes.append(h)
+    s_lanes, l_lanes = list(lanes), list(lanes)
+    if bond is not None and n_fluid > 0:
+        b = np.asarray(bond).view(np.uint32).astype(np.uint64).reshape(-1, bond_cap)[:n_fluid]
+        live = np.cumprod(b != 0, axis=1).astype(bool)          # the dense prefix of each row
+        bn = live.sum(axis=1).astype(np.uint64)
+        com = (np.where(live, _np_fmix(((b >> np.uint64(17)) & np.uint64(0x1FFF)) ^ np.uint64(BOND_SALT)), 0)
+               .sum(axis=1) & np.uint64(_M32))
+        scom = np.where(live, _np_fmix(b ^ np.uint64(BOND_SALT2)), 0).sum(axis=1) & np.uint64(_M32)
+        has = np.zeros(n_rec, bool)
+        has[:n_fluid] = bn != 0
+        bnf = np.zeros(n_rec, np.uint64); bnf[:n_fluid] = bn
+        comf = np.zeros(n_rec, np.uint64); comf[:n_fluid] = com
+        scomf = np.zeros(n_rec, np.uint64); scomf[:n_fluid] = scom
+        for lane in range(4):
+            t = _np_mm_step(lanes[lane], bnf)
+            s_lanes[lane] = np.where(has, t ^ comf, lanes[lane])
+            l_lanes[lane] = np.where(has, t ^ scomf, lanes[lane])
     n = np.uint64(hashed)
     m = np.uint64(hashed + 1)
     idx = np.arange(n_rec, dtype=np.uint64)
     for lane in range(4):
-        acc.append(int(_np_fmix(lanes[lane] ^ n).sum()) & _M32)
+        acc.append(int(_np_fmix(s_lanes[lane] ^ n).sum()) & _M32)
     for lane in range(4):
-        acc.append(int(_np_fmix(_np_mm_step(lanes[lane], idx) ^ m).sum()) & _M32)
+        acc.append(int(_np_fmix(_np_mm_step(l_lanes[lane], idx) ^ m).sum()) & _M32)
     return acc
 
 

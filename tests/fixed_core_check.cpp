@@ -128,13 +128,24 @@ int main() {
     i128 ab = a < 0 ? -(i128)a : (i128)a;
     EXPECT(oref::fp_abs(a) == sat32(ab) && sn::fp_abs(a, g) == sat32(ab), "abs");
   }
-  // ---- fp_div_s (oracle only: the product never calls it)
+  // ---- fp_div_s: oracle and product against the exact quotient
   for (int it = 0; it < 200000; ++it) {
     int32_t a = rnd(), b = rnd();
     unsigned s = (unsigned)(rng() % 32);
     if (s == 0) s = 16;
-    EXPECT(oref::fp_div_s(a, b, s) == ref_div_s(a, b, s), "div_s(%d,%d,%u)", a, b, s);
+    if (it % 4 == 1) b = (int32_t)(rng() % 9000) - 4500;  // small divisors: saturation and ties
+    if (it % 4 == 2) a = (int32_t)(rng() % 9000) - 4500;
+    const int32_t want = ref_div_s(a, b, s);
+    EXPECT(oref::fp_div_s(a, b, s) == want, "div_s(%d,%d,%u)", a, b, s);
+    EXPECT(sn::fp_div_s(a, b, s, g) == want, "sn div_s(%d,%d,%u)", a, b, s);
   }
+  for (int32_t a = -600; a <= 600; ++a)
+    for (int32_t b = -40; b <= 40; ++b)
+      EXPECT(sn::fp_div_s(a, b, 16, g) == ref_div_s(a, b, 16) && sn::fp_div_s(a, b, 1, g) == ref_div_s(a, b, 1),
+             "sn div_s small(%d,%d)", a, b);
+  EXPECT(sn::fp_div_s(INT32_MIN, 1, 1, g) == INT32_MIN && sn::fp_div_s(INT32_MIN, -1, 1, g) == INT32_MAX &&
+             sn::fp_div_s(-65536, 65536, 31, g) == INT32_MIN && sn::fp_div_s(65536, 65536, 31, g) == INT32_MAX,
+         "div_s rails");
   // ---- fp_divshift: product == oracle everywhere; closed-form structure
   for (int32_t dn = 32768; dn < 65536; ++dn) {
     int32_t want = (int32_t)(1073741824LL / dn);

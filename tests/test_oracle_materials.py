@@ -1,0 +1,127 @@
+"""The oracle's material passes against checks that do not restate them.
+
+* Weld bonds by brute force: a weld-tagged STONE slab at rest forms, on
+  substep 0, exactly the bonds an O(n^2) search over its final positions
+  gives (every same-tag pair with D2_FLOOR <= d2 <= BOND_R2 at 1/32-tick
+  resolution, rest length isqrt(d2)), src/sim.wgsl:2794-3097.
+* bondForm2 leaves only mutual bonds: right after a formation substep every
+  word has its mirror in the partner's row with the same rest length, and no
+  tombstone survives.
+* The bond fold of the state hash equals the vectorised digestState twin
+  (sunaengine_amd.tables.digest_words, after src/hash/statehash.js:80-159).
+* Friction is antisymmetric: with no boundary neighbour, no truncation and
+  no saturating term, the friction terms of a SAND heap cancel pairwise, so
+  one application of friction moves the heap's summed dp by zero
+  (src/sim.wgsl:1727-1781).
+"""
+import math
+
+import numpy as np
+
+from sunaengine_amd import materials as mt
+from sunaengine_amd import scenes, tables
+from tests import material_scenes as ms
+from tests.helpers import oracle_world
+
+ONE = 65536
+BOND_R2 = 17623204
+D2_FLOOR = 262144
+TOMB = 0x40000000
+
+
+def _shr_rne(x, s):
+    """RNE arithmetic shift, exact integer arithmetic."""
+    q, r = divmod(int(x), 1 << s)
+    half = 1 << (s - 1)
+    if r > half or (r == half and q & 1):
+        q += 1
+    return q
+
+
+def _rows(bond, nf):
+    out = []
+    for i in range(nf):
+        row = []
+        for w in bond[i]:
+            if w == 0:
+                break
+            row.append(int(w))
+        out.append(row)
+    return out
+
+
+def test_weld_bonds_match_brute_force():
+    rnd = scenes.xorshift32(0xB0D5)
+    floor = (30 - 3) * ONE
+    slab = ms._block(rnd, mt.STONE, 12, 4, 8 * ONE, floor, ONE, jitter=256)
+    s = ms._scene(slab, 30, 30, "stone_slab")
+    o = oracle_world(s, 4)
+    o.step(1)
+    st = o.read_state()
+    bond = o.read_bond()
+    nf = s.nFluid
+    got = {(i, w & 0x1FFFF, (w >> 17) & 0x1FFF) for i, row in enumerate(_rows(bond, nf)) for w in row}
+    want = set()
+    for i in range(nf):
+        for j in range(nf):
+            if i == j:
+                continue
+            d4x = _shr_rne(int(st[i, 0]) - int(st[j, 0]), 5)
+            d4y = _shr_rne(int(st[i, 1]) - int(st[j, 1]), 5)
+            if abs(d4x) > 5120 or abs(d4y) > 5120:
+                continue
+            d2 = d4x * d4x + d4y * d4y
+            if D2_FLOOR <= d2 <= BOND_R2:
+                want.add((i, j, math.isqrt(d2)))
+    assert len(want) > 4 * nf
+    assert got == want
+    # the weld tags are cleared by the same formation pass
+    assert not (st[:nf, 5] & 0x000FFF00).any()
+
+
+def test_bonds_are_mutual_after_formation():
+    s = ms.rice_on_rubber()
+    o = oracle_world(s, 4)
+    done = 0
+    for k in (1, 5, 21, 41):
+        o.step(k - done)
+        done = k
+        bond = o.read_bond()
+        assert not (bond & TOMB).any()
+        rows = _rows(bond, s.nFluid)
+        pairs = {(i, w & 0x1FFFF): (w >> 17) & 0x1FFF for i, row in enumerate(rows) for w in row}
+        assert pairs, "no bonds formed"
+        for (i, j), l0 in pairs.items():
+            assert pairs.get((j, i)) == l0, f"bond {i}->{j} has no mirror after substep {k}"
+
+
+def test_bond_fold_matches_digest_twin():
+    for build, k in ((ms.stone_in_water, 6), (ms.rice_on_rubber, 10), (ms.freezing_pool, 7)):
+        s = build()
+        o = oracle_world(s, 4)
+        o.step(k)
+        st = o.read_state()
+        bond = o.read_bond()
+        assert (bond != 0).any()
+        want = tables.digest_words(st.reshape(-1), 8, 8, bond=bond, n_fluid=s.nFluid)
+        assert want == [int(x) for x in o.digest()], s.name
+        # and the fold matters: without it the digest differs
+        assert tables.digest_words(st.reshape(-1), 8, 8) != want
+
+
+def test_friction_terms_cancel_pairwise():
+    from oracle import oracle as orc
+    rnd = scenes.xorshift32(0xF71C)
+    # a loose SAND cloud in mid-air, far from every wall: no boundary neighbour
+    parts = ms._block(rnd, mt.SAND, 14, 10, 20 * ONE, 30 * ONE, (ONE * 3) >> 2, jitter=4096)
+    s = ms._scene(parts, 60, 60, "sand_cloud")
+    vel = np.zeros(2 * s.n, np.int32)
+    r2 = scenes.xorshift32(0x1234)
+    for i in range(s.nFluid):
+        vel[2 * i] = (r2() % 8192) - 4096
+        vel[2 * i + 1] = (r2() % 8192) - 4096
+    s.vel = vel
+    w = oracle_world(s, 1)
+    sums = orc.friction_probe(w, s.nFluid)
+    assert sums["pairs"] > 500 and sums["boundary"] == 0 and sums["full"] == 0
+    assert sums["dp"] == (0, 0)
