@@ -59,7 +59,9 @@ KERNEL_OF = {"predict": "k_predict_count", "scan": "k_scan_reduce+k_scan_top+k_s
              "nbr_solveA": "k_nbr_tile",
              "solveA": "k_solveA", "solveB": "k_solveB", "apply": "k_apply", "xnc": "k_xnc",
              "st_thermal_hash": "k_st_thermal_hash", "hash_finish": "k_hash_finish",
-             "impulse": "k_impulse_*", "lists": "k_build_lists"}
+             "impulse": "k_impulse_*", "lists": "k_build_lists",
+             "materials": "k_friction+k_apply_fric+k_contain+k_finalize+k_contain_stress+k_bond_form1"
+                          "+k_bond_form2+k_hash_gen"}
 
 
 def class_bytes(F: int, N: int, C: int, k: int) -> dict:
@@ -82,6 +84,17 @@ def class_bytes(F: int, N: int, C: int, k: int) -> dict:
         "st_thermal_hash": 12 * N + 24 * F + 40 * F + 32 * N + 4 * F,
         "hash_finish": 0,
     }
+
+
+def material_bytes(F: int, k: int, b: float) -> int:
+    """Algorithmic HBM bytes per substep of the material kernels, on the same
+    per-particle-datum model (neighbour gathers not counted), for F fluid
+    particles of which a fraction b hold bond rows: friction 64 + 192b and
+    applyFric 44 per iteration, containment 8, containStress 20, formation
+    (bondForm1 236 + 192b, bondForm2 408) every 4th substep, and the bond
+    rows the hash folds in, 192b.  Finalize and the state hash are already in
+    class_bytes."""
+    return int(F * (k * (108 + 192 * b) + 8 + 20 + (644 + 192 * b) / 4 + 192 * b))
 
 
 def cpu_info():
@@ -107,10 +120,11 @@ def main():
     # the 1M configuration runs 500 substeps: 10 warm-up + 490 timed
     ap.add_argument("--steps", type=int, default=490)
     ap.add_argument("--warmup", type=int, default=10)
-    ap.add_argument("--workload", default="scene1m", choices=("scene1m", "batched", "serve", "3d"),
+    ap.add_argument("--workload", default="scene1m", choices=("scene1m", "batched", "serve", "3d", "materials"),
                     help="scene1m: the 1M-particle 2D scene (default); batched: 512 uneven scenes, "
                          "batched vs one at a time; serve: Poisson job stream with continuous batching; "
-                         "3d: the 2M-particle 3D block")
+                         "3d: the 2M-particle 3D block; materials: the 112,780-particle mixed-material "
+                         "scene (friction, bonds, containment)")
     ap.add_argument("--side", type=int, default=1024, help="fluid block side (1024 = 1M particles)")
     ap.add_argument("--cpu-substeps", type=int, default=8, help="CPU-baseline window in substeps")
     ap.add_argument("--cpu-threads", type=int, default=0, help="0 = all available host cores")
@@ -137,6 +151,8 @@ def main():
         return run_serve(args)
     if args.workload == "3d":
         return run_3d(args)
+    if args.workload == "materials":
+        return run_materials(args)
     return run_scene1m(args)
 
 
@@ -326,6 +342,100 @@ def run_3d(args):
     print(json.dumps(_line(value, "particle-substeps/s", args.steps, args.warmup, dt * 1e3 / args.steps,
                            cfg, roofline=roof)), flush=True)
     e.destroy()
+
+
+def run_materials(args):
+    """The material passes at scale: scenes.materials_mix (112,780 particles,
+    SAND / METAL / held STONE / RICE / water), 4 density iterations, the full
+    pass set (friction and bond gather per iteration, containment,
+    containStress, bond formation, bond-folded hash).  Timed by graph-replayed
+    substeps; then every kernel class event-timed over PROFILE_SUBSTEPS more;
+    the oracle on the same scene is the CPU baseline, hash-equal substep by
+    substep.  Unit = one fluid particle-substep."""
+    from sunaengine_amd import materials as mt
+    from sunaengine_amd import scenes
+    from sunaengine_amd.engine import Engine
+    iters = 4
+    t0 = time.time()
+    scene = scenes.materials_mix()
+    img = scenes.load_image(scene)
+    F, N, C = img.nFluid, img.n, img.cellTotal
+    print(f"# materials scene: n={N} nFluid={F} cells={C} ({time.time() - t0:.1f}s)", file=sys.stderr, flush=True)
+    eng = Engine(max_particles=N, max_cells=C, iters=iters, stream_cap=max(4096, args.steps + args.warmup + 64),
+                 debug=not args.no_debug)
+    eng.load_scene(scene)
+    eng.step(args.warmup)
+    eng.sync()
+    eng.reset_stream()
+    t1 = time.perf_counter()
+    eng.step(args.steps)
+    eng.sync()
+    wall = time.perf_counter() - t1
+    ms_per_step = wall * 1e3 / args.steps
+    value = F * args.steps / wall
+    bond = eng.read_bond()
+    bonded = int(((bond != 0).any(axis=1)).sum())
+    st = eng.read_state()
+    print(f"# timed {args.steps} substeps: {ms_per_step:.3f} ms/substep, {value:.4e} ps/s, "
+          f"{bonded} slots with bonds", file=sys.stderr, flush=True)
+    prof = eng.profile_step(PROFILE_SUBSTEPS)
+    b = bonded / F
+    cb = class_bytes(F, N, C, iters)
+    cb["materials"] = material_bytes(F, iters, b)
+    per = {}
+    for k, v in prof.items():
+        if k == "total" or v["calls"] == 0:
+            continue
+        avg_ms = v["ms"] / v["calls"]
+        byt = cb.get(k, 0) * PROFILE_SUBSTEPS / v["calls"]
+        per[k] = {"ms_per_substep": round(v["ms"] / PROFILE_SUBSTEPS, 4), "launches_per_substep": v["calls"] / PROFILE_SUBSTEPS,
+                  "avg_launch_us": round(avg_ms * 1e3, 2), "algorithmic_bytes_per_launch": int(byt),
+                  "achieved_GBs": round(byt / (avg_ms * 1e-3) / 1e9, 2) if avg_ms > 0 else 0.0}
+    dom = max(per, key=lambda k: per[k]["ms_per_substep"])
+    d = per[dom]
+    roof = {"bound": "hbm", "achieved": d["achieved_GBs"], "peak": HBM_PEAK_GBS, "unit": "GB/s",
+            "frac": round(d["achieved_GBs"] / HBM_PEAK_GBS, 6), "traffic": None,
+            "kernel": KERNEL_OF.get(dom, dom), "kernel_class": dom, "per_kernel": per,
+            "per_kernel_window": f"every kernel class event-timed over {PROFILE_SUBSTEPS} substeps after the timed ones"}
+    cpu_baseline = None
+    if not args.no_cpu:
+        try:
+            from oracle import OracleWorld
+            from oracle.oracle import set_threads
+            from sunaengine_amd import tables
+            model, avail = cpu_info()
+            threads = args.cpu_threads or min(avail, 16)
+            set_threads(threads)
+            o = OracleWorld(img, tables.load_luts(), mt.mats_image(), iters=iters)
+            g2 = Engine(max_particles=N, max_cells=C, iters=iters)
+            g2.load_scene(scene)
+            g2.step(args.cpu_substeps)
+            ref_gpu = g2.hash_stream()
+            g2.destroy()
+            tc = time.perf_counter()
+            ref_cpu = o.step(args.cpu_substeps, stream=True)
+            cw = time.perf_counter() - tc
+            same = bool(np.array_equal(ref_cpu, ref_gpu))
+            cpu_baseline = {"value": round(F * args.cpu_substeps / cw, 1), "unit": "particle-substeps/s",
+                            "cores": threads, "kind": "port",
+                            "sample": (f"first {args.cpu_substeps} substeps of the same scene from load, OpenMP over "
+                                       f"particles, {threads} threads on '{model}'; per-substep H_set/H_slot equal "
+                                       f"to the GPU's: {same}")}
+            o.close()
+        except Exception as ex:
+            cpu_baseline = {"value": None, "unit": "particle-substeps/s", "cores": 0, "kind": "port",
+                            "sample": f"unavailable: {ex}"}
+    cfg = {"workload": (f"2D mixed-material scene: {F:,} fluid slots (SAND 19,600, METAL 240, held STONE 1,200, "
+                        f"RICE 12,100 over the bond ramp, water 78,000) + {N - F:,} boundary in a walled 512 x 160 wu "
+                        f"box, {img.gridW}x{img.gridH} cells, {iters} density iterations, every material pass; "
+                        f"substeps {args.warmup + 1}..{args.warmup + args.steps} from load, graph-replayed; "
+                        "unit = one fluid particle-substep"),
+           "n_particles": N, "n_fluid": F, "cells": C, "iters": iters,
+           "slots_with_bonds_at_end": bonded, "bursts_at_end": int(((st[:, 5] & 64) != 0).sum()),
+           "debug_counters": not args.no_debug}
+    print(json.dumps(_line(value, "particle-substeps/s", args.steps, args.warmup, ms_per_step, cfg,
+                           roofline=roof, cpu_baseline=cpu_baseline)), flush=True)
+    eng.destroy()
 
 
 def run_scene1m(args):

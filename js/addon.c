@@ -72,10 +72,10 @@ static napi_value num(napi_env env, double x) {
 }
 
 /* create(maxParticles, maxCells, maxScenes, iters, streamCap, device, luts:Int32Array, mats:Int32Array
- *        [, debug = 1]) */
+ *        [, debug = 1 [, materials = 0]]) */
 static napi_value js_create(napi_env env, napi_callback_info info) {
-  napi_value a[9];
-  const int argc = get_args(env, info, 9, a);
+  napi_value a[10];
+  const int argc = get_args(env, info, 10, a);
   if (argc < 8) {
     napi_throw_error(env, NULL, "create: 8 arguments expected");
     return NULL;
@@ -83,6 +83,7 @@ static napi_value js_create(napi_env env, napi_callback_info info) {
   sn_opts o;
   memset(&o, 0, sizeof(o));
   o.no_debug = (argc > 8 && u32(env, a[8]) == 0) ? 1u : 0u;
+  o.materials = (argc > 9 && u32(env, a[9]) != 0) ? 1u : 0u;
   o.max_particles = u32(env, a[0]);
   o.max_cells = u32(env, a[1]);
   o.max_scenes = u32(env, a[2]);
@@ -237,6 +238,20 @@ static napi_value read_slots(napi_env env, napi_callback_info info, int derived)
   napi_value arr = new_typed(env, napi_int32_array, words ? words : 0, 4, &d);
   if (!words) return arr;
   int rc = derived ? sn_read_derived(e, scene, (int32_t*)d, words) : sn_read_state(e, scene, (int32_t*)d, words);
+  if (rc) return throw_sn(env, e, rc);
+  return arr;
+}
+/* readBond(h, scene, n) -> Uint32Array(n * 48) */
+static napi_value js_read_bond(napi_env env, napi_callback_info info) {
+  napi_value a[3];
+  get_args(env, info, 3, a);
+  sn_engine* e = handle(env, a[0]);
+  uint32_t scene = u32(env, a[1]), n = u32(env, a[2]);
+  size_t words = (size_t)n * 48;
+  void* d = NULL;
+  napi_value arr = new_typed(env, napi_uint32_array, words, 4, &d);
+  if (!words) return arr;
+  int rc = sn_read_bond(e, scene, (uint32_t*)d, words);
   if (rc) return throw_sn(env, e, rc);
   return arr;
 }
@@ -541,7 +556,7 @@ static napi_value init(napi_env env, napi_value exports) {
       {"step", js_step},              {"sync", js_sync},               {"readHash", js_read_hash},
       {"readChain", js_read_chain},   {"debugCounters", js_debug},     {"substepCount", js_substep_count},
       {"readHashStream", js_read_stream}, {"resetStream", js_reset_stream}, {"readState", js_read_state},
-      {"readDerived", js_read_derived}, {"applyImpulse", js_impulse},  {"spawnFluid", js_spawn},
+      {"readDerived", js_read_derived}, {"readBond", js_read_bond}, {"applyImpulse", js_impulse},  {"spawnFluid", js_spawn},
       {"setEvents", js_set_events},   {"setActive", js_set_active},
       {"reserveSlots", js_reserve_slots}, {"loadSlot", js_load_slot}, {"query", js_query},
       {"create3", js_create3},        {"destroy3", js_destroy3},       {"load3", js_load3},

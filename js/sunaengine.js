@@ -238,7 +238,8 @@ class Engine {
     this.iters = opts.iters;
     const img = Int32Array.from(mats().image);
     this._h = native().create(opts.maxParticles, opts.maxCells, opts.maxScenes || 1, opts.iters,
-      opts.streamCap || 4096, opts.device || 0, loadLuts(), img, opts.debug === false ? 0 : 1);
+      opts.streamCap || 4096, opts.device || 0, loadLuts(), img, opts.debug === false ? 0 : 1,
+      opts.materials ? 1 : 0);
     this.n = 0; this.nFluid = 0; this._scene = null; this.substepCount = 0;
   }
   get matCount() { return mats().rows.length; }
@@ -274,6 +275,7 @@ class Engine {
   resetStream() { native().resetStream(this._h); }
   async readState() { return native().readState(this._h, 0, this.n); }
   async readDerived() { return native().readDerived(this._h, 0, this.n); }
+  async readBond() { return native().readBond(this._h, 0, this.n); }
   async debugCounters() { return native().debugCounters(this._h, 0); }
   async checkErrors() { native().sync(this._h); }
   // The exact current state is what applyImpulse selects on, so the mirror
@@ -329,7 +331,8 @@ async function createEngine(opts) {
   const iters = opts.iters !== undefined ? opts.iters : ITERS;
   assert(Number.isInteger(iters) && iters >= 1 && iters <= 16, `iters = ${iters} must be an integer in 1..16`);
   return new Engine({ maxParticles: opts.maxParticles || 20000, maxCells: opts.maxCells || 65536, iters,
-    maxScenes: 1, streamCap: opts.streamCap || 4096, device: opts.device || 0, debug: opts.debug !== false });
+    maxScenes: 1, streamCap: opts.streamCap || 4096, device: opts.device || 0, debug: opts.debug !== false,
+    materials: !!opts.materials });
 }
 
 module.exports = {

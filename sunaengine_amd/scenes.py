@@ -438,3 +438,40 @@ def batch_scenes(count: int = 512, seed: int = 0xBA7C4, lo: int = 1024, hi: int 
         s.meta.update(block=(bw, bh), x0=int(x0))
         out.append(s)
     return out
+
+
+def materials_mix(seed: int = 0x3A7E11) -> Scene:
+    """A mixed-material 2D scene at the bonded-scene size limit, every block
+    at rest spacing (1 wu, as dambreak_v1): a SAND heap, a METAL bar, a
+    weld-tagged STONE slab with its bottom row held, RICE grains spread over
+    the bond ramp's temperatures resting on the slab, and a water column, in
+    a walled 1024 x 256 wu box.  109,928 particles (106,880 fluid), under
+    the 131,071 of MAX_PARTICLES_BONDED; every material pass has work in it
+    (friction, bonds and their breaks, containment, held-wall bursts,
+    formation)."""
+    rnd = xorshift32(seed)
+    floor = (256 - 3) * ONE
+    parts, flags, temps = [], [], []
+
+    def block(mat, nx, ny, x0_wu, y_floor, jitter, temp=None, anchor_rows=0):
+        for gy in range(ny):
+            for gx in range(nx):
+                jx = (rnd() % (2 * jitter)) - jitter
+                jy = (rnd() % (2 * jitter)) - jitter
+                parts.append((x0_wu * ONE + gx * ONE + jx, y_floor - gy * ONE + jy, mat))
+                flags.append(FLAG_ANCHOR if gy < anchor_rows else 0)
+                temps.append(mt.AMBIENT if temp is None else temp(gx, gy))
+
+    block(mt.SAND, 200, 120, 10, floor, 2048)
+    block(mt.METAL, 80, 6, 215, floor, 512)
+    block(mt.STONE, 400, 6, 300, floor, 512, anchor_rows=1)
+    block(mt.RICE, 200, 100, 320, floor - 7 * ONE, 2048, temp=lambda gx, gy: mt.DEG(40 + (gx * 7 + gy * 13) % 60))
+    block(mt.WATER, 300, 200, 710, floor, 2048)
+    s = _assemble(parts, boundary_ring(1024, 256, 256 - 6), 1024, 256, "materials_mix")
+    f = np.zeros(s.n, np.uint32)
+    f[: len(flags)] = flags
+    t = np.full(s.n, mt.AMBIENT, np.int64)
+    t[: len(temps)] = temps
+    s.flags, s.temp = f, t.astype(np.int32)
+    s.meta = dict(seed=seed)
+    return s

@@ -72,3 +72,38 @@ def test_node_tables3d_match_python():
     assert r.returncode == 0, r.stderr
     got = np.frombuffer(bytes.fromhex(r.stdout), dtype=np.int32)
     assert np.array_equal(got, tables3d.luts3d())
+
+
+@pytest.mark.gpu
+@needs_node
+@pytest.mark.parametrize("name", ["stone_in_water", "freezing_pool"])
+def test_node_material_scene_matches_oracle(tmp_path, name):
+    """The Node host runs the material passes: same per-substep hashes and
+    bond rows as the oracle (freezing_pool needs createEngine({materials: true}))."""
+    assert os.path.exists(ADDON), "N-API addon not built"
+    sys.path.insert(0, ROOT)
+    import numpy as np
+    from tests import material_scenes as ms
+    from tests.helpers import oracle_world
+    build, _, materials = ms.CASES[name]
+    s = build()
+    steps = 16
+    doc = {"n": s.n, "nFluid": s.nFluid, "domW": s.domW, "domH": s.domH, "materials": materials,
+           "pos": [int(v) for v in s.pos], "vel": [int(v) for v in s.vel], "matId": [int(v) for v in s.matId],
+           "flags": None if s.flags is None else [int(v) for v in s.flags],
+           "temp": None if s.temp is None else [int(v) for v in s.temp]}
+    path = tmp_path / "scene.json"
+    path.write_text(json.dumps(doc))
+    r = subprocess.run(["node", os.path.join(HERE, "js", "materials_gpu.js"), str(path), str(steps)],
+                       capture_output=True, text=True, timeout=300)
+    assert r.returncode == 0, r.stdout + r.stderr
+    got = json.loads(r.stdout.strip().splitlines()[-1])
+    o = oracle_world(s, 4)
+    ref = o.step(steps, stream=True)
+    want = ["".join("%08x" % int(w) for w in row) for row in ref]
+    assert got["stream"] == want
+    bond = o.read_bond().reshape(-1)
+    nz = np.nonzero(bond)[0]
+    assert got["bonds"] == nz.size > 0
+    assert got["bondSum"] == int((bond[nz].astype(np.uint64) * (nz.astype(np.uint64) + 1)).sum() % 2**32)
+
