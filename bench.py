@@ -51,6 +51,7 @@ def algorithmic_bytes(F: int, N: int, C: int, k: int) -> dict:
 
 
 DOMINANT = "nbr_solveA"     # the neighbour sweep: the kernel class timed inside the timed region
+MAT_DOMINANT = "friction"   # materials workload: k_friction + k_apply_fric, timed inside its timed region
 PROFILE_SUBSTEPS = 20       # follow-on substeps profiled with every kernel class timed
 
 # kernel behind each profiled class (the HIP symbol rocprofv3 reports)
@@ -60,8 +61,8 @@ KERNEL_OF = {"predict": "k_predict_count", "scan": "k_scan_reduce+k_scan_top+k_s
              "solveA": "k_solveA", "solveB": "k_solveB", "apply": "k_apply", "xnc": "k_xnc",
              "st_thermal_hash": "k_st_thermal_hash", "hash_finish": "k_hash_finish",
              "impulse": "k_impulse_*", "lists": "k_build_lists",
-             "materials": "k_friction+k_apply_fric+k_contain+k_finalize+k_contain_stress+k_bond_form1"
-                          "+k_bond_form2+k_hash_gen"}
+             "friction": "k_friction+k_apply_fric",
+             "materials": "k_contain+k_finalize+k_contain_stress+k_bond_form1+k_bond_form2+k_hash_gen"}
 
 
 def class_bytes(F: int, N: int, C: int, k: int) -> dict:
@@ -86,15 +87,16 @@ def class_bytes(F: int, N: int, C: int, k: int) -> dict:
     }
 
 
-def material_bytes(F: int, k: int, b: float) -> int:
+def material_bytes(F: int, k: int, b: float) -> dict:
     """Algorithmic HBM bytes per substep of the material kernels, on the same
     per-particle-datum model (neighbour gathers not counted), for F fluid
-    particles of which a fraction b hold bond rows: friction 64 + 192b and
-    applyFric 44 per iteration, containment 8, containStress 20, formation
-    (bondForm1 236 + 192b, bondForm2 408) every 4th substep, and the bond
-    rows the hash folds in, 192b.  Finalize and the state hash are already in
-    class_bytes."""
-    return int(F * (k * (108 + 192 * b) + 8 + 20 + (644 + 192 * b) / 4 + 192 * b))
+    particles of which a fraction b hold bond rows.  friction: k_friction
+    64 + 192b and k_apply_fric 44, per iteration.  materials: containment 8,
+    containStress 20, formation (bondForm1 236 + 192b, bondForm2 408) every
+    4th substep, and the bond rows the hash folds in, 192b.  Finalize and
+    the state hash are already in class_bytes."""
+    return {"friction": int(F * k * (108 + 192 * b)),
+            "materials": int(F * (8 + 20 + (644 + 192 * b) / 4 + 192 * b))}
 
 
 def cpu_info():
@@ -345,13 +347,14 @@ def run_3d(args):
 
 
 def run_materials(args):
-    """The material passes at scale: scenes.materials_mix (112,780 particles,
-    SAND / METAL / held STONE / RICE / water), 4 density iterations, the full
-    pass set (friction and bond gather per iteration, containment,
-    containStress, bond formation, bond-folded hash).  Timed by graph-replayed
-    substeps; then every kernel class event-timed over PROFILE_SUBSTEPS more;
-    the oracle on the same scene is the CPU baseline, hash-equal substep by
-    substep.  Unit = one fluid particle-substep."""
+    """The material passes at scale: scenes.materials_mix (109,928 particles
+    at rest spacing: SAND / METAL / held STONE / RICE / water), 4 density
+    iterations, the full pass set (friction and bond gather per iteration,
+    containment, containStress, bond formation, bond-folded hash).  Timed
+    eagerly with events around the friction class; then every kernel class
+    event-timed over PROFILE_SUBSTEPS more; the oracle on the same scene is
+    the CPU baseline, hash-equal substep by substep.  Unit = one fluid
+    particle-substep."""
     from sunaengine_amd import materials as mt
     from sunaengine_amd import scenes
     from sunaengine_amd.engine import Engine
@@ -367,8 +370,11 @@ def run_materials(args):
     eng.step(args.warmup)
     eng.sync()
     eng.reset_stream()
+    # timed region: launched eagerly like the 1M run, with a HIP event pair
+    # around every launch of the friction class (k_friction + k_apply_fric,
+    # the largest material kernels) on the engine's stream
     t1 = time.perf_counter()
-    eng.step(args.steps)
+    tprof = eng.profile_step(args.steps, classes=(MAT_DOMINANT,))
     eng.sync()
     wall = time.perf_counter() - t1
     ms_per_step = wall * 1e3 / args.steps
@@ -381,7 +387,7 @@ def run_materials(args):
     prof = eng.profile_step(PROFILE_SUBSTEPS)
     b = bonded / F
     cb = class_bytes(F, N, C, iters)
-    cb["materials"] = material_bytes(F, iters, b)
+    cb.update(material_bytes(F, iters, b))
     per = {}
     for k, v in prof.items():
         if k == "total" or v["calls"] == 0:
@@ -391,11 +397,17 @@ def run_materials(args):
         per[k] = {"ms_per_substep": round(v["ms"] / PROFILE_SUBSTEPS, 4), "launches_per_substep": v["calls"] / PROFILE_SUBSTEPS,
                   "avg_launch_us": round(avg_ms * 1e3, 2), "algorithmic_bytes_per_launch": int(byt),
                   "achieved_GBs": round(byt / (avg_ms * 1e-3) / 1e9, 2) if avg_ms > 0 else 0.0}
-    dom = max(per, key=lambda k: per[k]["ms_per_substep"])
-    d = per[dom]
-    roof = {"bound": "hbm", "achieved": d["achieved_GBs"], "peak": HBM_PEAK_GBS, "unit": "GB/s",
-            "frac": round(d["achieved_GBs"] / HBM_PEAK_GBS, 6), "traffic": None,
-            "kernel": KERNEL_OF.get(dom, dom), "kernel_class": dom, "per_kernel": per,
+    tv = tprof[MAT_DOMINANT]
+    t_avg_ms = tv["ms"] / tv["calls"]
+    t_bytes = cb[MAT_DOMINANT] * args.steps / tv["calls"]
+    t_ach = t_bytes / (t_avg_ms * 1e-3) / 1e9
+    roof = {"bound": "hbm", "achieved": round(t_ach, 2), "peak": HBM_PEAK_GBS, "unit": "GB/s",
+            "frac": round(t_ach / HBM_PEAK_GBS, 6), "traffic": None,
+            "kernel": KERNEL_OF[MAT_DOMINANT], "kernel_class": MAT_DOMINANT,
+            "kernel_avg_launch_us": round(t_avg_ms * 1e3, 2), "kernel_launches_per_substep": tv["calls"] / args.steps,
+            "kernel_algorithmic_bytes_per_launch": int(t_bytes),
+            "largest_class_in_profile": max(per, key=lambda k: per[k]["ms_per_substep"]),
+            "per_kernel": per,
             "per_kernel_window": f"every kernel class event-timed over {PROFILE_SUBSTEPS} substeps after the timed ones"}
     cpu_baseline = None
     if not args.no_cpu:
@@ -425,10 +437,10 @@ def run_materials(args):
         except Exception as ex:
             cpu_baseline = {"value": None, "unit": "particle-substeps/s", "cores": 0, "kind": "port",
                             "sample": f"unavailable: {ex}"}
-    cfg = {"workload": (f"2D mixed-material scene: {F:,} fluid slots (SAND 19,600, METAL 240, held STONE 1,200, "
-                        f"RICE 12,100 over the bond ramp, water 78,000) + {N - F:,} boundary in a walled 512 x 160 wu "
-                        f"box, {img.gridW}x{img.gridH} cells, {iters} density iterations, every material pass; "
-                        f"substeps {args.warmup + 1}..{args.warmup + args.steps} from load, graph-replayed; "
+    cfg = {"workload": (f"2D mixed-material scene at rest spacing: {F:,} fluid slots (SAND 24,000, METAL 480, "
+                        f"held STONE 2,400, RICE 20,000 over the bond ramp, water 60,000) + {N - F:,} boundary in a "
+                        f"walled 1024 x 256 wu box, {img.gridW}x{img.gridH} cells, {iters} density iterations, every material pass; "
+                        f"substeps {args.warmup + 1}..{args.warmup + args.steps} from load, launched eagerly; "
                         "unit = one fluid particle-substep"),
            "n_particles": N, "n_fluid": F, "cells": C, "iters": iters,
            "slots_with_bonds_at_end": bonded, "bursts_at_end": int(((st[:, 5] & 64) != 0).sum()),

@@ -212,6 +212,7 @@ struct Dev {
   uint32_t* bondCand;     // bondForm1's candidate rows, same layout
   uint32_t* posOf;        // partBase + slot -> sorted position (rebuilt by canon_place)
   uint32_t gen;
+  uint64_t sealRows;      // bit m: material row m is seal class
   uint32_t N, C, S, ringCap;
   uint32_t nEvents;
   uint32_t NS;  // neighbour-list stride: nbr[k * NS + p]
@@ -1766,24 +1767,31 @@ __device__ __forceinline__ int32_t sq2(int32_t x, int32_t y) { return wadd(wmul(
 
 // friction (src/sim.wgsl:1673-1934) for fluid slot p: tangential Coulomb
 // correction against every neighbour, then the bond springs of p's row
-// (breaking ones are tombstoned in place).  Writes dp only.
+// (breaking ones are tombstoned in place).  FR_L lanes share one particle:
+// lane `sub` takes neighbour entries and bond words k = sub, sub + FR_L, ...
+// and the lanes' wrapping sums are combined afterwards, which is exact (u32
+// sums are order-free, and each lane writes only its own bond words).  A
+// row is dense — every word after its first zero is zero — so a lane may
+// stop at the first zero it meets.
+constexpr uint32_t FR_L = 4;
+
+struct FricSums {
+  int32_t fx, fy, bx, by;
+};
+
 template <bool BND>
-__device__ void friction_at(const Dev& D, const SceneP& P, uint32_t p, Dbg& g) {
+__device__ FricSums friction_part(const Dev& D, const SceneP& P, uint32_t p, uint32_t sub, Dbg& g) {
+  FricSums r{0, 0, 0, 0};
   const int4 bi = D.ptY[p];
   const int32_t* Mi = MAT(D, (uint32_t)bi.x);
   const int32_t fi = Mi[M_FRIC];
-  const uint32_t flagsI = (uint32_t)bi.y;
-  if (fi == 0 && Mi[M_BONDK] == 0) {
-    D.dp[p] = make_int2(0, 0);
-    return;
-  }
+  if (fi == 0 && Mi[M_BONDK] == 0) return r;   // dp = (0, 0): the zero-work path
   const int2 pi = D.pred[p];
-  const int4 ai = D.psY[p];
-  const int2 dxi = make_int2(fp_sub_sat(pi.x, ai.x, g), fp_sub_sat(pi.y, ai.y, g));
-  int32_t fx = 0, fy = 0;
   if (fi != 0) {
+    const int4 ai = D.psY[p];
+    const int2 dxi = make_int2(fp_sub_sat(pi.x, ai.x, g), fp_sub_sat(pi.y, ai.y, g));
     const uint32_t cnt = D.nbrN[p] & 0x1fffffffu;
-    for (uint32_t k = 0; k < cnt; ++k) {
+    for (uint32_t k = sub; k < cnt; k += FR_L) {
       const uint32_t e = D.nbr[(size_t)k * D.NS + p];
       const uint32_t q = e & ~NBR_BND;
       const int2 pj = D.pred[q];
@@ -1810,18 +1818,17 @@ __device__ void friction_at(const Dev& D, const SceneP& P, uint32_t p, Dbg& g) {
       const int32_t wq = fp_shr_rne(D.lut4[id].x, 4);
       const int32_t sx = fp_mul_s(fp_mul_s(tx, wq, 16 - ACC - RPRE, g), mu, 16, g);
       const int32_t sy = fp_mul_s(fp_mul_s(ty, wq, 16 - ACC - RPRE, g), mu, 16, g);
-      fx = wsub(fx, clampi(sx, -FR_TERM, FR_TERM));
-      fy = wsub(fy, clampi(sy, -FR_TERM, FR_TERM));
+      r.fx = wsub(r.fx, clampi(sx, -FR_TERM, FR_TERM));
+      r.fy = wsub(r.fy, clampi(sy, -FR_TERM, FR_TERM));
     }
   }
-  int32_t bx = 0, by = 0;
-  if (Mi[M_BONDK] != 0 && (flagsI & FLAG_BURST) == 0u) {
+  if (Mi[M_BONDK] != 0 && ((uint32_t)bi.y & FLAG_BURST) == 0u) {
     const int32_t ki = bond_k(Mi, bi.z, g);
     const int32_t li = D.lam[p];
     uint32_t* row = bond_row(D, D.bond, P, D.idY[p]);
-    for (uint32_t k = 0; k < BOND_CAP; ++k) {
+    for (uint32_t k = sub; k < BOND_CAP; k += FR_L) {
       const uint32_t w = row[k];
-      if (w == 0u) break;                        // rows are dense
+      if (w == 0u) break;
       if (w & BOND_TOMB) continue;
       const uint32_t q = D.posOf[P.partBase + (w & BOND_J_MASK)];
       const int4 bj = D.ptY[q];
@@ -1848,22 +1855,33 @@ __device__ void friction_at(const Dev& D, const SceneP& P, uint32_t p, Dbg& g) {
       const int32_t C = fp_sub_sat(d2, L02, g);
       const int32_t qq = fp_divshift(C, wmul(d2, 4), 16, g);
       const int32_t qk = fp_mul_s(qq, kp, 16, g);
-      bx = wsub(bx, clampi(fp_mul_s(d4x, qk, 16 - BPRE - ACC, g), -BD_TERM, BD_TERM));
-      by = wsub(by, clampi(fp_mul_s(d4y, qk, 16 - BPRE - ACC, g), -BD_TERM, BD_TERM));
+      r.bx = wsub(r.bx, clampi(fp_mul_s(d4x, qk, 16 - BPRE - ACC, g), -BD_TERM, BD_TERM));
+      r.by = wsub(r.by, clampi(fp_mul_s(d4y, qk, 16 - BPRE - ACC, g), -BD_TERM, BD_TERM));
     }
   }
-  D.dp[p] = make_int2(fp_add_sat(fx, bx, g), fp_add_sat(fy, by, g));
+  return r;
 }
 
+// Launched over FR_L x (the per-particle kernels' blocks): logical block lb
+// serves 256 / FR_L slots of 256-slot block blockMap[lb / FR_L].
 __global__ __launch_bounds__(256) void k_friction(Dev D) {
-  const uint32_t p = slot_block(D) * 256u + threadIdx.x;
+  const uint32_t lb = xcd_block();
+  const uint32_t p = D.blockMap[lb / FR_L] * 256u + (lb % FR_L) * (256u / FR_L) + threadIdx.x / FR_L;
+  const uint32_t sub = threadIdx.x % FR_L;
   const uint32_t s = p < D.N ? D.sidOf[p] : INACTIVE;
   Dbg g;
   const bool bnd = wave_bounded(D, s);
-  if (s != INACTIVE && (D.info[p] & INF_FLUID)) {
-    if (bnd) friction_at<true>(D, D.sp[s], p, g);
-    else friction_at<false>(D, D.sp[s], p, g);
+  const bool act = s != INACTIVE && (D.info[p] & INF_FLUID);
+  FricSums r{0, 0, 0, 0};
+  if (act) r = bnd ? friction_part<true>(D, D.sp[s], p, sub, g) : friction_part<false>(D, D.sp[s], p, sub, g);
+#pragma unroll
+  for (uint32_t off = 1; off < FR_L; off <<= 1) {
+    r.fx = wadd(r.fx, __shfl_xor(r.fx, (int)off));
+    r.fy = wadd(r.fy, __shfl_xor(r.fy, (int)off));
+    r.bx = wadd(r.bx, __shfl_xor(r.bx, (int)off));
+    r.by = wadd(r.by, __shfl_xor(r.by, (int)off));
   }
+  if (act && sub == 0u) D.dp[p] = make_int2(fp_add_sat(r.fx, r.bx, g), fp_add_sat(r.fy, r.by, g));
   flush_dbg(D, s, g, 0);
 }
 
@@ -1929,6 +1947,42 @@ __device__ __forceinline__ void seal_segment_test(int2 pj, int2 pk, int2 pos, in
   pr = pos;
 }
 
+constexpr int64_t SEAL_NEAR = 98304;   // 1.5 wu: longer than any kept seal segment (1.4504 wu)
+
+// containOverflow's candidate sequence: up to three contiguous ranges of
+// sorted positions (one per grid row of the 3x3 block), filtered to the
+// seal-class fluid slots whose pred lies in the reach box.
+struct SealBox {
+  int32_t loX, hiX, loY, hiY;
+  uint32_t lo[3], hi[3];
+  uint32_t nr;
+};
+struct SealCursor {
+  uint32_t r, t;   // range index, position
+};
+__device__ __forceinline__ SealCursor seal_begin(const SealBox& B) {
+  SealCursor c;
+  c.r = 0;
+  c.t = B.nr ? B.lo[0] : 0u;
+  return c;
+}
+// Advance c to the first qualifying position at or after it; false at the end.
+__device__ __forceinline__ bool seal_next(const Dev& D, const SceneP& P, const SealBox& B, SealCursor& c) {
+  while (c.r < B.nr) {
+    if (c.t >= B.hi[c.r]) {
+      if (++c.r < B.nr) c.t = B.lo[c.r];
+      continue;
+    }
+    const uint32_t t = c.t;
+    if (D.idY[t] < P.nFluid && seal_class(MAT(D, (uint32_t)D.ptY[t].x))) {
+      const int2 pj = D.pred[t];
+      if (!(pj.x < B.loX || pj.x > B.hiX || pj.y < B.loY || pj.y > B.hiY)) return true;
+    }
+    c.t++;
+  }
+  return false;
+}
+
 // contain (src/sim.wgsl:2056-2136) for a list below MAXNBR and
 // containOverflow (:2147-2235) for a full one, in one launch: both write only
 // the pred of a non-seal slot and read only the preds of seal-class slots,
@@ -1945,7 +1999,46 @@ __device__ void contain_at(const Dev& D, const SceneP& P, uint32_t p, Dbg& g) {
   int2 pr = D.pred[p];
   const bool liquidI = seal_liquid(Mi);
   const uint32_t cnt = nn & 0x1fffffffu;
-  if (cnt < MAXNBR) {
+  if (cnt < MAXNBR && P.bounded) {
+    // The crossing tests only ever set pr back to pos, and once pr == pos no
+    // test fires again (s0 == s1), so the pass reverts exactly when some
+    // pair of seal neighbours, tested against the pushed pr, fires: an OR
+    // over the pairs.  A pair fires only if its segment's box meets the
+    // move's box, and a kept segment is at most 1.4504 wu long (e2 <=
+    // SEAL_SEG_MAX2), so an end farther than SEAL_NEAR from the move's box
+    // on either axis rules out every pair it is in.  Positions of a bounded
+    // scene cannot saturate, so skipped tests would have counted no event.
+    uint64_t sm = 0;   // neighbour entries that are seal-class fluid slots
+    for (uint32_t k = 0; k < cnt; ++k) {
+      const uint32_t e = D.nbr[(size_t)k * D.NS + p];
+      if (e & NBR_BND) continue;
+      if (!((D.sealRows >> ((uint32_t)D.ptY[e].x & 63u)) & 1ull)) continue;
+      sm |= 1ull << k;
+      seal_keep_push(D.pred[e], pr, g);
+    }
+    if (sm != 0u) {
+      const int64_t bx0 = (int64_t)mini(pos.x, pr.x) - SEAL_NEAR, bx1 = (int64_t)maxi(pos.x, pr.x) + SEAL_NEAR;
+      const int64_t by0 = (int64_t)mini(pos.y, pr.y) - SEAL_NEAR, by1 = (int64_t)maxi(pos.y, pr.y) + SEAL_NEAR;
+      uint64_t live = 0;
+      for (uint64_t m = sm; m; m &= m - 1u) {
+        const uint32_t k = (uint32_t)__builtin_ctzll(m);
+        const int2 pj = D.pred[D.nbr[(size_t)k * D.NS + p]];
+        if (pj.x >= bx0 && pj.x <= bx1 && pj.y >= by0 && pj.y <= by1) live |= 1ull << k;
+      }
+      // once reverted (pr == pos) nothing fires again: stop there
+      for (uint64_t m = live; m != 0u && (pr.x != pos.x || pr.y != pos.y); m &= m - 1u) {
+        const uint32_t k = (uint32_t)__builtin_ctzll(m);
+        const uint32_t e = D.nbr[(size_t)k * D.NS + p];
+        const int2 pj = D.pred[e];
+        const bool modeJ = liquidI && MAT(D, (uint32_t)D.ptY[e].x)[M_SEALMODE] != 0;
+        for (uint64_t m2 = live & ~((2ull << k) - 1ull); m2; m2 &= m2 - 1u) {
+          const uint32_t e2 = D.nbr[(size_t)__builtin_ctzll(m2) * D.NS + p];
+          if (modeJ && MAT(D, (uint32_t)D.ptY[e2].x)[M_SEALMODE] != 0) continue;
+          seal_segment_test(pj, D.pred[e2], pos, pr, g);
+        }
+      }
+    }
+  } else if (cnt < MAXNBR) {
     bool touched = false;
     for (uint32_t k = 0; k < cnt; ++k) {
       const uint32_t e = D.nbr[(size_t)k * D.NS + p];
@@ -1972,40 +2065,41 @@ __device__ void contain_at(const Dev& D, const SceneP& P, uint32_t p, Dbg& g) {
       }
     }
   } else {
-    // the seal-class slots of the 3x3 cells around pred, in canonical order,
-    // within SEAL_REACH of the swept box; the first 48 are kept
-    uint32_t seals[48];
-    uint32_t sealN = 0;
-    const int32_t loX = fp_sub_sat(mini(pos.x, pr.x), SEAL_REACH, g);
-    const int32_t hiX = fp_add_sat(maxi(pos.x, pr.x), SEAL_REACH, g);
-    const int32_t loY = fp_sub_sat(mini(pos.y, pr.y), SEAL_REACH, g);
-    const int32_t hiY = fp_add_sat(maxi(pos.y, pr.y), SEAL_REACH, g);
+    // The seal-class slots of the 3x3 cells around pred, in canonical order,
+    // within SEAL_REACH of the swept box; the first 48 are used.  The cells
+    // gx-1..gx+1 of one grid row are consecutive in the cell-sorted order, so
+    // the candidates are up to three contiguous position ranges, enumerated
+    // again for each use instead of stored (no private array, no scratch).
+    SealBox B;
+    B.loX = fp_sub_sat(mini(pos.x, pr.x), SEAL_REACH, g);
+    B.hiX = fp_add_sat(maxi(pos.x, pr.x), SEAL_REACH, g);
+    B.loY = fp_sub_sat(mini(pos.y, pr.y), SEAL_REACH, g);
+    B.hiY = fp_add_sat(maxi(pos.y, pr.y), SEAL_REACH, g);
+    B.nr = 0;
     const CC cc = cell_coord(P, pr);
+    const int32_t gx0 = maxi(cc.x - 1, 0), gx1 = mini(cc.x + 1, (int32_t)P.gridW - 1);
     for (int32_t gy = cc.y - 1; gy <= cc.y + 1; ++gy) {
       if (gy < 0 || gy >= (int32_t)P.gridH) continue;
-      for (int32_t gx = cc.x - 1; gx <= cc.x + 1; ++gx) {
-        if (gx < 0 || gx >= (int32_t)P.gridW) continue;
-        const uint32_t c = P.cellBase + (uint32_t)gy * P.gridW + (uint32_t)gx;
-        const uint32_t lo = (uint32_t)((int32_t)D.cellStart[c] + P.posShift);
-        const uint32_t hi = (uint32_t)((int32_t)D.cellStart[c + 1] + P.posShift);
-        for (uint32_t t = lo; t < hi; ++t) {
-          if (D.idY[t] >= P.nFluid) continue;
-          if (!seal_class(MAT(D, (uint32_t)D.ptY[t].x))) continue;
-          const int2 pj = D.pred[t];
-          if (pj.x < loX || pj.x > hiX || pj.y < loY || pj.y > hiY) continue;
-          if (sealN < 48u) seals[sealN++] = t;
-        }
-      }
+      const uint32_t c0 = P.cellBase + (uint32_t)gy * P.gridW + (uint32_t)gx0;
+      const uint32_t c1 = P.cellBase + (uint32_t)gy * P.gridW + (uint32_t)gx1;
+      B.lo[B.nr] = (uint32_t)((int32_t)D.cellStart[c0] + P.posShift);
+      B.hi[B.nr] = (uint32_t)((int32_t)D.cellStart[c1 + 1] + P.posShift);
+      B.nr++;
     }
-    for (uint32_t k = 0; k < sealN; ++k) seal_keep_push(D.pred[seals[k]], pr, g);
-    for (uint32_t k = 0; k < sealN; ++k) {
-      const int32_t* Mj = MAT(D, (uint32_t)D.ptY[seals[k]].x);
-      const int2 pj = D.pred[seals[k]];
-      for (uint32_t k2 = k + 1; k2 < sealN; ++k2) {
-        const int32_t* Mk = MAT(D, (uint32_t)D.ptY[seals[k2]].x);
+    SealCursor a = seal_begin(B);
+    for (uint32_t k = 0; k < 48u && seal_next(D, P, B, a); ++k, ++a.t) seal_keep_push(D.pred[a.t], pr, g);
+    a = seal_begin(B);
+    for (uint32_t k = 0; k < 48u && seal_next(D, P, B, a); ++k) {
+      const int32_t* Mj = MAT(D, (uint32_t)D.ptY[a.t].x);
+      const int2 pj = D.pred[a.t];
+      SealCursor b = a;
+      b.t++;
+      for (uint32_t k2 = k + 1; k2 < 48u && seal_next(D, P, B, b); ++k2, ++b.t) {
+        const int32_t* Mk = MAT(D, (uint32_t)D.ptY[b.t].x);
         if (liquidI && Mj[M_SEALMODE] != 0 && Mk[M_SEALMODE] != 0) continue;
-        seal_segment_test(pj, D.pred[seals[k2]], pos, pr, g);
+        seal_segment_test(pj, D.pred[b.t], pos, pr, g);
       }
+      a.t++;
     }
   }
   D.pred[p] = pr;
@@ -2514,7 +2608,7 @@ void enqueue_scan(sn_engine* e, uint32_t* count, uint32_t n, uint32_t* out, bool
 // Kernel classes timed by sn_profile_step (index into its outputs, + 1).
 enum ProfClass {
   PC_PREDICT = 0, PC_SCAN, PC_SCATTER, PC_CANON, PC_NBR, PC_SOLVEA, PC_COLOUR, PC_XNC, PC_STH, PC_FINISH,
-  PC_IMPULSE, PC_LISTS, PC_APPLY, PC_MATERIAL, PC_COUNT
+  PC_IMPULSE, PC_LISTS, PC_APPLY, PC_FRICTION, PC_MATERIAL, PC_COUNT
 };
 
 // Record an event before (begin) / after (end) one launch while profiling.
@@ -2600,8 +2694,8 @@ void enqueue_substep(sn_engine* e) {
                          (uint32_t)(!gen && it + 1 == iters));
     }
     if (gen) {
-      ProfScope ps(e, PC_MATERIAL);
-      hipLaunchKernelGGL(k_friction, dim3(e->actBlocks), dim3(256), 0, st, D);
+      ProfScope ps(e, PC_FRICTION);
+      hipLaunchKernelGGL(k_friction, dim3(e->actBlocks * FR_L), dim3(256), 0, st, D);
       hipLaunchKernelGGL(k_apply_fric, dim3(e->actBlocks), dim3(256), 0, st, D);
     }
   }
@@ -2783,6 +2877,12 @@ int sn_create(const sn_opts* opts, const int32_t* luts, const int32_t* mats, sn_
     int32_t* dm = dalloc<int32_t>(e, MAT_MAX * MAT_WORDS);
     HIPCHECK(hipMemcpy(dm, mats, sizeof(int32_t) * MAT_MAX * MAT_WORDS, hipMemcpyHostToDevice));
     D.mats = dm;
+    D.sealRows = 0;
+    for (uint32_t m = 0; m < MAT_MAX; ++m) {
+      const int32_t* M = mats + (size_t)m * MAT_WORDS;
+      if (M[M_BONDK] != 0 && M[M_BONDREFORM] == 0 && M[M_SOLIDMODE] != 0 && M[M_PHANTOM] == 0)
+        D.sealRows |= 1ull << m;   // seal_class (src/sim.wgsl:1978-1981)
+    }
     D.sp = dalloc<SceneP>(e, SC);
     D.sd = dalloc<SceneD>(e, SC);
     D.hacc = dalloc<uint32_t>(e, (size_t)SC * 8);

@@ -270,7 +270,7 @@ class Engine:
     # kernel classes of sn_profile_step, slot 0 = the whole span
     PROFILE_KEYS = ("total", "predict", "scan", "scatter", "canon_gather", "nbr_solveA", "solveA",
                     "solveB", "xnc", "st_thermal_hash", "hash_finish", "impulse", "lists", "apply",
-                    "materials")
+                    "friction", "materials")
 
     def profile_step(self, count: int = 1, classes: Optional[Iterable[str]] = None) -> dict:
         """Run `count` substeps eagerly with an event pair around every launch

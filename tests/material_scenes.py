@@ -106,6 +106,25 @@ def hot_metal(seed=0x3E7A1):
     return _scene(bar, 36, 30, "hot_metal", temps=temps)
 
 
+def seal_crossing(seed=0xC505):
+    """A held STONE line (seal class, anchored) and water probes just above
+    it, thrown down across it at 0.6 wu per substep, between the stones:
+    each probe's straight move crosses a seal segment, so contain reverts it
+    to where it started (src/sim.wgsl:2104-2135).  A second row of probes
+    is thrown straight at stones (the keep-out push)."""
+    rnd = scenes.xorshift32(seed)
+    y_line = 20 * ONE
+    stones = [(10 * ONE + k * ONE + (rnd() % 64), y_line, mt.STONE) for k in range(16)]
+    probes = [(10 * ONE + k * ONE + (ONE >> 1), y_line - (ONE // 5), mt.WATER) for k in range(0, 15, 2)]
+    probes += [(10 * ONE + k * ONE, y_line - (ONE // 3), mt.WATER) for k in range(1, 15, 2)]
+    s = _scene(stones + probes, 40, 40, "seal_crossing",
+               flags=[scenes.FLAG_ANCHOR] * len(stones) + [0] * len(probes))
+    for i in range(len(stones), len(stones) + len(probes)):
+        s.vel[2 * i + 1] = (ONE * 3) // 5
+    s.meta = dict(n_stones=len(stones), n_cross=8)
+    return s
+
+
 # name -> (builder, substeps, create the world with materials on)
 CASES = {
     "sand_and_water": (sand_and_water, 48, False),
@@ -114,4 +133,5 @@ CASES = {
     "freezing_pool": (freezing_pool, 40, True),
     "packed_against_stone": (packed_against_stone, 32, False),
     "hot_metal": (hot_metal, 32, False),
+    "seal_crossing": (seal_crossing, 16, False),
 }

@@ -9,6 +9,8 @@
   tombstone survives.
 * The bond fold of the state hash equals the vectorised digestState twin
   (sunaengine_amd.tables.digest_words, after src/hash/statehash.js:80-159).
+* A move across a held seal segment is reverted: the particle ends the
+  substep where it started (src/sim.wgsl:2104-2135).
 * Friction is antisymmetric: with no boundary neighbour, no truncation and
   no saturating term, the friction terms of a SAND heap cancel pairwise, so
   one application of friction moves the heap's summed dp by zero
@@ -107,6 +109,29 @@ def test_bond_fold_matches_digest_twin():
         assert want == [int(x) for x in o.digest()], s.name
         # and the fold matters: without it the digest differs
         assert tables.digest_words(st.reshape(-1), 8, 8) != want
+
+
+def test_crossing_a_seal_segment_reverts_the_move():
+    """Water thrown across a held STONE line (each probe's straight move
+    crosses the segment between two stones) ends the substep where it
+    started, with zero velocity; the held stones do not move."""
+    s = ms.seal_crossing()
+    o = oracle_world(s, 4)
+    st0 = o.read_state().copy()
+    o.step(1)
+    st = o.read_state()
+    ns = s.meta["n_stones"]
+    probes = slice(ns, s.nFluid)
+    assert np.array_equal(st[probes, :2], st0[probes, :2])
+    assert not st[probes, 2:4].any()
+    assert np.array_equal(st[:ns, :2], st0[:ns, :2])
+    # without the line, the same probes fall through: the revert is contain's doing
+    s2 = ms.seal_crossing()
+    s2.matId[:ns] = mt.WATER
+    s2.flags[:ns] = 0
+    o2 = oracle_world(s2, 4)
+    o2.step(1)
+    assert (o2.read_state()[probes, 1] > st0[probes, 1]).all()
 
 
 def test_friction_terms_cancel_pairwise():
