@@ -32,10 +32,14 @@
 //                  (:2409-2707, :3039-3097, src/hash/statehash.wgsl:83-159)
 //   hash_finish    chain fold (statehash.wgsl:164-170), H ring append
 //
-// Material paths: frictionless, bondless rows (no friction/bond/seal/rigid
-// terms — those passes are bit-inert for them, src/sim.wgsl:1698-1702,
-// 2062, 2747, 2830).  A transition into any other row sets the scene's
-// fault word and sn_sync reports SN_E_UNSUPPORTED.
+// Material passes (src/sim.wgsl:1673-2235, 2708-3097) run in worlds whose
+// scenes hold fluid rows with friction or bonds, or that were created with
+// materials = 1: friction + applyFric per iteration after the colour
+// passes, contain before a separate finalize, and containStress, bond
+// formation and a bond-folded hash after st_thermal.  Water-only worlds skip
+// them (they are bit-inert for frictionless, bondless rows, :1698-1702,
+// 2062, 2747, 2830); there a transition into a row with friction or bonds
+// sets the scene's fault word and sn_sync reports SN_E_UNSUPPORTED.
 #include <hip/hip_runtime.h>
 
 #include <algorithm>
@@ -1636,7 +1640,9 @@ __device__ __forceinline__ void hash_reduce(const Dev& D, uint32_t blk, uint32_t
 // src/sim.wgsl:2409-2494 (stAcc for the next substep), :2612-2707 (thermal,
 // elementwise), bondForm2's flag clear on formation substeps (:3086-3096;
 // bond rows stay empty for bondless rows), then the particle's state hash
-// (src/hash/statehash.wgsl:83-159) reduced per scene.
+// (src/hash/statehash.wgsl:83-159) reduced per scene.  With gen = 1 (the
+// material passes run) the flag clear and the hash are left to
+// k_bond_form2 and k_hash_gen.
 __global__ __launch_bounds__(256) void k_st_thermal_hash(Dev D, uint32_t gen) {
   const uint32_t blk = slot_block(D);
   uint32_t p = blk * 256u + threadIdx.x;
