@@ -2337,24 +2337,31 @@ __global__ __launch_bounds__(256) void k_hash_gen(Dev D) {
   hash_reduce(D, blk, s, hv);
 }
 
-// Digest of the current state without advancing (digest(), src/engine.js:4453).
+// Digest of the current state without advancing (digest(), src/engine.js:4453),
+// with the bond fold in a world that runs the material passes.
 __global__ __launch_bounds__(256) void k_hash_state(Dev D, uint32_t s) {
   const SceneP& P = D.sp[s];
   uint32_t t = blockIdx.x * 256u + threadIdx.x;
   if (t >= P.n) return;
   uint32_t p = P.partBase + t;
-  const uint32_t SEED[4] = {0x9e3779b1u, 0x85ebca77u, 0xc2b2ae3du, 0x27d4eb2fu};
-  const uint32_t SALT[4] = {0x00000000u, 0x51633e2du, 0x9c8f2a17u, 0xd0e5b943u};
   int4 a = D.ps[p], b = D.pt[p];
   uint32_t myid = D.id[p];
   uint32_t words[8] = {(uint32_t)a.x, (uint32_t)a.y, (uint32_t)a.z, (uint32_t)a.w,
                        (uint32_t)b.x, (uint32_t)b.y, (uint32_t)b.z, (uint32_t)b.w};
-  for (int l = 0; l < 4; ++l) {
-    uint32_t h = SEED[l];
-    for (int w = 0; w < 8; ++w) h = mm_step(h, words[w] ^ SALT[l]);
-    atomicAdd(&D.hacc[s * 8 + l], fmix32(h ^ 8u));
-    atomicAdd(&D.hacc[s * 8 + 4 + l], fmix32(mm_step(h, myid) ^ 9u));
+  uint32_t bn = 0, com = 0, scom = 0;
+  if (D.gen && myid < P.nFluid) {
+    const uint32_t* row = bond_row(D, D.bond, P, myid);
+    for (uint32_t k = 0; k < BOND_CAP; ++k) {
+      const uint32_t w = row[k];
+      if (w == 0u) break;
+      com += fmix32(((w >> 17) & 0x1fffu) ^ BOND_SALT);
+      scom += fmix32(w ^ BOND_SALT2);
+      bn++;
+    }
   }
+  uint32_t hv[8];
+  hash_terms(words, myid, bn, com, scom, hv);
+  for (int l = 0; l < 8; ++l) atomicAdd(&D.hacc[s * 8 + l], hv[l]);
 }
 
 // Per scene: fold the chain, append this substep's digest to the H ring,

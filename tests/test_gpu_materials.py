@@ -68,6 +68,37 @@ def test_material_scene_matches_oracle(name):
         assert o.debug_counters()[6] > 0  # full lists: containOverflow gathers the seals itself
 
 
+def test_spawn_in_a_material_world():
+    """spawnFluid in a bonded world: the existing fluid slots keep their bond
+    rows across the reload, the new slots start empty, and formation runs
+    the next substep (src/engine.js:5901-6020)."""
+    from sunaengine_amd import scenes
+    from sunaengine_amd.engine import Engine
+    from sunaengine_amd import materials as mt
+    s = ms.stone_in_water()
+    im = scenes.load_image(s)
+    cap = im.n + 64
+    o = oracle_world(s, 4, max_particles=cap)
+    e = Engine(max_particles=cap, max_cells=im.cellTotal, iters=4)
+    e.load_scene(s)
+    o.step(9)
+    e.step(9)
+    assert (e.read_bond() != 0).any()
+    drop = [dict(x=(14 + dx) * 65536, y=(12 + dy) * 65536, vx=0, vy=2000, mat=mt.SAND if dx < 3 else mt.WATER)
+            for dy in range(2) for dx in range(6)]
+    fl, bd = scenes.spawn_records(s, drop)
+    assert o.spawn(fl, bd) == e.spawn_fluid(drop)
+    done = 0
+    for k in (1, 5, 20):
+        o.step(k - done)
+        e.step(k - done)
+        done = k
+        assert np.array_equal(e.read_state(), o.read_state()), f"state after {k}"
+        assert np.array_equal(e.read_bond(), o.read_bond()), f"bonds after {k}"
+        assert np.array_equal(e.digest()["acc"], o.digest()), f"digest after {k}"
+    assert np.array_equal(e.chain(), o.chain())
+
+
 def test_stripped_build_on_material_scene():
     """debug:false runs the same material passes: identical stream and bonds."""
     s = ms.stone_in_water()
