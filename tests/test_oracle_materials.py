@@ -1,5 +1,8 @@
 """The oracle's material passes against checks that do not restate them.
 
+* Freezing: water poured below 0 C becomes ICE on substep 0 and forms
+  congeal bonds to exactly the brute-force set of same-row pairs in range;
+  water at 100 C becomes its boil row.
 * Weld bonds by brute force: a weld-tagged STONE slab at rest forms, on
   substep 0, exactly the bonds an O(n^2) search over its final positions
   gives (every same-tag pair with D2_FLOOR <= d2 <= BOND_R2 at 1/32-tick
@@ -79,6 +82,60 @@ def test_weld_bonds_match_brute_force():
     assert got == want
     # the weld tags are cleared by the same formation pass
     assert not (st[:nf, 5] & 0x000FFF00).any()
+
+
+def test_freezing_forms_congeal_bonds_by_brute_force():
+    """Water poured at -10 C freezes to its freezeTo row (ICE) on substep 0's
+    thermal pass, with FLAG_CONGEAL and FLAG_SEAL; substep 0 is a formation
+    substep, so bondForm1 bonds every same-row pair in range without the
+    velocity and pressure gates, bondForm2 keeps the mutual ones and clears
+    FLAG_CONGEAL (src/sim.wgsl:2612-2707, 2794-3097).  After one substep the
+    bond set equals the O(n^2) pair search over the final positions."""
+    rnd = scenes.xorshift32(0xF0E5)
+    floor = (30 - 3) * ONE
+    pool = ms._block(rnd, mt.WATER, 12, 6, 8 * ONE, floor, ONE, jitter=256)
+    s = ms._scene(pool, 30, 30, "cold_pool", temps=[mt.DEG(-10)] * len(pool))
+    o = oracle_world(s, 4)
+    o.step(1)
+    st = o.read_state()
+    nf = s.nFluid
+    ice = mt.MATS[mt.WATER][mt.F["freezeTo"]]
+    assert (st[:nf, 4] == ice).all()
+    assert ((st[:nf, 5] & 128) != 0).all() and not (st[:nf, 5] & 32).any()   # SEAL set, CONGEAL cleared
+    bond = o.read_bond()
+    got = {(i, w & 0x1FFFF, (w >> 17) & 0x1FFF) for i, row in enumerate(_rows(bond, nf)) for w in row}
+    want = set()
+    for i in range(nf):
+        for j in range(nf):
+            if i == j:
+                continue
+            d4x = _shr_rne(int(st[i, 0]) - int(st[j, 0]), 5)
+            d4y = _shr_rne(int(st[i, 1]) - int(st[j, 1]), 5)
+            if abs(d4x) > 5120 or abs(d4y) > 5120:
+                continue
+            d2 = d4x * d4x + d4y * d4y
+            if D2_FLOOR <= d2 <= BOND_R2:
+                want.add((i, j, math.isqrt(d2)))
+    assert len(want) > 4 * nf
+    assert got == want
+
+
+def test_boiling_water_turns_to_its_boil_row():
+    """Water held uniformly at 101 C (no conduction; cooling toward ambient
+    takes off 0.05 C) is past boilPt on substep 0's thermal pass and becomes
+    its boilTo row with that row's fuel0; at 99 C it stays water
+    (src/sim.wgsl:2612-2707)."""
+    steam = mt.MATS[mt.WATER][mt.F["boilTo"]]
+    for deg, want in ((101, steam), (99, mt.WATER)):
+        rnd = scenes.xorshift32(0xB011)
+        floor = (30 - 3) * ONE
+        pool = ms._block(rnd, mt.WATER, 12, 6, 8 * ONE, floor, ONE, jitter=256)
+        s = ms._scene(pool, 30, 30, "hot_pool", temps=[mt.DEG(deg)] * len(pool))
+        o = oracle_world(s, 4)
+        o.step(1)
+        st = o.read_state()
+        assert (st[: s.nFluid, 4] == want).all(), deg
+        assert (st[: s.nFluid, 7] == mt.MATS[want][mt.F["fuel0"]]).all()
 
 
 def test_bonds_are_mutual_after_formation():
