@@ -56,7 +56,7 @@ PROFILE_SUBSTEPS = 20       # follow-on substeps profiled with every kernel clas
 
 # kernel behind each profiled class (the HIP symbol rocprofv3 reports)
 KERNEL_OF = {"predict": "k_predict_count", "scan": "k_scan_reduce+k_scan_top+k_scan_down",
-             "scatter": "k_scatter", "canon_gather": "k_canon_gather+k_canon_heavy_rank+k_canon_heavy_place",
+             "scatter": "k_scatter", "canon_gather": "k_canon_gather+k_canon_heavy_sort+k_canon_heavy_rank+k_canon_heavy_place",
              "nbr_solveA": "k_nbr_tile",
              "solveA": "k_solveA", "solveB": "k_solveB", "apply": "k_apply", "xnc": "k_xnc",
              "st_thermal_hash": "k_st_thermal_hash", "hash_finish": "k_hash_finish",

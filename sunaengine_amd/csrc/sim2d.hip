@@ -507,12 +507,64 @@ __device__ __forceinline__ HeavySpan heavy_item(const Dev& D, uint32_t i) {
   return h;
 }
 
+// Heavy cells of up to SORT_MAX slots: one workgroup per cell sorts the
+// cell's slot ids in LDS (bitonic network over the next power of two, pads
+// sort last) and gives each slot the position of its id in the sorted run,
+// i.e. #{ids below it}: O(k log k) instead of the pairwise count's O(k^2).
+// Ids are distinct, so the position is exact.  Larger cells take the
+// pairwise count below.
+constexpr uint32_t SORT_MAX = 16384;
+constexpr uint32_t SORT_THREADS = 512;
+
+__global__ __launch_bounds__(SORT_THREADS) void k_canon_heavy_sort(Dev D) {
+  __shared__ uint32_t key[SORT_MAX];
+  const uint32_t n = (uint32_t)__builtin_amdgcn_readfirstlane((int)D.listCount[5]);
+  for (uint32_t w = blockIdx.x; w < n; w += gridDim.x) {
+    const uint2 item = D.dlist[5][w];
+    if (__builtin_amdgcn_readfirstlane((int)item.y) != 0) continue;   // the cell's first item only
+    const uint32_t c = (uint32_t)__builtin_amdgcn_readfirstlane((int)item.x);
+    const int32_t sh = D.sp[D.cellScene[c]].posShift;
+    const uint32_t lo = (uint32_t)((int32_t)D.cellStart[c] + sh);
+    const uint32_t k = (uint32_t)((int32_t)D.cellStart[c + 1] + sh) - lo;
+    if (k > SORT_MAX) continue;
+    uint32_t P = 1;
+    while (P < k) P <<= 1;
+    __syncthreads();   // the previous cell's keys are no longer read
+    for (uint32_t i = threadIdx.x; i < P; i += SORT_THREADS) key[i] = i < k ? D.bucketId[lo + i] : 0xffffffffu;
+    __syncthreads();
+    for (uint32_t size = 2; size <= P; size <<= 1) {
+      for (uint32_t stride = size >> 1; stride > 0; stride >>= 1) {
+        for (uint32_t i = threadIdx.x; i < (P >> 1); i += SORT_THREADS) {
+          const uint32_t a = 2u * i - (i & (stride - 1u)), b = a + stride;
+          const uint32_t x = key[a], y = key[b];
+          if ((x > y) == ((a & size) == 0u)) {
+            key[a] = y;
+            key[b] = x;
+          }
+        }
+        __syncthreads();
+      }
+    }
+    for (uint32_t i = threadIdx.x; i < k; i += SORT_THREADS) {
+      const uint32_t id = D.bucketId[lo + i];
+      uint32_t l = 0, r = k;
+      while (l < r) {
+        const uint32_t m = (l + r) >> 1;
+        if (key[m] < id) l = m + 1u;
+        else r = m;
+      }
+      D.crank[lo + i] = l;
+    }
+  }
+}
+
 __global__ __launch_bounds__(256) void k_canon_heavy_rank(Dev D) {
   __shared__ uint4 ids[CANON_TILE / 4];
   const uint32_t n = (uint32_t)__builtin_amdgcn_readfirstlane((int)D.listCount[5]) * CANON_SPLIT;
   for (uint32_t w = blockIdx.x; w < n; w += gridDim.x) {
     const HeavySpan h = heavy_item(D, w / CANON_SPLIT);
     const uint32_t part = w % CANON_SPLIT, k = h.hi - h.lo;
+    if (k <= SORT_MAX) continue;   // ranked by k_canon_heavy_sort (uniform over the block)
     const uint32_t nparts = min(CANON_SPLIT, (k + CANON_PART_MIN - 1u) / CANON_PART_MIN);
     if (part >= nparts) continue;   // uniform over the block
     const uint32_t span = (k + nparts - 1u) / nparts;
@@ -2682,6 +2734,7 @@ void enqueue_substep(sn_engine* e) {
   {
     ProfScope ps(e, PC_CANON);
     hipLaunchKernelGGL(k_canon_gather, dim3(e->actBlocks), dim3(256), 0, st, D);
+    hipLaunchKernelGGL(k_canon_heavy_sort, dim3(e->canonBlocks), dim3(SORT_THREADS), 0, st, D);
     hipLaunchKernelGGL(k_canon_heavy_rank, dim3(e->canonBlocks), dim3(256), 0, st, D);
     hipLaunchKernelGGL(k_canon_heavy_place, dim3(e->canonBlocks), dim3(256), 0, st, D);
   }
