@@ -1364,6 +1364,13 @@ __device__ __forceinline__ void chunk_range(const Dev& D, uint2 item, uint32_t& 
 // shorter; the (G, S) kernel tables sit in LDS.  k_apply moves the chunk's
 // particles and, on the last iteration, finalizes them
 // (src/sim.wgsl:2245-2265).
+// A lane takes entries k = sub + u * L, u < SB_E, of its particle's list
+// (L >= 8 lanes per particle, so SB_E = 6 covers MAXNBR).  All list words
+// are loaded first, then all neighbour preds and lambdas, then the terms
+// are formed: one gather round trip per item instead of one per pair of
+// entries.  Entries past the count are skipped before any test.
+constexpr uint32_t SB_E = MAXNBR / 8;
+static_assert(256u / CELL_CHUNK >= 8u && SB_E * 8u >= MAXNBR, "k_solveB gives every particle at least 8 lanes");
 template <bool BND>
 __device__ __forceinline__ int2 solveB_part(const Dev& D, uint32_t p, uint32_t sub, uint32_t L,
                                             const int2* lutGS, Dbg& g) {
@@ -1371,29 +1378,31 @@ __device__ __forceinline__ int2 solveB_part(const Dev& D, uint32_t p, uint32_t s
   const int32_t li = D.lam[p];
   const uint32_t cnt = D.nbrN[p] & 0x1fffffffu;
   const uint32_t* nb = D.nbr + p;
+  uint32_t e[SB_E];
+  int2 pj[SB_E];
+  int32_t lj[SB_E];
+#pragma unroll
+  for (uint32_t u = 0; u < SB_E; ++u) {
+    const uint32_t k = sub + u * L;
+    e[u] = k < cnt ? nb[(size_t)k * D.NS] : p;
+  }
+#pragma unroll
+  for (uint32_t u = 0; u < SB_E; ++u) {
+    const uint32_t q = e[u] & ~NBR_BND;
+    pj[u] = D.pred[q];
+    lj[u] = D.lam[q];
+  }
   int32_t dpx = 0, dpy = 0;
-  for (uint32_t k = sub; k < cnt; k += 2u * L) {
-    const uint32_t k1 = k + L;
-    const uint32_t e0 = nb[(size_t)k * D.NS];
-    const uint32_t e1 = k1 < cnt ? nb[(size_t)k1 * D.NS] : p;
-    const uint32_t q0 = e0 & ~NBR_BND, q1 = e1 & ~NBR_BND;
-    const int2 pj0 = D.pred[q0], pj1 = D.pred[q1];
-    const int32_t lj0 = D.lam[q0], lj1 = D.lam[q1];
-    const int2 d0 = pdelta<BND>(pi, pj0, g), d1 = pdelta<BND>(pi, pj1, g);
-    const int32_t i0 = lut_idx<BND>(d0, g);
-    const int32_t i1 = k1 < cnt ? lut_idx<BND>(d1, g) : -1;
-    if (i0 >= 0) {
-      const int2 T = lutGS[i0];
-      const int32_t co = fp_add_sat(fp_add_sat(li, (e0 & NBR_BND) ? li : lj0, g), T.y, g);
-      dpx = wadd(dpx, clampi(fp_mul_s(co, grad_q16(T.x, d0.x, g), 20 - ACC, g), -DP_TERM, DP_TERM));
-      dpy = wadd(dpy, clampi(fp_mul_s(co, grad_q16(T.x, d0.y, g), 20 - ACC, g), -DP_TERM, DP_TERM));
-    }
-    if (i1 >= 0) {
-      const int2 T = lutGS[i1];
-      const int32_t co = fp_add_sat(fp_add_sat(li, (e1 & NBR_BND) ? li : lj1, g), T.y, g);
-      dpx = wadd(dpx, clampi(fp_mul_s(co, grad_q16(T.x, d1.x, g), 20 - ACC, g), -DP_TERM, DP_TERM));
-      dpy = wadd(dpy, clampi(fp_mul_s(co, grad_q16(T.x, d1.y, g), 20 - ACC, g), -DP_TERM, DP_TERM));
-    }
+#pragma unroll
+  for (uint32_t u = 0; u < SB_E; ++u) {
+    if (sub + u * L >= cnt) continue;
+    const int2 d = pdelta<BND>(pi, pj[u], g);
+    const int32_t ix = lut_idx<BND>(d, g);
+    if (ix < 0) continue;
+    const int2 T = lutGS[ix];
+    const int32_t co = fp_add_sat(fp_add_sat(li, (e[u] & NBR_BND) ? li : lj[u], g), T.y, g);
+    dpx = wadd(dpx, clampi(fp_mul_s(co, grad_q16(T.x, d.x, g), 20 - ACC, g), -DP_TERM, DP_TERM));
+    dpy = wadd(dpy, clampi(fp_mul_s(co, grad_q16(T.x, d.y, g), 20 - ACC, g), -DP_TERM, DP_TERM));
   }
   return make_int2(dpx, dpy);
 }
