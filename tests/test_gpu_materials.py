@@ -124,3 +124,50 @@ def test_water_is_bit_inert_under_the_material_passes():
         e.step(k - done)
         done = k
         assert e.digest()["hex"] == g["checkpoints"][str(k)], f"substep {k}"
+
+
+def test_material_mode_refusals():
+    """What a world refuses, and how (the reference's assert and fault
+    conventions): a rigid-body flag at load; a bonded scene above
+    MAX_PARTICLES_BONDED (src/engine.js:261,3331); a water world created
+    without materials that freezes into ICE reports SN_E_UNSUPPORTED (at the
+    next call that checks the scene's fault word) instead of a wrong hash;
+    created with materials it runs."""
+    from sunaengine_amd import native, scenes
+    from sunaengine_amd import materials as mt
+    from sunaengine_amd.engine import Engine
+    # FLAG_RIGID cannot come from a scene's flags (legal mask 0x7), so it is
+    # put into the state image directly
+    s = ms.stone_in_water()
+    im = scenes.load_image(s)
+    e = Engine(max_particles=im.n, max_cells=im.cellTotal, iters=4)
+    im.state[0, 5] |= 16
+    arr = (native.sn_scene * 1)()
+    st = np.ascontiguousarray(im.state, np.int32)
+    arr[0] = native.sn_scene(im.n, im.nFluid, st.ctypes.data, im.domW, im.domH, im.cellShift, None)
+    rc = native.lib().sn_load_scenes(e._h, arr, 1)
+    assert rc == native.SN_E_UNSUPPORTED and b"FLAG_RIGID" in native.lib().sn_last_error(e._h)
+    e.destroy()
+    # a bonded scene of 131,072 slots
+    big = scenes.walled_block_scene(400, 326, 512, 256, 0xB16, pitch=65536 >> 1)
+    big.matId[: big.nFluid] = mt.STONE
+    bim = scenes.load_image(big)
+    assert bim.n > 131071
+    e = Engine(max_particles=bim.n, max_cells=bim.cellTotal, iters=4)
+    with pytest.raises(native.SnError) as ex:
+        e.load_scene(big)
+    assert ex.value.code == native.SN_E_CAPACITY and "MAX_PARTICLES_BONDED" in str(ex.value)
+    e.destroy()
+    # freezing without and with the material passes
+    cold = ms.freezing_pool()
+    plain = _engine(cold, False)
+    with pytest.raises(native.SnError) as ex:   # raised by step() or by the sync after it
+        plain.step(4)
+        plain.sync()
+    assert ex.value.code == native.SN_E_UNSUPPORTED
+    plain.destroy()
+    full = _engine(cold, True)
+    full.step(4)
+    full.sync()
+    full.destroy()
+
