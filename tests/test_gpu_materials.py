@@ -99,6 +99,52 @@ def test_spawn_in_a_material_world():
     assert np.array_equal(e.chain(), o.chain())
 
 
+def test_material_scenes_batched_equal_alone():
+    """Three material scenes in one world (their bond rows and slot maps side
+    by side): each scene's per-substep stream and bond rows equal its own
+    oracle run."""
+    from sunaengine_amd import scenes
+    from sunaengine_amd.engine import Engine
+    ss = [ms.stone_in_water(), ms.rice_on_rubber(), ms.freezing_pool()]
+    ims = [scenes.load_image(s) for s in ss]
+    e = Engine(max_particles=sum(i.n for i in ims), max_cells=sum(i.cellTotal for i in ims), iters=4,
+               max_scenes=len(ss))
+    e.load_scenes(ss)
+    e.step(24)
+    for k, s in enumerate(ss):
+        o = oracle_world(s, 4)
+        ref = o.step(24, stream=True)
+        bad = first_mismatch(e.hash_stream(k), ref)
+        assert bad is None, f"{s.name}: first differing substep {bad}"
+        assert np.array_equal(e.read_bond(k), o.read_bond()), s.name
+    e.destroy()
+
+
+def test_slotted_world_switches_to_material_passes():
+    """A slotted world running a water job takes a bonded job into a free
+    partition: the world switches to the material passes mid-run, and both
+    jobs' streams equal their own oracle runs (the water job's included:
+    the passes are bit-inert for it)."""
+    from sunaengine_amd import scenes
+    from sunaengine_amd.engine import Engine
+    water = scenes.batch_scenes(count=1, seed=93, lo=900, hi=900)[0]
+    stone = ms.stone_in_water()
+    caps_n, caps_c = [2000, 2000], [2048, 2048]
+    e = Engine(max_particles=sum(caps_n), max_cells=sum(caps_c), iters=4, max_scenes=2)
+    e.reserve_slots(caps_n, caps_c)
+    e.load_slot(0, water)
+    e.step(10)
+    e.load_slot(1, stone)
+    e.step(20)
+    for slot, s, steps in ((0, water, 30), (1, stone, 20)):
+        o = oracle_world(s, 4)
+        ref = o.step(steps, stream=True)
+        bad = first_mismatch(e.hash_stream(slot), ref)
+        assert bad is None, f"slot {slot}: first differing substep {bad}"
+    assert np.array_equal(e.read_bond(1), o.read_bond())   # o: the stone job's oracle, 20 substeps
+    e.destroy()
+
+
 def test_stripped_build_on_material_scene():
     """debug:false runs the same material passes: identical stream and bonds."""
     s = ms.stone_in_water()
