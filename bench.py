@@ -137,6 +137,7 @@ def main():
                     help="run on the stripped build (createEngine({debug: false})): same hashes, no debug counters")
     ap.add_argument("--scenes", type=int, default=512, help="batched: scene count")
     ap.add_argument("--solo-sample", type=int, default=16, help="batched: scenes also run one at a time")
+    ap.add_argument("--no-tune", action="store_true", help="batched: keep the default launch shape (no sn_autotune)")
     ap.add_argument("--rates", default="10,50,200,500", help="serve: arrival rates in jobs/s")
     ap.add_argument("--jobs", type=int, default=16, help="serve: jobs per arrival rate")
     ap.add_argument("--max-steps", type=int, default=5000, help="serve: upper bound of a job's substeps")
@@ -175,7 +176,8 @@ def run_batched(args):
     its own stir schedule (Bernoulli 1/50 per substep, radius 8 wu, magnitude
     uniform in [0, 2] wu per substep), advanced together in one world by
     hipGraph-replayed substeps, against a sample of the same scenes run one
-    at a time; every sampled scene's per-substep stream must be equal in both."""
+    at a time; every sampled scene's per-substep stream must be equal in both.
+    Each world's launch shape is autotuned on itself after the warm-up."""
     from sunaengine_amd import scenes, serve
     from sunaengine_amd.engine import Engine
     iters = 4
@@ -198,6 +200,12 @@ def run_batched(args):
     eng.set_events([(k,) + tuple(e) for k, es in enumerate(evs) for e in es])
     eng.step(W)
     eng.sync()
+    # launch shape tuned on the world itself, outside the timed region
+    # (state and streams are left as they were)
+    # trials start half way through the timed window, where the scenes have packed
+    tune = None if args.no_tune else eng.autotune(trial_substeps=16, lookahead=K // 2)
+    if tune:
+        print(f"# autotune: {tune['chosen']}", file=sys.stderr, flush=True)
     eng.reset_stream()
     t1 = time.perf_counter()
     eng.step(K)
@@ -216,6 +224,8 @@ def run_batched(args):
         e1.set_events([(0,) + tuple(e) for e in evs[k]])
         e1.step(W)
         e1.sync()
+        if not args.no_tune:
+            e1.autotune(trial_substeps=16, lookahead=K // 2)
         e1.reset_stream()
         t2 = time.perf_counter()
         e1.step(K)
@@ -239,6 +249,7 @@ def run_batched(args):
         "batched_over_one_at_a_time": round(agg / solo, 3),
         "streams_equal_alone_and_batched": equal,
         "debug_counters": not args.no_debug,
+        "autotune": tune,
     }
     print(json.dumps(_line(agg, "particle-substeps/s", K, W, dt * 1e3 / K, cfg)), flush=True)
     eng.destroy()

@@ -108,6 +108,24 @@ int sn_read_chain(sn_engine* e, uint32_t scene, uint32_t chain_out[8]);
 int sn_read_state(sn_engine* e, uint32_t scene, int32_t* out, size_t words);
 /* readDerived() (src/engine.js:4290): n * 16 words in slot order. */
 int sn_read_derived(sn_engine* e, uint32_t scene, int32_t* out, size_t words);
+/* No reference counterpart (the batched and serving worlds): tune the
+ * world's launch shape on the world itself.  Each neighbour-tile width
+ * (1, 2, 4, 8 cells) x hipGraph length (4, 8, 16 substeps) is timed over
+ * `trial_substeps` (0 = 16) graph-replayed substeps, all starting from the
+ * world advanced `lookahead` substeps (0 = now); the world is restored to
+ * its state before the call afterwards, and the fastest shape is kept.
+ * Results do not depend on the shape: state, streams and counters are as
+ * before the call.  Writes up to `max_trials` trials, their count to
+ * *n_trials and the kept shape to *chosen.  Loading scenes again resets the
+ * tile width to its occupancy heuristic. */
+typedef struct sn_tune_trial {
+  uint32_t tile_cells;      /* neighbour tile width in cells */
+  uint32_t graph_substeps;  /* substeps per captured hipGraph */
+  float ms_per_substep;     /* measured on the world */
+} sn_tune_trial;
+int sn_autotune(sn_engine* e, uint32_t trial_substeps, uint32_t lookahead, sn_tune_trial* trials,
+                uint32_t max_trials, uint32_t* n_trials, sn_tune_trial* chosen);
+
 /* readBond() (src/engine.js:4295): n * 48 words in slot order, each a packed
  * bond {partner slot:17, L0q:13, tombstone:1} (src/sim.wgsl:1590-1625);
  * zero rows in a world that does not run the material passes. */

@@ -361,6 +361,31 @@ static napi_value js_query(napi_env env, napi_callback_info info) {
   return r;
 }
 
+/* autotune(h, trialSubsteps, lookahead) -> {chosen: {tileCells, graphSubsteps, msPerSubstep}, trials: [...]} */
+static napi_value tune_obj(napi_env env, const sn_tune_trial* t) {
+  napi_value o;
+  napi_create_object(env, &o);
+  napi_set_named_property(env, o, "tileCells", num(env, t->tile_cells));
+  napi_set_named_property(env, o, "graphSubsteps", num(env, t->graph_substeps));
+  napi_set_named_property(env, o, "msPerSubstep", num(env, t->ms_per_substep));
+  return o;
+}
+static napi_value js_autotune(napi_env env, napi_callback_info info) {
+  napi_value a[3], r, arr;
+  const int argc = get_args(env, info, 3, a);
+  sn_engine* e = handle(env, a[0]);
+  sn_tune_trial trials[16], best;
+  uint32_t n = 0;
+  int rc = sn_autotune(e, u32(env, a[1]), argc > 2 ? u32(env, a[2]) : 0u, trials, 16, &n, &best);
+  if (rc) return throw_sn(env, e, rc);
+  napi_create_object(env, &r);
+  napi_create_array_with_length(env, n < 16 ? n : 16, &arr);
+  for (uint32_t i = 0; i < n && i < 16; ++i) napi_set_element(env, arr, i, tune_obj(env, &trials[i]));
+  napi_set_named_property(env, r, "trials", arr);
+  napi_set_named_property(env, r, "chosen", tune_obj(env, &best));
+  return r;
+}
+
 /* ---- the 3D solver ---- */
 static sn3_engine* handle3(napi_env env, napi_value v) {
   void* p = NULL;
@@ -556,7 +581,7 @@ static napi_value init(napi_env env, napi_value exports) {
       {"step", js_step},              {"sync", js_sync},               {"readHash", js_read_hash},
       {"readChain", js_read_chain},   {"debugCounters", js_debug},     {"substepCount", js_substep_count},
       {"readHashStream", js_read_stream}, {"resetStream", js_reset_stream}, {"readState", js_read_state},
-      {"readDerived", js_read_derived}, {"readBond", js_read_bond}, {"applyImpulse", js_impulse},  {"spawnFluid", js_spawn},
+      {"readDerived", js_read_derived}, {"readBond", js_read_bond}, {"autotune", js_autotune}, {"applyImpulse", js_impulse},  {"spawnFluid", js_spawn},
       {"setEvents", js_set_events},   {"setActive", js_set_active},
       {"reserveSlots", js_reserve_slots}, {"loadSlot", js_load_slot}, {"query", js_query},
       {"create3", js_create3},        {"destroy3", js_destroy3},       {"load3", js_load3},

@@ -276,6 +276,8 @@ class Engine {
   async readState() { return native().readState(this._h, 0, this.n); }
   async readDerived() { return native().readDerived(this._h, 0, this.n); }
   async readBond() { return native().readBond(this._h, 0, this.n); }
+  // Tune the launch shape on this world (results unchanged; see sn_autotune).
+  autotune(trialSubsteps, lookahead) { return native().autotune(this._h, trialSubsteps || 16, lookahead || 0); }
   async debugCounters() { return native().debugCounters(this._h, 0); }
   async checkErrors() { native().sync(this._h); }
   // The exact current state is what applyImpulse selects on, so the mirror

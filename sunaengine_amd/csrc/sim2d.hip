@@ -2591,6 +2591,7 @@ struct sn_engine {
   bool loaded = false;
   bool slotted = false;  // sn_reserve_slots layout: partitions keep their bases
   bool gen = false;      // the material passes run (D.gen); never switched off again
+  uint32_t graphG = 8;   // substeps per captured hipGraph (sn_autotune may change it)
   // profiling: an event pair around every launch, tagged with its class
   std::vector<hipEvent_t> evPool;
   std::vector<int> evClass;
@@ -3300,7 +3301,7 @@ int sn_step(sn_engine* e, uint32_t count) {
   if (count == 0) return SN_OK;
   try {
     HIPCHECK(hipSetDevice((int)e->opts.device));
-    const uint32_t G = 8;
+    const uint32_t G = e->graphG;
     if (!e->graph) {
       hipGraph_t g;
       HIPCHECK(hipStreamBeginCapture(e->stream, hipStreamCaptureModeRelaxed));
@@ -3328,6 +3329,108 @@ int sn_step(sn_engine* e, uint32_t count) {
     return fail(e, SN_E_DEVICE, d.msg);
   }
   return SN_OK;
+}
+
+// The launch shape of a world, tuned on the world itself: every neighbour
+// tile width (1, 2, 4, 8 cells) and hipGraph length (4, 8, 16 substeps) is
+// timed over graph-replayed substeps.  A world's cost changes as its scenes
+// evolve (blocks collapse and pack their cells), so the trials can start
+// `lookahead` substeps ahead: the world is advanced that far once, and every
+// trial starts from that state.  Before and after, the snapshot of what a
+// substep carries to the next (state, stAcc, ids, the per-scene counters
+// and cursors, bond rows) is restored, so tuning leaves no trace in any
+// result; the fastest shape is kept.  Both knobs only regroup independent
+// work: every list stays in canonical order and every sum is order-free.
+int sn_autotune(sn_engine* e, uint32_t trial_substeps, uint32_t lookahead, sn_tune_trial* trials,
+                uint32_t max_trials, uint32_t* n_trials, sn_tune_trial* chosen) {
+  if (!e || !n_trials || !chosen || (max_trials && !trials)) return SN_E_ARG;
+  if (!e->loaded) return fail(e, SN_E_STATE, "[aether] autotune() called before loadScene()");
+  if (trial_substeps == 0) trial_substeps = 16;
+  *n_trials = 0;
+  std::vector<std::pair<void*, size_t>> regions;
+  std::vector<void*> start, ahead;   // the world now, and `lookahead` substeps on
+  hipEvent_t t0 = nullptr, t1 = nullptr;
+  const uint32_t keepTile = e->tileG, keepG = e->graphG;
+  int rc = SN_OK;
+  auto put = [&](const std::vector<void*>& from) {
+    for (size_t i = 0; i < from.size(); ++i)
+      (void)hipMemcpyAsync(regions[i].first, from[i], regions[i].second, hipMemcpyDeviceToDevice, e->stream);
+    (void)hipStreamSynchronize(e->stream);
+  };
+  try {
+    HIPCHECK(hipSetDevice((int)e->opts.device));
+    HIPCHECK(hipStreamSynchronize(e->stream));
+    Dev& D = e->D;
+    const size_t N = std::max(e->N, 1u);
+    regions = {{D.ps, N * sizeof(int4)}, {D.pt, N * sizeof(int4)}, {D.id, N * sizeof(uint32_t)},
+               {D.st, N * sizeof(int2)}, {D.sd, (size_t)e->S * sizeof(SceneD)}};
+    if (e->gen) regions.push_back({D.bond, (size_t)std::max(e->opts.max_particles, 1u) * BOND_CAP * sizeof(uint32_t)});
+    for (auto& r : regions) {
+      void* p = nullptr;
+      HIPCHECK(hipMalloc(&p, r.second));
+      start.push_back(p);
+      HIPCHECK(hipMemcpyAsync(p, r.first, r.second, hipMemcpyDeviceToDevice, e->stream));
+    }
+    if (lookahead) {
+      rc = sn_step(e, lookahead);
+      HIPCHECK(hipStreamSynchronize(e->stream));
+      if (rc != SN_OK) throw DevError(e->err);
+    }
+    for (auto& r : regions) {
+      void* p = nullptr;
+      HIPCHECK(hipMalloc(&p, r.second));
+      ahead.push_back(p);
+      HIPCHECK(hipMemcpyAsync(p, r.first, r.second, hipMemcpyDeviceToDevice, e->stream));
+    }
+    HIPCHECK(hipEventCreate(&t0));
+    HIPCHECK(hipEventCreate(&t1));
+    const uint32_t tileW[4] = {1, 2, 4, 8}, graphLen[3] = {4, 8, 16};
+    sn_tune_trial best{0, 0, 0.0f};
+    for (uint32_t tw : tileW) {
+      for (uint32_t gl : graphLen) {
+        e->tileG = tw;
+        e->graphG = gl;
+        upload_layout(e);
+        drop_graph(e);
+        rc = sn_step(e, gl);   // capture and one warm replay
+        const uint32_t steps = (trial_substeps + gl - 1) / gl * gl;
+        if (rc == SN_OK) {
+          HIPCHECK(hipEventRecord(t0, e->stream));
+          rc = sn_step(e, steps);
+          HIPCHECK(hipEventRecord(t1, e->stream));
+        }
+        HIPCHECK(hipStreamSynchronize(e->stream));
+        put(ahead);
+        if (rc != SN_OK) throw DevError(e->err);
+        float ms = 0.0f;
+        HIPCHECK(hipEventElapsedTime(&ms, t0, t1));
+        const sn_tune_trial tr{tw, gl, ms / (float)steps};
+        if (*n_trials < max_trials) trials[*n_trials] = tr;
+        ++*n_trials;
+        if (best.graph_substeps == 0 || tr.ms_per_substep < best.ms_per_substep) best = tr;
+      }
+    }
+    e->tileG = best.tile_cells;
+    e->graphG = best.graph_substeps;
+    *chosen = best;
+  } catch (const DevError& d) {
+    e->tileG = keepTile;
+    e->graphG = keepG;
+    rc = fail(e, rc != SN_OK ? rc : SN_E_DEVICE, d.msg);
+  }
+  (void)hipStreamSynchronize(e->stream);
+  put(start);   // the world as it was before the call, whatever happened
+  try {
+    upload_layout(e);
+    drop_graph(e);
+  } catch (const DevError& d) {
+    if (rc == SN_OK) rc = fail(e, SN_E_DEVICE, d.msg);
+  }
+  for (void* p : start) hipFree(p);
+  for (void* p : ahead) hipFree(p);
+  if (t0) hipEventDestroy(t0);
+  if (t1) hipEventDestroy(t1);
+  return rc;
 }
 
 int sn_sync(sn_engine* e) {

@@ -202,6 +202,20 @@ class Engine:
         self._check(native.lib().sn_read_derived(self._h, scene, out.ctypes.data, out.size))
         return out[:n]
 
+    def autotune(self, trial_substeps: int = 16, lookahead: int = 0) -> dict:
+        """Tune the world's launch shape (neighbour tile width, hipGraph
+        length) on the world itself, timing every trial from the world
+        advanced `lookahead` substeps; state and streams are left as they
+        were.  Returns {"trials": [...], "chosen": {...}}."""
+        trials = (native.sn_tune_trial * 16)()
+        n = C.c_uint32()
+        best = native.sn_tune_trial()
+        self._check(native.lib().sn_autotune(self._h, int(trial_substeps), int(lookahead), trials, 16,
+                                             C.byref(n), C.byref(best)))
+        as_dict = lambda t: {"tile_cells": int(t.tile_cells), "graph_substeps": int(t.graph_substeps),
+                             "ms_per_substep": round(float(t.ms_per_substep), 4)}
+        return {"trials": [as_dict(trials[i]) for i in range(min(n.value, 16))], "chosen": as_dict(best)}
+
     def read_bond(self, scene: int = 0) -> np.ndarray:
         """readBond (src/engine.js:4295): (n, 48) packed bond words in slot order."""
         n = self.images[scene].n if scene < len(self.images) else 0
