@@ -126,6 +126,12 @@ typedef struct sn_tune_trial {
 int sn_autotune(sn_engine* e, uint32_t trial_substeps, uint32_t lookahead, sn_tune_trial* trials,
                 uint32_t max_trials, uint32_t* n_trials, sn_tune_trial* chosen);
 
+/* The buffer writes of restore(snap) with `derived` (src/demo_boot.js:308-321,
+ * after a loadScene): n * 8 state words and, optionally, n * 16 derived words
+ * in slot order.  Only the derived stAcc words (8..9) carry into the next
+ * substep; the engine-side counters are kept. */
+int sn_write_state_derived(sn_engine* e, uint32_t scene, const int32_t* st8, size_t st_words, const int32_t* dv16,
+                           size_t dv_words);
 /* readBond() (src/engine.js:4295): n * 48 words in slot order, each a packed
  * bond {partner slot:17, L0q:13, tombstone:1} (src/sim.wgsl:1590-1625);
  * zero rows in a world that does not run the material passes. */

@@ -3642,6 +3642,48 @@ int sn_read_derived(sn_engine* e, uint32_t scene, int32_t* out, size_t words) {
   }
 }
 
+// restore(snap) with `derived` (src/demo_boot.js:308-321): after loadScene,
+// the snapshot's state words and derived scratch are written back.  Of the
+// derived words only stAcc (8..9) carries into the next substep (the
+// others are recomputed by every substep), so the state and stAcc are
+// rewritten in slot order, which the next substep sorts; the engine-side
+// counters stay as they are.
+int sn_write_state_derived(sn_engine* e, uint32_t scene, const int32_t* st8, size_t st_words, const int32_t* dv16,
+                           size_t dv_words) {
+  if (!e || !st8) return SN_E_ARG;
+  if (!e->loaded) return fail(e, SN_E_STATE, "[aether] no scene loaded");
+  if (scene >= e->S || !e->active[scene]) return fail(e, SN_E_ARG, "[aether] scene index out of range");
+  const SceneP& P = e->sp[scene];
+  if (st_words < (size_t)P.n * 8) return fail(e, SN_E_ARG, "[aether] state buffer too small");
+  if (dv16 && dv_words < (size_t)P.n * 16) return fail(e, SN_E_ARG, "[aether] derived buffer too small");
+  for (uint32_t i = 0; i < P.n; ++i)
+    if ((uint32_t)st8[(size_t)i * 8 + 4] >= MAT_MAX) return fail(e, SN_E_ARG, "[aether] matId outside the material table");
+  try {
+    HIPCHECK(hipSetDevice((int)e->opts.device));
+    HIPCHECK(hipStreamSynchronize(e->stream));
+    std::vector<int4> ps(P.n), pt(P.n);
+    std::vector<uint32_t> id(P.n);
+    std::vector<int2> st(P.n, make_int2(0, 0));
+    for (uint32_t i = 0; i < P.n; ++i) {
+      const int32_t* w = st8 + (size_t)i * 8;
+      ps[i] = make_int4(w[0], w[1], w[2], w[3]);
+      pt[i] = make_int4(w[4], w[5], w[6], w[7]);
+      id[i] = i;
+      if (dv16) st[i] = make_int2(dv16[(size_t)i * 16 + 8], dv16[(size_t)i * 16 + 9]);
+    }
+    if (P.n) {
+      Dev& D = e->D;
+      HIPCHECK(hipMemcpy(D.ps + P.partBase, ps.data(), sizeof(int4) * P.n, hipMemcpyHostToDevice));
+      HIPCHECK(hipMemcpy(D.pt + P.partBase, pt.data(), sizeof(int4) * P.n, hipMemcpyHostToDevice));
+      HIPCHECK(hipMemcpy(D.id + P.partBase, id.data(), sizeof(uint32_t) * P.n, hipMemcpyHostToDevice));
+      HIPCHECK(hipMemcpy(D.st + P.partBase, st.data(), sizeof(int2) * P.n, hipMemcpyHostToDevice));
+    }
+  } catch (const DevError& d) {
+    return fail(e, SN_E_DEVICE, d.msg);
+  }
+  return SN_OK;
+}
+
 int sn_read_bond(sn_engine* e, uint32_t scene, uint32_t* out, size_t words) {
   if (!e || !out) return SN_E_ARG;
   try {

@@ -216,6 +216,15 @@ class Engine:
                              "ms_per_substep": round(float(t.ms_per_substep), 4)}
         return {"trials": [as_dict(trials[i]) for i in range(min(n.value, 16))], "chosen": as_dict(best)}
 
+    def write_state_derived(self, state, derived=None, scene: int = 0):
+        """The buffer writes of restore(snap) (src/demo_boot.js:308-321): state
+        (n, 8) and optionally derived (n, 16) words, slot order."""
+        st = np.ascontiguousarray(state, np.int32).reshape(-1)
+        dv = None if derived is None else np.ascontiguousarray(derived, np.int32).reshape(-1)
+        self._check(native.lib().sn_write_state_derived(self._h, scene, st.ctypes.data, st.size,
+                                                        None if dv is None else dv.ctypes.data,
+                                                        0 if dv is None else dv.size))
+
     def read_bond(self, scene: int = 0) -> np.ndarray:
         """readBond (src/engine.js:4295): (n, 48) packed bond words in slot order."""
         n = self.images[scene].n if scene < len(self.images) else 0

@@ -19,7 +19,7 @@ SN_OK, SN_E_ARG, SN_E_CAPACITY, SN_E_DEVICE, SN_E_UNSUPPORTED, SN_E_STATE = 0, -
 EXPORTS = [
     "sn_create", "sn_destroy", "sn_last_error", "sn_load_scene", "sn_load_scenes", "sn_step",
     "sn_sync", "sn_read_hash", "sn_read_hash_stream", "sn_reset_stream", "sn_read_chain",
-    "sn_read_state", "sn_read_derived", "sn_read_bond", "sn_autotune", "sn_debug_counters", "sn_substep_count",
+    "sn_read_state", "sn_read_derived", "sn_read_bond", "sn_autotune", "sn_write_state_derived", "sn_debug_counters", "sn_substep_count",
     "sn_apply_impulse", "sn_spawn_fluid", "sn_set_events", "sn_set_active", "sn_profile_step",
     "sn_reserve_slots", "sn_load_slot", "sn_query", "sn3_query",
     "sn3_create", "sn3_destroy", "sn3_last_error", "sn3_load", "sn3_set_inputs", "sn3_step",
@@ -106,6 +106,7 @@ def lib():
         "sn_read_derived": (C.c_int, [P, u32, P, sz]),
         "sn_read_bond": (C.c_int, [P, u32, P, sz]),
         "sn_autotune": (C.c_int, [P, u32, u32, P, u32, P, P]),
+        "sn_write_state_derived": (C.c_int, [P, u32, P, sz, P, sz]),
         "sn_debug_counters": (C.c_int, [P, u32, P]),
         "sn_substep_count": (C.c_int, [P, u32, C.POINTER(u32)]),
         "sn_apply_impulse": (C.c_int, [P, u32, i32, i32, i32, i32, i32, u32, C.POINTER(u32)]),

@@ -196,3 +196,29 @@ def test_full_scale_1m_first_substeps(wide):
     _print(f"{s.name}: n={s.n}, 4 substeps")
     e, o = _stream_parity(s, 4, 4)
     e.destroy()
+
+
+@pytest.mark.parametrize("name", ["pool", "stone_in_water"])
+def test_restore_state_derived_matches_oracle(name):
+    """restore(snap) with derived (src/demo_boot.js:308-321): a mid-run
+    snapshot written into a freshly loaded world continues exactly as the
+    oracle's restored world does; a water scene also rejoins its own
+    timeline (its formation passes are bit-inert)."""
+    from sunaengine_amd import scenes
+    from tests import material_scenes as ms
+    s = scenes.make_scene("pool_v1", w=40, h=12) if name == "pool" else ms.stone_in_water()
+    donor = oracle_world(s, 4)
+    donor.step(9)
+    st, dv = donor.read_state(), donor.read_derived()
+    o = oracle_world(s, 4)
+    o.write_state_derived(st, dv)
+    e = gpu_engine(s, 4)
+    e.write_state_derived(st, dv)
+    ref = o.step(20, stream=True)
+    e.step(20)
+    assert first_mismatch(e.hash_stream(), ref) is None
+    assert np.array_equal(e.read_state(), o.read_state())
+    if name == "pool":
+        assert first_mismatch(ref, donor.step(20, stream=True)) is None
+    e.destroy()
+
