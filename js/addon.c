@@ -241,6 +241,24 @@ static napi_value read_slots(napi_env env, napi_callback_info info, int derived)
   if (rc) return throw_sn(env, e, rc);
   return arr;
 }
+/* writeStateDerived(h, scene, state:Int32Array(n*8), derived:Int32Array(n*16) | null) */
+static napi_value js_write_state_derived(napi_env env, napi_callback_info info) {
+  napi_value a[4];
+  const int argc = get_args(env, info, 4, a);
+  sn_engine* e = handle(env, a[0]);
+  size_t sw = 0, dw = 0;
+  const int32_t* st = (const int32_t*)typed_data(env, a[2], &sw);
+  const int32_t* dv = NULL;
+  if (argc > 3) {
+    napi_valuetype t;
+    napi_typeof(env, a[3], &t);
+    if (t == napi_object) dv = (const int32_t*)typed_data(env, a[3], &dw);
+  }
+  int rc = sn_write_state_derived(e, u32(env, a[1]), st, sw, dv, dw);
+  if (rc) return throw_sn(env, e, rc);
+  return NULL;
+}
+
 /* readBond(h, scene, n) -> Uint32Array(n * 48) */
 static napi_value js_read_bond(napi_env env, napi_callback_info info) {
   napi_value a[3];
@@ -581,7 +599,7 @@ static napi_value init(napi_env env, napi_value exports) {
       {"step", js_step},              {"sync", js_sync},               {"readHash", js_read_hash},
       {"readChain", js_read_chain},   {"debugCounters", js_debug},     {"substepCount", js_substep_count},
       {"readHashStream", js_read_stream}, {"resetStream", js_reset_stream}, {"readState", js_read_state},
-      {"readDerived", js_read_derived}, {"readBond", js_read_bond}, {"autotune", js_autotune}, {"applyImpulse", js_impulse},  {"spawnFluid", js_spawn},
+      {"readDerived", js_read_derived}, {"readBond", js_read_bond}, {"autotune", js_autotune}, {"writeStateDerived", js_write_state_derived}, {"applyImpulse", js_impulse},  {"spawnFluid", js_spawn},
       {"setEvents", js_set_events},   {"setActive", js_set_active},
       {"reserveSlots", js_reserve_slots}, {"loadSlot", js_load_slot}, {"query", js_query},
       {"create3", js_create3},        {"destroy3", js_destroy3},       {"load3", js_load3},

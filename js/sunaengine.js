@@ -276,6 +276,11 @@ class Engine {
   async readState() { return native().readState(this._h, 0, this.n); }
   async readDerived() { return native().readDerived(this._h, 0, this.n); }
   async readBond() { return native().readBond(this._h, 0, this.n); }
+  // The buffer writes of restore(snap) (src/demo_boot.js:308-321): state and
+  // derived words of the loaded scene, slot order.
+  writeStateDerived(state, derived) {
+    native().writeStateDerived(this._h, 0, Int32Array.from(state), derived ? Int32Array.from(derived) : null);
+  }
   // Tune the launch shape on this world (results unchanged; see sn_autotune).
   autotune(trialSubsteps, lookahead) { return native().autotune(this._h, trialSubsteps || 16, lookahead || 0); }
   async debugCounters() { return native().debugCounters(this._h, 0); }

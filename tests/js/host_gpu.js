@@ -50,6 +50,20 @@ async function main() {
   rep.step(30);
   assert.strictEqual(await rep.digest(), a, 'restore(snapshot) continues identically');
   rep.destroy();
+
+  // the engine-level restore writes (writeStateDerived) rejoin a timeline too
+  const e1 = await eng.createEngine({ maxParticles: 8192, maxCells: 65536, iters: 4 });
+  e1.loadScene(eng.makeScene('dambreak_v1'));
+  e1.step(9);
+  const st = await e1.readState(), dv = await e1.readDerived();
+  e1.step(20);
+  const e2 = await eng.createEngine({ maxParticles: 8192, maxCells: 65536, iters: 4 });
+  e2.loadScene(eng.makeScene('dambreak_v1'));
+  e2.writeStateDerived(st, dv);
+  e2.step(20);
+  assert.strictEqual((await e2.digest()).hex, (await e1.digest()).hex, 'writeStateDerived continues identically');
+  e1.destroy();
+  e2.destroy();
   console.log('OK node host');
 }
 main().catch((err) => { console.error(err && err.stack ? err.stack : err); process.exit(1); });
