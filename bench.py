@@ -536,6 +536,10 @@ def run_scene1m(args):
             traffic = int(pm["hbm_bytes"])
             counters = {"valu_instructions_per_launch": pm["valu_insts"], "valu_busy_pct": round(pm["valu_busy_pct"], 1),
                         "valu_lane_util_pct": round(pm["valu_lane_util_pct"], 1),
+                        "valu_int32_ops_per_s": pm.get("valu_int32_ops_per_s"),
+                        "valu_int32_peak_ops_per_s": pm.get("valu_int32_peak_ops_per_s"),
+                        "valu_int32_frac": (round(pm["valu_int32_ops_per_s"] / pm["valu_int32_peak_ops_per_s"], 4)
+                                            if pm.get("valu_int32_peak_ops_per_s") else None),
                         "source": "rocprofv3 --pmc runs of this workload, profiles/substep2d_1m_pmc.md"}
     roofline = {
         "bound": "hbm",
