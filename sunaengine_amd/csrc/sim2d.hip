@@ -514,7 +514,7 @@ __device__ __forceinline__ HeavySpan heavy_item(const Dev& D, uint32_t i) {
 // Ids are distinct, so the position is exact.  Larger cells take the
 // pairwise count below.
 constexpr uint32_t SORT_MAX = 16384;
-constexpr uint32_t SORT_THREADS = 512;
+constexpr uint32_t SORT_THREADS = 1024;
 
 __global__ __launch_bounds__(SORT_THREADS) void k_canon_heavy_sort(Dev D) {
   __shared__ uint32_t key[SORT_MAX];
