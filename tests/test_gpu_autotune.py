@@ -53,3 +53,36 @@ def test_autotune_material_world_keeps_bonds():
     assert first_mismatch(e.hash_stream(), ref) is None
     assert np.array_equal(e.read_bond(), o.read_bond())
     e.destroy()
+
+
+def test_new_entry_points_refuse_bad_calls():
+    """autotune before any load is a state error; readBond with a short
+    buffer, restore into a scene that does not exist or with a matId outside
+    the table are argument errors; none of them changes the world."""
+    import ctypes as C
+    from sunaengine_amd import native, scenes
+    from sunaengine_amd.engine import Engine
+    s = scenes.make_scene("pool_v1", w=20, h=8)
+    im = scenes.load_image(s)
+    e = Engine(max_particles=im.n, max_cells=im.cellTotal, iters=4)
+    with pytest.raises(native.SnError) as ex:
+        e.autotune()
+    assert ex.value.code == native.SN_E_STATE
+    e.load_scene(s)
+    e.step(3)
+    st = e.read_state()
+    short = np.zeros(8, np.uint32)
+    assert native.lib().sn_read_bond(e._h, 0, short.ctypes.data, short.size) == native.SN_E_ARG
+    with pytest.raises(native.SnError) as ex:
+        e.write_state_derived(st, scene=1)
+    assert ex.value.code == native.SN_E_ARG
+    bad = st.copy()
+    bad[0, 4] = 64
+    with pytest.raises(native.SnError) as ex:
+        e.write_state_derived(bad)
+    assert ex.value.code == native.SN_E_ARG
+    assert np.array_equal(e.read_state(), st)
+    e.step(2)
+    ref = oracle_world(s, 4).step(5, stream=True)
+    assert first_mismatch(e.hash_stream(), ref) is None
+    e.destroy()
