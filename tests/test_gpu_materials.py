@@ -68,6 +68,26 @@ def test_material_scene_matches_oracle(name):
         assert o.debug_counters()[6] > 0  # full lists: containOverflow gathers the seals itself
 
 
+@pytest.mark.parametrize("name", ["stone_in_water", "seal_crossing"])
+def test_material_scene_in_an_unbounded_domain(name):
+    """The same scenes in a domain 2^29 ticks wide: friction and the bond
+    gather take the saturating pair forms, and contain runs the reference's
+    full pair loop (its bounded-scene box filter does not apply)."""
+    build, steps, materials = ms.CASES[name]
+    s = build()
+    s.domW = 1 << 29
+    o = oracle_world(s, 4)
+    e = _engine(s, materials)
+    ref = o.step(steps, stream=True)
+    e.step(steps)
+    bad = first_mismatch(e.hash_stream(), ref)
+    assert bad is None, f"{name} (unbounded): first differing substep {bad}"
+    assert np.array_equal(e.read_state(), o.read_state())
+    assert np.array_equal(e.read_bond(), o.read_bond())
+    assert np.array_equal(e.debug_counters(), o.debug_counters())
+    e.destroy()
+
+
 def test_spawn_in_a_material_world():
     """spawnFluid in a bonded world: the existing fluid slots keep their bond
     rows across the reload, the new slots start empty, and formation runs
