@@ -125,6 +125,23 @@ def seal_crossing(seed=0xC505):
     return s
 
 
+def dense_crossing(seed=0xDC05):
+    """Water packed at 0.25 wu (every list full, so containOverflow gathers
+    the seal slots itself) thrown down at 0.6 wu per substep onto a held
+    STONE line wider than the water: no water particle may end a substep
+    below the line."""
+    rnd = scenes.xorshift32(seed)
+    y_line = 24 * ONE
+    stones = [(8 * ONE + k * ONE + (rnd() % 64), y_line, mt.STONE) for k in range(24)]
+    water = _block(rnd, mt.WATER, 40, 8, 12 * ONE, y_line - (ONE // 4), ONE >> 2, jitter=128)
+    s = _scene(stones + water, 40, 40, "dense_crossing",
+               flags=[scenes.FLAG_ANCHOR] * len(stones) + [0] * len(water))
+    for i in range(len(stones), len(stones) + len(water)):
+        s.vel[2 * i + 1] = (ONE * 3) // 5
+    s.meta = dict(n_stones=len(stones), y_line=y_line)
+    return s
+
+
 # name -> (builder, substeps, create the world with materials on)
 CASES = {
     "sand_and_water": (sand_and_water, 48, False),
@@ -134,4 +151,5 @@ CASES = {
     "packed_against_stone": (packed_against_stone, 32, False),
     "hot_metal": (hot_metal, 32, False),
     "seal_crossing": (seal_crossing, 16, False),
+    "dense_crossing": (dense_crossing, 16, False),
 }

@@ -13,7 +13,8 @@
 * The bond fold of the state hash equals the vectorised digestState twin
   (sunaengine_amd.tables.digest_words, after src/hash/statehash.js:80-159).
 * A move across a held seal segment is reverted: the particle ends the
-  substep where it started (src/sim.wgsl:2104-2135).
+  substep where it started (src/sim.wgsl:2104-2135); with every list full
+  (containOverflow) packed water stays above a held line.
 * Friction is antisymmetric: with no boundary neighbour, no truncation and
   no saturating term, the friction terms of a SAND heap cancel pairwise, so
   one application of friction moves the heap's summed dp by zero
@@ -189,6 +190,27 @@ def test_crossing_a_seal_segment_reverts_the_move():
     o2 = oracle_world(s2, 4)
     o2.step(1)
     assert (o2.read_state()[probes, 1] > st0[probes, 1]).all()
+
+
+def test_dense_crossing_is_contained_with_full_lists():
+    """Water packed at 0.25 wu (every list full: containOverflow gathers the
+    seal slots from the cells itself) thrown across a held STONE line: after
+    one substep no water particle is below the line, and without the line
+    they fall through.  (From the next substep on the crushed wall bursts,
+    src/sim.wgsl:2742-2771, and the water breaks through.)"""
+    s = ms.dense_crossing()
+    ns, yl = s.meta["n_stones"], s.meta["y_line"]
+    o = oracle_world(s, 4)
+    o.step(1)
+    st = o.read_state()
+    assert o.debug_counters()[6] > 0
+    assert st[ns:s.nFluid, 1].max() <= yl
+    s2 = ms.dense_crossing()
+    s2.matId[:ns] = mt.WATER
+    s2.flags[:ns] = 0
+    o2 = oracle_world(s2, 4)
+    o2.step(1)
+    assert o2.read_state()[ns:s2.nFluid, 1].max() > yl
 
 
 def test_friction_terms_cancel_pairwise():
