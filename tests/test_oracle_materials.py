@@ -15,6 +15,9 @@
 * A move across a held seal segment is reverted: the particle ends the
   substep where it started (src/sim.wgsl:2104-2135); with every list full
   (containOverflow) packed water stays above a held line.
+* Burning spends fuel at exactly burnRate per substep; ICE melts past its
+  melting point; applyImpulse and spawnFluid's slot layout match their
+  closed forms.
 * Friction is antisymmetric: with no boundary neighbour, no truncation and
   no saturating term, the friction terms of a SAND heap cancel pairwise, so
   one application of friction moves the heap's summed dp by zero
@@ -229,3 +232,79 @@ def test_friction_terms_cancel_pairwise():
     sums = orc.friction_probe(w, s.nFluid)
     assert sums["pairs"] > 500 and sums["boundary"] == 0 and sums["full"] == 0
     assert sums["dp"] == (0, 0)
+
+
+def _pool(mat, deg, w=12, h=6, seed=0x7007):
+    rnd = scenes.xorshift32(seed)
+    floor = (30 - 3) * ONE
+    pool = ms._block(rnd, mat, w, h, 8 * ONE, floor, ONE, jitter=256)
+    return ms._scene(pool, 30, 30, f"pool_{mat}", temps=[mt.DEG(deg)] * len(pool))
+
+
+def test_burning_spends_fuel_at_the_burn_rate():
+    """A row with fuel, held past its ignition point, burns from substep 0:
+    FLAG_BURNING is set and fuel falls by exactly burnRate per substep while
+    it lasts, whatever the temperature does afterwards (burning is sticky,
+    src/sim.wgsl:2612-2707)."""
+    row = 4
+    R = mt.MATS[row]
+    F = mt.F
+    s = _pool(row, 250)
+    o = oracle_world(s, 4)
+    for k in (1, 5):
+        o.step(1 if k == 1 else 4)
+        st = o.read_state()
+        assert (st[: s.nFluid, 4] == row).all()
+        assert ((st[: s.nFluid, 5] & 1) != 0).all()
+        assert (st[: s.nFluid, 7] == R[F["fuel0"]] - k * R[F["burnRate"]]).all(), k
+
+
+def test_ice_melts_past_its_melting_point():
+    """ICE held at 5 C melts to its meltTo row on substep 0; at -5 C it stays."""
+    ice = mt.MATS[mt.WATER][mt.F["freezeTo"]]
+    for deg, want in ((5, mt.MATS[ice][mt.F["meltTo"]]), (-5, ice)):
+        s = _pool(ice, deg)
+        o = oracle_world(s, 4)
+        o.step(1)
+        assert (o.read_state()[: s.nFluid, 4] == want).all(), deg
+
+
+def test_apply_impulse_closed_form():
+    """applyImpulse (src/engine.js:6077-6098) on the current state: fluid
+    slots in ascending order whose centre is within r (the test in IEEE
+    doubles, as JavaScript computes it), at most `limit` of them, get (ix, iy)
+    added to their velocity, clamped to +-VMAX; nothing else changes."""
+    s = scenes.make_scene("pool_v1", w=40, h=12)
+    o = oracle_world(s, 4)
+    o.step(3)
+    st0 = o.read_state().copy()
+    cx, cy, r, ix, iy, limit = 30 * ONE, 60 * ONE, 6 * ONE, 40000, -50000, 25
+    dx = st0[: s.nFluid, 0].astype(np.float64) - float(cx)
+    dy = st0[: s.nFluid, 1].astype(np.float64) - float(cy)
+    inside = np.nonzero(dx * dx + dy * dy <= float(r) * float(r))[0]
+    assert inside.size > limit
+    hit = inside[:limit]
+    want = st0.copy()
+    want[hit, 2] = np.clip(want[hit, 2].astype(np.int64) + ix, -65536, 65536)
+    want[hit, 3] = np.clip(want[hit, 3].astype(np.int64) + iy, -65536, 65536)
+    assert o.apply_impulse(cx, cy, r, ix, iy, limit) == limit
+    assert np.array_equal(o.read_state(), want)
+
+
+def test_spawn_fluid_slot_layout():
+    """spawnFluid (src/engine.js:5901-6020): the new fluid records take slots
+    nFluid..nFluid+k-1, the boundary block moves up by k unchanged, existing
+    fluid slots keep their words, and the new slots' bond rows are empty."""
+    s = ms.stone_in_water()
+    o = oracle_world(s, 4, max_particles=s.n + 16)
+    o.step(5)
+    st0 = o.read_state().copy()
+    nf, nb = s.nFluid, s.n - s.nFluid
+    drop = [dict(x=(14 + dx) * ONE, y=12 * ONE, vx=0, vy=2000, mat=mt.SAND) for dx in range(5)]
+    fl, bd = scenes.spawn_records(s, drop)
+    assert o.spawn(fl, bd) == s.n + 5
+    st = o.read_state()
+    assert np.array_equal(st[:nf], st0[:nf])
+    assert np.array_equal(st[nf:nf + 5], fl)
+    assert np.array_equal(st[nf + 5:nf + 5 + nb], bd)
+    assert not o.read_bond()[nf:nf + 5].any()
