@@ -269,6 +269,19 @@ def test_ice_melts_past_its_melting_point():
         assert (o.read_state()[: s.nFluid, 4] == want).all(), deg
 
 
+def test_steam_condenses_below_its_condensation_point():
+    """The boil row of water held at 30 C (below its condPt) becomes its
+    condTo row on substep 0; at 90 C it stays."""
+    steam = mt.MATS[mt.WATER][mt.F["boilTo"]]
+    R = mt.MATS[steam]
+    assert R[mt.F["condPt"]] > mt.DEG(30) and R[mt.F["condPt"]] < mt.DEG(90)
+    for deg, want in ((30, R[mt.F["condTo"]]), (90, steam)):
+        s = _pool(steam, deg)
+        o = oracle_world(s, 4)
+        o.step(1)
+        assert (o.read_state()[: s.nFluid, 4] == want).all(), deg
+
+
 def test_apply_impulse_closed_form():
     """applyImpulse (src/engine.js:6077-6098) on the current state: fluid
     slots in ascending order whose centre is within r (the test in IEEE
