@@ -165,6 +165,23 @@ def test_slotted_world_switches_to_material_passes():
     e.destroy()
 
 
+def test_materials_mix_at_scale_first_substeps():
+    """The measured mixed-material scene itself (109,928 particles: SAND,
+    METAL, held STONE, RICE over the bond ramp, water) against the oracle
+    for its first 8 substeps: streams, state and bond rows."""
+    from sunaengine_amd import scenes
+    s = scenes.materials_mix()
+    print(f"materials_mix n={s.n} x8", flush=True)
+    o = oracle_world(s, 4)
+    e = _engine(s, False)
+    ref = o.step(8, stream=True)
+    e.step(8)
+    assert first_mismatch(e.hash_stream(), ref) is None
+    assert np.array_equal(e.read_state(), o.read_state())
+    assert np.array_equal(e.read_bond(), o.read_bond())
+    e.destroy()
+
+
 def test_stripped_build_on_material_scene():
     """debug:false runs the same material passes: identical stream and bonds."""
     s = ms.stone_in_water()
