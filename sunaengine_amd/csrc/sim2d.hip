@@ -1299,7 +1299,7 @@ __global__ __launch_bounds__(256) void k_nbr_tile(Dev D) {
 // ================================================================ solveA (k>0)
 
 template <bool BND>
-__device__ __forceinline__ void solveA_at(const Dev& D, uint32_t p, Dbg& g) {
+__device__ __forceinline__ void solveA_at(const Dev& D, uint32_t p, const int2* lutWG, Dbg& g) {
   const int2 pi = D.pred[p];
   const uint32_t cnt = D.nbrN[p] & 0x1fffffffu;
   const uint32_t* nb = D.nbr + p;
@@ -1315,7 +1315,8 @@ __device__ __forceinline__ void solveA_at(const Dev& D, uint32_t p, Dbg& g) {
     for (uint32_t u = 0; u < NBR_UNROLL; ++u) {
       d[u] = pdelta<BND>(pi, D.pred[q[u] & ~NBR_BND], g);
       idx[u] = (k + u < cnt) ? lut_idx<BND>(d[u], g) : -1;
-      L[u] = D.lut4[maxi(idx[u], 0)];
+      const int2 wg = lutWG[maxi(idx[u], 0)];
+      L[u] = make_int4(wg.x, wg.y, 0, 0);   // solveA uses W and G only
     }
 #pragma unroll
     for (uint32_t u = 0; u < NBR_UNROLL; ++u)
@@ -1330,13 +1331,20 @@ __device__ __forceinline__ bool wave_bounded(const Dev& D, uint32_t s) {
 }
 
 __global__ __launch_bounds__(256) void k_solveA(Dev D) {
+  // the (W, G) kernel tables in LDS: every pair term reads one entry
+  __shared__ int2 lutWG[TN];
+  for (uint32_t i = threadIdx.x; i < TN; i += 256u) {
+    const int4 L = D.lut4[i];
+    lutWG[i] = make_int2(L.x, L.y);
+  }
+  __syncthreads();
   const uint32_t p = slot_block(D) * 256u + threadIdx.x;
   const uint32_t s = p < D.N ? D.sidOf[p] : INACTIVE;
   Dbg g;
   const bool bnd = wave_bounded(D, s);
   if (s != INACTIVE && (D.info[p] & INF_FLUID)) {
-    if (bnd) solveA_at<true>(D, p, g);
-    else solveA_at<false>(D, p, g);
+    if (bnd) solveA_at<true>(D, p, lutWG, g);
+    else solveA_at<false>(D, p, lutWG, g);
   }
   flush_dbg(D, s, g, 0);
 }
