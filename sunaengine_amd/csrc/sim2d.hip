@@ -1498,7 +1498,7 @@ __global__ __launch_bounds__(256) void k_apply(Dev D, uint32_t colour, uint32_t 
 // neighbours; each keeps its own accumulators.  Writes the final position and
 // velocity of the substep (boundary slots: position kept, velocity zeroed).
 template <bool BND>
-__device__ __forceinline__ void xnc_fluid(const Dev& D, uint32_t p, Dbg& g) {
+__device__ __forceinline__ void xnc_fluid(const Dev& D, uint32_t p, const int2* lutWG, Dbg& g) {
   int4 b = D.ptY[p];
   uint32_t mi = (uint32_t)b.x;
   const int32_t* Mi = MAT(D, mi);
@@ -1526,7 +1526,8 @@ __device__ __forceinline__ void xnc_fluid(const Dev& D, uint32_t p, Dbg& g) {
       const uint32_t qj = e[u] & ~NBR_BND;
       d[u] = pdelta<BND>(pi, skip ? pi : D.pred[qj], g);
       idx[u] = skip ? -1 : lut_idx<BND>(d[u], g);
-      Ls[u] = D.lut4[maxi(idx[u], 0)];
+      const int2 wg = lutWG[maxi(idx[u], 0)];
+      Ls[u] = make_int4(wg.x, wg.y, 0, 0);   // xsph, normals and conduct use W and G only
       bjs[u] = D.ptY[qj];
       rhoj[u] = D.rho[qj];
       vjs[u] = D.vtmp[qj];
@@ -1572,6 +1573,13 @@ __device__ __forceinline__ void xnc_fluid(const Dev& D, uint32_t p, Dbg& g) {
 }
 
 __global__ __launch_bounds__(256) void k_xnc(Dev D) {
+  // the (W, G) kernel tables in LDS: every pair term reads one entry
+  __shared__ int2 lutWG[TN];
+  for (uint32_t i = threadIdx.x; i < TN; i += 256u) {
+    const int4 L = D.lut4[i];
+    lutWG[i] = make_int2(L.x, L.y);
+  }
+  __syncthreads();
   uint32_t p = slot_block(D) * 256u + threadIdx.x;
   uint32_t s = p < D.N ? D.sidOf[p] : INACTIVE;
   Dbg g;
@@ -1582,9 +1590,9 @@ __global__ __launch_bounds__(256) void k_xnc(Dev D) {
       int4 a = D.psY[p];
       D.ps[p] = make_int4(a.x, a.y, 0, 0);
     } else if (bnd) {
-      xnc_fluid<true>(D, p, g);
+      xnc_fluid<true>(D, p, lutWG, g);
     } else {
-      xnc_fluid<false>(D, p, g);
+      xnc_fluid<false>(D, p, lutWG, g);
     }
   }
   flush_dbg(D, s, g, 0);
@@ -1594,7 +1602,7 @@ __global__ __launch_bounds__(256) void k_xnc(Dev D) {
 // material row M: cohesion and curvature against fluid neighbours, adhesion
 // against boundary ones.
 template <bool BND>
-__device__ __forceinline__ int2 st_sum(const Dev& D, uint32_t p, const int32_t* M, Dbg& g) {
+__device__ __forceinline__ int2 st_sum(const Dev& D, uint32_t p, const int32_t* M, const int2* lutAC, Dbg& g) {
   int2 pi = D.pred[p], ni = D.nrm[p];
   int32_t rhoi = D.rho[p];
   uint32_t cnt = D.nbrN[p] & 0x1fffffffu;
@@ -1612,8 +1620,9 @@ __device__ __forceinline__ int2 st_sum(const Dev& D, uint32_t p, const int32_t* 
       d[u] = pdelta<BND>(pi, D.pred[qj], g);
       idx[u] = (k + u < cnt) ? lut_idx<BND>(d[u], g) : -1;
       uint32_t ii = (uint32_t)maxi(idx[u], 0);
-      ats[u] = D.lutA[ii];
-      cts[u] = D.lut4[ii].z;
+      const int2 ac = lutAC[ii];
+      ats[u] = ac.x;
+      cts[u] = ac.y;
       rhoj[u] = D.rho[qj];
       matj[u] = D.ptY[qj].x;
       njs[u] = D.nrm[qj];
@@ -1713,6 +1722,10 @@ __device__ __forceinline__ void hash_reduce(const Dev& D, uint32_t blk, uint32_t
 // material passes run) the flag clear and the hash are left to
 // k_bond_form2 and k_hash_gen.
 __global__ __launch_bounds__(256) void k_st_thermal_hash(Dev D, uint32_t gen) {
+  // the (adhesion, cohesion) kernel tables in LDS: every pair term reads one entry
+  __shared__ int2 lutAC[TN];
+  for (uint32_t i = threadIdx.x; i < TN; i += 256u) lutAC[i] = make_int2(D.lutA[i], D.lut4[i].z);
+  __syncthreads();
   const uint32_t blk = slot_block(D);
   uint32_t p = blk * 256u + threadIdx.x;
   uint32_t s = p < D.N ? D.sidOf[p] : INACTIVE;
@@ -1733,7 +1746,7 @@ __global__ __launch_bounds__(256) void k_st_thermal_hash(Dev D, uint32_t gen) {
       uint32_t mat0 = (uint32_t)b.x;
       const int32_t* M = MAT(D, mat0);
       // ---- surfaceTension
-      const int2 axy = bnd ? st_sum<true>(D, p, M, g) : st_sum<false>(D, p, M, g);
+      const int2 axy = bnd ? st_sum<true>(D, p, M, lutAC, g) : st_sum<false>(D, p, M, lutAC, g);
       const int32_t ax = axy.x, ay = axy.y;
       stOut = make_int2(fp_shr_rne(ax, ACC), fp_shr_rne(ay, ACC));
       // ---- thermal
