@@ -394,23 +394,40 @@ __global__ __launch_bounds__(256) void k_predict_count(Dev D) {
     D.pred0[p] = pr;
     D.cell0[p] = cell;
   }
-  // Arrival ranks, one atomic per run of equal cells in the wave (the old
-  // order is cell-sorted, so most of a wave shares a few cells): the run's
-  // first lane reserves the run's length, every lane takes base + offset.
-  // Any assignment of distinct ranks within a cell is valid (the canonical
+  // Arrival ranks, one device atomic per distinct cell of the workgroup.
+  // The old order is sorted by last substep's cell, but in a moving fluid
+  // half the particles change cell each substep, so runs of equal new cells
+  // are short (about 38 per wave), while a workgroup's 256 slots land in
+  // about 15 cells.  The workgroup groups its cells in an LDS hash table
+  // (linear probing, 512 entries for at most 256 keys), takes a local rank
+  // per thread from an LDS counter, reserves each cell's total with one
+  // atomic on cellCount, and every thread takes base + local rank.  Any
+  // assignment of distinct ranks within a cell is valid (the canonical
   // gather re-ranks by slot id).
   {
-    const uint32_t lane = threadIdx.x & 63u;
-    const uint32_t prev = (uint32_t)__shfl_up((int)cell, 1);
-    const uint64_t starts = __ballot(lane == 0u || prev != cell);
-    const uint64_t upto = starts & (lane == 63u ? ~0ull : ((2ull << lane) - 1ull));
-    const uint32_t lead = 63u - (uint32_t)__builtin_clzll(upto);
-    const uint64_t after = starts & ~(lane == 63u ? ~0ull : ((2ull << lane) - 1ull));
-    const uint32_t next = after ? (uint32_t)__builtin_ctzll(after) : 64u;
-    uint32_t base = 0;
-    if (lane == lead && cell != INACTIVE) base = atomicAdd(&D.cellCount[cell], next - lead);
-    base = (uint32_t)__shfl((int)base, (int)lead);
-    if (cell != INACTIVE) D.rank0[p] = base + (lane - lead);
+    constexpr uint32_t HT = 512u;
+    __shared__ uint32_t hkey[HT], hcnt[HT], hbase[HT];
+    const uint32_t t = threadIdx.x;
+    for (uint32_t i = t; i < HT; i += 256u) {
+      hkey[i] = INACTIVE;
+      hcnt[i] = 0u;
+    }
+    __syncthreads();
+    uint32_t h = 0, local = 0;
+    if (cell != INACTIVE) {
+      h = (cell * 2654435761u) >> 23;   // 9 bits
+      while (true) {
+        const uint32_t old = atomicCAS(&hkey[h], INACTIVE, cell);
+        if (old == INACTIVE || old == cell) break;
+        h = (h + 1u) & (HT - 1u);
+      }
+      local = atomicAdd(&hcnt[h], 1u);
+    }
+    __syncthreads();
+    for (uint32_t i = t; i < HT; i += 256u)
+      if (hkey[i] != INACTIVE) hbase[i] = atomicAdd(&D.cellCount[hkey[i]], hcnt[i]);
+    __syncthreads();
+    if (cell != INACTIVE) D.rank0[p] = hbase[h] + local;
   }
   flush_dbg(D, s, g, 0);
 }
