@@ -14,6 +14,40 @@ namespace {
 // counts for the next substep.  Integer adds in a fixed tree: exact.
 constexpr uint32_t SCAN_PER_THREAD = 16, SCAN_BLOCK = 256 * SCAN_PER_THREAD;
 
+// Arrival ranks in the cell histogram, one device atomic per distinct cell
+// of a 256-thread workgroup.  A thread whose `cell` is `none` takes no rank
+// (returns 0).  The workgroup groups its cells in an LDS hash table (linear
+// probing, 512 entries for at most 256 keys: a probe always ends), takes a
+// local rank per thread from an LDS counter, reserves each cell's total
+// with one atomic on counts[cell], and every thread gets base + local rank.
+// The ranks within a cell are distinct, in no particular order.  Every
+// thread of the workgroup must call it.
+__device__ __forceinline__ uint32_t wg_cell_rank(uint32_t* counts, uint32_t cell, uint32_t none) {
+  constexpr uint32_t HT = 512u;
+  __shared__ uint32_t hkey[HT], hcnt[HT], hbase[HT];
+  const uint32_t t = threadIdx.x;
+  for (uint32_t i = t; i < HT; i += 256u) {
+    hkey[i] = none;
+    hcnt[i] = 0u;
+  }
+  __syncthreads();
+  uint32_t h = 0, local = 0;
+  if (cell != none) {
+    h = (cell * 2654435761u) >> 23;   // 9 bits
+    while (true) {
+      const uint32_t old = atomicCAS(&hkey[h], none, cell);
+      if (old == none || old == cell) break;
+      h = (h + 1u) & (HT - 1u);
+    }
+    local = atomicAdd(&hcnt[h], 1u);
+  }
+  __syncthreads();
+  for (uint32_t i = t; i < HT; i += 256u)
+    if (hkey[i] != none) hbase[i] = atomicAdd(&counts[hkey[i]], hcnt[i]);
+  __syncthreads();
+  return cell != none ? hbase[h] + local : 0u;
+}
+
 __device__ __forceinline__ uint32_t block_excl_scan(uint32_t v, uint32_t* sh, uint32_t& total) {
   // 256 threads, Hillis-Steele in LDS
   uint32_t t = threadIdx.x;

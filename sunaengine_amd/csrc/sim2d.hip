@@ -394,40 +394,15 @@ __global__ __launch_bounds__(256) void k_predict_count(Dev D) {
     D.pred0[p] = pr;
     D.cell0[p] = cell;
   }
-  // Arrival ranks, one device atomic per distinct cell of the workgroup.
-  // The old order is sorted by last substep's cell, but in a moving fluid
-  // half the particles change cell each substep, so runs of equal new cells
-  // are short (about 38 per wave), while a workgroup's 256 slots land in
-  // about 15 cells.  The workgroup groups its cells in an LDS hash table
-  // (linear probing, 512 entries for at most 256 keys), takes a local rank
-  // per thread from an LDS counter, reserves each cell's total with one
-  // atomic on cellCount, and every thread takes base + local rank.  Any
-  // assignment of distinct ranks within a cell is valid (the canonical
-  // gather re-ranks by slot id).
+  // Arrival ranks (wg_cell_rank): the old order is sorted by last substep's
+  // cell, but in a moving fluid half the particles change cell each
+  // substep, so a wave holds about 38 runs of equal new cells while a
+  // workgroup's 256 slots land in about 15 cells.  Any assignment of
+  // distinct ranks within a cell is valid (the canonical gather re-ranks by
+  // slot id).
   {
-    constexpr uint32_t HT = 512u;
-    __shared__ uint32_t hkey[HT], hcnt[HT], hbase[HT];
-    const uint32_t t = threadIdx.x;
-    for (uint32_t i = t; i < HT; i += 256u) {
-      hkey[i] = INACTIVE;
-      hcnt[i] = 0u;
-    }
-    __syncthreads();
-    uint32_t h = 0, local = 0;
-    if (cell != INACTIVE) {
-      h = (cell * 2654435761u) >> 23;   // 9 bits
-      while (true) {
-        const uint32_t old = atomicCAS(&hkey[h], INACTIVE, cell);
-        if (old == INACTIVE || old == cell) break;
-        h = (h + 1u) & (HT - 1u);
-      }
-      local = atomicAdd(&hcnt[h], 1u);
-    }
-    __syncthreads();
-    for (uint32_t i = t; i < HT; i += 256u)
-      if (hkey[i] != INACTIVE) hbase[i] = atomicAdd(&D.cellCount[hkey[i]], hcnt[i]);
-    __syncthreads();
-    if (cell != INACTIVE) D.rank0[p] = hbase[h] + local;
+    const uint32_t r = wg_cell_rank(D.cellCount, cell, INACTIVE);
+    if (cell != INACTIVE) D.rank0[p] = r;
   }
   flush_dbg(D, s, g, 0);
 }

@@ -207,7 +207,7 @@ __device__ __forceinline__ void flush_dbg3(const Dev3& D, uint32_t s, const Dbg&
 }
 
 // ------------------------------------------------------------------ predict
-__device__ __forceinline__ void predict3(const Dev3& D, uint32_t p, uint32_t s, Dbg& g) {
+__device__ __forceinline__ uint32_t predict3(const Dev3& D, uint32_t p, uint32_t s, Dbg& g) {
   const Scene3P& P = D.sp[s];
   int4 a = D.ps[p];
   int4 v4 = D.vel[p];
@@ -225,14 +225,19 @@ __device__ __forceinline__ void predict3(const Dev3& D, uint32_t p, uint32_t s, 
   uint32_t c = cell_of(P, pr);
   D.pr0[p] = make_int4(pr.x, pr.y, pr.z, 0);
   D.cell0[p] = c;
-  D.rank0[p] = atomicAdd(&D.cellCount[c], 1u);
+  return c;
 }
 __global__ __launch_bounds__(256) void k3_predict_count(Dev3 D) {
   uint32_t p = blockIdx.x * 256u + threadIdx.x;
   if (blockIdx.x == 0 && threadIdx.x == 0) D.listCount[0] = 0u;  // rebuilt after the scan
   uint32_t s = p < D.N ? D.sidOf[p] : INACTIVE;
   Dbg g;
-  if (s != INACTIVE) predict3(D, p, s, g);
+  const uint32_t c = s != INACTIVE ? predict3(D, p, s, g) : INACTIVE;
+  // arrival rank in the cell: one atomic per distinct cell of the workgroup
+  // instead of one per particle (any distinct ranks are valid: canonicalize
+  // re-ranks by slot id)
+  const uint32_t r = wg_cell_rank(D.cellCount, c, INACTIVE);
+  if (c != INACTIVE) D.rank0[p] = r;
   flush_dbg3(D, s, g, 0);
 }
 
