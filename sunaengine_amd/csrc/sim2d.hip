@@ -633,8 +633,8 @@ __device__ __forceinline__ uint32_t nbr_at(const Dev& D, const uint32_t* nb, uin
 // every occupied neighbour tile, one item {tile, chunk} per TILE_CHUNK of its
 // particles.  Chunking bounds every item's work, so a cell that walls or
 // collisions have packed with hundreds of particles is spread over many
-// waves instead of stalling one.  Items are appended with wave-aggregated
-// atomics; every item is processed independently of the others, so list
+// waves instead of stalling one.  Items are appended with one atomic per
+// workgroup and list; every item is processed independently of the others, so list
 // order (which the atomics leave open) affects no result.
 constexpr int NBR_PPL = 1;             // particles per lane in the neighbour sweep
 constexpr uint32_t CELL_CHUNK = 32;
@@ -653,12 +653,24 @@ __device__ __forceinline__ void append_chunks(uint32_t n, uint32_t* counter, uin
     const uint32_t v = (uint32_t)__shfl_up((int)incl, off);
     if (lane >= off) incl += v;
   }
-  const uint32_t total = (uint32_t)__shfl((int)incl, 63);
-  if (total == 0u) return;
-  uint32_t base = 0;
-  if (lane == 0u) base = atomicAdd(counter, total);
-  base = (uint32_t)__shfl((int)base, 0) + incl - n;
+  // one reservation per workgroup: every wave's appends land on the same few
+  // counters, whose atomics are performed one at a time at the memory side
+  __shared__ uint32_t wtot[4], wbase[4];
+  const uint32_t wv = threadIdx.x >> 6;
+  if (lane == 63u) wtot[wv] = incl;
+  __syncthreads();
+  if (threadIdx.x == 0u) {
+    const uint32_t s1 = wtot[0], s2 = s1 + wtot[1], s3 = s2 + wtot[2], tot = s3 + wtot[3];
+    const uint32_t b = tot ? atomicAdd(counter, tot) : 0u;
+    wbase[0] = b;
+    wbase[1] = b + s1;
+    wbase[2] = b + s2;
+    wbase[3] = b + s3;
+  }
+  __syncthreads();
+  const uint32_t base = wbase[wv] + incl - n;
   for (uint32_t k = 0; k < n; ++k) list[base + k] = make_uint2(key, k);
+  __syncthreads();   // wtot / wbase are reused by the next call
 }
 
 // Called with every lane of every wave active.
