@@ -14,6 +14,38 @@ namespace {
 // counts for the next substep.  Integer adds in a fixed tree: exact.
 constexpr uint32_t SCAN_PER_THREAD = 16, SCAN_BLOCK = 256 * SCAN_PER_THREAD;
 
+// Work-list append from a 256-thread workgroup: thread i appends n_i items
+// {key_i, 0..n_i-1} to list, reserving the workgroup's total with one
+// atomic on *counter (every wave's appends land on the same few counters,
+// whose atomics are performed one at a time at the memory side).  The
+// order of items in the list is left open.  Every thread of the workgroup
+// must call it.
+__device__ __forceinline__ void wg_append_chunks(uint32_t n, uint32_t* counter, uint2* list, uint32_t key) {
+  const uint32_t lane = threadIdx.x & 63u;
+  uint32_t incl = n;
+#pragma unroll
+  for (uint32_t off = 1; off < 64u; off <<= 1) {
+    const uint32_t v = (uint32_t)__shfl_up((int)incl, off);
+    if (lane >= off) incl += v;
+  }
+  __shared__ uint32_t wtot[4], wbase[4];
+  const uint32_t wv = threadIdx.x >> 6;
+  if (lane == 63u) wtot[wv] = incl;
+  __syncthreads();
+  if (threadIdx.x == 0u) {
+    const uint32_t s1 = wtot[0], s2 = s1 + wtot[1], s3 = s2 + wtot[2], tot = s3 + wtot[3];
+    const uint32_t b = tot ? atomicAdd(counter, tot) : 0u;
+    wbase[0] = b;
+    wbase[1] = b + s1;
+    wbase[2] = b + s2;
+    wbase[3] = b + s3;
+  }
+  __syncthreads();
+  const uint32_t base = wbase[wv] + incl - n;
+  for (uint32_t k = 0; k < n; ++k) list[base + k] = make_uint2(key, k);
+  __syncthreads();   // wtot / wbase are reused by the next call
+}
+
 // Arrival ranks in the cell histogram, one device atomic per distinct cell
 // of a 256-thread workgroup.  A thread whose `cell` is `none` takes no rank
 // (returns 0).  The workgroup groups its cells in an LDS hash table (linear

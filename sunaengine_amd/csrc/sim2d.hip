@@ -645,34 +645,6 @@ constexpr uint32_t TILE_CHUNK = 64;    // 64 lanes x NBR_PPL
 // such work spreads over the whole chip instead of one XCD's list range.
 constexpr uint32_t NBR_HEAVY_CAND = 4096;
 
-__device__ __forceinline__ void append_chunks(uint32_t n, uint32_t* counter, uint2* list, uint32_t key) {
-  const uint32_t lane = threadIdx.x & 63u;
-  uint32_t incl = n;
-#pragma unroll
-  for (uint32_t off = 1; off < 64u; off <<= 1) {
-    const uint32_t v = (uint32_t)__shfl_up((int)incl, off);
-    if (lane >= off) incl += v;
-  }
-  // one reservation per workgroup: every wave's appends land on the same few
-  // counters, whose atomics are performed one at a time at the memory side
-  __shared__ uint32_t wtot[4], wbase[4];
-  const uint32_t wv = threadIdx.x >> 6;
-  if (lane == 63u) wtot[wv] = incl;
-  __syncthreads();
-  if (threadIdx.x == 0u) {
-    const uint32_t s1 = wtot[0], s2 = s1 + wtot[1], s3 = s2 + wtot[2], tot = s3 + wtot[3];
-    const uint32_t b = tot ? atomicAdd(counter, tot) : 0u;
-    wbase[0] = b;
-    wbase[1] = b + s1;
-    wbase[2] = b + s2;
-    wbase[3] = b + s3;
-  }
-  __syncthreads();
-  const uint32_t base = wbase[wv] + incl - n;
-  for (uint32_t k = 0; k < n; ++k) list[base + k] = make_uint2(key, k);
-  __syncthreads();   // wtot / wbase are reused by the next call
-}
-
 // Called with every lane of every wave active.
 __global__ __launch_bounds__(256) void k_build_lists(Dev D) {
   const uint32_t t = blockIdx.x * 256u + threadIdx.x;
@@ -689,7 +661,7 @@ __global__ __launch_bounds__(256) void k_build_lists(Dev D) {
     }
     const uint32_t nch = (cnt + CELL_CHUNK - 1u) / CELL_CHUNK;
 #pragma unroll
-    for (uint32_t c = 0; c < 4; ++c) append_chunks(col == c ? nch : 0u, &D.listCount[c], D.dlist[c], t);
+    for (uint32_t c = 0; c < 4; ++c) wg_append_chunks(col == c ? nch : 0u, &D.listCount[c], D.dlist[c], t);
   }
   {
     uint32_t nch = 0;
@@ -713,8 +685,8 @@ __global__ __launch_bounds__(256) void k_build_lists(Dev D) {
         heavy = cand > NBR_HEAVY_CAND;
       }
     }
-    append_chunks(heavy ? 0u : nch, &D.listCount[4], D.dlist[4], t);
-    append_chunks(heavy ? nch : 0u, &D.listCount[6], D.dlist[6], t);
+    wg_append_chunks(heavy ? 0u : nch, &D.listCount[4], D.dlist[4], t);
+    wg_append_chunks(heavy ? nch : 0u, &D.listCount[6], D.dlist[6], t);
   }
   {
     uint32_t nch = 0;
@@ -722,7 +694,7 @@ __global__ __launch_bounds__(256) void k_build_lists(Dev D) {
       const uint32_t cnt = D.cellStart[t + 1] - D.cellStart[t];
       if (cnt > CANON_HEAVY) nch = (cnt + CANON_ITEM - 1u) / CANON_ITEM;
     }
-    append_chunks(nch, &D.listCount[5], D.dlist[5], t);
+    wg_append_chunks(nch, &D.listCount[5], D.dlist[5], t);
   }
 }
 

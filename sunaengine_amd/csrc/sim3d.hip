@@ -315,31 +315,15 @@ __device__ __forceinline__ void dens_finish(const Dev3& D, const Scene3P& P, uin
 
 // ============================================================ per-substep work
 // After the scan: one item {cell, chunk} per CHUNK3 particles of every
-// occupied cell of an active scene, appended with wave-aggregated atomics.
+// occupied cell of an active scene, appended with one atomic per workgroup.
 // Items are independent, so list order (left open by the atomics) affects
 // no result; chunking bounds each item's work however densely a cell packs.
-__device__ __forceinline__ void append_chunks3(uint32_t n, uint32_t* counter, uint2* list, uint32_t key) {
-  const uint32_t lane = threadIdx.x & 63u;
-  uint32_t incl = n;
-#pragma unroll
-  for (uint32_t off = 1; off < 64u; off <<= 1) {
-    const uint32_t v = (uint32_t)__shfl_up((int)incl, off);
-    if (lane >= off) incl += v;
-  }
-  const uint32_t total = (uint32_t)__shfl((int)incl, 63);
-  if (total == 0u) return;
-  uint32_t base = 0;
-  if (lane == 0u) base = atomicAdd(counter, total);
-  base = (uint32_t)__shfl((int)base, 0) + incl - n;
-  for (uint32_t k = 0; k < n; ++k) list[base + k] = make_uint2(key, k);
-}
-
 // Called with every lane of every wave active.
 __global__ __launch_bounds__(256) void k3_build_lists(Dev3 D) {
   const uint32_t t = blockIdx.x * 256u + threadIdx.x;
   uint32_t nch = 0;
   if (t < D.C && D.cellScene[t] != INACTIVE) nch = (D.cellStart[t + 1] - D.cellStart[t] + CHUNK3 - 1u) / CHUNK3;
-  append_chunks3(nch, &D.listCount[0], D.dlist, t);
+  wg_append_chunks(nch, &D.listCount[0], D.dlist, t);
 }
 
 // XCD x (= blockIdx.x % 8) walks the x-th eighth of a device list.
