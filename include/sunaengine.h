@@ -198,6 +198,27 @@ int sn_load_slot(sn_engine* e, uint32_t slot, const sn_scene* scene);
 int sn_profile_step(sn_engine* e, uint32_t count, uint32_t class_mask, float out_ms[SN_PROF_SLOTS],
                     uint32_t out_calls[SN_PROF_SLOTS]);
 
+/* ============================================================ state digest
+ * digestState(state, {particleCount, strideWords, hashedWords, nFluid, bond,
+ * bondCap}) (src/hash/statehash.js:80-159) over DEVICE-resident words, with
+ * no engine: `state` (particle_count records of stride_words u32 words, of
+ * which the first hashed_words are hashed) and the optional `bond` rows
+ * (bond_cap words per fluid row; NULL = no bond fold, as an absent p.bond)
+ * are device pointers; state_words is the length of `state`.  acc_out is a
+ * host array: acc[0..3] = H_set, acc[4..7] = H_slot.  The work runs on
+ * `stream` (a hipStream_t; NULL = the null stream) and the call returns when
+ * it is done.  The reference's thrown errors ("hashedWords > strideWords",
+ * "state buffer too small", bond without bondCap) return SN_E_ARG; the
+ * message of the last failed digest call on this thread is
+ * sn_digest_last_error().  The 2D engine's H of a loaded scene is this digest
+ * with stride 8 = hashed 8 over readState's words and, in a world that runs
+ * the material passes, readBond's rows with bond_cap 48; the 3D H_slot is it
+ * with stride 16, hashed 8. */
+const char* sn_digest_last_error(void);
+int sn_digest_state(const uint32_t* state, size_t state_words, uint32_t particle_count, uint32_t stride_words,
+                    uint32_t hashed_words, uint32_t n_fluid, const uint32_t* bond, uint32_t bond_cap,
+                    uint32_t acc_out[8], void* stream);
+
 /* ===================================================================== 3D
  * The 3D solver (3d/sim3d.wgsl, host 3d/app.js:606-743): App.runSubsteps(k)
  * and App.chain().  A 3D record is 16 int32 words {p.xyzw, v.xyzw, pr.xyzw,

@@ -26,6 +26,7 @@ EXPORTS = [
     "sn3_sync", "sn3_read_hash", "sn3_read_stream", "sn3_reset_stream", "sn3_read_chain",
     "sn3_read_state", "sn3_debug_counters", "sn3_fnv_digest", "sn3_set_events", "sn3_set_active",
     "sn3_profile_step", "sn3_reserve_slots", "sn3_load_slot",
+    "sn_digest_last_error", "sn_digest_state",
 ]
 
 
@@ -137,6 +138,8 @@ def lib():
         "sn3_reserve_slots": (C.c_int, [P, u32, P, P]),
         "sn3_load_slot": (C.c_int, [P, u32, P]),
         "sn3_profile_step": (C.c_int, [P, u32, P]),
+        "sn_digest_last_error": (C.c_char_p, []),
+        "sn_digest_state": (C.c_int, [P, sz, u32, u32, u32, u32, P, u32, P, P]),
     }
     for name, (res, args) in sig.items():
         f = getattr(L, name)
@@ -144,6 +147,24 @@ def lib():
         f.argtypes = args
     _LIB = L
     return L
+
+
+def digest_state(state_ptr: int, state_words: int, particle_count: int, stride_words: int,
+                 hashed_words: int, n_fluid=None, bond_ptr=None, bond_cap: int = 0, stream=None,
+                 entry: str = "sn_digest_state"):
+    """digestState (src/hash/statehash.js:80-159) of device-resident words.
+
+    `state_ptr` / `bond_ptr` are device addresses; returns the 8 accumulator
+    words as a numpy uint32 array.  `entry` names the C entry point to call
+    (each takes the same arguments)."""
+    import numpy as np
+    out = np.zeros(8, np.uint32)
+    nf = particle_count if n_fluid is None else n_fluid
+    rc = getattr(lib(), entry)(state_ptr, state_words, particle_count, stride_words, hashed_words, nf,
+                               bond_ptr, bond_cap, out.ctypes.data, stream)
+    if rc != SN_OK:
+        raise SnError(rc, (lib().sn_digest_last_error() or b"").decode() or f"sn error {rc}")
+    return out
 
 
 def check(handle, rc: int, three_d: bool = False) -> None:

@@ -36,3 +36,21 @@ def test_missing_library_fails_loudly(monkeypatch):
         assert "not built" in str(e)
     else:
         raise AssertionError("expected NativeMissing")
+
+
+def test_digest_state_argument_errors():
+    """digestState's thrown errors (src/hash/statehash.js:97-108) come back as
+    SN_E_ARG before any device work, so this runs without a GPU."""
+    import pytest
+    bogus = 0x1000  # never dereferenced: every case fails its argument check first
+    cases = [
+        (dict(state_words=64, particle_count=8, stride_words=8, hashed_words=9), "hashedWords > strideWords"),
+        (dict(state_words=63, particle_count=8, stride_words=8, hashed_words=8), "state buffer too small"),
+        (dict(state_words=64, particle_count=8, stride_words=8, hashed_words=8, bond_ptr=bogus, bond_cap=0),
+         "digestState: p.bond given without p.bondCap"),
+    ]
+    for kw, msg in cases:
+        with pytest.raises(native.SnError) as ei:
+            native.digest_state(bogus, **kw)
+        assert ei.value.code == native.SN_E_ARG
+        assert str(ei.value) == msg
