@@ -1917,6 +1917,7 @@ struct sn_engine {
   uint32_t S = 0, N = 0, C = 0;
   uint32_t maxEventsPerSubstep = 0;
   Event* dEvents = nullptr;
+  Event* dImpulse = nullptr;         // sn_apply_impulse's one-event schedule (allocated with the engine)
   uint32_t eventCap = 0;
   std::vector<void*> allocs;
   hipGraphExec_t graph = nullptr;
@@ -2275,6 +2276,7 @@ int sn_create(const sn_opts* opts, const int32_t* luts, const int32_t* mats, sn_
     D.hpart = dalloc<uint32_t>(e, (size_t)div_up(NC, 256) * HPART_WORDS);
     D.hitFlag = dalloc<uint32_t>(e, NC);
     D.hitScan = dalloc<uint32_t>(e, (size_t)NC + 1);
+    e->dImpulse = dalloc<Event>(e, 1);
     int4* lut4 = dalloc<int4>(e, TN);
     int32_t* lutA = dalloc<int32_t>(e, TN);
     std::vector<int4> h4(TN);
@@ -3041,30 +3043,34 @@ int sn_read_bond(sn_engine* e, uint32_t scene, uint32_t* out, size_t words) {
 int sn_apply_impulse(sn_engine* e, uint32_t scene, int32_t cx, int32_t cy, int32_t r, int32_t ix, int32_t iy,
                      uint32_t limit, uint32_t* hits) {
   if (!e || !hits) return SN_E_ARG;
+  // The event kernels read each scene's cursor into the schedule; for the
+  // duration of this call every scene's cursor is pointed at a one-event
+  // schedule (this scene) or at nothing (the others), and the saved cursors
+  // are put back on every exit path, including a failed device call.
+  std::vector<SceneD> saved;
+  bool swapped = false;
   try {
     SceneD sd;
     int rc = read_sd(e, scene, sd);
     if (rc) return rc;
-    // a one-event schedule at this scene's current substep, applied now
     Event ev{scene, sd.substep, cx, cy, r, ix, iy, limit};
-    Event* dev = nullptr;
-    HIPCHECK(hipMalloc((void**)&dev, sizeof(Event)));
-    HIPCHECK(hipMemcpy(dev, &ev, sizeof(Event), hipMemcpyHostToDevice));
+    HIPCHECK(hipMemcpy(e->dImpulse, &ev, sizeof(Event), hipMemcpyHostToDevice));
     Dev D = e->D;
-    D.events = dev;
-    // temporarily point this scene's cursor at the single event
-    std::vector<SceneD> all(e->S);
-    HIPCHECK(hipMemcpy(all.data(), e->D.sd, sizeof(SceneD) * e->S, hipMemcpyDeviceToHost));
-    std::vector<SceneD> tmp = all;
+    D.events = e->dImpulse;
+    saved.resize(e->S);
+    HIPCHECK(hipMemcpy(saved.data(), e->D.sd, sizeof(SceneD) * e->S, hipMemcpyDeviceToHost));
+    std::vector<SceneD> tmp = saved;
     for (uint32_t s = 0; s < e->S; ++s) {
       tmp[s].evBase = 0;
       tmp[s].evCursor = 0;
       tmp[s].evCount = (s == scene) ? 1u : 0u;
     }
+    swapped = true;
     HIPCHECK(hipMemcpy(e->D.sd, tmp.data(), sizeof(SceneD) * e->S, hipMemcpyHostToDevice));
     hipLaunchKernelGGL(k_impulse_mark, dim3(e->actBlocks), dim3(256), 0, e->stream, D, 0u);
     enqueue_scan(e, D.hitFlag, D.N, D.hitScan, false);
     hipLaunchKernelGGL(k_impulse_apply, dim3(e->actBlocks), dim3(256), 0, e->stream, D, 0u);
+    HIPCHECK(hipGetLastError());
     HIPCHECK(hipStreamSynchronize(e->stream));
     const SceneP& P = e->sp[scene];
     uint32_t total = 0;
@@ -3076,9 +3082,10 @@ int sn_apply_impulse(sn_engine* e, uint32_t scene, int32_t cx, int32_t cy, int32
       total = lastScan + lastFlag - base;
     }
     *hits = std::min(total, limit);
-    HIPCHECK(hipMemcpy(e->D.sd, all.data(), sizeof(SceneD) * e->S, hipMemcpyHostToDevice));
-    hipFree(dev);
+    HIPCHECK(hipMemcpy(e->D.sd, saved.data(), sizeof(SceneD) * e->S, hipMemcpyHostToDevice));
+    swapped = false;
   } catch (const DevError& d) {
+    if (swapped) (void)hipMemcpy(e->D.sd, saved.data(), sizeof(SceneD) * e->S, hipMemcpyHostToDevice);
     return fail(e, SN_E_DEVICE, d.msg);
   }
   return SN_OK;

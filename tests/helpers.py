@@ -37,3 +37,45 @@ def first_mismatch(a, b):
         return f"shape {a.shape} != {b.shape}"
     bad = np.nonzero((a != b).reshape(a.shape[0], -1).any(axis=1))[0]
     return None if bad.size == 0 else int(bad[0])
+
+
+class DeviceWords:
+    """A device copy of a host array, allocated through the same HIP runtime
+    the library links (/opt/rocm's libamdhip64): test plumbing for entry
+    points that take device pointers.  torch is not used for this because it
+    loads a second, bundled HIP runtime into the process."""
+
+    _hip = None
+
+    @classmethod
+    def hip(cls):
+        if cls._hip is None:
+            import ctypes as C
+            from sunaengine_amd import native
+            native.lib()   # the library's runtime is loaded first; the same soname resolves to it
+            h = C.CDLL("libamdhip64.so.7")
+            h.hipMalloc.argtypes = [C.POINTER(C.c_void_p), C.c_size_t]
+            h.hipMemcpy.argtypes = [C.c_void_p, C.c_void_p, C.c_size_t, C.c_int]
+            h.hipFree.argtypes = [C.c_void_p]
+            h.hipDeviceSynchronize.argtypes = []
+            cls._hip = h
+        return cls._hip
+
+    def __init__(self, a):
+        import ctypes as C
+        host = np.ascontiguousarray(a).view(np.uint32).reshape(-1)
+        self.words = host.size
+        p = C.c_void_p()
+        h = self.hip()
+        assert h.hipMalloc(C.byref(p), max(4, host.nbytes)) == 0, "hipMalloc failed"
+        self.ptr = p.value
+        if host.nbytes:
+            assert h.hipMemcpy(self.ptr, host.ctypes.data, host.nbytes, 1) == 0, "hipMemcpy failed"
+        assert h.hipDeviceSynchronize() == 0
+
+    def free(self):
+        if self.ptr:
+            self.hip().hipFree(self.ptr)
+            self.ptr = None
+
+    __del__ = free

@@ -16,18 +16,16 @@ pytestmark = pytest.mark.gpu
 
 
 def _dev(a):
-    import torch
-    t = torch.from_numpy(np.ascontiguousarray(a).view(np.int32).reshape(-1).copy()).to("cuda:0")
-    torch.cuda.synchronize()
-    return t
+    from tests.helpers import DeviceWords
+    return DeviceWords(a)
 
 
 def _digest(words, stride, hashed, bond=None, n_fluid=None, bond_cap=0, entry="sn_digest_state"):
     t = _dev(words)
     b = None if bond is None else _dev(bond)
-    n = t.numel() // stride if stride else 0
-    return native.digest_state(t.data_ptr(), t.numel(), n, stride, hashed, n_fluid,
-                               None if b is None else b.data_ptr(), bond_cap, entry=entry)
+    n = t.words // stride if stride else 0
+    return native.digest_state(t.ptr, t.words, n, stride, hashed, n_fluid,
+                               None if b is None else b.ptr, bond_cap, entry=entry)
 
 
 def test_table_hash_on_device():
@@ -97,8 +95,11 @@ def test_random_words_match_twin(count, stride, hashed):
     rng = np.random.default_rng(count * 31 + stride)
     w = rng.integers(0, 2 ** 32, size=count * stride, dtype=np.uint64).astype(np.uint32)
     print(f"digest: random count={count} stride={stride} hashed={hashed}", flush=True)
-    got = _digest(w, stride, hashed) if count else native.digest_state(
-        _dev(np.zeros(1, np.uint32)).data_ptr(), 1, 0, stride, hashed)
+    if count:
+        got = _digest(w, stride, hashed)
+    else:
+        one = _dev(np.zeros(1, np.uint32))
+        got = native.digest_state(one.ptr, 1, 0, stride, hashed)
     assert got.tolist() == tables.digest_words(w, stride, hashed)
 
 
