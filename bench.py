@@ -53,6 +53,12 @@ def algorithmic_bytes(F: int, N: int, C: int, k: int) -> dict:
 DOMINANT = "nbr_solveA"     # the neighbour sweep: the kernel class timed inside the timed region
 MAT_DOMINANT = "friction"   # materials workload: k_friction + k_apply_fric, timed inside its timed region
 PROFILE_SUBSTEPS = 20       # follow-on substeps profiled with every kernel class timed
+# The full 500-substep run steps in calls of this many substeps with a sync
+# after each (three 8-substep graphs in flight at most): rocprofv3's kernel
+# trace of the bench command ended in a crash inside sn_step when the whole
+# run's ~60 graph launches were queued at once.  A sync costs tens of us
+# against ~110 ms of work per call.
+FULL_CHUNK = 24
 
 # kernel behind each profiled class (the HIP symbol rocprofv3 reports)
 KERNEL_OF = {"predict": "k_predict_count", "scan": "k_scan_reduce+k_scan_top+k_scan_down",
@@ -143,7 +149,15 @@ def main():
     ap.add_argument("--max-steps", type=int, default=5000, help="serve: upper bound of a job's substeps")
     ap.add_argument("--host", default="python", choices=("python", "js"),
                     help="serve: the host that runs the serving loop (js = node js/serve_bench.js over the addon)")
+    ap.add_argument("--full-substeps", type=int, default=500,
+                    help="1M run: substeps of the whole configuration, replayed from load in the same process "
+                         "for config.full_run (0 = skip)")
+    ap.add_argument("--trace-summary", metavar="KERNEL_TRACE_CSV",
+                    help="print per-kernel launch averages of a rocprofv3 kernel trace of a 1M run made with the "
+                         "same --steps/--warmup, over the timed window and the follow-on profile window, and exit")
     args = ap.parse_args()
+    if args.trace_summary:
+        return trace_summary(args.trace_summary, args.warmup, args.steps)
     if args.gpus != 1:
         print(json.dumps({"error": "this build runs on one GPU; --gpus must be 1",
                           "n_gpus": args.gpus}), flush=True)
@@ -157,6 +171,51 @@ def main():
     if args.workload == "materials":
         return run_materials(args)
     return run_scene1m(args)
+
+
+def trace_summary(path: str, warmup: int, steps: int):
+    """Per-kernel launch durations of a rocprofv3 --kernel-trace CSV of the 1M
+    run.  Substeps are delimited by the dispatches of k_predict_count (the
+    first kernel of every substep, one per substep); the bench's timed engine
+    runs first, so its substeps are the first warmup + steps + the follow-on
+    PROFILE_SUBSTEPS of the trace.  Prints a markdown table: all launches,
+    the timed window (substeps warmup+1 .. warmup+steps) and the follow-on
+    window the line's per_kernel figures come from."""
+    import csv
+    import re
+    rows = []
+    with open(path, newline="") as f:
+        for r in csv.DictReader(f):
+            m = re.search(r"(k\w+|__amd\w+)", r["Kernel_Name"])
+            rows.append((int(r["Start_Timestamp"]), int(r["End_Timestamp"]), m.group(1) if m else r["Kernel_Name"]))
+    rows.sort()
+    marks = [t0 for t0, _, k in rows if k == "k_predict_count"]
+    import bisect
+    win_t = (warmup + 1, warmup + steps)
+    win_p = (warmup + steps + 1, warmup + steps + PROFILE_SUBSTEPS)
+    agg = {}
+    for t0, t1, k in rows:
+        g = bisect.bisect_right(marks, t0)          # 1-based substep holding this dispatch (0 = before any)
+        a = agg.setdefault(k, {"n": 0, "us": 0.0, "nt": 0, "ust": 0.0, "np": 0, "usp": 0.0})
+        us = (t1 - t0) / 1e3
+        a["n"] += 1
+        a["us"] += us
+        if win_t[0] <= g <= win_t[1]:
+            a["nt"] += 1
+            a["ust"] += us
+        if win_p[0] <= g <= win_p[1]:
+            a["np"] += 1
+            a["usp"] += us
+    print(f"substeps in the trace: {len(marks)}; timed window = substeps {win_t[0]}..{win_t[1]}, "
+          f"follow-on window = {win_p[0]}..{win_p[1]}")
+    print()
+    print("| kernel | calls (all) | avg us (all) | calls (timed) | avg us (timed) | calls (follow-on) | "
+          "avg us (follow-on) | total ms (all) |")
+    print("|---|---|---|---|---|---|---|---|")
+    avg = lambda u, n: f"{u / n:.1f}" if n else "-"
+    for k, a in sorted(agg.items(), key=lambda kv: -kv[1]["us"]):
+        print(f"| {k} | {a['n']} | {avg(a['us'], a['n'])} | {a['nt']} | {avg(a['ust'], a['nt'])} | {a['np']} | "
+              f"{avg(a['usp'], a['np'])} | {a['us'] / 1e3:.3f} |")
 
 
 def _line(value, unit, steps, warmup, ms_per_step, config, roofline=None, cpu_baseline=None,
@@ -439,6 +498,17 @@ def run_materials(args):
             ref_cpu = o.step(args.cpu_substeps, stream=True)
             cw = time.perf_counter() - tc
             same = bool(np.array_equal(ref_cpu, ref_gpu))
+            o.close()
+            # one thread: the scalar restatement, first substep from load
+            set_threads(1)
+            o1 = OracleWorld(img, tables.load_luts(), materials.mats_image(), iters=iters)
+            t1c = time.perf_counter()
+            r1 = o1.step(1, stream=True)
+            w1 = time.perf_counter() - t1c
+            o1.close()
+            single = {"value": round(F / w1, 1), "cores": 1,
+                      "sample": f"substep 1 of the same scene from load, one thread; H equal to the GPU's: "
+                                f"{bool(np.array_equal(r1[0], ref_gpu[0]))}"}
             cpu_baseline = {"value": round(F * args.cpu_substeps / cw, 1), "unit": "particle-substeps/s",
                             "cores": threads, "kind": "port",
                             "sample": (f"first {args.cpu_substeps} substeps of the same scene from load, OpenMP over "
@@ -561,6 +631,12 @@ def run_scene1m(args):
                               f"{args.warmup + args.steps + PROFILE_SUBSTEPS} (after the timed region)"),
     }
 
+    # Engines made after the timed one are destroyed only after the line is
+    # printed: under rocprofv3's kernel trace, the first step of an engine
+    # created after another engine had been destroyed ended in a crash inside
+    # the profiler.
+    keep = []
+
     # ---- the other build (debug counters on <-> off) over the same substeps,
     # graph-replayed; its per-substep stream must equal the timed one
     other_build = None
@@ -579,9 +655,35 @@ def run_scene1m(args):
                        "ms_per_step": round(w3 * 1e3 / args.steps, 4),
                        "stream_equal": bool(np.array_equal(other.hash_stream(), hs)),
                        "timing": "hipGraph-replayed substeps, no per-launch events"}
-        other.destroy()
+        keep.append(other)
         print(f"# other build (debug={not args.no_debug} -> {args.no_debug}): {other_build['ms_per_step']:.3f} "
               f"ms/substep, stream equal: {other_build['stream_equal']}", file=sys.stderr, flush=True)
+
+    # ---- the whole configuration (substeps 1..full_substeps from load),
+    # graph-replayed on a fresh engine of the same build in this process; its
+    # stream over the timed window must equal the timed engine's
+    full_run = None
+    if args.full_substeps:
+        K = args.full_substeps
+        fe = Engine(max_particles=N, max_cells=C, iters=iters, stream_cap=max(4096, K + 64),
+                    debug=not args.no_debug)
+        fe.load_scene(scene)
+        fe.sync()
+        t5 = time.perf_counter()
+        for k0 in range(0, K, FULL_CHUNK):
+            fe.step(min(FULL_CHUNK, K - k0))
+            fe.sync()
+        w5 = time.perf_counter() - t5
+        fs = fe.hash_stream()
+        lo, hi = args.warmup, args.warmup + args.steps
+        full_run = {"substeps": K, "value": round(F * K / w5, 1), "ms_per_step": round(w5 * 1e3 / K, 4),
+                    "timing": ("substeps 1..%d from load, hipGraph-replayed in calls of %d substeps with a "
+                               "device sync after each, one wall-clock span" % (K, FULL_CHUNK)),
+                    "stream_equal_over_timed_window": (bool(np.array_equal(fs[lo:hi], hs)) if hi <= K else None),
+                    "dbg_truncation_events": int(fe.debug_counters()[6])}
+        keep.append(fe)
+        print(f"# full run, substeps 1..{K}: {full_run['ms_per_step']:.3f} ms/substep, {full_run['value']:.4e} "
+              f"ps/s", file=sys.stderr, flush=True)
 
     # ---- CPU baseline: the C++ restatement on this host's cores, same scene
     cpu_baseline = None
@@ -598,11 +700,22 @@ def run_scene1m(args):
             eng2.load_scene(scene)
             eng2.step(args.cpu_substeps)
             ref_gpu = eng2.hash_stream()
-            eng2.destroy()
+            keep.append(eng2)
             tc = time.perf_counter()
             ref_cpu = o.step(args.cpu_substeps, stream=True)
             cpu_wall = time.perf_counter() - tc
             same = bool(np.array_equal(ref_cpu, ref_gpu))
+            o.close()
+            # one thread: the scalar restatement, first substep from load
+            set_threads(1)
+            o1 = OracleWorld(img, tables.load_luts(), materials.mats_image(), iters=iters)
+            t1c = time.perf_counter()
+            r1 = o1.step(1, stream=True)
+            w1 = time.perf_counter() - t1c
+            o1.close()
+            single = {"value": round(F / w1, 1), "cores": 1,
+                      "sample": f"substep 1 of the same scene from load, one thread; H equal to the GPU's: "
+                                f"{bool(np.array_equal(r1[0], ref_gpu[0]))}"}
             cpu_baseline = {
                 "value": round(F * args.cpu_substeps / cpu_wall, 1),
                 "unit": "particle-substeps/s",
@@ -611,8 +724,8 @@ def run_scene1m(args):
                 "sample": (f"first {args.cpu_substeps} substeps of the same 1M-particle scene from load, "
                            f"OpenMP over particles, {threads} threads on '{model}' "
                            f"({avail} cores available); per-substep H_set/H_slot equal to the GPU's: {same}"),
+                "single_thread": single,
             }
-            o.close()
         except Exception as ex:  # the baseline is reported, never the measured value
             cpu_baseline = {"value": None, "unit": "particle-substeps/s", "cores": 0, "kind": "port",
                             "sample": f"unavailable: {ex}"}
@@ -644,11 +757,14 @@ def run_scene1m(args):
             "dbg_truncation_events": int(dbg[6]),
             "dbg": [int(x) for x in dbg],
             "other_build": other_build,
+            "full_run": full_run,
         },
         "roofline": roofline,
         "cpu_baseline": cpu_baseline,
     }
     print(json.dumps(out), flush=True)
+    for x in keep:
+        x.destroy()
     eng.destroy()
 
 
