@@ -218,6 +218,19 @@ const char* sn_digest_last_error(void);
 int sn_digest_state(const uint32_t* state, size_t state_words, uint32_t particle_count, uint32_t stride_words,
                     uint32_t hashed_words, uint32_t n_fluid, const uint32_t* bond, uint32_t bond_cap,
                     uint32_t acc_out[8], void* stream);
+/* No counterpart in the reference.  sn_digest_state for callers that cannot
+ * allocate device memory (code under HIP graph capture, tight serving loops,
+ * harnesses that forbid hipMalloc): same arguments, same result and errors,
+ * but the partial sums live in a fixed device array of the library (256 KiB
+ * per GPU, held until the process ends) and the call makes no allocation.
+ * particle_count above SN_DIGEST_NOALLOC_MAX_PARTICLES returns
+ * SN_E_CAPACITY.  All calls on one GPU share that array: calls that overlap
+ * on one GPU (other host threads or streams) race and must be serialised by
+ * the caller. */
+#define SN_DIGEST_NOALLOC_MAX_PARTICLES 8388608u
+int sn_digest_state_noalloc(const uint32_t* state, size_t state_words, uint32_t particle_count,
+                            uint32_t stride_words, uint32_t hashed_words, uint32_t n_fluid, const uint32_t* bond,
+                            uint32_t bond_cap, uint32_t acc_out[8], void* stream);
 
 /* ===================================================================== 3D
  * The 3D solver (3d/sim3d.wgsl, host 3d/app.js:606-743): App.runSubsteps(k)

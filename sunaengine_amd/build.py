@@ -24,7 +24,7 @@ FLAGS = [
     "-Xclang", "-target-feature", "-Xclang", "-packed-fp32-ops",
     "-Wall", "-Wno-unused-function", "-Wno-unused-variable",
 ]
-SOURCES = ["sim2d.hip", "sim3d.hip", "digest.hip"]
+SOURCES = ["sim2d.hip", "sim3d.hip", "digest.hip", "digest_noalloc.hip"]
 
 
 def build(verbose: bool = False) -> str:

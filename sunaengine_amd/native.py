@@ -14,6 +14,8 @@ LIB_PATH = os.path.join(HERE, "libsunaengine_amd.so")
 
 # error codes (include/sunaengine.h)
 SN_OK, SN_E_ARG, SN_E_CAPACITY, SN_E_DEVICE, SN_E_UNSUPPORTED, SN_E_STATE = 0, -1, -2, -3, -4, -5
+#: largest particle_count sn_digest_state_noalloc accepts (its fixed device array)
+SN_DIGEST_NOALLOC_MAX_PARTICLES = 8388608
 
 #: every symbol the header declares; tests check the library exports them all
 EXPORTS = [
@@ -26,7 +28,7 @@ EXPORTS = [
     "sn3_sync", "sn3_read_hash", "sn3_read_stream", "sn3_reset_stream", "sn3_read_chain",
     "sn3_read_state", "sn3_debug_counters", "sn3_fnv_digest", "sn3_set_events", "sn3_set_active",
     "sn3_profile_step", "sn3_reserve_slots", "sn3_load_slot",
-    "sn_digest_last_error", "sn_digest_state",
+    "sn_digest_last_error", "sn_digest_state", "sn_digest_state_noalloc",
 ]
 
 
@@ -140,6 +142,7 @@ def lib():
         "sn3_profile_step": (C.c_int, [P, u32, P]),
         "sn_digest_last_error": (C.c_char_p, []),
         "sn_digest_state": (C.c_int, [P, sz, u32, u32, u32, u32, P, u32, P, P]),
+        "sn_digest_state_noalloc": (C.c_int, [P, sz, u32, u32, u32, u32, P, u32, P, P]),
     }
     for name, (res, args) in sig.items():
         f = getattr(L, name)
