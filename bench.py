@@ -53,12 +53,6 @@ def algorithmic_bytes(F: int, N: int, C: int, k: int) -> dict:
 DOMINANT = "nbr_solveA"     # the neighbour sweep: the kernel class timed inside the timed region
 MAT_DOMINANT = "friction"   # materials workload: k_friction + k_apply_fric, timed inside its timed region
 PROFILE_SUBSTEPS = 20       # follow-on substeps profiled with every kernel class timed
-# The full 500-substep run steps in calls of this many substeps with a sync
-# after each (three 8-substep graphs in flight at most): rocprofv3's kernel
-# trace of the bench command ended in a crash inside sn_step when the whole
-# run's ~60 graph launches were queued at once.  A sync costs tens of us
-# against ~110 ms of work per call.
-FULL_CHUNK = 24
 
 # kernel behind each profiled class (the HIP symbol rocprofv3 reports)
 KERNEL_OF = {"predict": "k_predict_count", "scan": "k_scan_reduce+k_scan_top+k_scan_down",
@@ -152,9 +146,10 @@ def main():
     ap.add_argument("--full-substeps", type=int, default=500,
                     help="1M run: substeps of the whole configuration, replayed from load in the same process "
                          "for config.full_run (0 = skip)")
-    ap.add_argument("--trace-summary", metavar="KERNEL_TRACE_CSV",
-                    help="print per-kernel launch averages of a rocprofv3 kernel trace of a 1M run made with the "
-                         "same --steps/--warmup, over the timed window and the follow-on profile window, and exit")
+    ap.add_argument("--trace-summary", metavar="TRACE",
+                    help="print per-kernel launch averages of a rocprofv3 kernel trace (its rocpd .db, or a "
+                         "*_kernel_trace.csv) of a 1M run made with the same --steps/--warmup, over the timed "
+                         "window and the follow-on profile window, and exit")
     args = ap.parse_args()
     if args.trace_summary:
         return trace_summary(args.trace_summary, args.warmup, args.steps)
@@ -183,11 +178,19 @@ def trace_summary(path: str, warmup: int, steps: int):
     window the line's per_kernel figures come from."""
     import csv
     import re
+    raw = []
+    if path.endswith(".db"):           # rocprofv3's rocpd (SQLite) output
+        import sqlite3
+        con = sqlite3.connect(path)
+        raw = [(int(a), int(b), n) for a, b, n in con.execute("select start, end, name from kernels")]
+        con.close()
+    else:                              # --output-format csv: *_kernel_trace.csv
+        with open(path, newline="") as f:
+            raw = [(int(r["Start_Timestamp"]), int(r["End_Timestamp"]), r["Kernel_Name"]) for r in csv.DictReader(f)]
     rows = []
-    with open(path, newline="") as f:
-        for r in csv.DictReader(f):
-            m = re.search(r"(k\w+|__amd\w+)", r["Kernel_Name"])
-            rows.append((int(r["Start_Timestamp"]), int(r["End_Timestamp"]), m.group(1) if m else r["Kernel_Name"]))
+    for t0, t1, name in raw:
+        m = re.search(r"(k\w+|__amd\w+)", name)
+        rows.append((t0, t1, m.group(1) if m else name))
     rows.sort()
     marks = [t0 for t0, _, k in rows if k == "k_predict_count"]
     import bisect
@@ -543,8 +546,8 @@ def run_scene1m(args):
     print(f"# scene built: n={N} nFluid={F} cells={C} ({time.time() - t0:.1f}s)",
           file=sys.stderr, flush=True)
 
-    eng = Engine(max_particles=N, max_cells=C, iters=iters, stream_cap=max(4096, args.steps + args.warmup + 64),
-                 debug=not args.no_debug)
+    eng = Engine(max_particles=N, max_cells=C, iters=iters,
+                 stream_cap=max(4096, args.steps + args.warmup + 64, args.full_substeps + 64), debug=not args.no_debug)
     eng.load_scene(scene)
     eng.step(args.warmup)
     eng.sync()
@@ -631,11 +634,7 @@ def run_scene1m(args):
                               f"{args.warmup + args.steps + PROFILE_SUBSTEPS} (after the timed region)"),
     }
 
-    # Engines made after the timed one are destroyed only after the line is
-    # printed: under rocprofv3's kernel trace, the first step of an engine
-    # created after another engine had been destroyed ended in a crash inside
-    # the profiler.
-    keep = []
+    keep = []   # engines destroyed after the line is printed
 
     # ---- the other build (debug counters on <-> off) over the same substeps,
     # graph-replayed; its per-substep stream must equal the timed one
@@ -658,32 +657,6 @@ def run_scene1m(args):
         keep.append(other)
         print(f"# other build (debug={not args.no_debug} -> {args.no_debug}): {other_build['ms_per_step']:.3f} "
               f"ms/substep, stream equal: {other_build['stream_equal']}", file=sys.stderr, flush=True)
-
-    # ---- the whole configuration (substeps 1..full_substeps from load),
-    # graph-replayed on a fresh engine of the same build in this process; its
-    # stream over the timed window must equal the timed engine's
-    full_run = None
-    if args.full_substeps:
-        K = args.full_substeps
-        fe = Engine(max_particles=N, max_cells=C, iters=iters, stream_cap=max(4096, K + 64),
-                    debug=not args.no_debug)
-        fe.load_scene(scene)
-        fe.sync()
-        t5 = time.perf_counter()
-        for k0 in range(0, K, FULL_CHUNK):
-            fe.step(min(FULL_CHUNK, K - k0))
-            fe.sync()
-        w5 = time.perf_counter() - t5
-        fs = fe.hash_stream()
-        lo, hi = args.warmup, args.warmup + args.steps
-        full_run = {"substeps": K, "value": round(F * K / w5, 1), "ms_per_step": round(w5 * 1e3 / K, 4),
-                    "timing": ("substeps 1..%d from load, hipGraph-replayed in calls of %d substeps with a "
-                               "device sync after each, one wall-clock span" % (K, FULL_CHUNK)),
-                    "stream_equal_over_timed_window": (bool(np.array_equal(fs[lo:hi], hs)) if hi <= K else None),
-                    "dbg_truncation_events": int(fe.debug_counters()[6])}
-        keep.append(fe)
-        print(f"# full run, substeps 1..{K}: {full_run['ms_per_step']:.3f} ms/substep, {full_run['value']:.4e} "
-              f"ps/s", file=sys.stderr, flush=True)
 
     # ---- CPU baseline: the C++ restatement on this host's cores, same scene
     cpu_baseline = None
@@ -729,6 +702,31 @@ def run_scene1m(args):
         except Exception as ex:  # the baseline is reported, never the measured value
             cpu_baseline = {"value": None, "unit": "particle-substeps/s", "cores": 0, "kind": "port",
                             "sample": f"unavailable: {ex}"}
+
+    # ---- last: the whole configuration (substeps 1..full_substeps from load)
+    # on the timed engine, reloaded, in one graph-replayed call; its stream
+    # over the timed window must equal the timed one.  It comes after
+    # everything else so that the run without it (--full-substeps 0) is an
+    # exact prefix of this one: rocprofv3's kernel trace ends in a crash
+    # inside the profiler during this leg, and the committed trace is of that
+    # prefix.
+    full_run = None
+    if args.full_substeps:
+        K = args.full_substeps
+        eng.load_scene(scene)
+        eng.sync()
+        t5 = time.perf_counter()
+        eng.step(K)
+        eng.sync()
+        w5 = time.perf_counter() - t5
+        fs = eng.hash_stream()
+        lo, hi = args.warmup, args.warmup + args.steps
+        full_run = {"substeps": K, "value": round(F * K / w5, 1), "ms_per_step": round(w5 * 1e3 / K, 4),
+                    "timing": "substeps 1..%d from load, hipGraph-replayed, one wall-clock span" % K,
+                    "stream_equal_over_timed_window": bool(np.array_equal(fs[lo:hi], hs)) if hi <= K else None,
+                    "dbg_truncation_events": int(eng.debug_counters()[6])}
+        print(f"# full run, substeps 1..{K}: {full_run['ms_per_step']:.3f} ms/substep, {full_run['value']:.4e} "
+              f"ps/s", file=sys.stderr, flush=True)
 
     out = {
         "metric": METRIC,
