@@ -54,6 +54,7 @@ def _lib():
         L.or_spawn.restype = C.c_uint32
         L.or_spawn.argtypes = [P, C.c_uint32, i32p, i32p]
         L.or_friction_probe.argtypes = [P, np.ctypeslib.ndpointer(np.int64, flags="C")]
+        L.or_nbr_probe.argtypes = [P, i32p, u32p, u32p, C.POINTER(C.c_uint32)]
         _LIB = L
     return _LIB
 
@@ -233,6 +234,20 @@ class OracleWorld3:
         a = np.zeros(8, np.uint32)
         _lib3().or3_fnv_digest(self._h, a)
         return a
+
+
+def nbr_probe(world: "OracleWorld") -> dict:
+    """Run the next substep of `world` through buildNbr only and return its
+    lists: pred [n, 2], nbr [n, 48] (list order), nbrN [n] (count plus the
+    seal / ext-seal / congeal bits) and the pairs dropped by truncation
+    (the world is left mid-substep)."""
+    n = world.n
+    pred = np.zeros(n * 2, np.int32)
+    nbr = np.zeros(n * 48, np.uint32)
+    nbrn = np.zeros(n, np.uint32)
+    dropped = C.c_uint32()
+    _lib().or_nbr_probe(world._h, pred, nbr, nbrn, C.byref(dropped))
+    return {"pred": pred.reshape(-1, 2), "nbr": nbr.reshape(-1, 48), "nbrN": nbrn, "dropped": int(dropped.value)}
 
 
 def friction_probe(world: "OracleWorld", n_fluid: int) -> dict:

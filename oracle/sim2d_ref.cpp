@@ -1200,6 +1200,28 @@ void or_friction_probe(or_world* w, int64_t out[5]) {
   out[0] = sx; out[1] = sy; out[2] = pairs; out[3] = bnd; out[4] = full;
 }
 
+// Test probe: the next substep's predict, grid and buildNbr only, then
+// pred (2 words per slot), the neighbour lists (MAXNBR words per slot, in
+// list order), nbrN (count | seal/ext-seal/congeal bits) and the truncation
+// count buildNbr added to dbg[6].  The world is left mid-substep (only for
+// tests that discard it).
+void or_nbr_probe(or_world* w, int32_t* pred2, uint32_t* nbr48, uint32_t* nbrN, uint32_t* dropped) {
+  World& W = *reinterpret_cast<World*>(w);
+  g_dbg = W.dbg;
+  const Particle* sin = W.state[W.parity].data();
+  const uint32_t before = W.dbg[6].load();
+  predict(W, sin);
+  build_grid(W);
+  build_nbr(W, sin);
+  *dropped = W.dbg[6].load() - before;
+  for (uint32_t i = 0; i < W.n; ++i) {
+    pred2[2 * i] = W.derived[i].pred.x;
+    pred2[2 * i + 1] = W.derived[i].pred.y;
+    nbrN[i] = W.nbrN[i];
+  }
+  std::memcpy(nbr48, W.nbr.data(), (size_t)W.n * MAXNBR * 4);
+}
+
 // applyImpulse on the exact current state (src/engine.js:6077-6098): fluid
 // slots in ascending order, JS-double circle test, up to `limit` hits, velocity
 // VMAX-clamped.  Returns the hit count.
