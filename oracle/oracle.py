@@ -172,6 +172,8 @@ def _lib3():
         L.or3_read.argtypes = [P, i32p]
         L.or3_debug_counters.argtypes = [P, u32p]
         L.or3_fnv_digest.argtypes = [P, u32p]
+        L.or3_grid_probe.argtypes = [P, u32p, u32p, u32p, u32p, C.POINTER(C.c_uint32)]
+        L.or3_dp_probe.argtypes = [P]
         L._or3_ready = True
     return L
 
@@ -186,6 +188,7 @@ class OracleWorld3:
         lut = np.ascontiguousarray(luts3, np.int32)
         self._h = L.or3_create(int(max_particles or self.n), int(max_cells or cells), lut, lut.size,
                                int(iters))
+        self._cells = cells
         rc = L.or3_load(self._h, self.n, np.ascontiguousarray(scene3.parts, np.int32).reshape(-1),
                         scene3.gx, scene3.gy, scene3.gz)
         if rc:
@@ -234,6 +237,26 @@ class OracleWorld3:
         a = np.zeros(8, np.uint32)
         _lib3().or3_fnv_digest(self._h, a)
         return a
+
+    def grid_probe(self):
+        """Run the next substep through buildNbr only (the world is left
+        mid-substep): returns cellStart, sortedIds, nbr [n, 48], nbrN and the
+        pairs truncation dropped; read() then holds the predicted positions."""
+        cells = self._cells
+        cs = np.zeros(cells + 1, np.uint32)
+        ids = np.zeros(self.n, np.uint32)
+        nbr = np.zeros(self.n * 48, np.uint32)
+        nn = np.zeros(self.n, np.uint32)
+        dropped = C.c_uint32()
+        _lib3().or3_grid_probe(self._h, cs, ids, nbr, nn, C.byref(dropped))
+        return {"cellStart": cs, "sortedIds": ids, "nbr": nbr.reshape(-1, 48), "nbrN": nn,
+                "dropped": int(dropped.value)}
+
+    def dp_probe(self):
+        """Run the next substep through iteration 0's solveDp (left
+        mid-substep) and return the records: pr, lambda, rho and dp."""
+        _lib3().or3_dp_probe(self._h)
+        return self.read()
 
 
 def nbr_probe(world: "OracleWorld") -> dict:

@@ -101,7 +101,9 @@ inline bool ball_term(const World3& W, V3 pi, int32_t& w, V3& g) {
 
 inline int32_t s_term(int32_t a) { return clampi(fp_mul_s(a, a, S_ACC_S), 0, S_TERM); }
 
-void substep3(World3& W, uint32_t* acc_out) {
+// The next substep up to and including `stop`: 0 = buildNbr, 1 = iteration
+// 0's solveDp (dp not yet applied), 2 = the whole substep.
+void substep3_until(World3& W, int stop) {
   g_dbg = W.dbg;
   const uint32_t n = W.n;
   // predict (3d/sim3d.wgsl:188-215)
@@ -172,6 +174,7 @@ void substep3(World3& W, uint32_t* acc_out) {
     if (dropped) W.dbg[6].fetch_add(dropped, std::memory_order_relaxed);
     W.nbrN[i] = cnt;
   }
+  if (stop == 0) return;
   for (int it = 0; it < W.iters; ++it) {
     // density (3d/sim3d.wgsl:397-468)
     PFOR3(n) {
@@ -263,6 +266,7 @@ void substep3(World3& W, uint32_t* acc_out) {
       }
       P.dp[0] = acc.x; P.dp[1] = acc.y; P.dp[2] = acc.z;
     }
+    if (stop == 1) return;
     // applyDp (3d/sim3d.wgsl:529-537)
     PFOR3(n) {
       Part& P = W.parts[(size_t)ii];
@@ -282,6 +286,8 @@ void substep3(World3& W, uint32_t* acc_out) {
   }
   W.substep++;
 }
+
+void substep3(World3& W) { substep3_until(W, 2); }
 
 // digestState over 16-word records, hashing words 0..7 (p.xyzw, v.xyzw)
 inline uint32_t rotl32(uint32_t x, uint32_t r) { return (x << (r & 31u)) | (x >> ((32u - r) & 31u)); }
@@ -366,7 +372,7 @@ void or3_set_inputs(or3_world* w, int32_t stirAmp, int32_t stirPhase, uint32_t b
 void or3_step(or3_world* w, uint32_t count, uint32_t* acc_stream) {
   World3& W = *reinterpret_cast<World3*>(w);
   for (uint32_t s = 0; s < count; ++s) {
-    substep3(W, nullptr);
+    substep3(W);
     digest3(W, W.acc);
     uint32_t next[8];
     for (int j = 0; j < 8; ++j) next[j] = mm_step(W.chain[j], W.acc[j] ^ rotl32(W.acc[(j + 1) & 7], 7));
@@ -389,6 +395,27 @@ void or3_debug_counters(or3_world* w, uint32_t out[16]) {
   World3& W = *reinterpret_cast<World3*>(w);
   for (int j = 0; j < 16; ++j) out[j] = W.dbg[j].load();
 }
+
+// Test probe: the next substep through buildNbr only.  Outputs the grid
+// (cellStart, cellTotal + 1 words; sortedIds, n words), the lists (48 words
+// per slot, list order), their counts and the pairs truncation dropped; the
+// predicted positions are in the records (or3_read, words 8..10).  The world
+// is left mid-substep (only for tests that discard it).
+void or3_grid_probe(or3_world* w, uint32_t* cell_start, uint32_t* sorted_ids, uint32_t* nbr48, uint32_t* nbr_n,
+                    uint32_t* dropped) {
+  World3& W = *reinterpret_cast<World3*>(w);
+  const uint32_t before = W.dbg[6].load();
+  substep3_until(W, 0);
+  *dropped = W.dbg[6].load() - before;
+  std::memcpy(cell_start, W.cellStart.data(), ((size_t)W.cellTotal + 1) * 4);
+  std::memcpy(sorted_ids, W.sortedIds.data(), (size_t)W.n * 4);
+  std::memcpy(nbr48, W.nbr.data(), (size_t)W.n * MAXNBR * 4);
+  std::memcpy(nbr_n, W.nbrN.data(), (size_t)W.n * 4);
+}
+
+// Test probe: the next substep through iteration 0's solveDp; the records
+// (or3_read) then hold pr, lambda (pr.w), rho (dp.w) and the unapplied dp.
+void or3_dp_probe(or3_world* w) { substep3_until(*reinterpret_cast<World3*>(w), 1); }
 
 // The reference's own 3D digest: 256 stripes of FNV-1a over p.xyz, v.xyz in
 // slot order (3d/sim3d.wgsl:571-590), folded into 8 lanes on the host
