@@ -174,6 +174,8 @@ def _lib3():
         L.or3_fnv_digest.argtypes = [P, u32p]
         L.or3_grid_probe.argtypes = [P, u32p, u32p, u32p, u32p, C.POINTER(C.c_uint32)]
         L.or3_dp_probe.argtypes = [P]
+        L.or3_apply_impulse.restype = C.c_uint32
+        L.or3_apply_impulse.argtypes = [P] + [C.c_int32] * 7 + [C.c_uint32]
         L._or3_ready = True
     return L
 
@@ -237,6 +239,11 @@ class OracleWorld3:
         a = np.zeros(8, np.uint32)
         _lib3().or3_fnv_digest(self._h, a)
         return a
+
+    def apply_impulse(self, cx, cy, cz, r, ix, iy, iz, limit=3000) -> int:
+        """The 3D stir event on the current state (slots in ascending order
+        within r, at most `limit`); returns the hit count."""
+        return int(_lib3().or3_apply_impulse(self._h, cx, cy, cz, r, ix, iy, iz, limit))
 
     def grid_probe(self):
         """Run the next substep through buildNbr only (the world is left

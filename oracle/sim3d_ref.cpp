@@ -396,6 +396,33 @@ void or3_debug_counters(or3_world* w, uint32_t out[16]) {
   for (int j = 0; j < 16; ++j) out[j] = W.dbg[j].load();
 }
 
+// The 3D stir event (this build's definition; the reference has no 3D
+// impulse), applied between substeps on the current state: slots in
+// ascending order whose p lies within r of (cx, cy, cz) (IEEE double test,
+// as applyImpulse's src/engine.js:6077-6098 in 2D), at most `limit`, get
+// (ix, iy, iz) added to v, each component clamped to +-65536.  Returns the
+// hit count.
+uint32_t or3_apply_impulse(or3_world* w, int32_t cx, int32_t cy, int32_t cz, int32_t r, int32_t ix, int32_t iy,
+                           int32_t iz, uint32_t limit) {
+  World3& W = *reinterpret_cast<World3*>(w);
+  const double r2 = (double)r * (double)r;
+  uint32_t hits = 0;
+  auto cl = [](int64_t x) { return (int32_t)std::max<int64_t>(-65536, std::min<int64_t>(65536, x)); };
+  for (uint32_t i = 0; i < W.n && hits < limit; ++i) {
+    Part& P = W.parts[i];
+    double dx = (double)P.p[0] - (double)cx, dy = (double)P.p[1] - (double)cy, dz = (double)P.p[2] - (double)cz;
+    double q = dx * dx;
+    q = q + dy * dy;
+    q = q + dz * dz;
+    if (q > r2) continue;
+    P.v[0] = cl((int64_t)P.v[0] + ix);
+    P.v[1] = cl((int64_t)P.v[1] + iy);
+    P.v[2] = cl((int64_t)P.v[2] + iz);
+    hits++;
+  }
+  return hits;
+}
+
 // Test probe: the next substep through buildNbr only.  Outputs the grid
 // (cellStart, cellTotal + 1 words; sortedIds, n words), the lists (48 words
 // per slot, list order), their counts and the pairs truncation dropped; the

@@ -79,3 +79,32 @@ class DeviceWords:
             self.ptr = None
 
     __del__ = free
+
+
+def oracle_stream_2d(scene, iters, steps, events):
+    """The oracle's per-substep stream of a 2D scene with stir events
+    (substep, cx, cy, r, ix, iy[, limit]) applied immediately before their
+    substep, in the order given (arming order, src/demo_boot.js:244-272)."""
+    o = oracle_world(scene, iters)
+    rows = []
+    for s in range(steps):
+        for ev in events:
+            if ev[0] == s:
+                o.apply_impulse(*ev[1:6], ev[6] if len(ev) > 6 else 3000)
+        rows.append(o.step(1, stream=True)[0])
+    return o, np.array(rows).reshape(-1, 8)
+
+
+def oracle_stream_3d(scene, iters, steps, events):
+    """As oracle_stream_2d for a 3D scene: events (substep, cx, cy, cz, r,
+    ix, iy, iz[, limit])."""
+    from oracle import OracleWorld3
+    from sunaengine_amd import tables3d
+    o = OracleWorld3(scene, tables3d.luts3d(), iters)
+    rows = []
+    for s in range(steps):
+        for ev in events:
+            if ev[0] == s:
+                o.apply_impulse(*ev[1:8], ev[8] if len(ev) > 8 else 3000)
+        rows.append(o.step(1, stream=True)[0])
+    return o, np.array(rows).reshape(-1, 8)

@@ -132,3 +132,36 @@ def test_apply_impulse_keeps_schedule_cursors():
     e.step(6)
     assert first_mismatch(e.hash_stream(), np.array(rows)) is None
     e.destroy()
+
+
+def test_3d_schedule_matches_oracle():
+    """The 3D stir events (sn3_set_events) against the oracle applying them
+    between substeps: substep 0, three on one substep with clamping overlap,
+    a clipped limit, graph boundaries, stepping in pieces."""
+    from sunaengine_amd import scenes3d
+    from sunaengine_amd.engine3d import Engine3D
+    from tests.helpers import oracle_stream_3d
+    s = scenes3d.column_scene(3000, gx=12, gy=14, gz=12)
+    p = s.parts[:, :3].astype(np.int64)
+    c = [int(x) for x in p.mean(axis=0)]
+    evs = [(0, c[0], c[1], c[2], 4 * ONE, 40000, 0, -30000, 3000),
+           (3, c[0], c[1], c[2], 3 * ONE, 60000, 20000, 0, 3000),
+           (3, c[0] + ONE, c[1], c[2], 3 * ONE, -65536, 0, 65536, 3000),
+           (3, c[0], c[1] - ONE, c[2] + ONE, 2 * ONE, 30000, -65536, 30000, 3000),
+           (8, c[0], c[1], c[2], 20 * ONE, 1234, -4321, 999, 7),
+           (15, c[0], c[1], c[2], 5 * ONE, -20000, 50000, 0, 3000),
+           (16, c[0], c[1], c[2], 5 * ONE, 20000, -50000, 0, 0)]
+    steps = 26
+    o, ref = oracle_stream_3d(s, 3, steps, evs)
+    e = Engine3D(max_particles=s.n, max_cells=s.cell_total, iters=3)
+    e.load([s])
+    e.set_events([(0,) + ev for ev in evs])
+    for piece in (2, 7, 17):
+        e.step(piece)
+    print(f"events: 3d column n={s.n}", flush=True)
+    bad = first_mismatch(e.hash_stream(0, max_substeps=steps), ref)
+    assert bad is None, f"first differing substep {bad}"
+    assert np.array_equal(e.read_state()[:, :8], o.read()[:, :8])
+    e.destroy()
+    o2, plain = oracle_stream_3d(s, 3, steps, [])
+    assert not np.array_equal(plain, ref)

@@ -1,10 +1,12 @@
 """Serving loop on the GPU: every job's per-substep stream is the same when it
 runs in the continuous-batching mix (sharing worlds, admitted and retired
-between rounds, partitions reused) as when it runs alone."""
+between rounds, partitions reused) as when it runs alone, and both equal the
+oracle's stream of the job (its scene with its stir events applied between
+substeps in arming order)."""
 import numpy as np
 import pytest
 
-from tests.helpers import first_mismatch
+from tests.helpers import first_mismatch, oracle_stream_2d, oracle_stream_3d
 
 pytestmark = pytest.mark.gpu
 
@@ -52,3 +54,8 @@ def test_serving_mix_equals_alone():
         assert j.stream is not None and j.stream.shape == (j.substeps, 8)
         ref = _solo(j)
         assert first_mismatch(j.stream, ref) is None, f"job {j.jid}"
+        if j.kind == "2d":
+            _, orc = oracle_stream_2d(j.scene, 4, j.substeps, j.events)
+        else:
+            _, orc = oracle_stream_3d(j.scene, 3, j.substeps, j.events)
+        assert first_mismatch(j.stream, orc) is None, f"job {j.jid}: differs from the oracle"
