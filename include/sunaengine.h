@@ -148,12 +148,22 @@ int sn_substep_count(sn_engine* e, uint32_t scene, uint32_t* out);
 int sn_apply_impulse(sn_engine* e, uint32_t scene, int32_t cx, int32_t cy, int32_t r, int32_t ix,
                      int32_t iy, uint32_t limit, uint32_t* hits);
 
-/* spawnFluid(list, {noWeld}) (src/engine.js:5901-6020) on a one-scene world:
- * k new fluid records (8 words each) are appended after the fluid block and
- * the boundary block (given again as nb records) moves up by k slots.
- * *n_out receives the new n, unchanged when capacity would be exceeded. */
-int sn_spawn_fluid(sn_engine* e, const int32_t* fluid8, uint32_t k, const int32_t* boundary8,
-                   uint32_t nb, uint32_t* n_out);
+/* spawnFluid(list, {noWeld, ownGroup}) (src/engine.js:5901-6020) on a
+ * one-scene world: k new fluid records (8 words each, flags already holding
+ * the caller's FLAG_BURNING/FLAG_ANCHOR and the material-derived FLAG_DEAD /
+ * FLAG_SEAL, as the host builds them) are appended after the fluid block,
+ * and the world's own boundary block moves up by k slots, rewritten as the
+ * reference rewrites it (position, material, temperature and fuel kept;
+ * velocity and flags zero).  The engine owns the weld-group sequence: each
+ * spawn takes the next 12-bit group id (loadScene restarts it), written into
+ * the flags of every record whose row welds on spawn (bondGroup >= 0,
+ * bondReform == 0) unless SN_SPAWN_NO_WELD is set.  SN_SPAWN_OWN_GROUP is
+ * accepted and changes nothing: this build opens no weld gesture, so every
+ * spawn already owns its group.  *n_out receives the new n, unchanged when
+ * capacity would be exceeded (no error, as the reference). */
+#define SN_SPAWN_NO_WELD 1u   /* {noWeld: true} */
+#define SN_SPAWN_OWN_GROUP 2u /* {ownGroup: true} */
+int sn_spawn_fluid(sn_engine* e, const int32_t* fluid8, uint32_t k, uint32_t flags, uint32_t* n_out);
 
 /* Device-resident input schedule for batched / served runs: `count` stir
  * events {scene, substep, cx, cy, r, ix, iy, limit}, sorted by (scene,

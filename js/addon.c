@@ -288,16 +288,18 @@ static napi_value js_impulse(napi_env env, napi_callback_info info) {
   return num(env, hits);
 }
 
-/* spawnFluid(h, fluid8:Int32Array, boundary8:Int32Array) -> new n */
+/* spawnFluid(h, fluid8:Int32Array, flags:number) -> new n
+ * (flags: 1 = {noWeld: true}, 2 = {ownGroup: true}) */
 static napi_value js_spawn(napi_env env, napi_callback_info info) {
   napi_value a[3];
   get_args(env, info, 3, a);
   sn_engine* e = handle(env, a[0]);
-  size_t nf = 0, nb = 0;
+  size_t nf = 0;
   const int32_t* f = (const int32_t*)typed_data(env, a[1], &nf);
-  const int32_t* b = (const int32_t*)typed_data(env, a[2], &nb);
+  uint32_t flags = 0;
+  napi_get_value_uint32(env, a[2], &flags);
   uint32_t n = 0;
-  int rc = sn_spawn_fluid(e, f, (uint32_t)(nf / 8), nb ? b : NULL, (uint32_t)(nb / 8), &n);
+  int rc = sn_spawn_fluid(e, f, (uint32_t)(nf / 8), flags, &n);
   if (rc) return throw_sn(env, e, rc);
   return num(env, n);
 }

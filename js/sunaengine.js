@@ -308,15 +308,9 @@ class Engine {
       fl[o + 6] = Math.max(TEMP_MIN, Math.min(TEMP_MAX, p.temp !== undefined ? p.temp : row[F('spawnTemp')])) | 0;
       fl[o + 7] = Math.max(0, p.fuel !== undefined ? p.fuel : row[F('fuel0')]) | 0;
     });
-    const sc = this._scene, nB = this.n - this.nFluid;
-    const bd = new Int32Array(nB * 8);
-    for (let b = 0; b < nB; b++) {
-      const src = sc.nFluid + b, bm = sc.matId[src] >>> 0, o = b * 8;
-      bd[o] = sc.pos[2 * src]; bd[o + 1] = sc.pos[2 * src + 1]; bd[o + 4] = bm;
-      bd[o + 6] = sc.temp ? sc.temp[src] : AMBIENT;
-      bd[o + 7] = sc.fuel ? sc.fuel[src] : rows[bm][F('fuel0')];
-    }
-    const n = native().spawnFluid(this._h, fl, bd);
+    // the engine shifts its own boundary block and assigns the spawn's weld
+    // group (src/engine.js:5933-5971)
+    const n = native().spawnFluid(this._h, fl, (opts.noWeld ? 1 : 0) | (opts.ownGroup === true ? 2 : 0));
     if (n !== this.n) { this.nFluid += k; this.n = n; }
     return this.n;
   }

@@ -84,6 +84,7 @@ constexpr int32_t CORR_NEVER = 2147483647;
 constexpr uint32_t FLAG_BURNING = 1, FLAG_DEAD = 2, FLAG_ANCHOR = 4, FLAG_RIGID = 16,
                    FLAG_CONGEAL = 32, FLAG_BURST = 64, FLAG_SEAL = 128;
 constexpr uint32_t WELD_TAG_MASK = 0x000fff00u;
+constexpr uint32_t WELD_SEQ_MAX = 4095;   // 12-bit spawn-event ids, 0 = no tag (src/engine.js:268-275)
 constexpr uint32_t NBR_SEAL_BIT = 0x80000000u, NBR_EXT_SEAL_BIT = 0x40000000u,
                    NBR_CONGEAL_BIT = 0x20000000u;
 constexpr uint32_t NBR_BND = 0x80000000u;  // neighbour entry: j is a boundary slot
@@ -1924,6 +1925,7 @@ struct sn_engine {
   uint32_t graphSubsteps = 0;
   bool loaded = false;
   bool slotted = false;  // sn_reserve_slots layout: partitions keep their bases
+  uint32_t weldSeq = 1;  // the 12-bit spawn-event id of the last weld group (`_weldSeq`, src/engine.js:3801)
   bool gen = false;      // the material passes run (D.gen); never switched off again
   uint32_t graphG = 8;   // substeps per captured hipGraph (sn_autotune may change it)
   // profiling: an event pair around every launch, tagged with its class
@@ -2487,6 +2489,7 @@ int sn_load_scenes(sn_engine* e, const sn_scene* scenes, uint32_t count) {
     }
     upload_layout(e);
     e->loaded = true;
+    e->weldSeq = 1;   // one loadScene is one bonding group; the sequence restarts (src/engine.js:3795-3801)
   } catch (const DevError& d) {
     return fail(e, SN_E_DEVICE, d.msg);
   }
@@ -3163,15 +3166,15 @@ int sn_set_active(sn_engine* e, const uint8_t* active, uint32_t count) {
   return SN_OK;
 }
 
-int sn_spawn_fluid(sn_engine* e, const int32_t* fluid8, uint32_t k, const int32_t* boundary8, uint32_t nb,
-                   uint32_t* n_out) {
-  if (!e || !n_out || (k && !fluid8) || (nb && !boundary8)) return SN_E_ARG;
+int sn_spawn_fluid(sn_engine* e, const int32_t* fluid8, uint32_t k, uint32_t flags, uint32_t* n_out) {
+  if (!e || !n_out || (k && !fluid8)) return SN_E_ARG;
   if (!e->loaded) return fail(e, SN_E_STATE, "[aether] no scene loaded");
   if (e->S != 1) return fail(e, SN_E_ARG, "[aether] spawnFluid needs a one-scene world");
+  if (flags & ~(SN_SPAWN_NO_WELD | SN_SPAWN_OWN_GROUP)) return fail(e, SN_E_ARG, "[aether] spawnFluid: unknown option bits");
   const SceneP P = e->sp[0];
   *n_out = P.n;
   if (k == 0) return SN_OK;
-  if (nb != P.n - P.nFluid) return fail(e, SN_E_ARG, "[aether] spawnFluid: boundary block size mismatch");
+  const uint32_t nb = P.n - P.nFluid;
   uint32_t newN = P.n + k;
   if (newN > e->opts.max_particles) return SN_OK;  // refused deterministically, n unchanged
   if (e->gen && newN > MAX_PARTICLES_BONDED) return SN_OK;
@@ -3182,10 +3185,33 @@ int sn_spawn_fluid(sn_engine* e, const int32_t* fluid8, uint32_t k, const int32_
     std::vector<int32_t> st((size_t)P.n * 8), dv((size_t)P.n * 16);
     rc = read_slots(e, 0, st.data(), dv.data());
     if (rc) return rc;
+    std::vector<int32_t> mats(MAT_MAX * MAT_WORDS);
+    HIPCHECK(hipMemcpy(mats.data(), e->D.mats, sizeof(int32_t) * mats.size(), hipMemcpyDeviceToHost));
+    // One spawn event is one weld group: no gesture can be open in this
+    // build, so every spawn takes the next id, with or without ownGroup
+    // (src/engine.js:4570-4580,5933-5935).
+    const uint32_t seq = (e->weldSeq % WELD_SEQ_MAX) + 1u;
+    const uint32_t tag = (seq << 8) & WELD_TAG_MASK;
     std::vector<int32_t> nst((size_t)newN * 8), acc((size_t)newN * 2, 0);
     std::memcpy(nst.data(), st.data(), (size_t)P.nFluid * 32);
-    std::memcpy(nst.data() + (size_t)P.nFluid * 8, fluid8, (size_t)k * 32);
-    if (nb) std::memcpy(nst.data() + (size_t)(P.nFluid + k) * 8, boundary8, (size_t)nb * 32);
+    for (uint32_t i = 0; i < k; ++i) {
+      int32_t* o = nst.data() + (size_t)(P.nFluid + i) * 8;
+      std::memcpy(o, fluid8 + (size_t)i * 8, 32);
+      if ((uint32_t)o[4] >= MAT_MAX) return fail(e, SN_E_ARG, "[aether] spawnFluid: matId outside the material table");
+      const int32_t* M = &mats[(size_t)(uint32_t)o[4] * MAT_WORDS];
+      if (!(flags & SN_SPAWN_NO_WELD) && M[M_BONDGROUP] >= 0 && M[M_BONDREFORM] == 0)
+        o[5] = (int32_t)((uint32_t)o[5] | tag);
+    }
+    // the boundary block moves up by k slots, rewritten as the reference
+    // rewrites it from the loaded scene: position, material, temperature and
+    // fuel kept (boundary slots never move and the thermal passes skip
+    // them), velocity and flags zero (src/engine.js:5964-5971)
+    for (uint32_t b = 0; b < nb; ++b) {
+      const int32_t* src = st.data() + (size_t)(P.nFluid + b) * 8;
+      int32_t* o = nst.data() + (size_t)(P.nFluid + k + b) * 8;
+      o[0] = src[0]; o[1] = src[1]; o[2] = 0; o[3] = 0;
+      o[4] = src[4]; o[5] = 0; o[6] = src[6]; o[7] = src[7];
+    }
     for (uint32_t i = 0; i < P.nFluid; ++i) {
       acc[2 * i] = dv[(size_t)i * 16 + 8];
       acc[2 * i + 1] = dv[(size_t)i * 16 + 9];
@@ -3201,6 +3227,7 @@ int sn_spawn_fluid(sn_engine* e, const int32_t* fluid8, uint32_t k, const int32_
     sn_scene sc{newN, P.nFluid + k, nst.data(), P.domW, P.domH, P.cellShift, acc.data()};
     rc = sn_load_scenes(e, &sc, 1);
     if (rc) return rc;
+    e->weldSeq = seq;
     if (!rows.empty())
       HIPCHECK(hipMemcpy(e->D.bond + (size_t)e->sp[0].partBase * BOND_CAP, rows.data(),
                          sizeof(uint32_t) * rows.size(), hipMemcpyHostToDevice));

@@ -258,19 +258,19 @@ class Engine:
                                                   int(ix), int(iy), int(limit), C.byref(hits)))
         return int(hits.value)
 
-    def spawn_fluid(self, particles: Iterable[dict], no_weld: bool = True) -> int:
-        """spawnFluid(list, {noWeld}) (src/engine.js:5901); returns the new n."""
+    def spawn_fluid(self, particles: Iterable[dict], no_weld: bool = False, own_group: bool = False) -> int:
+        """spawnFluid(list, {noWeld, ownGroup}) (src/engine.js:5901); returns
+        the new n.  The engine shifts its own boundary block and assigns the
+        spawn's weld group."""
         plist = list(particles)
         if not plist:
             return self.n
         scene = self.scenes[0]
-        fl, bd = sc.spawn_records(scene, plist, self.mats_rows)
+        fl, _ = sc.spawn_records(scene, plist, self.mats_rows)
         n_out = C.c_uint32()
         fl = np.ascontiguousarray(fl, np.int32)
-        bd = np.ascontiguousarray(bd, np.int32)
-        self._check(native.lib().sn_spawn_fluid(self._h, fl.ctypes.data, len(plist),
-                                                bd.ctypes.data if bd.size else None,
-                                                bd.shape[0], C.byref(n_out)))
+        flags = (native.SN_SPAWN_NO_WELD if no_weld else 0) | (native.SN_SPAWN_OWN_GROUP if own_group else 0)
+        self._check(native.lib().sn_spawn_fluid(self._h, fl.ctypes.data, len(plist), flags, C.byref(n_out)))
         new_n = int(n_out.value)
         if new_n != self.images[0].n:
             im = self.images[0]

@@ -14,6 +14,7 @@ LIB_PATH = os.path.join(HERE, "libsunaengine_amd.so")
 
 # error codes (include/sunaengine.h)
 SN_OK, SN_E_ARG, SN_E_CAPACITY, SN_E_DEVICE, SN_E_UNSUPPORTED, SN_E_STATE = 0, -1, -2, -3, -4, -5
+SN_SPAWN_NO_WELD, SN_SPAWN_OWN_GROUP = 1, 2
 #: largest particle_count sn_digest_state_noalloc accepts (its fixed device array)
 SN_DIGEST_NOALLOC_MAX_PARTICLES = 8388608
 
@@ -113,7 +114,7 @@ def lib():
         "sn_debug_counters": (C.c_int, [P, u32, P]),
         "sn_substep_count": (C.c_int, [P, u32, C.POINTER(u32)]),
         "sn_apply_impulse": (C.c_int, [P, u32, i32, i32, i32, i32, i32, u32, C.POINTER(u32)]),
-        "sn_spawn_fluid": (C.c_int, [P, P, u32, P, u32, C.POINTER(u32)]),
+        "sn_spawn_fluid": (C.c_int, [P, P, u32, u32, C.POINTER(u32)]),
         "sn_set_events": (C.c_int, [P, P, u32]),
         "sn_set_active": (C.c_int, [P, P, u32]),
         "sn_reserve_slots": (C.c_int, [P, u32, P, P]),

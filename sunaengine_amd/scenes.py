@@ -304,8 +304,13 @@ def load_image(scene: Scene, mats=None, max_particles: Optional[int] = None,
 
 
 def spawn_records(scene: Scene, particles, mats=None):
-    """The k fluid records and the rebuilt boundary block of spawnFluid with
-    noWeld (src/engine.js:5901-5962); both int32 [*, 8]."""
+    """The k fluid records and the rebuilt boundary block of spawnFluid
+    (src/engine.js:5901-5971); both int32 [*, 8].  The fluid records carry
+    the caller's FLAG_BURNING/FLAG_ANCHOR and the material-derived FLAG_DEAD
+    and FLAG_SEAL but no weld tag (the engine adds the spawn's group id);
+    the boundary block is the loaded scene's, velocity and flags zero, as
+    the reference rewrites it (the engine takes it from its own state; the
+    oracle is given it)."""
     rows = mt.MATS if mats is None else mats
     F = mt.F
     mat_count = len(rows)

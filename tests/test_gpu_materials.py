@@ -254,3 +254,47 @@ def test_material_mode_refusals():
     full.sync()
     full.destroy()
 
+
+
+def test_spawn_weld_groups_and_boundary_rewrite():
+    """spawnFluid's weld tags (src/engine.js:4570-4580,5933-5962): after a
+    load the first spawn is group 2, the next group 3, each tag written only
+    on rows that weld on spawn (STONE; not WATER) and not at all with
+    noWeld; the boundary block moves up with its flags cleared.  The oracle
+    is handed the records with the tags the reference assigns and the
+    boundary block rebuilt from the scene; the engine makes both itself."""
+    from sunaengine_amd import scenes
+    from sunaengine_amd.engine import Engine
+    from sunaengine_amd import materials as mt
+    s = ms.stone_in_water()
+    im = scenes.load_image(s)
+    cap = im.n + 64
+    o = oracle_world(s, 4, max_particles=cap)
+    e = Engine(max_particles=cap, max_cells=im.cellTotal, iters=4)
+    e.load_scene(s)
+    o.step(5)
+    e.step(5)
+    tags = []
+    for group, no_weld in ((2, False), (3, False), (None, True)):
+        drop = [dict(x=(10 + dx) * 65536, y=(8 + dy) * 65536, vy=1500, mat=mt.STONE if dx < 4 else mt.WATER)
+                for dy in range(2) for dx in range(6)]
+        fl, bd = scenes.spawn_records(s, drop)
+        for i, p in enumerate(drop):
+            if group is not None and p["mat"] == mt.STONE:
+                fl[i, 5] |= group << 8
+        assert o.spawn(fl, bd) == e.spawn_fluid(drop, no_weld=no_weld)
+        st = e.read_state()
+        nf = o.nFluid
+        tags.append(sorted({int(f) >> 8 & 0xFFF for f in st[nf - len(drop):nf, 5]}))
+        assert not st[nf:, 5].any(), "boundary flags are cleared by the rewrite"
+        assert np.array_equal(st, o.read_state())
+        o.step(3)
+        e.step(3)
+    assert tags == [[0, 2], [0, 3], [0]]
+    o.step(12)
+    e.step(12)
+    assert np.array_equal(e.read_state(), o.read_state())
+    assert np.array_equal(e.read_bond(), o.read_bond())
+    assert (e.read_bond() != 0).any()
+    assert np.array_equal(e.chain(), o.chain())
+    e.destroy()
