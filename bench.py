@@ -183,7 +183,10 @@ def trace_summary(path: str, warmup: int, steps: int):
         import sqlite3
         con = sqlite3.connect(path)
         raw = [(int(a), int(b), n) for a, b, n in con.execute("select start, end, name from kernels")]
+        pmc = list(con.execute("select start, kernel_name, counter_name, value from counters_collection"))
         con.close()
+        if pmc:
+            return _pmc_summary(raw, pmc, warmup, steps)
     else:                              # --output-format csv: *_kernel_trace.csv
         with open(path, newline="") as f:
             raw = [(int(r["Start_Timestamp"]), int(r["End_Timestamp"]), r["Kernel_Name"]) for r in csv.DictReader(f)]
@@ -219,6 +222,40 @@ def trace_summary(path: str, warmup: int, steps: int):
     for k, a in sorted(agg.items(), key=lambda kv: -kv[1]["us"]):
         print(f"| {k} | {a['n']} | {avg(a['us'], a['n'])} | {a['nt']} | {avg(a['ust'], a['nt'])} | {a['np']} | "
               f"{avg(a['usp'], a['np'])} | {a['us'] / 1e3:.3f} |")
+
+
+def _pmc_summary(raw, pmc, warmup, steps):
+    """Counter means per dispatch of a rocprofv3 --pmc database of the 1M run,
+    over the timed window and the follow-on window (substeps delimited by the
+    k_predict_count dispatches, as in trace_summary)."""
+    import bisect
+    import re
+    short = lambda n: (re.search(r"(k\w+|__amd\w+)", n) or [n])[0]
+    marks = sorted(t0 for t0, _, n in raw if short(n) == "k_predict_count")
+    win_t = (warmup + 1, warmup + steps)
+    win_p = (warmup + steps + 1, warmup + steps + PROFILE_SUBSTEPS)
+    agg = {}
+    for t0, name, counter, value in pmc:
+        g = bisect.bisect_right(marks, int(t0))
+        a = agg.setdefault((short(name), counter), {"n": 0, "v": 0.0, "nt": 0, "vt": 0.0, "np": 0, "vp": 0.0})
+        a["n"] += 1
+        a["v"] += float(value)
+        if win_t[0] <= g <= win_t[1]:
+            a["nt"] += 1
+            a["vt"] += float(value)
+        if win_p[0] <= g <= win_p[1]:
+            a["np"] += 1
+            a["vp"] += float(value)
+    print(f"substeps in the run: {len(marks)}; timed window = substeps {win_t[0]}..{win_t[1]}, "
+          f"follow-on window = {win_p[0]}..{win_p[1]}; values are means per dispatch")
+    print()
+    print("| kernel | counter | dispatches (timed) | mean (timed) | dispatches (follow-on) | mean (follow-on) | "
+          "mean (all) |")
+    print("|---|---|---|---|---|---|---|")
+    avg = lambda v, n: f"{v / n:.1f}" if n else "-"
+    for (k, c), a in sorted(agg.items(), key=lambda kv: -kv[1]["v"]):
+        print(f"| {k} | {c} | {a['nt']} | {avg(a['vt'], a['nt'])} | {a['np']} | {avg(a['vp'], a['np'])} | "
+              f"{avg(a['v'], a['n'])} |")
 
 
 def _line(value, unit, steps, warmup, ms_per_step, config, roofline=None, cpu_baseline=None,
