@@ -146,11 +146,19 @@ def main():
     ap.add_argument("--full-substeps", type=int, default=500,
                     help="1M run: substeps of the whole configuration, replayed from load in the same process "
                          "for config.full_run (0 = skip)")
+    ap.add_argument("--pmc-json", metavar="OUT",
+                    help="write the per-kernel counter file the 1M line reads (profiles/substep2d_1m_pmc.json) from "
+                         "the rocprofv3 --pmc databases given by --pmc-db and the copy calibrations by --cal-db "
+                         "(FETCH_SIZE run, then WRITE_SIZE run), over the timed window of --steps/--warmup, and exit")
+    ap.add_argument("--pmc-db", nargs="*", default=[])
+    ap.add_argument("--cal-db", nargs=2, metavar=("FETCH_CAL", "WRITE_CAL"))
     ap.add_argument("--trace-summary", metavar="TRACE",
                     help="print per-kernel launch averages of a rocprofv3 kernel trace (its rocpd .db, or a "
                          "*_kernel_trace.csv) of a 1M run made with the same --steps/--warmup, over the timed "
                          "window and the follow-on profile window, and exit")
     args = ap.parse_args()
+    if args.pmc_json:
+        return pmc_json(args.pmc_json, args.pmc_db, args.cal_db, args.warmup, args.steps)
     if args.trace_summary:
         return trace_summary(args.trace_summary, args.warmup, args.steps)
     if args.gpus != 1:
@@ -222,6 +230,75 @@ def trace_summary(path: str, warmup: int, steps: int):
     for k, a in sorted(agg.items(), key=lambda kv: -kv[1]["us"]):
         print(f"| {k} | {a['n']} | {avg(a['us'], a['n'])} | {a['nt']} | {avg(a['ust'], a['nt'])} | {a['np']} | "
               f"{avg(a['usp'], a['np'])} | {a['us'] / 1e3:.3f} |")
+
+
+COPY_BYTES = 1 << 30      # the calibration copy: 2^28 int32 words read and written
+CUS, XCDS, LANE_CLOCK = 256, 8, 2.4e9
+
+
+def pmc_json(out, dbs, cal, warmup, steps):
+    """Per-kernel means over the timed window (substeps warmup+1 ..
+    warmup+steps of the timed engine) of every counter in the --pmc
+    databases of the 1M run; HBM bytes from FETCH_SIZE / WRITE_SIZE scaled by
+    what the same counters read for a copy of COPY_BYTES each way; VALUBusy =
+    SQ_ACTIVE_INST_VALU / CUs / (GRBM_GUI_ACTIVE / XCDs), lane utilisation =
+    SQ_THREAD_CYCLES_VALU / (64 SQ_ACTIVE_INST_VALU), int32 lane-ops/s =
+    SQ_INSTS_VALU_INT32 x 64 x lane utilisation / the kernel's mean duration
+    in the same counter run, against CUs x 64 lanes x 2.4 GHz."""
+    import bisect
+    import re
+    import sqlite3
+    short = lambda n: (re.search(r"(k\w+|__amd\w+)", n) or [n])[0]
+    win = (warmup + 1, warmup + steps)
+    acc = {}
+    for db in dbs:
+        con = sqlite3.connect(db)
+        marks = sorted(int(t) for t, n in con.execute("select start, name from kernels") if short(n) == "k_predict_count")
+        for t0, t1, name, counter, value in con.execute(
+                "select start, end, kernel_name, counter_name, value from counters_collection"):
+            g = bisect.bisect_right(marks, int(t0))
+            if not (win[0] <= g <= win[1]):
+                continue
+            a = acc.setdefault(short(name), {})
+            v = a.setdefault(counter, [0, 0.0, 0.0])
+            v[0] += 1
+            v[1] += float(value)
+            v[2] += (int(t1) - int(t0)) * 1e-3       # us, this counter run's dispatch
+        con.close()
+    factors = {}
+    for db, ctr in zip(cal or [], ("FETCH_SIZE", "WRITE_SIZE")):
+        con = sqlite3.connect(db)
+        kib = max(float(v) for n, c, v in con.execute("select kernel_name, counter_name, value from counters_collection")
+                  if c == ctr and short(n) == "__amd_rocclr_copyBuffer")
+        con.close()
+        factors[ctr] = COPY_BYTES / (kib * 1024.0)
+    kernels = {}
+    for k, a in acc.items():
+        m = {c: v[1] / v[0] for c, v in a.items()}
+        d = {}
+        if "FETCH_SIZE" in m and "WRITE_SIZE" in m and len(factors) == 2:
+            d.update(fetch_kib=m["FETCH_SIZE"], write_kib=m["WRITE_SIZE"],
+                     hbm_bytes=m["FETCH_SIZE"] * 1024 * factors["FETCH_SIZE"] + m["WRITE_SIZE"] * 1024 * factors["WRITE_SIZE"])
+        if "SQ_ACTIVE_INST_VALU" in m and m["SQ_ACTIVE_INST_VALU"] > 0:
+            lane = m["SQ_THREAD_CYCLES_VALU"] / (64.0 * m["SQ_ACTIVE_INST_VALU"])
+            us = a["SQ_INSTS_VALU"][2] / a["SQ_INSTS_VALU"][0]
+            d.update(valu_insts=m["SQ_INSTS_VALU"], valu_int32=m.get("SQ_INSTS_VALU_INT32"),
+                     valu_int64=m.get("SQ_INSTS_VALU_INT64"),
+                     valu_busy_pct=100.0 * m["SQ_ACTIVE_INST_VALU"] / CUS / (m["GRBM_GUI_ACTIVE"] / XCDS),
+                     valu_lane_util_pct=100.0 * lane, counter_run_avg_us=us,
+                     valu_int32_ops_per_s=m.get("SQ_INSTS_VALU_INT32", 0.0) * 64 * lane / (us * 1e-6),
+                     valu_int32_peak_ops_per_s=CUS * 64 * LANE_CLOCK)
+        if d:
+            d["dispatches"] = max(v[0] for v in a.values())
+            kernels[k] = d
+    doc = {"window": f"substeps {win[0]}..{win[1]} of the timed engine (python bench.py --steps {steps} "
+                     f"--warmup {warmup} --full-substeps 0 under rocprofv3 --pmc, one run per counter set)",
+           "calibration": {"copy_bytes": COPY_BYTES, "fetch_factor": factors.get("FETCH_SIZE"),
+                           "write_factor": factors.get("WRITE_SIZE")},
+           "kernels": kernels}
+    with open(out, "w") as f:
+        json.dump(doc, f, indent=1, sort_keys=True)
+    print(f"wrote {out}: {len(kernels)} kernels")
 
 
 def _pmc_summary(raw, pmc, warmup, steps):
@@ -650,7 +727,7 @@ def run_scene1m(args):
                         "valu_int32_peak_ops_per_s": pm.get("valu_int32_peak_ops_per_s"),
                         "valu_int32_frac": (round(pm["valu_int32_ops_per_s"] / pm["valu_int32_peak_ops_per_s"], 4)
                                             if pm.get("valu_int32_peak_ops_per_s") else None),
-                        "source": "rocprofv3 --pmc runs of this workload, profiles/substep2d_1m_pmc.md"}
+                        "source": "rocprofv3 --pmc runs of this command's timed window, profiles/substep2d_1m_pmc.md"}
     roofline = {
         "bound": "hbm",
         "achieved": d["achieved_GBs"],

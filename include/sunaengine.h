@@ -176,6 +176,11 @@ typedef struct sn_event {
   uint32_t limit;
 } sn_event;
 int sn_set_events(sn_engine* e, const sn_event* events, uint32_t count);
+/* The particles moved so far by stir events and sn_apply_impulse in scene
+ * `scene` since its load (a wrapping u32 count; each event contributes its
+ * hit count, at most its limit).  The harness's pushHits
+ * (src/demo_boot.js:137,256) for pushes run on the device. */
+int sn_read_push_hits(sn_engine* e, uint32_t scene, uint32_t* out);
 
 /* Scene activity mask for continuous batching: inactive scenes keep their
  * state and are not advanced by sn_step. */

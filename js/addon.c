@@ -288,6 +288,17 @@ static napi_value js_impulse(napi_env env, napi_callback_info info) {
   return num(env, hits);
 }
 
+/* pushHits(h, scene) -> particles moved by stir events and applyImpulse since load */
+static napi_value js_push_hits(napi_env env, napi_callback_info info) {
+  napi_value a[2];
+  get_args(env, info, 2, a);
+  sn_engine* e = handle(env, a[0]);
+  uint32_t v = 0;
+  int rc = sn_read_push_hits(e, u32(env, a[1]), &v);
+  if (rc) return throw_sn(env, e, rc);
+  return num(env, v);
+}
+
 /* spawnFluid(h, fluid8:Int32Array, flags:number) -> new n
  * (flags: 1 = {noWeld: true}, 2 = {ownGroup: true}) */
 static napi_value js_spawn(napi_env env, napi_callback_info info) {
@@ -601,7 +612,7 @@ static napi_value init(napi_env env, napi_value exports) {
       {"step", js_step},              {"sync", js_sync},               {"readHash", js_read_hash},
       {"readChain", js_read_chain},   {"debugCounters", js_debug},     {"substepCount", js_substep_count},
       {"readHashStream", js_read_stream}, {"resetStream", js_reset_stream}, {"readState", js_read_state},
-      {"readDerived", js_read_derived}, {"readBond", js_read_bond}, {"autotune", js_autotune}, {"writeStateDerived", js_write_state_derived}, {"applyImpulse", js_impulse},  {"spawnFluid", js_spawn},
+      {"readDerived", js_read_derived}, {"readBond", js_read_bond}, {"autotune", js_autotune}, {"writeStateDerived", js_write_state_derived}, {"applyImpulse", js_impulse},  {"spawnFluid", js_spawn}, {"pushHits", js_push_hits},
       {"setEvents", js_set_events},   {"setActive", js_set_active},
       {"reserveSlots", js_reserve_slots}, {"loadSlot", js_load_slot}, {"query", js_query},
       {"create3", js_create3},        {"destroy3", js_destroy3},       {"load3", js_load3},

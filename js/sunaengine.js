@@ -241,6 +241,9 @@ class Engine {
       opts.streamCap || 4096, opts.device || 0, loadLuts(), img, opts.debug === false ? 0 : 1,
       opts.materials ? 1 : 0);
     this.n = 0; this.nFluid = 0; this._scene = null; this.substepCount = 0;
+    // the per-slot RGBA8 colour sidecar (src/engine.js:4382-4402): not
+    // simulation state, never hashed, kept on the host
+    this._tints = new Uint32Array(Math.max(1, opts.maxParticles));
   }
   get matCount() { return mats().rows.length; }
   // stAcc (optional): n*2 words of carried surface-tension acceleration
@@ -255,8 +258,36 @@ class Engine {
     this.n = im.n; this.nFluid = im.nFluid; this._scene = scene;
     this.domW = im.domW; this.domH = im.domH; this.cellShift = im.cellShift;
     this.substepCount = 0;
+    this._tints.fill(0);
     return this;
   }
+  // `_substep` (src/engine.js:5825): substeps encoded since the load; it
+  // decides the formation schedule and survives spawnFluid's reload
+  get formationClock() { return native().substepCount(this._h, 0); }
+  async readTints(count) {
+    if (count === undefined) count = this.n;
+    return this._tints.slice(0, Math.max(1, count));
+  }
+  setTints(first, words) {
+    assert(Number.isInteger(first) && first >= 0 && first + words.length <= this.maxParticles,
+      `setTints: [${first}, ${first + words.length}) is outside [0, ${this.maxParticles})`);
+    if (!words.length) return 0;
+    for (let i = 0; i < words.length; i++) this._tints[first + i] = words[i] >>> 0;
+    return words.length;
+  }
+  // A device-resident push schedule (no reference counterpart: the
+  // reference applies each push from the host): events {substep, cx, cy, r,
+  // ix, iy, limit?} in arming order, each applied as applyImpulse
+  // immediately before its substep, inside the captured substeps.
+  setEvents(events) {
+    const w = new Int32Array(Math.max(1, events.length) * 8);
+    events.forEach((ev, k) => {
+      w.set([0, ev.substep, ev.cx, ev.cy, ev.r, ev.ix, ev.iy, ev.limit === undefined ? 3000 : ev.limit], k * 8);
+    });
+    native().setEvents(this._h, events.length ? w : w.subarray(0, 0));
+  }
+  // Particles moved by scheduled events and applyImpulse since the load.
+  pushHits() { return native().pushHits(this._h, 0); }
   step(count) {
     if (count === undefined) count = 1;
     assert(this.n > 0, 'step() called before loadScene()');
@@ -311,7 +342,14 @@ class Engine {
     // the engine shifts its own boundary block and assigns the spawn's weld
     // group (src/engine.js:5933-5971)
     const n = native().spawnFluid(this._h, fl, (opts.noWeld ? 1 : 0) | (opts.ownGroup === true ? 2 : 0));
-    if (n !== this.n) { this.nFluid += k; this.n = n; }
+    if (n !== this.n) {
+      // the colour block moves with the matter: the new slots take their
+      // tints, the shifted boundary block is cleared (src/engine.js:5993-6004)
+      const nB = this.n - this.nFluid;
+      for (let i = 0; i < k; i++) this._tints[this.nFluid + i] = (list[i].tint !== undefined ? list[i].tint : 0) >>> 0;
+      this._tints.fill(0, this.nFluid + k, this.nFluid + k + nB);
+      this.nFluid += k; this.n = n;
+    }
     return this.n;
   }
   async assertSpecValidRegime(where) {

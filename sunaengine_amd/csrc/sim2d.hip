@@ -144,6 +144,7 @@ struct SceneD {
   uint32_t fired;      // substep of the event group being applied (bookkeeping)
   uint32_t flagged;    // nonzero once this substep's canonicalize met a SEAL or CONGEAL
                        // particle; cleared at the end of every substep
+  uint32_t pushHits;   // particles moved by stir events and applyImpulse since load (wrapping)
 };
 
 struct Event {
@@ -1850,6 +1851,7 @@ __global__ __launch_bounds__(256) void k_impulse_apply(Dev D, uint32_t group) {
   a.z = (int32_t)vx;
   a.w = (int32_t)vy;
   D.ps[p] = a;
+  atomicAdd(&D.sd[s].pushHits, 1u);   // a u32 count: order-free
 }
 
 // Advance each scene's event cursor past the group just applied.
@@ -3085,10 +3087,24 @@ int sn_apply_impulse(sn_engine* e, uint32_t scene, int32_t cx, int32_t cy, int32
       total = lastScan + lastFlag - base;
     }
     *hits = std::min(total, limit);
+    saved[scene].pushHits += *hits;   // the saved records predate this call's count
     HIPCHECK(hipMemcpy(e->D.sd, saved.data(), sizeof(SceneD) * e->S, hipMemcpyHostToDevice));
     swapped = false;
   } catch (const DevError& d) {
     if (swapped) (void)hipMemcpy(e->D.sd, saved.data(), sizeof(SceneD) * e->S, hipMemcpyHostToDevice);
+    return fail(e, SN_E_DEVICE, d.msg);
+  }
+  return SN_OK;
+}
+
+int sn_read_push_hits(sn_engine* e, uint32_t scene, uint32_t* out) {
+  if (!e || !out) return SN_E_ARG;
+  try {
+    SceneD sd;
+    int rc = read_sd(e, scene, sd);
+    if (rc) return rc;
+    *out = sd.pushHits;
+  } catch (const DevError& d) {
     return fail(e, SN_E_DEVICE, d.msg);
   }
   return SN_OK;

@@ -277,6 +277,12 @@ class Engine:
             im.n, im.nFluid = new_n, im.nFluid + len(plist)
         return new_n
 
+    def push_hits(self, scene: int = 0) -> int:
+        """Particles moved by stir events and apply_impulse since the load."""
+        out = C.c_uint32()
+        self._check(native.lib().sn_read_push_hits(self._h, scene, C.byref(out)))
+        return int(out.value)
+
     def set_events(self, events: Sequence[tuple]):
         """Device-resident stir schedule: (scene, substep, cx, cy, r, ix, iy[, limit])."""
         evs = sorted(((e[0], e[1], k, e) for k, e in enumerate(events)))

@@ -29,7 +29,7 @@ EXPORTS = [
     "sn3_sync", "sn3_read_hash", "sn3_read_stream", "sn3_reset_stream", "sn3_read_chain",
     "sn3_read_state", "sn3_debug_counters", "sn3_fnv_digest", "sn3_set_events", "sn3_set_active",
     "sn3_profile_step", "sn3_reserve_slots", "sn3_load_slot",
-    "sn_digest_last_error", "sn_digest_state", "sn_digest_state_noalloc",
+    "sn_digest_last_error", "sn_digest_state", "sn_digest_state_noalloc", "sn_read_push_hits",
 ]
 
 
@@ -116,6 +116,7 @@ def lib():
         "sn_apply_impulse": (C.c_int, [P, u32, i32, i32, i32, i32, i32, u32, C.POINTER(u32)]),
         "sn_spawn_fluid": (C.c_int, [P, P, u32, u32, C.POINTER(u32)]),
         "sn_set_events": (C.c_int, [P, P, u32]),
+        "sn_read_push_hits": (C.c_int, [P, u32, C.POINTER(u32)]),
         "sn_set_active": (C.c_int, [P, P, u32]),
         "sn_reserve_slots": (C.c_int, [P, u32, P, P]),
         "sn_query": (C.c_int, [P]),

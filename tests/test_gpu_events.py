@@ -165,3 +165,32 @@ def test_3d_schedule_matches_oracle():
     e.destroy()
     o2, plain = oracle_stream_3d(s, 3, steps, [])
     assert not np.array_equal(plain, ref)
+
+
+def test_push_hits_count_scheduled_and_host_pushes():
+    """sn_read_push_hits: the particles moved by scheduled events and by
+    host applyImpulse calls since the load, each event counting its hits
+    (at most its limit), equal to the oracle's hit counts."""
+    from sunaengine_amd import scenes
+    from tests.helpers import gpu_engine
+    s = scenes.fluid_block_scene(2000, seed=0xE3E3, x_pick=20)
+    p = s.pos.reshape(-1, 2)[:s.nFluid]
+    cx, cy = int(p[:, 0].mean()), int(p[:, 1].mean())
+    sched = [(1, cx, cy, 5 * ONE, 20000, 0, 3000), (3, cx, cy, 30 * ONE, 0, -20000, 17)]
+    o = oracle_world(s, 4)
+    want = 0
+    for t in range(6):
+        for ev in sched:
+            if ev[0] == t:
+                want += o.apply_impulse(*ev[1:])
+        if t == 4:
+            want += o.apply_impulse(cx, cy + ONE, 4 * ONE, 0, 30000, 3000)
+        o.step(1)
+    e = gpu_engine(s, 4)
+    e.set_events([(0,) + ev for ev in sched])
+    e.step(4)
+    e.apply_impulse(cx, cy + ONE, 4 * ONE, 0, 30000, 3000)
+    e.step(2)
+    print(f"events: push hits {want}", flush=True)
+    assert e.push_hits() == want and want > 17
+    e.destroy()
