@@ -59,15 +59,19 @@ def test_pileup_76k_matches_oracle_fixture(debug):
     e.destroy()
 
 
-def test_thermal_1m_matches_oracle_fixture():
+@pytest.mark.parametrize("debug", [True, False])
+def test_thermal_1m_matches_oracle_fixture(debug):
     from sunaengine_amd import scenes
     fx = _fixture("thermal_1m")
     ref = fx["stream"]
     assert int(fx["maxcell"].max()) > 4000, "the fixture must reach cells of thousands"
-    print(f"thermal_1m x{ref.shape[0]}", flush=True)
-    e = _run(scenes.thermal_1m(), ref.shape[0], True)
+    print(f"thermal_1m x{ref.shape[0]} debug={debug}", flush=True)
+    e = _run(scenes.thermal_1m(), ref.shape[0], debug)
     bad = first_mismatch(e.hash_stream(), ref)
     assert bad is None, f"first differing substep {bad} (largest cell there {fx['maxcell'][bad]})"
     assert np.array_equal(e.chain(), fx["chain"])
-    assert np.array_equal(e.debug_counters(), fx["dbg"])
+    if debug:
+        assert np.array_equal(e.debug_counters(), fx["dbg"])
+    else:
+        assert not e.debug_counters().any()
     e.destroy()
