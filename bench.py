@@ -58,7 +58,7 @@ PROFILE_SUBSTEPS = 20       # follow-on substeps profiled with every kernel clas
 KERNEL_OF = {"predict": "k_predict_count", "scan": "k_scan_reduce+k_scan_top+k_scan_down",
              "scatter": "k_scatter", "canon_gather": "k_canon_gather+k_canon_heavy_sort+k_canon_heavy_rank+k_canon_heavy_place",
              "nbr_solveA": "k_nbr_tile",
-             "solveA": "k_solveA", "solveB": "k_solveB", "apply": "k_apply", "xnc": "k_xnc",
+             "solveA": "k_solveA", "colour": "k_colour", "apply": "k_apply", "xnc": "k_xnc",
              "st_thermal_hash": "k_st_thermal_hash", "hash_finish": "k_hash_finish",
              "impulse": "k_impulse_*", "lists": "k_build_lists",
              "friction": "k_friction+k_apply_fric",
@@ -68,8 +68,8 @@ KERNEL_OF = {"predict": "k_predict_count", "scan": "k_scan_reduce+k_scan_top+k_s
 def class_bytes(F: int, N: int, C: int, k: int) -> dict:
     """The same algorithmic bytes per substep, assigned to the kernel that
     carries each phase: the sort's cell-array traffic to the scan, its
-    per-particle traffic to scatter, a colour sweep's reads to solveB and its
-    pred update to apply, finalize's state write to xnc (which writes the
+    per-particle traffic to scatter, a colour sweep's reads to k_colour and
+    its pred update to apply, finalize's state write to xnc (which writes the
     substep's final state), thermal and the hash to the fused
     surface-tension kernel."""
     return {
@@ -79,7 +79,7 @@ def class_bytes(F: int, N: int, C: int, k: int) -> dict:
         "canon_gather": 0,
         "nbr_solveA": 16 * N + 8 * F + 4 * C,
         "solveA": (k - 1) * (8 * N + 8 * F) + 4 * F,
-        "solveB": k * (16 * N + 16 * F),
+        "colour": k * (16 * N + 16 * F),
         "apply": k * (16 * N + 16 * F),
         "xnc": 68 * F + 64 * N + 24 * F,
         "st_thermal_hash": 12 * N + 24 * F + 40 * F + 32 * N + 4 * F,

@@ -163,6 +163,36 @@ SN_HD int32_t fp_div_s(int32_t a, int32_t b, unsigned s, Dbg& g) {
   return (int32_t)q32;
 }
 
+// ---- exact binary64 forms of fp_mul_s for bounded operands ----------------
+// When |a * b| < 2^53, fp_mul_s(a, b, s) equals q_sat(q_rne((double)a *
+// ((double)b * 2^-s))): the power-of-two scale is exact, the product of the
+// two is the exact rational a*b / 2^s (its significand needs at most 53
+// bits), and rint rounds it half to even (v_rndne_f64).  Saturation and the
+// dbg[0] event then follow from the rounded value.  gfx950 issues binary64
+// multiply, rint, min/max and the i32 <-> f64 conversions at the full VALU
+// rate; the exact 64-bit integer product costs two 64-bit multiply-adds and a
+// 64-bit add / shift / compare chain per multiply.  Every caller states the
+// operand bound that makes its product exact.
+SN_HD double q_rne(double x) { return __builtin_rint(x); }
+// An integer-valued q with |q| < 2^52 saturated to i32; the event is the
+// one fp_mul_s counts: q outside [I32_MIN, I32_MAX] <=> |q + 0.5| > 2^31
+// (q + 0.5 is exact for |q| < 2^52).
+SN_HD uint32_t q_ovf(double q) { return __builtin_fabs(q + 0.5) > 2147483648.0 ? 1u : 0u; }
+SN_HD int32_t q_sat(double q, Dbg& g) {
+  g.mul += q_ovf(q);
+  return (int32_t)__builtin_fmin(__builtin_fmax(q, -2147483648.0), 2147483647.0);
+}
+// clampi(q_sat(q), lo, hi) for lo <= hi (clamping the saturated value or the
+// unsaturated one gives the same result), with q_sat's event.
+SN_HD int32_t q_sat_clamp(double q, int32_t lo, int32_t hi, Dbg& g) {
+  g.mul += q_ovf(q);
+  return (int32_t)__builtin_fmin(__builtin_fmax(q, (double)lo), (double)hi);
+}
+// The whole multiply in this form (for checks of the equivalence above).
+SN_HD int32_t fp_mul_s_q(int32_t a, int32_t b, unsigned s, Dbg& g) {
+  return q_sat(q_rne((double)a * __builtin_ldexp((double)b, -(int)s)), g);
+}
+
 // isqrt64 (src/fixed.wgsl:318-332) for a 32-bit input: floor(sqrt(n)).
 SN_HD uint32_t isqrt32(uint32_t n) {
   uint32_t rem = 0, root = 0;

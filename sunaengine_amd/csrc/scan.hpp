@@ -14,13 +14,13 @@ namespace {
 // counts for the next substep.  Integer adds in a fixed tree: exact.
 constexpr uint32_t SCAN_PER_THREAD = 16, SCAN_BLOCK = 256 * SCAN_PER_THREAD;
 
-// Work-list append from a 256-thread workgroup: thread i appends n_i items
-// {key_i, 0..n_i-1} to list, reserving the workgroup's total with one
-// atomic on *counter (every wave's appends land on the same few counters,
-// whose atomics are performed one at a time at the memory side).  The
-// order of items in the list is left open.  Every thread of the workgroup
-// must call it.
-__device__ __forceinline__ void wg_append_chunks(uint32_t n, uint32_t* counter, uint2* list, uint32_t key) {
+// Work-list reservation from a 256-thread workgroup: thread i reserves n_i
+// consecutive entries of a list and gets the index of its first one; the
+// workgroup's total is reserved with one atomic on *counter (every wave's
+// appends land on the same few counters, whose atomics are performed one at
+// a time at the memory side).  The order of the threads' runs in the list is
+// left open.  Every thread of the workgroup must call it.
+__device__ __forceinline__ uint32_t wg_reserve(uint32_t n, uint32_t* counter) {
   const uint32_t lane = threadIdx.x & 63u;
   uint32_t incl = n;
 #pragma unroll
@@ -42,8 +42,15 @@ __device__ __forceinline__ void wg_append_chunks(uint32_t n, uint32_t* counter, 
   }
   __syncthreads();
   const uint32_t base = wbase[wv] + incl - n;
-  for (uint32_t k = 0; k < n; ++k) list[base + k] = make_uint2(key, k);
   __syncthreads();   // wtot / wbase are reused by the next call
+  return base;
+}
+
+// Work-list append: thread i appends the n_i items {key_i, 0..n_i-1}.  Every
+// thread of the workgroup must call it.
+__device__ __forceinline__ void wg_append_chunks(uint32_t n, uint32_t* counter, uint2* list, uint32_t key) {
+  const uint32_t base = wg_reserve(n, counter);
+  for (uint32_t k = 0; k < n; ++k) list[base + k] = make_uint2(key, k);
 }
 
 // Arrival ranks in the cell histogram, one device atomic per distinct cell

@@ -23,9 +23,9 @@
 //   scatter        bucket = arrival order               (:1117-1124)
 //   canon_gather   ascending-id rank within the cell, gather state (:1155-1168)
 //   nbr_solveA     buildNbr fused with iteration 0's solveA (:1190-1318)
-//   colour x4      solveB + applyDp of one colour, one wave per cell group,
-//                  race-free because same-colour neighbours share a cell
-//                  (:1337-1474); the last iteration also finalizes (:2245-2265)
+//   colour x4      solveB (k_colour) + applyDp (k_apply) of one colour, a
+//                  workgroup / wave per cell chunk (:1337-1474); the last
+//                  iteration also finalizes (:2245-2265)
 //   solveA         iterations 1..k-1 (:1270-1318), then colour x4 again
 //   xnc            xsph + normals + conduct (:2300-2582)
 //   st_thermal     surfaceTension + thermal + formation flags + state hash
@@ -128,7 +128,8 @@ struct SceneP {
   uint32_t cellShift, cellBase, cellTotal, partBase;
   int32_t domW, domH, vmax, posShift;   // posShift = partBase - (particles of earlier scenes)
   uint32_t weldHold, cap, cellCap;
-  uint32_t bounded;   // every position of the scene stays within +-2^28 (see pdelta)
+  uint32_t bounded;   // every position of the scene stays within +-2^28 (see pdelta) and the
+                      // world's G table is below G_BOUND (see solveA_term)
 };
 struct SceneD {
   uint32_t substep;    // substeps encoded since load (`_substep`, src/engine.js:3811)
@@ -599,12 +600,29 @@ __global__ __launch_bounds__(256) void k_canon_heavy_place(Dev D) {
 struct SolveAAcc {
   int32_t rho, S, sgx, sgy;
 };
+// Bounded pairs (BND) take the binary64 forms of fixed.hpp: |d| <= REACH <
+// 2^17.33 per axis and |G| < 2^27 (G_BOUND, checked when the table is
+// uploaded; worlds whose table exceeds it run every scene unbounded), so
+// |G d| < 2^44.33 and each gradient is exact with |g| < 2^18.33 (no
+// saturation); g^2 / 2^11 < 2^25.7 is exact and below the i32 rails, so the
+// S terms only take their [0, S_TERM] clamp (g^2 >= 0).
+constexpr int32_t G_BOUND = 1 << 27;
+template <bool BND>
 __device__ __forceinline__ void solveA_term(SolveAAcc& a, int2 d, int4 L, Dbg& g) {
   a.rho = wadd(a.rho, L.x);
-  int32_t gx = grad_q16(L.y, d.x, g), gy = grad_q16(L.y, d.y, g);
-  a.sgx = wadd(a.sgx, gx);
-  a.sgy = wadd(a.sgy, gy);
-  a.S = wadd(wadd(a.S, clampi(fp_mul_s(gx, gx, 11, g), 0, S_TERM)), clampi(fp_mul_s(gy, gy, 11, g), 0, S_TERM));
+  if constexpr (BND) {
+    const double gs = (double)L.y * 0x1p-26;
+    const double gx = q_rne(gs * (double)d.x), gy = q_rne(gs * (double)d.y);
+    a.sgx = wadd(a.sgx, (int32_t)gx);
+    a.sgy = wadd(a.sgy, (int32_t)gy);
+    a.S = wadd(wadd(a.S, (int32_t)__builtin_fmin(q_rne(gx * gx * 0x1p-11), (double)S_TERM)),
+               (int32_t)__builtin_fmin(q_rne(gy * gy * 0x1p-11), (double)S_TERM));
+  } else {
+    int32_t gx = grad_q16(L.y, d.x, g), gy = grad_q16(L.y, d.y, g);
+    a.sgx = wadd(a.sgx, gx);
+    a.sgy = wadd(a.sgy, gy);
+    a.S = wadd(wadd(a.S, clampi(fp_mul_s(gx, gx, 11, g), 0, S_TERM)), clampi(fp_mul_s(gy, gy, 11, g), 0, S_TERM));
+  }
 }
 __device__ __forceinline__ void solveA_finish(const Dev& D, uint32_t p, SolveAAcc& a, int32_t eps, Dbg& g) {
   a.S = wadd(wadd(a.S, clampi(fp_mul_s(a.sgx, a.sgx, 11, g), 0, S_TERM)),
@@ -631,7 +649,8 @@ __device__ __forceinline__ uint32_t nbr_at(const Dev& D, const uint32_t* nb, uin
 
 // ============================================================ per-substep work
 // After the scan: work lists of this substep.  [0..3]: every occupied cell
-// of colour c, one item {cell, chunk} per CELL_CHUNK of its particles; [4]:
+// of colour c, one item per CELL_CHUNK of its particles, resolved to {first
+// sorted slot, count | scene << COL_SCENE_SHIFT}; [4]:
 // every occupied neighbour tile, one item {tile, chunk} per TILE_CHUNK of its
 // particles.  Chunking bounds every item's work, so a cell that walls or
 // collisions have packed with hundreds of particles is spread over many
@@ -640,6 +659,8 @@ __device__ __forceinline__ uint32_t nbr_at(const Dev& D, const uint32_t* nb, uin
 // order (which the atomics leave open) affects no result.
 constexpr int NBR_PPL = 1;             // particles per lane in the neighbour sweep
 constexpr uint32_t CELL_CHUNK = 32;
+constexpr uint32_t COLOUR_THREADS = 256;   // k_colour: 8 lanes for each particle of a 32-particle item
+constexpr uint32_t COL_COUNT = 0xffu, COL_SCENE_SHIFT = 8;
 constexpr uint32_t TILE_CHUNK = 64;    // 64 lanes x NBR_PPL
 // A tile whose candidate rows hold more than this many slots (a pile-up in
 // or next to it; 576 at 4x rest density) goes to dlist[6], whose chunks every
@@ -652,19 +673,26 @@ constexpr uint32_t NBR_HEAVY_CAND = 2048;
 __global__ __launch_bounds__(256) void k_build_lists(Dev D) {
   const uint32_t t = blockIdx.x * 256u + threadIdx.x;
   {
-    uint32_t cnt = 0, col = 0;
+    uint32_t cnt = 0, col = 0, lo = 0, s = 0;
     if (t < D.C) {
-      const uint32_t s = D.cellScene[t];
+      s = D.cellScene[t];
       if (s != INACTIVE) {
         cnt = D.cellStart[t + 1] - D.cellStart[t];
         const SceneP& P = D.sp[s];
         const uint32_t lc = t - P.cellBase;
         col = ((lc % P.gridW) & 1u) | (((lc / P.gridW) & 1u) << 1);
+        lo = (uint32_t)((int32_t)D.cellStart[t] + P.posShift);
       }
     }
     const uint32_t nch = (cnt + CELL_CHUNK - 1u) / CELL_CHUNK;
 #pragma unroll
-    for (uint32_t c = 0; c < 4; ++c) wg_append_chunks(col == c ? nch : 0u, &D.listCount[c], D.dlist[c], t);
+    for (uint32_t c = 0; c < 4; ++c) {
+      const uint32_t base = wg_reserve(col == c ? nch : 0u, &D.listCount[c]);
+      if (col == c)
+        for (uint32_t k = 0; k < nch; ++k)
+          D.dlist[c][base + k] =
+              make_uint2(lo + k * CELL_CHUNK, min(CELL_CHUNK, cnt - k * CELL_CHUNK) | (s << COL_SCENE_SHIFT));
+    }
   }
   {
     uint32_t nch = 0;
@@ -824,7 +852,7 @@ __device__ __forceinline__ void nbr_tail(const Dev& D, const SceneP& P, int32_t 
       const uint32_t room = MAXNBR - c;
       if (ok && rank < room) {
         D.nbr[(size_t)(c + rank) * D.NS + pkL] = q | ((ci & INF_FLUID) ? 0u : NBR_BND);
-        if (fluidL) solveA_term(part, d, D.lut4[idx], g);
+        if (fluidL) solveA_term<BND>(part, d, D.lut4[idx], g);
       }
       c += min((uint32_t)__popcll(m), room);
     }
@@ -995,7 +1023,7 @@ __device__ __forceinline__ void nbr_chunk(const Dev& D, uint32_t c0, uint32_t nc
             }
             D.nbr[(size_t)cnt[k] * D.NS + pk[k]] = q | ((qi & INF_FLUID) ? 0u : NBR_BND);
             cnt[k]++;
-            if (inf[k] & INF_FLUID) solveA_term(acc[k], d, D.lut4[idx], g);
+            if (inf[k] & INF_FLUID) solveA_term<BND>(acc[k], d, D.lut4[idx], g);
           }
         }
       }
@@ -1158,7 +1186,7 @@ __device__ __forceinline__ bool nbr_chunk_staged(const Dev& D, uint16_t* lst, ui
     }
 #pragma unroll
     for (uint32_t u = 0; u < NBR_UNROLL; ++u)
-      if (idx[u] >= 0) solveA_term(acc, d[u], L[u], g);
+      if (idx[u] >= 0) solveA_term<BND>(acc, d[u], L[u], g);
   }
   if (valid) {
     D.nbrN[p] = live ? (cnt | ((bits & INF_SEAL) ? NBR_SEAL_BIT : 0u) | ((bits & INF_CONGEAL) ? NBR_CONGEAL_BIT : 0u))
@@ -1299,7 +1327,7 @@ __device__ __forceinline__ void solveA_at(const Dev& D, uint32_t p, const int2* 
     }
 #pragma unroll
     for (uint32_t u = 0; u < NBR_UNROLL; ++u)
-      if (idx[u] >= 0) solveA_term(acc, d[u], L[u], g);
+      if (idx[u] >= 0) solveA_term<BND>(acc, d[u], L[u], g);
   }
   solveA_finish(D, p, acc, MAT(D, (uint32_t)D.ptY[p].x)[M_EPS], g);
 }
@@ -1328,15 +1356,18 @@ __global__ __launch_bounds__(256) void k_solveA(Dev D) {
   flush_dbg(D, s, g, 0);
 }
 
-// The particles of cell-chunk item {cell, chunk}: scene, first sorted slot, count.
-__device__ __forceinline__ void chunk_range(const Dev& D, uint2 item, uint32_t& s, uint32_t& lo, uint32_t& m) {
-  const uint32_t c = item.x;
-  s = D.cellScene[c];
-  const int32_t sh = D.sp[s].posShift;
-  const uint32_t clo = (uint32_t)((int32_t)D.cellStart[c] + sh);
-  const uint32_t chi = (uint32_t)((int32_t)D.cellStart[c + 1] + sh);
-  lo = clo + item.y * CELL_CHUNK;
-  m = (uint32_t)mini((int32_t)(chi - lo), (int32_t)CELL_CHUNK);
+// A colour-list item {first sorted slot, count | scene << COL_SCENE_SHIFT},
+// read uniformly by the workgroup.
+struct ColItem {
+  uint32_t lo, m, s;
+};
+__device__ __forceinline__ ColItem col_item(uint2 item) {
+  ColItem c;
+  c.lo = (uint32_t)__builtin_amdgcn_readfirstlane((int)item.x);
+  const uint32_t y = (uint32_t)__builtin_amdgcn_readfirstlane((int)item.y);
+  c.m = y & COL_COUNT;
+  c.s = y >> COL_SCENE_SHIFT;
+  return c;
 }
 
 
@@ -1344,20 +1375,24 @@ __device__ __forceinline__ void chunk_range(const Dev& D, uint2 item, uint32_t& 
 // solveB_c then applyDp_c (src/sim.wgsl:1337-1474) for one colour, as two
 // launches over the colour's cell chunks, like the reference's two
 // dispatches: every dp of the colour is formed before any pred moves.
-// k_solveB gives each chunk one workgroup and splits each particle's
+// k_colour gives each chunk one workgroup and splits each particle's
 // neighbour list over L lanes (entries k = sub, sub + L, ...; L = 256 /
 // particles, a power of two up to 16) whose wrapping partial sums are
 // combined with xor shuffles, so the 48-deep gather chains run L-fold
 // shorter; the (G, S) kernel tables sit in LDS.  k_apply moves the chunk's
 // particles and, on the last iteration, finalizes them
-// (src/sim.wgsl:2245-2265).
+// (src/sim.wgsl:2245-2265).  (Moving a one-chunk cell's particles inside
+// k_colour after a workgroup barrier is race-free, since a cell's only
+// same-colour neighbour is itself, but measured slower on the 1M dam break:
+// the barrier idles the workgroup, 3.02 against 2.96 ms per substep.)
 // A lane takes entries k = sub + u * L, u < SB_E, of its particle's list
 // (L >= 8 lanes per particle, so SB_E = 6 covers MAXNBR).  All list words
 // are loaded first, then all neighbour preds and lambdas, then the terms
 // are formed: one gather round trip per item instead of one per pair of
 // entries.  Entries past the count are skipped before any test.
 constexpr uint32_t SB_E = MAXNBR / 8;
-static_assert(256u / CELL_CHUNK >= 8u && SB_E * 8u >= MAXNBR, "k_solveB gives every particle at least 8 lanes");
+static_assert(COLOUR_THREADS / CELL_CHUNK >= 8u && SB_E * 8u >= MAXNBR, "k_colour gives every particle at least 8 lanes");
+static_assert(CELL_CHUNK <= COL_COUNT, "an item fits the count field");
 template <bool BND>
 __device__ __forceinline__ int2 solveB_part(const Dev& D, uint32_t p, uint32_t sub, uint32_t L,
                                             const int2* lutGS, Dbg& g) {
@@ -1388,16 +1423,50 @@ __device__ __forceinline__ int2 solveB_part(const Dev& D, uint32_t p, uint32_t s
     if (ix < 0) continue;
     const int2 T = lutGS[ix];
     const int32_t co = fp_add_sat(fp_add_sat(li, (e[u] & NBR_BND) ? li : lj[u], g), T.y, g);
-    dpx = wadd(dpx, clampi(fp_mul_s(co, grad_q16(T.x, d.x, g), 20 - ACC, g), -DP_TERM, DP_TERM));
-    dpy = wadd(dpy, clampi(fp_mul_s(co, grad_q16(T.x, d.y, g), 20 - ACC, g), -DP_TERM, DP_TERM));
+    if constexpr (BND) {
+      // binary64 forms (bounds as in solveA_term): exact gradients with
+      // |g| < 2^18.33, then |co g| < 2^31 * 2^18.33 < 2^53 is exact too
+      const double gs = (double)T.x * 0x1p-26, cf = (double)co * 0x1p-12;
+      const double gx = q_rne(gs * (double)d.x), gy = q_rne(gs * (double)d.y);
+      dpx = wadd(dpx, q_sat_clamp(q_rne(cf * gx), -DP_TERM, DP_TERM, g));
+      dpy = wadd(dpy, q_sat_clamp(q_rne(cf * gy), -DP_TERM, DP_TERM, g));
+    } else {
+      dpx = wadd(dpx, clampi(fp_mul_s(co, grad_q16(T.x, d.x, g), 20 - ACC, g), -DP_TERM, DP_TERM));
+      dpy = wadd(dpy, clampi(fp_mul_s(co, grad_q16(T.x, d.y, g), 20 - ACC, g), -DP_TERM, DP_TERM));
+    }
   }
   return make_int2(dpx, dpy);
 }
 
 
-__global__ __launch_bounds__(256) void k_solveB(Dev D, uint32_t colour) {
+// applyDp for fluid particle p of scene P (src/sim.wgsl:1458-1474), then on
+// the last iteration finalize (src/sim.wgsl:2245-2265).
+__device__ __forceinline__ void apply_at(const Dev& D, const SceneP& P, uint32_t p, int2 dp, uint32_t last,
+                                         Dbg& g) {
+  const int4 b = D.ptY[p];
+  const uint32_t fl = (uint32_t)b.y;
+  int2 pr = D.pred[p];
+  if ((fl & (FLAG_RIGID | FLAG_ANCHOR)) == 0u) {
+    const int32_t* M = MAT(D, (uint32_t)b.x);
+    const int32_t dm = M[M_DPMAX], dmN = fp_neg_sat(dm, g);
+    const int32_t ddx = clampi(fp_shr_rne(dp.x, ACC), dmN, dm);
+    const int32_t ddy = clampi(fp_shr_rne(dp.y, ACC), dmN, dm);
+    pr = wall_clamp(P, make_int2(fp_add_sat(pr.x, ddx, g), fp_add_sat(pr.y, ddy, g)));
+    D.pred[p] = pr;
+  }
+  if (last) {
+    const int4 a = D.psY[p];
+    const int32_t vmaxN = fp_neg_sat(P.vmax, g);
+    const int32_t vx = clampi(fp_sub_sat(pr.x, a.x, g), vmaxN, P.vmax);
+    const int32_t vy = clampi(fp_sub_sat(pr.y, a.y, g), vmaxN, P.vmax);
+    D.vtmp[p] = make_int2(vx, vy);
+    D.rinv[p] = clampi(fp_divshift(RHO0, maxi(D.rho[p], RHO_FLOOR_RINV), 16, g), 0, RINV_MAX);
+  }
+}
+
+__global__ __launch_bounds__(COLOUR_THREADS) void k_colour(Dev D, uint32_t colour) {
   __shared__ int2 lutGS[TN];
-  for (uint32_t i = threadIdx.x; i < TN; i += 256u) {
+  for (uint32_t i = threadIdx.x; i < TN; i += COLOUR_THREADS) {
     const int4 L = D.lut4[i];
     lutGS[i] = make_int2(L.y, L.w);
   }
@@ -1406,69 +1475,37 @@ __global__ __launch_bounds__(256) void k_solveB(Dev D, uint32_t colour) {
   const uint2* list = D.dlist[colour];
   const ListRange R = xcd_range(n, 1u, 0u);
   for (uint32_t w = R.first; w < R.hi; w += R.stride) {
-    uint2 item = list[w];
-    item.x = __builtin_amdgcn_readfirstlane(item.x);
-    item.y = __builtin_amdgcn_readfirstlane(item.y);
-    uint32_t s, lo, m;
-    chunk_range(D, item, s, lo, m);
-    s = __builtin_amdgcn_readfirstlane(s);
-    m = __builtin_amdgcn_readfirstlane(m);
+    const ColItem it = col_item(list[w]);
+    const bool bnd = D.sp[it.s].bounded != 0u;
     Dbg g;
     uint32_t L = 16;
-    while (L > 1u && L * m > 256u) L >>= 1;
+    while (L > 1u && L * it.m > COLOUR_THREADS) L >>= 1;
     const uint32_t pl = threadIdx.x / L, sub = threadIdx.x & (L - 1u);
-    const uint32_t p = lo + pl;
-    const bool act = pl < m && (D.info[p] & INF_FLUID);
+    const uint32_t p = it.lo + pl;
+    const bool act = pl < it.m && (D.info[p] & INF_FLUID);
     int2 dp = make_int2(0, 0);
-    if (act) dp = D.sp[s].bounded ? solveB_part<true>(D, p, sub, L, lutGS, g)
-                                  : solveB_part<false>(D, p, sub, L, lutGS, g);
+    if (act) dp = bnd ? solveB_part<true>(D, p, sub, L, lutGS, g) : solveB_part<false>(D, p, sub, L, lutGS, g);
     for (uint32_t off = 1; off < L; off <<= 1) {
       dp.x = wadd(dp.x, __shfl_xor(dp.x, (int)off));
       dp.y = wadd(dp.y, __shfl_xor(dp.y, (int)off));
     }
     if (act && sub == 0u) D.dp[p] = dp;
-    flush_dbg(D, s, g, 0);
+    flush_dbg(D, it.s, g, 0);
   }
 }
 
+// applyDp of the colour's chunks, one wave per chunk.
 __global__ __launch_bounds__(256) void k_apply(Dev D, uint32_t colour, uint32_t last) {
   const uint32_t lane = threadIdx.x & 63u;
   const uint32_t n = (uint32_t)__builtin_amdgcn_readfirstlane((int)D.listCount[colour]);
   const uint2* list = D.dlist[colour];
   const ListRange R = xcd_range(n, 4u, threadIdx.x >> 6);
   for (uint32_t w = R.first; w < R.hi; w += R.stride) {
-    uint2 item = list[w];
-    item.x = __builtin_amdgcn_readfirstlane(item.x);
-    item.y = __builtin_amdgcn_readfirstlane(item.y);
-    uint32_t s, lo, m;
-    chunk_range(D, item, s, lo, m);
-    s = __builtin_amdgcn_readfirstlane(s);
-    const SceneP& P = D.sp[s];
+    const ColItem it = col_item(list[w]);
     Dbg g;
-    const uint32_t p = lo + lane;
-    if (lane < m && (D.info[p] & INF_FLUID)) {
-      const int4 b = D.ptY[p];
-      const uint32_t fl = (uint32_t)b.y;
-      int2 pr = D.pred[p];
-      if ((fl & (FLAG_RIGID | FLAG_ANCHOR)) == 0u) {
-        const int32_t* M = MAT(D, (uint32_t)b.x);
-        const int32_t dm = M[M_DPMAX], dmN = fp_neg_sat(dm, g);
-        const int2 dp = D.dp[p];
-        const int32_t ddx = clampi(fp_shr_rne(dp.x, ACC), dmN, dm);
-        const int32_t ddy = clampi(fp_shr_rne(dp.y, ACC), dmN, dm);
-        pr = wall_clamp(P, make_int2(fp_add_sat(pr.x, ddx, g), fp_add_sat(pr.y, ddy, g)));
-        D.pred[p] = pr;
-      }
-      if (last) {
-        const int4 a = D.psY[p];
-        const int32_t vmaxN = fp_neg_sat(P.vmax, g);
-        const int32_t vx = clampi(fp_sub_sat(pr.x, a.x, g), vmaxN, P.vmax);
-        const int32_t vy = clampi(fp_sub_sat(pr.y, a.y, g), vmaxN, P.vmax);
-        D.vtmp[p] = make_int2(vx, vy);
-        D.rinv[p] = clampi(fp_divshift(RHO0, maxi(D.rho[p], RHO_FLOOR_RINV), 16, g), 0, RINV_MAX);
-      }
-    }
-    flush_dbg(D, s, g, 0);
+    const uint32_t p = it.lo + lane;
+    if (lane < it.m && (D.info[p] & INF_FLUID)) apply_at(D, D.sp[it.s], p, D.dp[p], last, g);
+    flush_dbg(D, it.s, g, 0);
   }
 }
 
@@ -1930,6 +1967,7 @@ struct sn_engine {
   uint32_t weldSeq = 1;  // the 12-bit spawn-event id of the last weld group (`_weldSeq`, src/engine.js:3801)
   bool gen = false;      // the material passes run (D.gen); never switched off again
   uint32_t graphG = 8;   // substeps per captured hipGraph (sn_autotune may change it)
+  bool gBounded = true;  // every G entry of the table is below G_BOUND in magnitude (bounded pair forms)
   // profiling: an event pair around every launch, tagged with its class
   std::vector<hipEvent_t> evPool;
   std::vector<int> evClass;
@@ -2101,7 +2139,7 @@ void enqueue_substep(sn_engine* e) {
     for (uint32_t c = 0; c < 4; ++c) {
       {
         ProfScope ps(e, PC_COLOUR);
-        hipLaunchKernelGGL(k_solveB, dim3(e->colourBlocks), dim3(256), 0, st, D, c);
+        hipLaunchKernelGGL(k_colour, dim3(e->colourBlocks), dim3(COLOUR_THREADS), 0, st, D, c);
       }
       ProfScope ps(e, PC_APPLY);
       hipLaunchKernelGGL(k_apply, dim3(e->applyBlocks), dim3(256), 0, st, D, c,
@@ -2230,6 +2268,11 @@ int sn_create(const sn_opts* opts, const int32_t* luts, const int32_t* mats, sn_
     *out = e;
     return rc;
   }
+  if (e->opts.max_scenes > (1u << (32 - COL_SCENE_SHIFT))) {   // scene index field of the work-list items
+    int rc = fail(e, SN_E_ARG, "[aether] maxScenes above " + std::to_string(1u << (32 - COL_SCENE_SHIFT)));
+    *out = e;
+    return rc;
+  }
   try {
     HIPCHECK(hipSetDevice((int)e->opts.device));
     HIPCHECK(hipStreamCreateWithFlags(&e->stream, hipStreamNonBlocking));
@@ -2284,7 +2327,10 @@ int sn_create(const sn_opts* opts, const int32_t* luts, const int32_t* mats, sn_
     int4* lut4 = dalloc<int4>(e, TN);
     int32_t* lutA = dalloc<int32_t>(e, TN);
     std::vector<int4> h4(TN);
-    for (uint32_t i = 0; i < TN; ++i) h4[i] = make_int4(luts[i], luts[TN + i], luts[2 * TN + i], luts[4 * TN + i]);
+    for (uint32_t i = 0; i < TN; ++i) {
+      h4[i] = make_int4(luts[i], luts[TN + i], luts[2 * TN + i], luts[4 * TN + i]);
+      e->gBounded = e->gBounded && luts[TN + i] > -G_BOUND && luts[TN + i] < G_BOUND;
+    }
     HIPCHECK(hipMemcpy(lut4, h4.data(), sizeof(int4) * TN, hipMemcpyHostToDevice));
     HIPCHECK(hipMemcpy(lutA, luts + 3 * TN, sizeof(int32_t) * TN, hipMemcpyHostToDevice));
     D.lut4 = lut4;
@@ -2309,7 +2355,9 @@ int sn_create(const sn_opts* opts, const int32_t* luts, const int32_t* mats, sn_
     D.cellScene = dalloc<uint32_t>(e, CC);
     D.tileAll = dalloc<uint2>(e, CC);
     // a list holds at most one item per cell (tile) plus one per full chunk
-    for (uint32_t c = 0; c < 7; ++c) D.dlist[c] = dalloc<uint2>(e, (size_t)CC + NC / CELL_CHUNK + 64);
+    // at most one item per cell or tile plus one per full 32 slots (every list's
+    // chunks are of at least 32)
+    for (uint32_t c = 0; c < 7; ++c) D.dlist[c] = dalloc<uint2>(e, (size_t)CC + NC / 32 + 64);
     D.listCount = dalloc<uint32_t>(e, 8);
     // persistent grids: as many workgroups as can be resident, a multiple of 8 (XCDs)
     {
@@ -2317,7 +2365,7 @@ int sn_create(const sn_opts* opts, const int32_t* luts, const int32_t* mats, sn_
       HIPCHECK(hipGetDeviceProperties(&prop, (int)e->opts.device));
       int bN = 0, bC = 0, bA = 0;
       HIPCHECK(hipOccupancyMaxActiveBlocksPerMultiprocessor(&bN, reinterpret_cast<const void*>(k_nbr_tile), 256, 0));
-      HIPCHECK(hipOccupancyMaxActiveBlocksPerMultiprocessor(&bC, reinterpret_cast<const void*>(k_solveB), 256, 0));
+      HIPCHECK(hipOccupancyMaxActiveBlocksPerMultiprocessor(&bC, reinterpret_cast<const void*>(k_colour), COLOUR_THREADS, 0));
       HIPCHECK(hipOccupancyMaxActiveBlocksPerMultiprocessor(&bA, reinterpret_cast<const void*>(k_apply), 256, 0));
       const uint32_t cus = (uint32_t)std::max(prop.multiProcessorCount, 1);
       e->nbrBlocks = std::max(8u, (cus * (uint32_t)std::max(bN, 1)) & ~7u);
@@ -2398,7 +2446,7 @@ int sn_load_scenes(sn_engine* e, const sn_scene* scenes, uint32_t count) {
     P.domH = sc.dom_h;
     P.vmax = 65536;
     P.weldHold = 0;
-    P.bounded = bounded ? 1u : 0u;
+    P.bounded = (bounded && e->gBounded) ? 1u : 0u;
     totN += sc.n;
     totC += (uint64_t)gw * gh;
   }
@@ -2602,7 +2650,7 @@ int sn_load_slot(sn_engine* e, uint32_t slot, const sn_scene* sc) {
     P.cellTotal = gw * gh;
     P.domW = sc->dom_w;
     P.domH = sc->dom_h;
-    P.bounded = bounded ? 1u : 0u;
+    P.bounded = (bounded && e->gBounded) ? 1u : 0u;
     e->sp[slot] = P;
     Dev& D = e->D;
     std::vector<int4> ps(sc->n), pt(sc->n);

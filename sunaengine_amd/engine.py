@@ -298,7 +298,7 @@ class Engine:
 
     # kernel classes of sn_profile_step, slot 0 = the whole span
     PROFILE_KEYS = ("total", "predict", "scan", "scatter", "canon_gather", "nbr_solveA", "solveA",
-                    "solveB", "xnc", "st_thermal_hash", "hash_finish", "impulse", "lists", "apply",
+                    "colour", "xnc", "st_thermal_hash", "hash_finish", "impulse", "lists", "apply",
                     "friction", "materials")
 
     def profile_step(self, count: int = 1, classes: Optional[Iterable[str]] = None) -> dict:

@@ -96,6 +96,40 @@ int main() {
     sn::fp_mul_s(a, b, s, g1);
     EXPECT(g1.mul == (sat ? 1u : 0u), "saturation count fp_mul_s(%d,%d,%u)", a, b, s);
   }
+  // the binary64 form: equal value and equal saturation event whenever
+  // |a * b| < 2^53 (b drawn below 2^(53 - bits(a))), every shift, with ties
+  // and both rails; and the fused clamp equals clampi of the saturated value
+  for (int it = 0; it < 600000; ++it) {
+    int32_t a = rnd();
+    int32_t b = rnd();
+    const int abits = a == 0 ? 0 : 64 - __builtin_clzll((uint64_t)(a < 0 ? -(int64_t)a : (int64_t)a));
+    const int room = 53 - abits;   // |b| < 2^room keeps |a b| < 2^53
+    if (room < 31) b = (int32_t)((int64_t)b >> (31 - room));
+    if (it % 5 == 0) {   // an exact tie at half of the last kept bit
+      const unsigned s0 = 1 + (unsigned)(rng() % 31);
+      a = (int32_t)((((int64_t)(rng() % 4001) - 2000) << s0) + ((int64_t)1 << (s0 - 1)));
+      if ((int64_t)a != (((int64_t)(a) >> s0) << s0) + ((int64_t)1 << (s0 - 1))) continue;
+      b = 1;
+    }
+    const i128 v = (i128)a * b;
+    if (v >= ((i128)1 << 53) || v <= -((i128)1 << 53)) continue;
+    const unsigned s = 1 + (unsigned)(rng() % 31);
+    sn::Dbg g1, g2;
+    const int32_t want = sn::fp_mul_s(a, b, s, g1);
+    EXPECT(sn::fp_mul_s_q(a, b, s, g2) == want && g1.mul == g2.mul, "binary64 fp_mul_s(%d,%d,%u)", a, b, s);
+    const int32_t lo = -(int32_t)(rng() % 0x7fffffffu), hi = (int32_t)(rng() % 0x7fffffffu);
+    sn::Dbg g3;
+    const double q = sn::q_rne((double)a * __builtin_ldexp((double)b, -(int)s));
+    EXPECT(sn::q_sat_clamp(q, lo, hi, g3) == sn::clampi(want, lo, hi) && g3.mul == g1.mul,
+           "binary64 clamp(%d,%d,%u,[%d,%d])", a, b, s, lo, hi);
+  }
+  for (unsigned s = 1; s <= 31; ++s)
+    for (int32_t a : {INT32_MIN, INT32_MAX, INT32_MIN + 1, -1, 1, 0})
+      for (int32_t b : {-(1 << 21), (1 << 21) - 1, 1 << 20, -3, 3, 0}) {
+        sn::Dbg g1, g2;
+        EXPECT(sn::fp_mul_s_q(a, b, s, g2) == sn::fp_mul_s(a, b, s, g1) && g1.mul == g2.mul,
+               "binary64 rails(%d,%d,%u)", a, b, s);
+      }
   // exhaustive small operands, all shifts
   for (int32_t a = -300; a <= 300; ++a)
     for (int32_t b = -40; b <= 40; ++b)
