@@ -601,8 +601,8 @@ struct SolveAAcc {
   int32_t rho, S, sgx, sgy;
 };
 // Bounded pairs (BND) take the binary64 forms of fixed.hpp: |d| <= REACH <
-// 2^17.33 per axis and |G| < 2^27 (G_BOUND, checked when the table is
-// uploaded; worlds whose table exceeds it run every scene unbounded), so
+// 2^17.33 per axis and |G| < 2^27 (G_BOUND; pair_bounds checks the world's
+// tables and materials, and a world that fails runs every scene unbounded), so
 // |G d| < 2^44.33 and each gradient is exact with |g| < 2^18.33 (no
 // saturation); g^2 / 2^11 < 2^25.7 is exact and below the i32 rails, so the
 // S terms only take their [0, S_TERM] clamp (g^2 >= 0).
@@ -1554,6 +1554,39 @@ __device__ __forceinline__ void xnc_fluid(const Dev& D, uint32_t p, const int2* 
       if (idx[u] < 0) continue;
       const int4 L = Ls[u], bj = bjs[u];
       const int32_t* Mj = MAT(D, (uint32_t)bj.x);
+      if constexpr (BND) {
+        // binary64 forms; the bounds that make each product exact hold in
+        // bounded scenes of worlds whose tables and materials pass
+        // pair_bounds (checked when the world is created)
+        if (doX) {
+          // |W| < 2^18, 0 <= K <= 2^18: w exact, |w| < 2^20; |dv| <= 2^31:
+          // |dv w| < 2^51; the saturated t times |cij| < 2^22: < 2^53
+          const int32_t K = sym_k(rhoi, rhoj[u], g);
+          const double wf = q_rne((double)L.x * ((double)K * 0x1p-16)) * 0x1p-12;
+          const double cf = (double)mat_mix(ci, Mj[M_XSPH]) * 0x1p-16;
+          const int2 vj = vjs[u];
+          const double tx = (double)q_sat(q_rne((double)fp_sub_sat(vj.x, vi.x, g) * wf), g);
+          const double ty = (double)q_sat(q_rne((double)fp_sub_sat(vj.y, vi.y, g) * wf), g);
+          ax = wadd(ax, q_sat_clamp(q_rne(tx * cf), -XS_TERM, XS_TERM, g));
+          ay = wadd(ay, q_sat_clamp(q_rne(ty * cf), -XS_TERM, XS_TERM, g));
+        }
+        // exact gradients (|g| < 2^18.33) times rinv <= 2^18: < 2^36.33
+        const double gs = (double)L.y * 0x1p-26, rf = (double)rjs[u] * 0x1p-16;
+        sx = wadd(sx, (int32_t)q_rne(q_rne(gs * (double)d[u].x) * rf));
+        sy = wadd(sy, (int32_t)q_rne(q_rne(gs * (double)d[u].y) * rf));
+        // |W| < 2^18, |cond| < 2^26: |wk| < 2^28; |dT| <= 2^24: < 2^52
+        const double wk = q_rne((double)L.x * ((double)mat_mix(ki, Mj[M_COND]) * 0x1p-16));
+        const int32_t dT = clampi(fp_sub_sat(bj.z, Ti, g), -DT_CLAMP, DT_CLAMP);
+        q = wadd(q, q_sat_clamp(q_rne((double)dT * wk * 0x1p-12), -HT_TERM, HT_TERM, g));
+        if ((uint32_t)bj.x != mi) {
+          const int32_t attack = Mj[M_CORRODE];
+          const int32_t spend = (Mj[M_CORRPT] != CORR_NEVER) ? corri : 0;
+          const int32_t str = wadd(attack, spend);
+          // |W| < 2^18, |str| <= 2^31: < 2^49
+          if (str != 0) c = wadd(c, q_sat_clamp(q_rne((double)L.x * ((double)str * 0x1p-12)), 0, CORR_TERM, g));
+        }
+        continue;
+      }
       if (doX) {
         int32_t K = sym_k(rhoi, rhoj[u], g);
         int32_t w = fp_mul_s(L.x, K, 16, g);
@@ -1647,6 +1680,35 @@ __device__ __forceinline__ int2 st_sum(const Dev& D, uint32_t p, const int32_t* 
     for (uint32_t u = 0; u < XNC_UNROLL; ++u) {
     if (idx[u] < 0) continue;
     const int2 d1 = d[u];
+    if constexpr (BND) {
+      // binary64 forms (bounds as in xnc_fluid): |d| < 2^17.33 per axis
+      if (e[u] & NBR_BND) {
+        // |A d| < 2^48.33, exact with |a| < 2^26.33; times |adh| < 2^22: < 2^48.33
+        const double as = (double)ats[u] * 0x1p-22, hf = (double)M[M_ADH] * 0x1p-8;
+        ax = wsub(ax, q_sat_clamp(q_rne(q_rne(as * (double)d1.x) * hf), -ST_TERM, ST_TERM, g));
+        ay = wsub(ay, q_sat_clamp(q_rne(q_rne(as * (double)d1.y) * hf), -ST_TERM, ST_TERM, g));
+        continue;
+      }
+      const int32_t K = sym_k(rhoi, rhoj[u], g);
+      const int32_t* Mj = MAT(D, (uint32_t)matj[u]);
+      // |C d| < 2^48.33 exact with |c| < 2^26.33; times |gCoh| < 2^22: < 2^48.33
+      const double cs = (double)cts[u] * 0x1p-22;
+      const double hf = (double)mat_mix(M[M_GCOH], Mj[M_GCOH]) * 0x1p-16;
+      int32_t fx = fp_neg_sat(q_sat(q_rne(q_rne(cs * (double)d1.x) * hf), g), g);
+      int32_t fy = fp_neg_sat(q_sat(q_rne(q_rne(cs * (double)d1.y) * hf), g), g);
+      // |dn| <= 2^31 times |gCur| < 2^22: < 2^53
+      const int2 nj = njs[u];
+      const double uf = (double)mat_mix(M[M_GCUR], Mj[M_GCUR]) * 0x1p-16;
+      const int32_t cx = q_sat(q_rne((double)fp_sub_sat(ni.x, nj.x, g) * uf), g);
+      const int32_t cy = q_sat(q_rne((double)fp_sub_sat(ni.y, nj.y, g) * uf), g);
+      fx = fp_add_sat(fx, fp_neg_sat(cx, g), g);
+      fy = fp_add_sat(fy, fp_neg_sat(cy, g), g);
+      // |f| <= 2^31 times 0 <= K <= 2^18: < 2^49
+      const double kf = (double)K * 0x1p-8;
+      ax = wadd(ax, q_sat_clamp(q_rne((double)fx * kf), -ST_TERM, ST_TERM, g));
+      ay = wadd(ay, q_sat_clamp(q_rne((double)fy * kf), -ST_TERM, ST_TERM, g));
+      continue;
+    }
     if (e[u] & NBR_BND) {
       int32_t at = ats[u];
       ax = wsub(ax, clampi(fp_mul_s(fp_mul_s(at, d1.x, 22, g), M[M_ADH], 16 - ACC, g), -ST_TERM, ST_TERM));
@@ -1967,7 +2029,7 @@ struct sn_engine {
   uint32_t weldSeq = 1;  // the 12-bit spawn-event id of the last weld group (`_weldSeq`, src/engine.js:3801)
   bool gen = false;      // the material passes run (D.gen); never switched off again
   uint32_t graphG = 8;   // substeps per captured hipGraph (sn_autotune may change it)
-  bool gBounded = true;  // every G entry of the table is below G_BOUND in magnitude (bounded pair forms)
+  bool pairBounded = true;  // the tables and materials meet pair_bounds (bounded pair forms)
   // profiling: an event pair around every launch, tagged with its class
   std::vector<hipEvent_t> evPool;
   std::vector<int> evClass;
@@ -2255,6 +2317,25 @@ int check_fault(sn_engine* e) {
 // =================================================================== C ABI
 extern "C" {
 
+// The operand bounds behind the binary64 pair forms (solveA_term, solveB_part,
+// xnc_fluid, st_sum): |G| < 2^27 and |W| < 2^18 in the kernel tables, and in
+// every material row |xsph|, |gCoh|, |gCur|, |adhesion| < 2^22 and
+// |conductivity| < 2^26 (mat_mix of two such values stays below the same
+// bound).  The frozen image and material table meet them; a world whose
+// tables or materials do not runs every scene with the integer forms.
+bool pair_bounds(const int32_t* luts, const int32_t* mats) {
+  auto below = [](int32_t v, int32_t b) { return v > -b && v < b; };
+  for (uint32_t i = 0; i < TN; ++i)
+    if (!below(luts[TN + i], G_BOUND) || !below(luts[i], 1 << 18)) return false;
+  for (uint32_t m = 0; m < MAT_MAX; ++m) {
+    const int32_t* M = mats + (size_t)m * MAT_WORDS;
+    if (!below(M[M_XSPH], 1 << 22) || !below(M[M_GCOH], 1 << 22) || !below(M[M_GCUR], 1 << 22) ||
+        !below(M[M_ADH], 1 << 22) || !below(M[M_COND], 1 << 26))
+      return false;
+  }
+  return true;
+}
+
 int sn_create(const sn_opts* opts, const int32_t* luts, const int32_t* mats, sn_engine** out) {
   if (!opts || !luts || !mats || !out) return SN_E_ARG;
   *out = nullptr;
@@ -2327,10 +2408,8 @@ int sn_create(const sn_opts* opts, const int32_t* luts, const int32_t* mats, sn_
     int4* lut4 = dalloc<int4>(e, TN);
     int32_t* lutA = dalloc<int32_t>(e, TN);
     std::vector<int4> h4(TN);
-    for (uint32_t i = 0; i < TN; ++i) {
-      h4[i] = make_int4(luts[i], luts[TN + i], luts[2 * TN + i], luts[4 * TN + i]);
-      e->gBounded = e->gBounded && luts[TN + i] > -G_BOUND && luts[TN + i] < G_BOUND;
-    }
+    for (uint32_t i = 0; i < TN; ++i) h4[i] = make_int4(luts[i], luts[TN + i], luts[2 * TN + i], luts[4 * TN + i]);
+    e->pairBounded = pair_bounds(luts, mats);
     HIPCHECK(hipMemcpy(lut4, h4.data(), sizeof(int4) * TN, hipMemcpyHostToDevice));
     HIPCHECK(hipMemcpy(lutA, luts + 3 * TN, sizeof(int32_t) * TN, hipMemcpyHostToDevice));
     D.lut4 = lut4;
@@ -2446,7 +2525,7 @@ int sn_load_scenes(sn_engine* e, const sn_scene* scenes, uint32_t count) {
     P.domH = sc.dom_h;
     P.vmax = 65536;
     P.weldHold = 0;
-    P.bounded = (bounded && e->gBounded) ? 1u : 0u;
+    P.bounded = (bounded && e->pairBounded) ? 1u : 0u;
     totN += sc.n;
     totC += (uint64_t)gw * gh;
   }
@@ -2650,7 +2729,7 @@ int sn_load_slot(sn_engine* e, uint32_t slot, const sn_scene* sc) {
     P.cellTotal = gw * gh;
     P.domW = sc->dom_w;
     P.domH = sc->dom_h;
-    P.bounded = (bounded && e->gBounded) ? 1u : 0u;
+    P.bounded = (bounded && e->pairBounded) ? 1u : 0u;
     e->sp[slot] = P;
     Dev& D = e->D;
     std::vector<int4> ps(sc->n), pt(sc->n);
