@@ -1393,23 +1393,30 @@ __device__ __forceinline__ ColItem col_item(uint2 item) {
 constexpr uint32_t SB_E = MAXNBR / 8;
 static_assert(COLOUR_THREADS / CELL_CHUNK >= 8u && SB_E * 8u >= MAXNBR, "k_colour gives every particle at least 8 lanes");
 static_assert(CELL_CHUNK <= COL_COUNT, "an item fits the count field");
-template <bool BND>
+// S0: the world's S table is all zero (the frozen image's is), so the LDS
+// table holds G alone (12.8 KB instead of 25.6 KB: more workgroups per CU
+// to hide the gathers).  The list words are loaded without waiting for the
+// count (every k < MAXNBR is inside the list array) and replaced by the
+// particle itself past it, so the count, the lists and the particle's own
+// data arrive in one round trip.
+template <bool BND, bool S0>
 __device__ __forceinline__ int2 solveB_part(const Dev& D, uint32_t p, uint32_t sub, uint32_t L,
-                                            const int2* lutGS, Dbg& g) {
+                                            const int32_t* lutT, Dbg& g) {
+  const uint32_t* nb = D.nbr + p;
+  uint32_t e[SB_E];
+#pragma unroll
+  for (uint32_t u = 0; u < SB_E; ++u) {
+    const uint32_t k = min(sub + u * L, MAXNBR - 1u);
+    e[u] = nb[(size_t)k * D.NS];
+  }
   const int2 pi = D.pred[p];
   const int32_t li = D.lam[p];
   const uint32_t cnt = D.nbrN[p] & 0x1fffffffu;
-  const uint32_t* nb = D.nbr + p;
-  uint32_t e[SB_E];
   int2 pj[SB_E];
   int32_t lj[SB_E];
 #pragma unroll
   for (uint32_t u = 0; u < SB_E; ++u) {
-    const uint32_t k = sub + u * L;
-    e[u] = k < cnt ? nb[(size_t)k * D.NS] : p;
-  }
-#pragma unroll
-  for (uint32_t u = 0; u < SB_E; ++u) {
+    if (sub + u * L >= cnt) e[u] = p;
     const uint32_t q = e[u] & ~NBR_BND;
     pj[u] = D.pred[q];
     lj[u] = D.lam[q];
@@ -1421,7 +1428,7 @@ __device__ __forceinline__ int2 solveB_part(const Dev& D, uint32_t p, uint32_t s
     const int2 d = pdelta<BND>(pi, pj[u], g);
     const int32_t ix = lut_idx<BND>(d, g);
     if (ix < 0) continue;
-    const int2 T = lutGS[ix];
+    const int2 T = S0 ? make_int2(lutT[ix], 0) : reinterpret_cast<const int2*>(lutT)[ix];
     const int32_t co = fp_add_sat(fp_add_sat(li, (e[u] & NBR_BND) ? li : lj[u], g), T.y, g);
     if constexpr (BND) {
       // binary64 forms (bounds as in solveA_term): exact gradients with
@@ -1437,7 +1444,6 @@ __device__ __forceinline__ int2 solveB_part(const Dev& D, uint32_t p, uint32_t s
   }
   return make_int2(dpx, dpy);
 }
-
 
 // applyDp for fluid particle p of scene P (src/sim.wgsl:1458-1474), then on
 // the last iteration finalize (src/sim.wgsl:2245-2265).
@@ -1464,11 +1470,13 @@ __device__ __forceinline__ void apply_at(const Dev& D, const SceneP& P, uint32_t
   }
 }
 
+template <bool S0>
 __global__ __launch_bounds__(COLOUR_THREADS) void k_colour(Dev D, uint32_t colour) {
-  __shared__ int2 lutGS[TN];
+  __shared__ int32_t lutT[S0 ? TN : 2 * TN];   // G, or (G, S) pairs
   for (uint32_t i = threadIdx.x; i < TN; i += COLOUR_THREADS) {
     const int4 L = D.lut4[i];
-    lutGS[i] = make_int2(L.y, L.w);
+    if (S0) lutT[i] = L.y;
+    else reinterpret_cast<int2*>(lutT)[i] = make_int2(L.y, L.w);
   }
   __syncthreads();
   const uint32_t n = (uint32_t)__builtin_amdgcn_readfirstlane((int)D.listCount[colour]);
@@ -1484,7 +1492,7 @@ __global__ __launch_bounds__(COLOUR_THREADS) void k_colour(Dev D, uint32_t colou
     const uint32_t p = it.lo + pl;
     const bool act = pl < it.m && (D.info[p] & INF_FLUID);
     int2 dp = make_int2(0, 0);
-    if (act) dp = bnd ? solveB_part<true>(D, p, sub, L, lutGS, g) : solveB_part<false>(D, p, sub, L, lutGS, g);
+    if (act) dp = bnd ? solveB_part<true, S0>(D, p, sub, L, lutT, g) : solveB_part<false, S0>(D, p, sub, L, lutT, g);
     for (uint32_t off = 1; off < L; off <<= 1) {
       dp.x = wadd(dp.x, __shfl_xor(dp.x, (int)off));
       dp.y = wadd(dp.y, __shfl_xor(dp.y, (int)off));
@@ -2030,6 +2038,7 @@ struct sn_engine {
   bool gen = false;      // the material passes run (D.gen); never switched off again
   uint32_t graphG = 8;   // substeps per captured hipGraph (sn_autotune may change it)
   bool pairBounded = true;  // the tables and materials meet pair_bounds (bounded pair forms)
+  bool sZero = true;        // the S kernel table is all zero (k_colour's LDS table holds G alone)
   // profiling: an event pair around every launch, tagged with its class
   std::vector<hipEvent_t> evPool;
   std::vector<int> evClass;
@@ -2201,7 +2210,8 @@ void enqueue_substep(sn_engine* e) {
     for (uint32_t c = 0; c < 4; ++c) {
       {
         ProfScope ps(e, PC_COLOUR);
-        hipLaunchKernelGGL(k_colour, dim3(e->colourBlocks), dim3(COLOUR_THREADS), 0, st, D, c);
+        if (e->sZero) hipLaunchKernelGGL(k_colour<true>, dim3(e->colourBlocks), dim3(COLOUR_THREADS), 0, st, D, c);
+        else hipLaunchKernelGGL(k_colour<false>, dim3(e->colourBlocks), dim3(COLOUR_THREADS), 0, st, D, c);
       }
       ProfScope ps(e, PC_APPLY);
       hipLaunchKernelGGL(k_apply, dim3(e->applyBlocks), dim3(256), 0, st, D, c,
@@ -2410,6 +2420,8 @@ int sn_create(const sn_opts* opts, const int32_t* luts, const int32_t* mats, sn_
     std::vector<int4> h4(TN);
     for (uint32_t i = 0; i < TN; ++i) h4[i] = make_int4(luts[i], luts[TN + i], luts[2 * TN + i], luts[4 * TN + i]);
     e->pairBounded = pair_bounds(luts, mats);
+    e->sZero = true;
+    for (uint32_t i = 0; i < TN; ++i) e->sZero = e->sZero && luts[4 * TN + i] == 0;
     HIPCHECK(hipMemcpy(lut4, h4.data(), sizeof(int4) * TN, hipMemcpyHostToDevice));
     HIPCHECK(hipMemcpy(lutA, luts + 3 * TN, sizeof(int32_t) * TN, hipMemcpyHostToDevice));
     D.lut4 = lut4;
@@ -2444,7 +2456,9 @@ int sn_create(const sn_opts* opts, const int32_t* luts, const int32_t* mats, sn_
       HIPCHECK(hipGetDeviceProperties(&prop, (int)e->opts.device));
       int bN = 0, bC = 0, bA = 0;
       HIPCHECK(hipOccupancyMaxActiveBlocksPerMultiprocessor(&bN, reinterpret_cast<const void*>(k_nbr_tile), 256, 0));
-      HIPCHECK(hipOccupancyMaxActiveBlocksPerMultiprocessor(&bC, reinterpret_cast<const void*>(k_colour), COLOUR_THREADS, 0));
+      HIPCHECK(hipOccupancyMaxActiveBlocksPerMultiprocessor(&bC, e->sZero ? reinterpret_cast<const void*>(k_colour<true>)
+                                                                       : reinterpret_cast<const void*>(k_colour<false>),
+                                                              COLOUR_THREADS, 0));
       HIPCHECK(hipOccupancyMaxActiveBlocksPerMultiprocessor(&bA, reinterpret_cast<const void*>(k_apply), 256, 0));
       const uint32_t cus = (uint32_t)std::max(prop.multiProcessorCount, 1);
       e->nbrBlocks = std::max(8u, (cus * (uint32_t)std::max(bN, 1)) & ~7u);
