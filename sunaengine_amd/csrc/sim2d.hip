@@ -1068,7 +1068,7 @@ __device__ __forceinline__ bool nbr_chunk_staged(const Dev& D, uint16_t* lst, ui
   const int2 pi = valid ? D.pred[p] : make_int2(0, 0);
   const int32_t cxk = valid ? (int32_t)(D.cellY[p] - myRow) : 0;
   const bool live = valid && !(inf & INF_DEAD);
-  uint32_t cnt = 0, bits = 0, dropped = 0;
+  uint32_t cnt = 0, bits = 0, dropped = 0, nin = 0;   // nin: in-range candidates taken or dropped
   int32_t bx0 = 0, bx1 = 0, by0 = 0, by1 = 0;
   if constexpr (BND) {
     bx0 = wave_min(live ? pi.x : I32_MAX) - REACH;
@@ -1091,6 +1091,18 @@ __device__ __forceinline__ bool nbr_chunk_staged(const Dev& D, uint16_t* lst, ui
     if (rhi[r] - rlo[r] > ROW_OFF_MAX) return false;
   }
   const bool stopFull = !D.dbgOn && D.sd[s].flagged == 0u;   // as in nbr_chunk
+  // The candidate loops below are branch-free per candidate: a lane counts
+  // its in-range candidates in nin (its list keeps the first MAXNBR, so cnt =
+  // min(nin, MAXNBR) and the truncation count is nin - cnt), and every lane
+  // stores the candidate's entry at its own next slot, lst[min(nin,
+  // MAXNBR)][lane], whether or not it takes the candidate: a slot past the
+  // lane's count is never read and the next accepted candidate overwrites
+  // it, and a full list writes the spare row MAXNBR.  Per candidate the wave then issues a few
+  // scalar instructions instead of a branch per condition (the scalar unit,
+  // shared by the CU's SIMDs, was the limit).  In the one-cell form (!WIN)
+  // the radius test takes |a - b| + 8 of positions biased by 2^31 (unsigned
+  // order equals signed order, and |d| < 2^19), one v_sad_u32 per axis.
+  const uint32_t pux = (uint32_t)pi.x ^ 0x80000000u, puy = (uint32_t)pi.y ^ 0x80000000u;
   bool done = false;
 #pragma unroll
   for (uint32_t r = 0; r < 3; ++r) {
@@ -1118,46 +1130,44 @@ __device__ __forceinline__ bool nbr_chunk_staged(const Dev& D, uint16_t* lst, ui
       const uint64_t mBnd = __ballot(lane < qn && !(ci & INF_FLUID));
       const uint64_t mSeal = __ballot(lane < qn && (ci & INF_SEAL));
       const uint64_t mCong = __ballot(lane < qn && (ci & INF_CONGEAL));
+      const bool flagged = (mSeal | mCong) != 0ull;   // uniform: no SEAL/CONGEAL candidate in this block
       const uint32_t selfj = p - q0;                 // lane index of the particle itself, if in this block
-      if (__all(!live || cnt >= MAXNBR)) {
-        // every list full: count and flag only, branch-free (as in nbr_chunk)
-        if (!D.dbgOn) todo &= mSeal | mCong;
-        while (todo) {
-          const uint32_t j = (uint32_t)__builtin_ctzll(todo);
-          todo &= todo - 1ull;
-          const int2 cj = make_int2(__builtin_amdgcn_readlane(cp.x, j), __builtin_amdgcn_readlane(cp.y, j));
-          const int32_t qc = WIN ? __builtin_amdgcn_readlane(cc, j) : 0;
-          const int2 d = pdelta<BND>(pi, cj, g);
-          const int32_t idx = (BND && !WIN) ? lut_idx_near(d) : lut_idx<BND>(d, g);
-          bool a = live && j != selfj && idx >= 0;
-          if (WIN) a = a && qc >= cxk - 1 && qc <= cxk + 1;
-          dropped += a ? 1u : 0u;
-          bits |= a ? ((((mSeal >> j) & 1ull) ? INF_SEAL : 0u) | (((mCong >> j) & 1ull) ? INF_CONGEAL : 0u)) : 0u;
-        }
-        continue;
-      }
+      const uint32_t cux = (uint32_t)cp.x ^ 0x80000000u, cuy = (uint32_t)cp.y ^ 0x80000000u;
+      const bool allFull = __all(!live || cnt >= MAXNBR);
+      // every list full: only `dropped` (debug counters) and the SEAL /
+      // CONGEAL seen bits can change, so without debug counters only
+      // candidates carrying those bits are left
+      if (allFull && !D.dbgOn) todo &= mSeal | mCong;
+      // this lane's candidate as a list entry, read per candidate with one lane read
+      const uint32_t ent = (r << 14) + q0 - rlo[r] + lane + (((mBnd >> lane) & 1ull) ? (1u << 13) : 0u);
       while (todo) {
         const uint32_t j = (uint32_t)__builtin_ctzll(todo);
-        todo &= todo - 1ull;
-        const int32_t px = __builtin_amdgcn_readlane(cp.x, j);
-        const int32_t py = __builtin_amdgcn_readlane(cp.y, j);
-        const int32_t qc = WIN ? __builtin_amdgcn_readlane(cc, j) : 0;
-        const int2 d = pdelta<BND>(pi, make_int2(px, py), g);
-        const int32_t idx = (BND && !WIN) ? lut_idx_near(d) : lut_idx<BND>(d, g);
-        bool acc = live && j != selfj && idx >= 0;
-        if (WIN) acc = acc && qc >= cxk - 1 && qc <= cxk + 1;
-        if (!acc) continue;
-        bits |= (((mSeal >> j) & 1ull) ? INF_SEAL : 0u) | (((mCong >> j) & 1ull) ? INF_CONGEAL : 0u);
-        if (cnt >= MAXNBR) {
-          dropped++;
-          continue;
+        todo ^= 1ull << j;
+        bool inr;
+        if constexpr (BND && !WIN) {
+          const uint32_t jx = (uint32_t)__builtin_amdgcn_readlane((int)cux, j);
+          const uint32_t jy = (uint32_t)__builtin_amdgcn_readlane((int)cuy, j);
+          const uint32_t ax = (max(pux, jx) - min(pux, jx) + 8u) >> 4;
+          const uint32_t ay = (max(puy, jy) - min(puy, jy) + 8u) >> 4;
+          inr = __umul24(ax, ax) + __umul24(ay, ay) < (uint32_t)H2S;
+        } else {
+          const int2 cj = make_int2(__builtin_amdgcn_readlane(cp.x, j), __builtin_amdgcn_readlane(cp.y, j));
+          inr = lut_idx<BND>(pdelta<BND>(pi, cj, g), g) >= 0;
         }
-        const uint32_t off = q0 - rlo[r] + j;
-        lst[cnt * 64u + lane] = (uint16_t)((r << 14) | (((mBnd >> j) & 1ull) ? (1u << 13) : 0u) | off);
-        cnt++;
+        bool acc = live && j != selfj && inr;
+        if constexpr (WIN) {
+          const int32_t qc = __builtin_amdgcn_readlane(cc, j);
+          acc = acc && qc >= cxk - 1 && qc <= cxk + 1;
+        }
+        if (flagged)
+          bits |= acc ? ((((mSeal >> j) & 1ull) ? INF_SEAL : 0u) | (((mCong >> j) & 1ull) ? INF_CONGEAL : 0u)) : 0u;
+        if (!allFull) lst[min(nin, MAXNBR) * 64u + lane] = (uint16_t)__builtin_amdgcn_readlane((int)ent, j);
+        nin += acc ? 1u : 0u;
       }
+      cnt = min(nin, MAXNBR);
     }
   }
+  dropped = nin - cnt;
   // write-out (coalesced per k) and iteration 0 of solveA over the lists,
   // NBR_UNROLL entries at a time with their gathers issued together
   SolveAAcc acc{W0, 0, 0, 0};
@@ -1285,7 +1295,7 @@ __device__ __forceinline__ void nbr_item(const Dev& D, uint2 item, uint16_t* lst
 }
 
 __global__ __launch_bounds__(256) void k_nbr_tile(Dev D) {
-  __shared__ uint16_t lists[4][MAXNBR * 64];
+  __shared__ uint16_t lists[4][(MAXNBR + 1) * 64];   // a spare row per wave: see nbr_chunk_staged
   const uint32_t lane = threadIdx.x & 63u;
   uint16_t* lst = lists[threadIdx.x >> 6];
   const int32_t W0 = D.lut4[0].x;
