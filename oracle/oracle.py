@@ -55,6 +55,7 @@ def _lib():
         L.or_spawn.argtypes = [P, C.c_uint32, i32p, i32p]
         L.or_friction_probe.argtypes = [P, np.ctypeslib.ndpointer(np.int64, flags="C")]
         L.or_nbr_probe.argtypes = [P, i32p, u32p, u32p, C.POINTER(C.c_uint32)]
+        L.or_friction_dp.argtypes = [P, i32p, i32p]
         _LIB = L
     return _LIB
 
@@ -278,6 +279,17 @@ def nbr_probe(world: "OracleWorld") -> dict:
     dropped = C.c_uint32()
     _lib().or_nbr_probe(world._h, pred, nbr, nbrn, C.byref(dropped))
     return {"pred": pred.reshape(-1, 2), "nbr": nbr.reshape(-1, 48), "nbrN": nbrn, "dropped": int(dropped.value)}
+
+
+def friction_dp(world: "OracleWorld", n_fluid: int) -> dict:
+    """Run the next substep of `world` up to its first friction pass and
+    return each fluid slot's friction-pass dp (ticks << ACC, friction plus
+    the bond gather, before applyFric) and pred, both (n_fluid, 2).  The
+    world is left mid-substep."""
+    dp = np.zeros(2 * n_fluid, np.int32)
+    pred = np.zeros(2 * n_fluid, np.int32)
+    _lib().or_friction_dp(world._h, dp, pred)
+    return {"dp": dp.reshape(n_fluid, 2), "pred": pred.reshape(n_fluid, 2)}
 
 
 def friction_probe(world: "OracleWorld", n_fluid: int) -> dict:

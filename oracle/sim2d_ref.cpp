@@ -1200,6 +1200,32 @@ void or_friction_probe(or_world* w, int64_t out[5]) {
   out[0] = sx; out[1] = sy; out[2] = pairs; out[3] = bnd; out[4] = full;
 }
 
+// Test probe: the next substep up to its first friction pass (predict, grid,
+// buildNbr, iteration 0's solveA and four colour passes, then friction with
+// the bond gather, src/sim.wgsl:1673-1934), and each fluid slot's friction
+// dp (2 words, ticks << ACC, before applyFric) and pred (2 words).  The world
+// is left mid-substep (only for tests that discard it).
+void or_friction_dp(or_world* w, int32_t* dp2, int32_t* pred2) {
+  World& W = *reinterpret_cast<World*>(w);
+  g_dbg = W.dbg;
+  const Particle* sin = W.state[W.parity].data();
+  predict(W, sin);
+  build_grid(W);
+  build_nbr(W, sin);
+  solve_a(W, sin);
+  for (uint32_t c = 0; c < 4; ++c) {
+    solve_b(W, sin, c);
+    apply_dp(W, sin, c);
+  }
+  friction(W, sin);
+  for (uint32_t i = 0; i < W.nFluid; ++i) {
+    dp2[2 * i] = W.derived[i].dp.x;
+    dp2[2 * i + 1] = W.derived[i].dp.y;
+    pred2[2 * i] = W.derived[i].pred.x;
+    pred2[2 * i + 1] = W.derived[i].pred.y;
+  }
+}
+
 // Test probe: the next substep's predict, grid and buildNbr only, then
 // pred (2 words per slot), the neighbour lists (MAXNBR words per slot, in
 // list order), nbrN (count | seal/ext-seal/congeal bits) and the truncation
