@@ -56,6 +56,7 @@ def _lib():
         L.or_friction_probe.argtypes = [P, np.ctypeslib.ndpointer(np.int64, flags="C")]
         L.or_nbr_probe.argtypes = [P, i32p, u32p, u32p, C.POINTER(C.c_uint32)]
         L.or_friction_dp.argtypes = [P, i32p, i32p]
+        L.or_contain_probe.argtypes = [P, i32p, i32p]
         _LIB = L
     return _LIB
 
@@ -290,6 +291,16 @@ def friction_dp(world: "OracleWorld", n_fluid: int) -> dict:
     pred = np.zeros(2 * n_fluid, np.int32)
     _lib().or_friction_dp(world._h, dp, pred)
     return {"dp": dp.reshape(n_fluid, 2), "pred": pred.reshape(n_fluid, 2)}
+
+
+def contain_probe(world: "OracleWorld", n_fluid: int) -> dict:
+    """Run the next substep of `world` through its solver iterations and
+    return each fluid slot's pred just before and just after the containment
+    passes, both (n_fluid, 2).  The world is left mid-substep."""
+    before = np.zeros(2 * n_fluid, np.int32)
+    after = np.zeros(2 * n_fluid, np.int32)
+    _lib().or_contain_probe(world._h, before, after)
+    return {"before": before.reshape(n_fluid, 2), "after": after.reshape(n_fluid, 2)}
 
 
 def friction_probe(world: "OracleWorld", n_fluid: int) -> dict:

@@ -1226,6 +1226,38 @@ void or_friction_dp(or_world* w, int32_t* dp2, int32_t* pred2) {
   }
 }
 
+// Test probe: the next substep through its solver iterations, then pred of
+// every fluid slot (2 words) just before contain + containOverflow
+// (src/sim.wgsl:2056-2235) and just after them.  The world is left
+// mid-substep (only for tests that discard it).
+void or_contain_probe(or_world* w, int32_t* before2, int32_t* after2) {
+  World& W = *reinterpret_cast<World*>(w);
+  g_dbg = W.dbg;
+  const Particle* sin = W.state[W.parity].data();
+  predict(W, sin);
+  build_grid(W);
+  build_nbr(W, sin);
+  for (int it = 0; it < W.iters; ++it) {
+    solve_a(W, sin);
+    for (uint32_t c = 0; c < 4; ++c) {
+      solve_b(W, sin, c);
+      apply_dp(W, sin, c);
+    }
+    friction(W, sin);
+    apply_fric(W, sin);
+  }
+  for (uint32_t i = 0; i < W.nFluid; ++i) {
+    before2[2 * i] = W.derived[i].pred.x;
+    before2[2 * i + 1] = W.derived[i].pred.y;
+  }
+  contain(W, sin);
+  contain_overflow(W, sin);
+  for (uint32_t i = 0; i < W.nFluid; ++i) {
+    after2[2 * i] = W.derived[i].pred.x;
+    after2[2 * i + 1] = W.derived[i].pred.y;
+  }
+}
+
 // Test probe: the next substep's predict, grid and buildNbr only, then
 // pred (2 words per slot), the neighbour lists (MAXNBR words per slot, in
 // list order), nbrN (count | seal/ext-seal/congeal bits) and the truncation
