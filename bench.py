@@ -451,11 +451,16 @@ def run_serve(args):
         st = srv.run(jobs)
         summ = serve.summarize(jobs, st["wall_s"])
         summ["rate_jobs_per_s"] = rate
+        summ["world_busy_s"] = {k: round(v, 3) for k, v in st["busy_s_by_world"].items()}
+        summ["world_rate"] = {k: (st["round_work_by_world"][k] / v if v > 0 else 0.0)
+                              for k, v in st["busy_s_by_world"].items()}
         per.append(summ)
         work += summ["particle_substeps"]
         busy += st["wall_s"]
         print(f"# rate {rate}/s: {summ['jobs']} jobs, {summ['throughput']:.3e} ps/s, "
-              f"p50 {summ['p50_latency_s']:.2f}s p99 {summ['p99_latency_s']:.2f}s", file=sys.stderr, flush=True)
+              f"p50 {summ['p50_latency_s']:.2f}s p99 {summ['p99_latency_s']:.2f}s, world busy "
+              f"{summ['world_busy_s']} s, world rate {dict((k, f'{v:.3e}') for k, v in summ['world_rate'].items())}",
+              file=sys.stderr, flush=True)
     srv.close()
     value = work / busy
     cfg = {

@@ -870,7 +870,7 @@ __device__ __forceinline__ void nbr_tail(const Dev& D, const SceneP& P, int32_t 
   }
 }
 
-template <bool BND, bool WIN, int PPL>
+template <bool BND, bool WIN, int PPL, bool STRIP>
 __device__ __forceinline__ void nbr_chunk(const Dev& D, uint32_t c0, uint32_t ncell, uint32_t s,
                                           uint32_t plo, uint32_t phi, uint32_t lane, int32_t W0, Dbg& g,
                                           const uint32_t* perm) {
@@ -883,7 +883,7 @@ __device__ __forceinline__ void nbr_chunk(const Dev& D, uint32_t c0, uint32_t nc
   const uint32_t myRow = cellBase + (uint32_t)cy * gridW;
   // without debug counters, and with no SEAL/CONGEAL particle in the scene,
   // a wave whose lists are all full has nothing left to find
-  const bool stopFull = !D.dbgOn && D.sd[s].flagged == 0u;
+  const bool stopFull = STRIP && D.sd[s].flagged == 0u;
   for (uint32_t pbase = plo; pbase < phi; pbase += 64u * PPL) {
     uint32_t pk[PPL], inf[PPL], cnt[PPL], bits[PPL], dropped[PPL];
     int2 pik[PPL];
@@ -1051,7 +1051,7 @@ __device__ __forceinline__ void nbr_chunk(const Dev& D, uint32_t c0, uint32_t nc
 // 13-bit offset; the caller then takes nbr_chunk.
 constexpr uint32_t ROW_OFF_MAX = 1u << 13;
 
-template <bool BND, bool WIN>
+template <bool BND, bool WIN, bool STRIP>
 __device__ __forceinline__ bool nbr_chunk_staged(const Dev& D, uint16_t* lst, uint32_t c0,
                                                  uint32_t ncell, uint32_t s, uint32_t plo, uint32_t phi,
                                                  uint32_t lane, int32_t W0, Dbg& g, const uint32_t* perm) {
@@ -1090,7 +1090,7 @@ __device__ __forceinline__ bool nbr_chunk_staged(const Dev& D, uint16_t* lst, ui
     rhi[r] = (uint32_t)((int32_t)D.cellStart[rowc + (uint32_t)xb + 1u] + sh);
     if (rhi[r] - rlo[r] > ROW_OFF_MAX) return false;
   }
-  const bool stopFull = !D.dbgOn && D.sd[s].flagged == 0u;   // as in nbr_chunk
+  const bool stopFull = STRIP && D.sd[s].flagged == 0u;   // as in nbr_chunk
   // The candidate loops below are branch-free per candidate: a lane counts
   // its in-range candidates in nin (its list keeps the first MAXNBR, so cnt =
   // min(nin, MAXNBR) and the truncation count is nin - cnt), and every lane
@@ -1259,6 +1259,7 @@ __global__ __launch_bounds__(256) void k_heavy_order(Dev D) {
   }
 }
 
+template <bool STRIP>
 __device__ __forceinline__ void nbr_item(const Dev& D, uint2 item, uint16_t* lst, uint32_t lane, int32_t W0,
                                          bool heavy) {
   const uint32_t tix = __builtin_amdgcn_readfirstlane(item.x);
@@ -1282,19 +1283,23 @@ __device__ __forceinline__ void nbr_item(const Dev& D, uint2 item, uint16_t* lst
     // one-cell tiles of cells no wider than 4 wu: every particle's own
     // stencil is the whole candidate union (no window) and |d| < 2^19
     if (ncell == 1u && D.sp[s].cellShift <= 18u) {
-      if (!(one && nbr_chunk_staged<true, false>(D, lst, c0, ncell, s, plo, phi, lane, W0, g, perm)))
-        nbr_chunk<true, false, NBR_PPL>(D, c0, ncell, s, plo, phi, lane, W0, g, perm);
+      if (!(one && nbr_chunk_staged<true, false, STRIP>(D, lst, c0, ncell, s, plo, phi, lane, W0, g, perm)))
+        nbr_chunk<true, false, NBR_PPL, STRIP>(D, c0, ncell, s, plo, phi, lane, W0, g, perm);
     } else {
-      if (!(one && nbr_chunk_staged<true, true>(D, lst, c0, ncell, s, plo, phi, lane, W0, g, perm)))
-        nbr_chunk<true, true, NBR_PPL>(D, c0, ncell, s, plo, phi, lane, W0, g, perm);
+      if (!(one && nbr_chunk_staged<true, true, STRIP>(D, lst, c0, ncell, s, plo, phi, lane, W0, g, perm)))
+        nbr_chunk<true, true, NBR_PPL, STRIP>(D, c0, ncell, s, plo, phi, lane, W0, g, perm);
     }
   } else {
-    nbr_chunk<false, true, NBR_PPL>(D, c0, ncell, s, plo, phi, lane, W0, g, perm);
+    nbr_chunk<false, true, NBR_PPL, STRIP>(D, c0, ncell, s, plo, phi, lane, W0, g, perm);
   }
   flush_dbg(D, s, g, 0);
 }
 
-__global__ __launch_bounds__(256) void k_nbr_tile(Dev D) {
+// STRIP: the build without debug counters (a full list ends a particle's
+// sweep, with the per-particle tails); the debug build's kernel carries
+// none of that code, so its register budget is its own.
+template <bool STRIP>
+__global__ __launch_bounds__(256, 6) void k_nbr_tile(Dev D) {
   __shared__ uint16_t lists[4][(MAXNBR + 1) * 64];   // a spare row per wave: see nbr_chunk_staged
   const uint32_t lane = threadIdx.x & 63u;
   uint16_t* lst = lists[threadIdx.x >> 6];
@@ -1306,11 +1311,11 @@ __global__ __launch_bounds__(256) void k_nbr_tile(Dev D) {
     if (lane == 0u) w = atomicAdd(&D.listCount[7], 1u);
     w = (uint32_t)__builtin_amdgcn_readfirstlane((int)w);
     if (w >= nheavy) break;
-    nbr_item(D, D.dlist[6][w], lst, lane, W0, true);
+    nbr_item<STRIP>(D, D.dlist[6][w], lst, lane, W0, true);
   }
   const uint32_t nwork = (uint32_t)__builtin_amdgcn_readfirstlane((int)D.listCount[4]);
   const ListRange R = xcd_range(nwork, 4u, threadIdx.x >> 6);
-  for (uint32_t w = R.first; w < R.hi; w += R.stride) nbr_item(D, D.dlist[4][w], lst, lane, W0, false);
+  for (uint32_t w = R.first; w < R.hi; w += R.stride) nbr_item<STRIP>(D, D.dlist[4][w], lst, lane, W0, false);
 }
 
 // ================================================================ solveA (k>0)
@@ -2208,7 +2213,8 @@ void enqueue_substep(sn_engine* e) {
   {
     ProfScope ps(e, PC_NBR);
     hipLaunchKernelGGL(k_heavy_order, dim3(e->canonBlocks), dim3(256), 0, st, D);
-    hipLaunchKernelGGL(k_nbr_tile, dim3(e->nbrBlocks), dim3(256), 0, st, D);
+    if (D.dbgOn) hipLaunchKernelGGL(k_nbr_tile<false>, dim3(e->nbrBlocks), dim3(256), 0, st, D);
+    else hipLaunchKernelGGL(k_nbr_tile<true>, dim3(e->nbrBlocks), dim3(256), 0, st, D);
   }
   const uint32_t iters = e->opts.iters;
   const bool gen = D.gen != 0u;   // the material passes run (friction, containment, bonds)
@@ -2465,7 +2471,9 @@ int sn_create(const sn_opts* opts, const int32_t* luts, const int32_t* mats, sn_
       hipDeviceProp_t prop;
       HIPCHECK(hipGetDeviceProperties(&prop, (int)e->opts.device));
       int bN = 0, bC = 0, bA = 0;
-      HIPCHECK(hipOccupancyMaxActiveBlocksPerMultiprocessor(&bN, reinterpret_cast<const void*>(k_nbr_tile), 256, 0));
+      HIPCHECK(hipOccupancyMaxActiveBlocksPerMultiprocessor(
+          &bN, D.dbgOn ? reinterpret_cast<const void*>(k_nbr_tile<false>) : reinterpret_cast<const void*>(k_nbr_tile<true>),
+          256, 0));
       HIPCHECK(hipOccupancyMaxActiveBlocksPerMultiprocessor(&bC, e->sZero ? reinterpret_cast<const void*>(k_colour<true>)
                                                                        : reinterpret_cast<const void*>(k_colour<false>),
                                                               COLOUR_THREADS, 0));

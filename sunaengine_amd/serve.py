@@ -236,6 +236,9 @@ class Server:
         queue = {k: [] for k in self.engines}
         inflight = {k: 0 for k in self.engines}      # substeps of the round on the device, 0 = idle
         rounds = {k: 0 for k in self.engines}
+        busy_s = {k: 0.0 for k in self.engines}       # launch to read-back of each world's rounds
+        work = {k: 0 for k in self.engines}           # particle-substeps its rounds advanced (running jobs)
+        launched = {k: 0.0 for k in self.engines}
         t0 = clock()
 
         def finish_round(kind: str):
@@ -244,6 +247,7 @@ class Server:
             k = inflight[kind]
             inflight[kind] = 0
             rounds[kind] += 1
+            busy_s[kind] += clock() - launched[kind]
             retired = False
             for j in self.busy[kind]:
                 if j is None:
@@ -278,8 +282,10 @@ class Server:
             running = [j for j in self.busy[kind] if j is not None]
             if running:
                 k = min(self.chunk, max(j.substeps - j.done for j in running))
+                launched[kind] = clock()
                 eng.step(k)                 # enqueued; finish_round reads it back
                 inflight[kind] = k
+                work[kind] += k * sum(j.n for j in running)
 
         order = list(self.engines)
         turn = 0              # the world whose round is tried first next time
@@ -310,7 +316,8 @@ class Server:
                     break
                 nxt = min(p[0].arrival for p in pending.values() if p)
                 time.sleep(max(0.0, min(nxt - (clock() - t0), 0.005)))
-        return {"wall_s": clock() - t0, "rounds": sum(rounds.values()), "rounds_by_world": rounds}
+        return {"wall_s": clock() - t0, "rounds": sum(rounds.values()), "rounds_by_world": rounds,
+                "busy_s_by_world": busy_s, "round_work_by_world": work}
 
 
 def summarize(jobs: Sequence[Job], wall: float) -> dict:
