@@ -1042,14 +1042,17 @@ __device__ __forceinline__ void nbr_chunk(const Dev& D, uint32_t c0, uint32_t nc
 
 // The one-wave form of the sweep (one particle per lane) with the lists
 // staged in LDS.  The candidate loop only tests and records: an accepted
-// candidate is stored as 16 bits, (row << 14) | (boundary << 13) | offset in
-// its row range, at list[k][lane].  Afterwards the lists are written out
-// transposed, one coalesced 256-byte store per k, and iteration 0 of solveA
-// runs over them with every lane busy, instead of inside the candidate loop
-// where only the few accepting lanes of each candidate would run it.
-// Returns false, having done nothing, when a row range is too long for the
-// 13-bit offset; the caller then takes nbr_chunk.
-constexpr uint32_t ROW_OFF_MAX = 1u << 13;
+// candidate is stored as 16 bits, (boundary << 15) | its offset in the
+// three row ranges laid end to end, at list[k][lane].  Afterwards the lists
+// are written out transposed, one coalesced 256-byte store per k, and
+// iteration 0 of solveA runs over them with every lane busy, instead of
+// inside the candidate loop where only the few accepting lanes of each
+// candidate would run it.  Returns false, having done nothing, when the rows
+// hold more candidates than the 15-bit offset reaches; the caller then takes
+// nbr_chunk.  (A corner pile-up of a 256 x 128 wu box holds up to about
+// 27,000.)
+constexpr uint32_t CAND_OFF_MAX = 1u << 15;
+constexpr uint32_t DUP_MIN = 16;   // equal neighbours in a block that select the grouped loop
 
 template <bool BND, bool WIN, bool STRIP>
 __device__ __forceinline__ bool nbr_chunk_staged(const Dev& D, uint16_t* lst, uint32_t c0,
@@ -1088,8 +1091,11 @@ __device__ __forceinline__ bool nbr_chunk_staged(const Dev& D, uint16_t* lst, ui
     const uint32_t rowc = cellBase + (uint32_t)gy * gridW;
     rlo[r] = (uint32_t)((int32_t)D.cellStart[rowc + (uint32_t)xa] + sh);
     rhi[r] = (uint32_t)((int32_t)D.cellStart[rowc + (uint32_t)xb + 1u] + sh);
-    if (rhi[r] - rlo[r] > ROW_OFF_MAX) return false;
   }
+  // the rows' starts in the end-to-end offset space of the list entries
+  const uint32_t cb1 = rhi[0] - rlo[0], cb2 = cb1 + (rhi[1] - rlo[1]);
+  if (cb2 + (rhi[2] - rlo[2]) > CAND_OFF_MAX) return false;
+  const uint32_t cbase[3] = {0u, cb1, cb2};
   const bool stopFull = STRIP && D.sd[s].flagged == 0u;   // as in nbr_chunk
   // The candidate loops below are branch-free per candidate: a lane counts
   // its in-range candidates in nin (its list keeps the first MAXNBR, so cnt =
@@ -1139,7 +1145,47 @@ __device__ __forceinline__ bool nbr_chunk_staged(const Dev& D, uint16_t* lst, ui
       // candidates carrying those bits are left
       if (allFull && !D.dbgOn) todo &= mSeal | mCong;
       // this lane's candidate as a list entry, read per candidate with one lane read
-      const uint32_t ent = (r << 14) + q0 - rlo[r] + lane + (((mBnd >> lane) & 1ull) ? (1u << 13) : 0u);
+      const uint32_t ent = cbase[r] + q0 - rlo[r] + lane + (((mBnd >> lane) & 1ull) ? CAND_OFF_MAX : 0u);
+      bool grouped = false;
+      if constexpr (BND && !WIN) {
+        // A block mostly at one exact position: wall_clamp stacks a pile-up
+        // on a corner point, thousands of slots deep.  Lanes that cannot
+        // reach it never fill, so without this every such candidate costs an
+        // iteration on the pile's longest wave.
+        const uint32_t upx = (uint32_t)__shfl_up((int)cux, 1), upy = (uint32_t)__shfl_up((int)cuy, 1);
+        grouped = __builtin_popcountll(__ballot(lane > 0u && lane < qn && cux == upx && cuy == upy)) >= DUP_MIN;
+      }
+      if (grouped) {
+        // Candidate j is taken together with every later todo candidate of
+        // the block at the same position, whenever each lane that accepts
+        // them already holds MAXNBR entries: the list is then unchanged and
+        // the count and seen bits are order-free.  Otherwise j alone, as below.
+        const uint64_t mine = selfj < 64u ? (1ull << selfj) : 0ull;
+        while (todo) {
+          const uint32_t j = (uint32_t)__builtin_ctzll(todo);
+          const uint32_t jx = (uint32_t)__builtin_amdgcn_readlane((int)cux, j);
+          const uint32_t jy = (uint32_t)__builtin_amdgcn_readlane((int)cuy, j);
+          const uint64_t eq = __ballot(cux == jx && cuy == jy) & todo;
+          const uint32_t ax = (max(pux, jx) - min(pux, jx) + 8u) >> 4;
+          const uint32_t ay = (max(puy, jy) - min(puy, jy) + 8u) >> 4;
+          const bool inr = __umul24(ax, ax) + __umul24(ay, ay) < (uint32_t)H2S;
+          const uint64_t take = eq & ~mine;
+          const bool accAll = live && inr && take != 0ull;
+          if (__all(!accAll || nin >= MAXNBR)) {
+            if (flagged)
+              bits |= accAll ? (((mSeal & take) ? INF_SEAL : 0u) | ((mCong & take) ? INF_CONGEAL : 0u)) : 0u;
+            nin += accAll ? (uint32_t)__builtin_popcountll(take) : 0u;
+            todo &= ~eq;
+          } else {
+            todo ^= 1ull << j;
+            const bool acc = live && j != selfj && inr;
+            if (flagged)
+              bits |= acc ? ((((mSeal >> j) & 1ull) ? INF_SEAL : 0u) | (((mCong >> j) & 1ull) ? INF_CONGEAL : 0u)) : 0u;
+            if (!allFull) lst[min(nin, MAXNBR) * 64u + lane] = (uint16_t)__builtin_amdgcn_readlane((int)ent, j);
+            nin += acc ? 1u : 0u;
+          }
+        }
+      }
       while (todo) {
         const uint32_t j = (uint32_t)__builtin_ctzll(todo);
         todo ^= 1ull << j;
@@ -1183,9 +1229,9 @@ __device__ __forceinline__ bool nbr_chunk_staged(const Dev& D, uint16_t* lst, ui
       q[u] = p;
       if (k + u < cnt) {
         const uint32_t e = lst[(k + u) * 64u + lane];
-        const uint32_t r = e >> 14;
-        q[u] = (r == 0u ? rlo[0] : (r == 1u ? rlo[1] : rlo[2])) + (e & (ROW_OFF_MAX - 1u));
-        D.nbr[(size_t)(k + u) * D.NS + p] = q[u] | ((e & (1u << 13)) ? NBR_BND : 0u);
+        const uint32_t o = e & (CAND_OFF_MAX - 1u);
+        q[u] = o < cb1 ? rlo[0] + o : (o < cb2 ? rlo[1] + (o - cb1) : rlo[2] + (o - cb2));
+        D.nbr[(size_t)(k + u) * D.NS + p] = q[u] | ((e & CAND_OFF_MAX) ? NBR_BND : 0u);
       }
     }
 #pragma unroll

@@ -2,11 +2,12 @@
 
 Walls and collisions can pile thousands of particles into one grid cell.
 Such cells take their own paths in 2D: the canonical rank of a cell above
-256 particles runs in k_canon_heavy_rank / _place, and a neighbour row range
-above 8192 slots takes the unstaged sweep.  In 3D they run the ordinary
-kernels, whose per-cell work then grows with the cell.  These scenes start with
-3,600 / 10,000 particles (2D) or 4,096 (3D) inside one cell and keep cells of
-several hundred to thousands for every substep checked.
+256 particles runs in k_canon_heavy_rank / _place, and a tile whose three
+candidate rows hold more than 32,768 slots takes the unstaged sweep.  In 3D
+they run the ordinary kernels, whose per-cell work then grows with the cell.
+These scenes start with 3,600 / 10,000 / 36,100 particles (2D) or 4,096 (3D)
+inside one cell and keep cells of several hundred to thousands for every
+substep checked; the largest starts on the unstaged sweep.
 
 The same scenes also pin the debug:false build (src/engine.js:2884-2894):
 identical hashes, chain and state, and debug counters that stay zero.  In
@@ -42,15 +43,15 @@ def _gpu2d(scene, iters, steps, debug):
     return e
 
 
-@pytest.mark.parametrize("side,pitch_shift", [(60, 5), (100, 6)])
-def test_packed_cell_2d(side, pitch_shift):
+@pytest.mark.parametrize("side,pitch_shift,steps", [(60, 5, 12), (100, 6, 12), (190, 7, 6)])
+def test_packed_cell_2d(side, pitch_shift, steps):
     s = _packed2d(side, pitch_shift)
-    print(f"{s.name} x12 (debug on and off)", flush=True)
+    print(f"{s.name} x{steps} (debug on and off)", flush=True)
     o = oracle_world(s, 4)
-    ref = o.step(12, stream=True)
+    ref = o.step(steps, stream=True)
     assert o.debug_counters()[6] > 0
     for debug in (True, False):
-        e = _gpu2d(s, 4, 12, debug)
+        e = _gpu2d(s, 4, steps, debug)
         got = e.hash_stream()
         bad = first_mismatch(got, ref)
         assert bad is None, f"{s.name} debug={debug}: first differing substep {bad}"
@@ -105,10 +106,10 @@ def test_packed_cell_3d():
 
 def test_corner_pileup_debug_off_long_run():
     """A 76,175-particle dam break in the 256 x 128 wu box piles about 26,000
-    particles into its floor corner cell within a few hundred substeps; most
-    chunks of that cell then hold a particle with fewer than 48 neighbours in
-    range.  Without debug counters such lists are finished by the per-particle
-    tail of the sweep.  600 substeps: the stripped build's stream equals the
+    particles into its floor corner cell within a few hundred substeps, nearly
+    all on the one corner point; the chunks of that cell that hold a particle
+    off the point, with fewer than 48 neighbours in range, sweep every block of
+    the pile, position by position.  600 substeps: the stripped build's stream equals the
     instrumented build's (itself pinned to the oracle above and, here, over
     the first 20 substeps)."""
     from sunaengine_amd import scenes, serve
@@ -134,3 +135,52 @@ def test_corner_pileup_debug_off_long_run():
     st = off[2]
     cells = (st[:, 1].astype(np.int64) >> 18) * 64 + (st[:, 0].astype(np.int64) >> 18)
     assert np.unique(cells, return_counts=True)[1].max() > 8192   # the pile-up formed
+
+
+def _corner_point_pile(n_pile, n_spread, W=48, Hh=24, seed=0xC0C0):
+    """n_pile water particles on the exact corner point the wall clamp
+    holds them at (the bottom-right one, where a floor corner pile-up ends),
+    with n_spread others scattered over the 3 wu beside it, every
+    (n_pile + n_spread) / n_spread-th in slot order.  A cell's slots are
+    ordered by id, so nearly every 64-slot block of the cell's candidates is
+    one exact position."""
+    from sunaengine_amd import materials as mt
+    from sunaengine_amd import scenes
+    from tests import material_scenes as ms
+    rnd = scenes.xorshift32(seed)
+    wall = ONE >> 1
+    cx, cy = W * ONE - wall, Hh * ONE - wall
+    every = (n_pile + n_spread) // n_spread
+    parts = []
+    for i in range(n_pile + n_spread):
+        if i % every == 0:
+            parts.append((cx - rnd() % (3 * ONE), cy - rnd() % (3 * ONE), mt.WATER))
+        else:
+            parts.append((cx, cy, mt.WATER))
+    return ms._scene(parts, W, Hh, "corner_point_pile")
+
+
+def test_corner_point_pile():
+    """A pile-up stacked on one point: most candidate blocks are a single
+    position, which the sweep takes per position instead of per slot once
+    every lane that reaches it holds a full list.  Both builds against the
+    oracle: stream, chain, state, and (debug) the truncation count, which
+    the grouped blocks add up by position."""
+    s = _corner_point_pile(3000, 300)
+    print(f"{s.name} {s.n} x12 (debug on and off)", flush=True)
+    o = oracle_world(s, 4)
+    ref = o.step(12, stream=True)
+    st = o.read_state()
+    assert np.unique(st[:, :2], axis=0, return_counts=True)[1].max() > 2500   # still one stack
+    assert o.debug_counters()[6] > 0
+    for debug in (True, False):
+        e = _gpu2d(s, 4, 12, debug)
+        bad = first_mismatch(e.hash_stream(), ref)
+        assert bad is None, f"{s.name} debug={debug}: first differing substep {bad}"
+        assert np.array_equal(e.chain(), o.chain())
+        assert np.array_equal(e.read_state(), st)
+        if debug:
+            assert np.array_equal(e.debug_counters(), o.debug_counters())
+        else:
+            assert not e.debug_counters().any()
+        e.destroy()
