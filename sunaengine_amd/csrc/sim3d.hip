@@ -396,9 +396,10 @@ __global__ __launch_bounds__(256) void k3_nbr_tile(Dev3 D) {
     }
     DensAcc A{wmul(lutW[0], wadd(256, rhoW[mi])) >> 8, 0, 0, 0, 0};
     uint32_t cnt = 0, dropped = 0;
-    const int32_t bx0 = wave_min3(valid ? pi.x : I32_MAX) - (H - 1), bx1 = wave_max3(valid ? pi.x : I32_MIN) + (H - 1);
-    const int32_t by0 = wave_min3(valid ? pi.y : I32_MAX) - (H - 1), by1 = wave_max3(valid ? pi.y : I32_MIN) + (H - 1);
-    const int32_t bz0 = wave_min3(valid ? pi.z : I32_MAX) - (H - 1), bz1 = wave_max3(valid ? pi.z : I32_MIN) + (H - 1);
+    int32_t bx0 = wave_min3(valid ? pi.x : I32_MAX) - (H - 1), bx1 = wave_max3(valid ? pi.x : I32_MIN) + (H - 1);
+    int32_t by0 = wave_min3(valid ? pi.y : I32_MAX) - (H - 1), by1 = wave_max3(valid ? pi.y : I32_MIN) + (H - 1);
+    int32_t bz0 = wave_min3(valid ? pi.z : I32_MAX) - (H - 1), bz1 = wave_max3(valid ? pi.z : I32_MIN) + (H - 1);
+    uint64_t boxLanes = __ballot(valid);   // the lanes the reach box is taken over
     for (int32_t dz = -1; dz <= 1; ++dz) {
       const int32_t gz = cz + dz;
       if (gz < 0 || gz >= (int32_t)P.gz) continue;
@@ -409,8 +410,25 @@ __global__ __launch_bounds__(256) void k3_nbr_tile(Dev3 D) {
         const uint32_t lo = (uint32_t)((int32_t)D.cellStart[rowc + (uint32_t)xlo] + sh);
         const uint32_t hi = (uint32_t)((int32_t)D.cellStart[rowc + (uint32_t)xhi + 1u] + sh);
         for (uint32_t q0 = lo; q0 < hi; q0 += 64u) {
-          // without debug counters, a wave whose lists are all full is done
-          if (!D.dbgOn && __all(!valid || cnt >= MAXNBR)) break;
+          if (!D.dbgOn) {
+            // Without debug counters a full list has nothing left to find:
+            // a wave whose lists are all full is done, and the reach box
+            // shrinks to the lanes still filling.  In a packed cell the
+            // lanes far from the pile fill last, and the pile's candidates
+            // then drop out of the ballot instead of costing an iteration each.
+            const bool open = valid && cnt < MAXNBR;
+            const uint64_t openLanes = __ballot(open);
+            if (openLanes == 0ull) break;
+            if (openLanes != boxLanes) {
+              boxLanes = openLanes;
+              bx0 = wave_min3(open ? pi.x : I32_MAX) - (H - 1);
+              bx1 = wave_max3(open ? pi.x : I32_MIN) + (H - 1);
+              by0 = wave_min3(open ? pi.y : I32_MAX) - (H - 1);
+              by1 = wave_max3(open ? pi.y : I32_MIN) + (H - 1);
+              bz0 = wave_min3(open ? pi.z : I32_MAX) - (H - 1);
+              bz1 = wave_max3(open ? pi.z : I32_MIN) + (H - 1);
+            }
+          }
           const uint32_t qn = (uint32_t)mini((int32_t)(hi - q0), 64);
           int4 cp = make_int4(0, 0, 0, 0);
           uint32_t cm = 0;
