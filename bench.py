@@ -448,7 +448,15 @@ def run_serve(args):
     busy = 0.0
     for ri, rate in enumerate(rates):
         jobs = serve.poisson_jobs(args.jobs, rate, seed=0x5E7E + 101 * ri, steps_hi=args.max_steps)
-        st = srv.run(jobs)
+        finished = []
+
+        def progress(job, rate=rate, t0=time.perf_counter(), finished=finished):
+            finished.append(job.jid)
+            if len(finished) % 10 == 0:
+                print(f"# rate {rate}/s: {len(finished)}/{len(jobs)} jobs done, {time.perf_counter() - t0:.1f}s",
+                      file=sys.stderr, flush=True)
+
+        st = srv.run(jobs, on_finish=progress)
         summ = serve.summarize(jobs, st["wall_s"])
         summ["rate_jobs_per_s"] = rate
         summ["world_busy_s"] = {k: round(v, 3) for k, v in st["busy_s_by_world"].items()}

@@ -192,6 +192,7 @@ struct Dev {
   uint32_t* drop;         // candidates dropped by truncation, per particle
   uint32_t* crank;        // partial canonical ranks of heavy-cell slots (zero between uses)
   uint32_t* hperm;        // packed one-cell tiles: their slots in spatial (Morton bin) order
+  int4* blkBox;           // per substep: box of the live positions of each aligned 64-slot block
   const uint32_t* blockMap;  // per-particle kernels: logical block -> 256-slot block holding live slots
   uint32_t* hpart;        // per-block partial sums [block][HPART_WORDS]
   const uint32_t* cellScene;  // global cell -> scene, INACTIVE if its scene is inactive
@@ -793,6 +794,58 @@ __device__ __forceinline__ bool row_span(const SceneP& P, int32_t gy, int32_t& x
   return xa <= xb;
 }
 
+// Block boxes.  Per substep, after the canonical order: the bounding box of
+// the live positions of every aligned 64-slot block of the sorted order, as
+// (min x, min y, max x, max y); a block with no live slot has an empty box
+// (min > max).  Heavy items of the sweep visit only the blocks whose boxes
+// meet their reach box, found 64 blocks at a time with one vector load and
+// a ballot: every candidate of a block passed over is out of reach of every
+// lane the box covers, so only the work changes.  A corner pile-up of tens
+// of thousands of slots on one point is then crossed in a few loads by the
+// waves that cannot reach it, instead of one dependent load per 64 slots.
+__device__ __forceinline__ bool box_miss(int4 b, int32_t x0, int32_t x1, int32_t y0, int32_t y1) {
+  return b.x > x1 || b.z < x0 || b.y > y1 || b.w < y0;
+}
+
+__global__ __launch_bounds__(256) void k_block_box(Dev D) {
+  const uint32_t q = slot_block(D) * 256u + threadIdx.x;
+  const bool live = q < D.N && D.sidOf[q] != INACTIVE;
+  const int2 p = live ? D.pred[q] : make_int2(0, 0);
+  const int32_t x0 = wave_min(live ? p.x : I32_MAX), y0 = wave_min(live ? p.y : I32_MAX);
+  const int32_t x1 = wave_max(live ? p.x : I32_MIN), y1 = wave_max(live ? p.y : I32_MIN);
+  if ((threadIdx.x & 63u) == 0u) D.blkBox[q >> 6] = make_int4(x0, y0, x1, y1);
+}
+
+// Skipping blocks.  A sweep over a row range [q0, hi) keeps the mask of the
+// aligned blocks of one 64-block group whose boxes meet its reach box [x0,
+// x1] x [y0, y1], loaded with one vector load and a ballot.  skip_blocks
+// moves q0 to the start of the first such block at or after it (hi when
+// there is none): every slot passed over lies in a block whose box misses
+// the reach box.  Uniform state and arguments; every lane active.
+struct BlockMask {
+  uint32_t group = 0xffffffffu;   // first block of the loaded group (none yet)
+  uint64_t need = 0ull;
+};
+__device__ __forceinline__ uint32_t skip_blocks(const Dev& D, BlockMask& m, uint32_t q0, uint32_t hi, uint32_t lane,
+                                                int32_t x0, int32_t x1, int32_t y0, int32_t y1) {
+  const uint32_t b1 = (hi - 1u) >> 6;
+  uint32_t b = q0 >> 6;
+  while (b <= b1) {
+    const uint32_t g = b & ~63u;
+    if (m.group != g) {
+      const uint32_t bk = g + lane;
+      bool meet = false;
+      if (bk <= b1) meet = !box_miss(D.blkBox[bk], x0, x1, y0, y1);
+      m.group = g;
+      m.need = __ballot(meet);
+    }
+    const uint64_t rest = m.need & (~0ull << (b - g));
+    if (rest) return max(q0, (g + (uint32_t)__builtin_ctzll(rest)) << 6);
+    b = g + 64u;
+  }
+  return hi;
+}
+
 // The sweep's tail for one particle (lane L of the wave) whose list is not
 // yet full while the rest of the wave is: the remaining candidates, from row
 // dy0 at slot q0s on, are tested one per lane, 64 at a time, and the
@@ -807,6 +860,7 @@ __device__ __forceinline__ bool row_span(const SceneP& P, int32_t gy, int32_t& x
 // broadcast sweep about 20 per candidate for the whole wave, so up to about
 // 50 open lists the tail is cheaper; 32 keeps a margin.
 constexpr uint32_t TAIL_LANES = 32;   // at most this many unfilled lists switch to the tail
+constexpr uint32_t DUP_MIN = 16;      // equal neighbours in a block that select the grouped loop
 
 template <bool BND, bool WIN>
 __device__ __forceinline__ void nbr_tail(const Dev& D, const SceneP& P, int32_t cy, int32_t xlo, int32_t xhi,
@@ -829,7 +883,13 @@ __device__ __forceinline__ void nbr_tail(const Dev& D, const SceneP& P, int32_t 
     const uint32_t rowc = P.cellBase + (uint32_t)gy * P.gridW;
     const uint32_t rlo = (uint32_t)((int32_t)D.cellStart[rowc + (uint32_t)xa] + sh);
     const uint32_t rhi = (uint32_t)((int32_t)D.cellStart[rowc + (uint32_t)xb + 1u] + sh);
+    BlockMask bm;
     for (uint32_t q0 = dy == dy0 ? max(q0s, rlo) : rlo; q0 < rhi && c < MAXNBR; q0 += 64u) {
+      if constexpr (BND) {
+        // bounded scenes: positions within +-2^28, so the box cannot wrap
+        q0 = skip_blocks(D, bm, q0, rhi, lane, pL.x - REACH, pL.x + REACH, pL.y - REACH, pL.y + REACH);
+        if (q0 >= rhi) break;
+      }
       const uint32_t q = q0 + lane;
       bool ok = q < rhi;
       int2 cp = make_int2(0, 0);
@@ -930,22 +990,11 @@ __device__ __forceinline__ void nbr_chunk(const Dev& D, uint32_t c0, uint32_t nc
       const uint32_t rlo = (uint32_t)((int32_t)D.cellStart[rowc + (uint32_t)xa] + sh);
       const uint32_t rhi = (uint32_t)((int32_t)D.cellStart[rowc + (uint32_t)xb + 1u] + sh);
       for (uint32_t q0 = rlo; q0 < rhi; q0 += 64u) {
+        uint32_t open = 0;   // lists still filling, over the wave
         if (stopFull) {
-          uint32_t open = 0;   // lists still filling, over the wave
 #pragma unroll
           for (int k = 0; k < PPL; ++k) open += (uint32_t)__popcll(__ballot(live[k] && cnt[k] < MAXNBR));
-          if (open <= TAIL_LANES) {
-            // few lists left: finish each alone over the remaining candidates
-#pragma unroll
-            for (int k = 0; k < PPL; ++k) {
-              uint64_t m = __ballot(live[k] && cnt[k] < MAXNBR);
-              while (m) {
-                const uint32_t L = (uint32_t)__builtin_ctzll(m);
-                m &= m - 1ull;
-                nbr_tail<BND, WIN>(D, P, cy, xlo, xhi, dy, q0, lane, L, pik[k], pk[k], inf[k], cxk[k], cnt[k],
-                                   acc[k], g);
-              }
-            }
+          if (open == 0u) {
             done = true;
             break;
           }
@@ -963,9 +1012,74 @@ __device__ __forceinline__ void nbr_chunk(const Dev& D, uint32_t c0, uint32_t nc
         // wait for this block's candidates here, so that the candidate
         // loop below never waits on the neighbour-list stores it issues
         __builtin_amdgcn_s_waitcnt(0x0F70);  // vmcnt(0)
+        // a block mostly at one exact position (a corner pile-up): taken
+        // position by position below, see nbr_chunk_staged
+        bool grouped = false;
+        if constexpr (BND && !WIN && PPL == 1) {
+          const int32_t upx = __shfl_up(cp.x, 1), upy = __shfl_up(cp.y, 1);
+          grouped = __builtin_popcountll(__ballot(lane > 0u && lane < qn && cp.x == upx && cp.y == upy)) >= DUP_MIN;
+        }
+        if (stopFull && !grouped && open <= TAIL_LANES) {
+          // few lists left: finish each alone over the remaining candidates
+#pragma unroll
+          for (int k = 0; k < PPL; ++k) {
+            uint64_t m = __ballot(live[k] && cnt[k] < MAXNBR);
+            while (m) {
+              const uint32_t L = (uint32_t)__builtin_ctzll(m);
+              m &= m - 1ull;
+              nbr_tail<BND, WIN>(D, P, cy, xlo, xhi, dy, q0, lane, L, pik[k], pk[k], inf[k], cxk[k], cnt[k],
+                                 acc[k], g);
+            }
+          }
+          done = true;
+          break;
+        }
         bool want = lane < qn && !(ci & INF_DEAD);   // nobody sees a corpse
         if constexpr (BND) want = want && cp.x >= bx0 && cp.x <= bx1 && cp.y >= by0 && cp.y <= by1;
         uint64_t todo = __ballot(want);
+        if constexpr (BND && !WIN && PPL == 1) {
+          if (grouped) {
+            // as in nbr_chunk_staged: candidate j with every later todo
+            // candidate of the block at its position, while each lane that
+            // accepts them holds a full list (only `dropped` and the seen
+            // bits change); otherwise j alone
+            const uint64_t mS = __ballot(lane < qn && (ci & INF_SEAL));
+            const uint64_t mC = __ballot(lane < qn && (ci & INF_CONGEAL));
+            if (!D.dbgOn && __all(!live[0] || cnt[0] >= MAXNBR)) todo &= mS | mC;
+            const uint32_t selfj = pk[0] - q0;
+            const uint64_t mine = selfj < 64u ? (1ull << selfj) : 0ull;
+            while (todo) {
+              const uint32_t j = (uint32_t)__builtin_ctzll(todo);
+              const int2 cj = make_int2(__builtin_amdgcn_readlane(cp.x, j), __builtin_amdgcn_readlane(cp.y, j));
+              const uint64_t eq = __ballot(cp.x == cj.x && cp.y == cj.y) & todo;
+              const int2 d = pdelta<true>(pik[0], cj, g);
+              const int32_t idx = lut_idx_near(d);
+              const uint64_t take = eq & ~mine;
+              const bool accAll = live[0] && idx >= 0 && take != 0ull;
+              if (__all(!accAll || cnt[0] >= MAXNBR)) {
+                const uint64_t tk = accAll ? take : 0ull;
+                bits[0] |= ((mS & tk) ? INF_SEAL : 0u) | ((mC & tk) ? INF_CONGEAL : 0u);
+                dropped[0] += (uint32_t)__builtin_popcountll(tk);
+                todo &= ~eq;
+              } else {
+                todo ^= 1ull << j;
+                const uint32_t q = q0 + j;
+                const uint32_t qi = (uint32_t)__builtin_amdgcn_readlane((int)ci, j);
+                if (live[0] && q != pk[0] && idx >= 0) {
+                  bits[0] |= qi & (INF_SEAL | INF_CONGEAL);
+                  if (cnt[0] >= MAXNBR) {
+                    dropped[0]++;
+                  } else {
+                    D.nbr[(size_t)cnt[0] * D.NS + pk[0]] = q | ((qi & INF_FLUID) ? 0u : NBR_BND);
+                    cnt[0]++;
+                    if (inf[0] & INF_FLUID) solveA_term<BND>(acc[0], d, D.lut4[idx], g);
+                  }
+                }
+              }
+            }
+            continue;
+          }
+        }
         bool full = true;
 #pragma unroll
         for (int k = 0; k < PPL; ++k) full = full && (!live[k] || cnt[k] >= MAXNBR);
@@ -1052,7 +1166,6 @@ __device__ __forceinline__ void nbr_chunk(const Dev& D, uint32_t c0, uint32_t nc
 // nbr_chunk.  (A corner pile-up of a 256 x 128 wu box holds up to about
 // 27,000.)
 constexpr uint32_t CAND_OFF_MAX = 1u << 15;
-constexpr uint32_t DUP_MIN = 16;   // equal neighbours in a block that select the grouped loop
 
 template <bool BND, bool WIN, bool STRIP>
 __device__ __forceinline__ bool nbr_chunk_staged(const Dev& D, uint16_t* lst, uint32_t c0,
@@ -2258,6 +2371,7 @@ void enqueue_substep(sn_engine* e) {
   }
   {
     ProfScope ps(e, PC_NBR);
+    hipLaunchKernelGGL(k_block_box, dim3(e->actBlocks), dim3(256), 0, st, D);
     hipLaunchKernelGGL(k_heavy_order, dim3(e->canonBlocks), dim3(256), 0, st, D);
     if (D.dbgOn) hipLaunchKernelGGL(k_nbr_tile<false>, dim3(e->nbrBlocks), dim3(256), 0, st, D);
     else hipLaunchKernelGGL(k_nbr_tile<true>, dim3(e->nbrBlocks), dim3(256), 0, st, D);
@@ -2466,6 +2580,14 @@ int sn_create(const sn_opts* opts, const int32_t* luts, const int32_t* mats, sn_
     D.drop = dalloc<uint32_t>(e, NC);
     D.crank = dalloc<uint32_t>(e, NC);
     D.hperm = dalloc<uint32_t>(e, NC);
+    // every aligned 64-slot block of the 256-slot blocks; empty boxes until
+    // first written (a sweep reads only blocks holding live candidates)
+    {
+      const size_t nb = (size_t)div_up(std::max(NC, 1u), 256) * 4u;
+      std::vector<int4> empty(nb, make_int4(I32_MAX, I32_MAX, I32_MIN, I32_MIN));
+      D.blkBox = dalloc<int4>(e, nb);
+      HIPCHECK(hipMemcpy(D.blkBox, empty.data(), sizeof(int4) * nb, hipMemcpyHostToDevice));
+    }
     e->dBlockMap = dalloc<uint32_t>(e, div_up(std::max(NC, 1u), 256));
     D.blockMap = e->dBlockMap;
     {

@@ -215,9 +215,12 @@ class Server:
 
     def run(self, jobs: Sequence[Job], clock: Callable[[], float] = time.perf_counter,
             on_finish: Optional[Callable[[Job], None]] = None, realtime: bool = True,
-            poll_s: float = 0.0002) -> dict:
+            poll_s: float = 0.0002, trace: Optional[list] = None) -> dict:
         """Serve `jobs` (arrival times in seconds from the start).  With
-        realtime=False every job is treated as already arrived.
+        realtime=False every job is treated as already arrived.  `trace`, if
+        given, receives one tuple per round: (world, launch time, substeps,
+        jobs running, their particles, seconds from launch to read-back,
+        jobs queued for that world).
 
         One host thread drives both worlds, one round on the device at a
         time, alternating between the worlds that have running jobs; it
@@ -248,6 +251,10 @@ class Server:
             inflight[kind] = 0
             rounds[kind] += 1
             busy_s[kind] += clock() - launched[kind]
+            if trace is not None:
+                running = [j for j in self.busy[kind] if j is not None]
+                trace.append((kind, launched[kind] - t0, k, len(running), sum(j.n for j in running),
+                              clock() - launched[kind], len(queue[kind])))
             retired = False
             for j in self.busy[kind]:
                 if j is None:
