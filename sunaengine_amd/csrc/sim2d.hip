@@ -192,6 +192,11 @@ struct Dev {
   uint32_t* drop;         // candidates dropped by truncation, per particle
   uint32_t* crank;        // partial canonical ranks of heavy-cell slots (zero between uses)
   uint32_t* hperm;        // packed one-cell tiles: their slots in spatial (Morton bin) order
+  uint32_t* cellBin;      // per substep: cell -> its bin table slot (BIN_NONE: the cell is not binned)
+  uint4* binTab;          // [slot][64] per Morton bin {first, end (offsets into hperm from the cell's
+                          // first slot), live (not DEAD) slots, 0}
+  int4* binBox;           // [slot][64] box of the live slots' predicted positions (x0, y0, x1, y1)
+  uint32_t binCap;        // slots of binTab / binBox
   int4* blkBox;           // per substep: box of the live positions of each aligned 64-slot block
   const uint32_t* blockMap;  // per-particle kernels: logical block -> 256-slot block holding live slots
   uint32_t* hpart;        // per-block partial sums [block][HPART_WORDS]
@@ -200,7 +205,8 @@ struct Dev {
   uint2* dlist[7];        // this substep's occupied cells per colour [0..3], occupied tiles [4],
                           // chunks of heavy cells for the canonical rank [5], tiles with
                           // heavy candidate rows [6]
-  uint32_t* listCount;    // [7] entries of dlist, [7] next item of dlist[6] (cleared by predict)
+  uint32_t* listCount;    // [0..6] entries of dlist, [7] next item of dlist[6], [8] bin table slots
+                          // taken (cleared by predict)
   uint32_t nTiles;
   uint32_t dbgOn;         // 0: the reference's debug:false build, counters stay zero
   const int4* lut4;       // {W, G, C, S} per LUT bin
@@ -369,7 +375,7 @@ __device__ __forceinline__ uint32_t slot_block(const Dev& D) {
 // which replaces the reference's second atomic pass in `scatter`.
 __global__ __launch_bounds__(256) void k_predict_count(Dev D) {
   uint32_t p = slot_block(D) * 256u + threadIdx.x;
-  if (blockIdx.x == 0 && threadIdx.x < 8u) D.listCount[threadIdx.x] = 0u;  // rebuilt after the scan
+  if (blockIdx.x == 0 && threadIdx.x < 9u) D.listCount[threadIdx.x] = 0u;  // rebuilt after the scan
   uint32_t s = p < D.N ? D.sidOf[p] : INACTIVE;
   Dbg g;
   uint32_t cell = INACTIVE;
@@ -669,6 +675,7 @@ constexpr uint32_t TILE_CHUNK = 64;    // 64 lanes x NBR_PPL
 // such work spreads over the whole chip instead of one XCD's list range.
 // On the 1M dam break: 1024..2048 measure alike, 4096 is 2% slower, 8192 50%.
 constexpr uint32_t NBR_HEAVY_CAND = 2048;
+constexpr uint32_t BIN_NONE = 0xffffffffu;   // cellBin: the cell has no bin table this substep
 
 // Called with every lane of every wave active.
 __global__ __launch_bounds__(256) void k_build_lists(Dev D) {
@@ -676,6 +683,7 @@ __global__ __launch_bounds__(256) void k_build_lists(Dev D) {
   {
     uint32_t cnt = 0, col = 0, lo = 0, s = 0;
     if (t < D.C) {
+      D.cellBin[t] = BIN_NONE;   // k_heavy_order bins this substep's packed cells
       s = D.cellScene[t];
       if (s != INACTIVE) {
         cnt = D.cellStart[t + 1] - D.cellStart[t];
@@ -846,6 +854,145 @@ __device__ __forceinline__ uint32_t skip_blocks(const Dev& D, BlockMask& m, uint
   return hi;
 }
 
+__device__ __forceinline__ uint32_t morton_bin(const SceneP& P, uint32_t c0, int2 pr) {
+  const uint32_t lc0 = c0 - P.cellBase;
+  const int32_t x0 = (int32_t)((lc0 % P.gridW) << P.cellShift), y0 = (int32_t)((lc0 / P.gridW) << P.cellShift);
+  const int32_t bs = maxi((int32_t)P.cellShift - 3, 0);
+  const uint32_t bx = (uint32_t)clampi((pr.x - x0) >> bs, 0, 7), by = (uint32_t)clampi((pr.y - y0) >> bs, 0, 7);
+  return (bx & 1u) | ((by & 1u) << 1) | ((bx & 2u) << 1) | ((by & 2u) << 2) | ((bx & 4u) << 2) | ((by & 4u) << 3);
+}
+
+// Exact in-range counts for the debug build.  Once every list of a wave is
+// full, a further candidate changes only its lane's truncation count (and,
+// in scenes with SEAL or CONGEAL particles, the seen bits), so in a bounded
+// scene without such particles a one-cell chunk can take the rest of each
+// lane's count at once: the number of live candidates not yet visited,
+// other than itself, that the sweep's test accepts.  In a cell with a bin table
+// (a packed cell, binned by k_heavy_order) a bin whose box lies wholly
+// inside a lane's radius adds its live count to that lane and one wholly
+// outside adds nothing; only bins that the radius crosses for some lane are
+// tested slot by slot.  The test is monotone: r2q below is non-decreasing in
+// each of |dx| and |dy|, so the box's farthest and nearest corners bound
+// every slot in it.
+__device__ __forceinline__ uint32_t r2q_abs(uint32_t a, uint32_t b) {
+  // |d| < 2^29 in a bounded scene; past HS every pair is out of range
+  const uint32_t ax = min((a + 8u) >> 4, (uint32_t)HS + 1u), ay = min((b + 8u) >> 4, (uint32_t)HS + 1u);
+  return __umul24(ax, ax) + __umul24(ay, ay);
+}
+
+// Lanes with `on` count the slots map(i), i in [a, b), that are live, inside
+// the chunk's reach box, not their own slot pk, and in range by the sweep's
+// one-cell test (biased positions, |d| < 2^19 inside the reach box of a
+// one-cell chunk).  Uniform arguments; every lane active.
+__device__ __forceinline__ uint32_t count_exact(const Dev& D, const uint32_t* perm, uint32_t a, uint32_t b, bool on,
+                                                uint32_t pk, uint32_t pux, uint32_t puy, int32_t bx0, int32_t bx1,
+                                                int32_t by0, int32_t by1, uint32_t lane) {
+  uint32_t n = 0;
+  for (uint32_t i0 = a; i0 < b; i0 += 64u) {
+    const uint32_t i = i0 + lane;
+    uint32_t q = 0, ci = INF_DEAD;
+    int2 cp = make_int2(0, 0);
+    if (i < b) {
+      q = perm ? perm[i] : i;
+      cp = D.pred[q];
+      ci = D.info[q];
+    }
+    const bool want = !(ci & INF_DEAD) && cp.x >= bx0 && cp.x <= bx1 && cp.y >= by0 && cp.y <= by1;
+    uint64_t todo = __ballot(want);
+    const uint32_t cux = (uint32_t)cp.x ^ 0x80000000u, cuy = (uint32_t)cp.y ^ 0x80000000u;
+    while (todo) {
+      const uint32_t j = (uint32_t)__builtin_ctzll(todo);
+      todo &= todo - 1ull;
+      const uint32_t jx = (uint32_t)__builtin_amdgcn_readlane((int)cux, j);
+      const uint32_t jy = (uint32_t)__builtin_amdgcn_readlane((int)cuy, j);
+      const uint32_t qj = (uint32_t)__builtin_amdgcn_readlane((int)q, j);
+      const uint32_t ax = (max(pux, jx) - min(pux, jx) + 8u) >> 4;
+      const uint32_t ay = (max(puy, jy) - min(puy, jy) + 8u) >> 4;
+      n += (uint32_t)on & (uint32_t)(qj != pk) & (uint32_t)(__umul24(ax, ax) + __umul24(ay, ay) < (uint32_t)H2S);
+    }
+  }
+  return n;
+}
+
+// A lane's in-range count over the whole of binned cell c (bin table at
+// `slot`); c0 is the chunk's own cell and mybin the lane's bin in it.
+__device__ __forceinline__ uint32_t bins_count(const Dev& D, uint32_t slot, uint32_t lo, uint32_t c, uint32_t c0,
+                                               uint32_t mybin, bool live, uint32_t pk, int2 pi, int32_t bx0,
+                                               int32_t bx1, int32_t by0, int32_t by1, uint32_t lane) {
+  const uint32_t pux = (uint32_t)pi.x ^ 0x80000000u, puy = (uint32_t)pi.y ^ 0x80000000u;
+  const uint4* T = D.binTab + (size_t)slot * 64u;
+  const int4* B = D.binBox + (size_t)slot * 64u;
+  uint32_t n = 0;
+  for (uint32_t b = 0; b < 64u; ++b) {
+    const uint4 tb = T[b];
+    const uint32_t nlive = (uint32_t)__builtin_amdgcn_readfirstlane((int)tb.z);
+    if (nlive == 0u) continue;
+    const int4 bb = B[b];
+    const int32_t x0 = __builtin_amdgcn_readfirstlane(bb.x), y0 = __builtin_amdgcn_readfirstlane(bb.y);
+    const int32_t x1 = __builtin_amdgcn_readfirstlane(bb.z), y1 = __builtin_amdgcn_readfirstlane(bb.w);
+    if (x0 > bx1 || x1 < bx0 || y0 > by1 || y1 < by0) continue;   // out of every lane's reach
+    const uint32_t fx = (uint32_t)maxi(abs(pi.x - x0), abs(pi.x - x1));
+    const uint32_t fy = (uint32_t)maxi(abs(pi.y - y0), abs(pi.y - y1));
+    const uint32_t nx = (uint32_t)(pi.x < x0 ? x0 - pi.x : (pi.x > x1 ? pi.x - x1 : 0));
+    const uint32_t ny = (uint32_t)(pi.y < y0 ? y0 - pi.y : (pi.y > y1 ? pi.y - y1 : 0));
+    const bool full = live && r2q_abs(fx, fy) < (uint32_t)H2S;
+    const bool part = live && !full && r2q_abs(nx, ny) < (uint32_t)H2S;
+    if (full) n += nlive - ((c == c0 && b == mybin) ? 1u : 0u);
+    if (__ballot(part)) {
+      const uint32_t first = (uint32_t)__builtin_amdgcn_readfirstlane((int)tb.x);
+      const uint32_t end = (uint32_t)__builtin_amdgcn_readfirstlane((int)tb.y);
+      n += count_exact(D, D.hperm + lo, first, end, part, pk, pux, puy, bx0, bx1, by0, by1, lane);
+    }
+  }
+  return n;
+}
+
+// A lane's in-range count over the slots [a, b) of stencil row gy, cells xlo
+// .. xhi, of one-cell tile c0 (bounded scene, cell no wider than 2^18; 0 for
+// a lane that is not live).  The chunk's reach box [bx0, bx1] x [by0, by1]
+// holds every lane's reach.  A binned cell of more than BIN_MIN slots in the
+// range is counted from its bins, less the slots of the cell outside the
+// range counted exactly when those are fewer.
+constexpr uint32_t BIN_MIN = 256;
+__device__ __forceinline__ uint32_t range_count(const Dev& D, const SceneP& P, int32_t gy, int32_t xlo, int32_t xhi,
+                                                uint32_t a, uint32_t b, uint32_t c0, bool live, uint32_t pk,
+                                                int2 pi, int32_t bx0, int32_t bx1, int32_t by0, int32_t by1,
+                                                uint32_t lane) {
+  const int32_t sh = P.posShift;
+  const uint32_t pux = (uint32_t)pi.x ^ 0x80000000u, puy = (uint32_t)pi.y ^ 0x80000000u;
+  const uint32_t rowc = P.cellBase + (uint32_t)gy * P.gridW;
+  uint32_t n = 0;
+  for (int32_t x = xlo; x <= xhi; ++x) {
+    const uint32_t c = rowc + (uint32_t)x;
+    const uint32_t lo = (uint32_t)((int32_t)D.cellStart[c] + sh);
+    const uint32_t hi = (uint32_t)((int32_t)D.cellStart[c + 1] + sh);
+    const uint32_t u = max(lo, a), v = min(hi, b);
+    if (u >= v) continue;
+    const uint32_t slot = (uint32_t)__builtin_amdgcn_readfirstlane((int)D.cellBin[c]);
+    if (slot == BIN_NONE || v - u <= BIN_MIN || (u - lo) + (hi - v) >= v - u) {
+      n += count_exact(D, nullptr, u, v, live, pk, pux, puy, bx0, bx1, by0, by1, lane);
+      continue;
+    }
+    n += bins_count(D, slot, lo, c, c0, morton_bin(P, c0, pi), live, pk, pi, bx0, bx1, by0, by1, lane);
+    n -= count_exact(D, nullptr, lo, u, live, pk, pux, puy, bx0, bx1, by0, by1, lane);
+    n -= count_exact(D, nullptr, v, hi, live, pk, pux, puy, bx0, bx1, by0, by1, lane);
+  }
+  return n;
+}
+
+// The in-range count of the candidates a one-cell sweep has not yet visited:
+// row gy0 from slot q0 on, then every later row of the stencil (rows below
+// gy0 + 1 .. cy + 1, cells xlo .. xhi).
+__device__ __forceinline__ uint32_t rest_count(const Dev& D, const SceneP& P, int32_t cy, int32_t xlo, int32_t xhi,
+                                               int32_t gy0, uint32_t q0, uint32_t c0, bool live, uint32_t pk,
+                                               int2 pi, int32_t bx0, int32_t bx1, int32_t by0, int32_t by1,
+                                               uint32_t lane) {
+  uint32_t n = range_count(D, P, gy0, xlo, xhi, q0, 0xffffffffu, c0, live, pk, pi, bx0, bx1, by0, by1, lane);
+  for (int32_t gy = gy0 + 1; gy <= mini(cy + 1, (int32_t)P.gridH - 1); ++gy)
+    n += range_count(D, P, gy, xlo, xhi, 0u, 0xffffffffu, c0, live, pk, pi, bx0, bx1, by0, by1, lane);
+  return n;
+}
+
 // The sweep's tail for one particle (lane L of the wave) whose list is not
 // yet full while the rest of the wave is: the remaining candidates, from row
 // dy0 at slot q0s on, are tested one per lane, 64 at a time, and the
@@ -944,6 +1091,10 @@ __device__ __forceinline__ void nbr_chunk(const Dev& D, uint32_t c0, uint32_t nc
   // without debug counters, and with no SEAL/CONGEAL particle in the scene,
   // a wave whose lists are all full has nothing left to find
   const bool stopFull = STRIP && D.sd[s].flagged == 0u;
+  // with debug counters, and with no SEAL/CONGEAL particle in the scene, a
+  // wave whose lists are all full takes the rest of each lane's count from
+  // rest_count
+  const bool binCount = !STRIP && D.dbgOn && D.sd[s].flagged == 0u;
   for (uint32_t pbase = plo; pbase < phi; pbase += 64u * PPL) {
     uint32_t pk[PPL], inf[PPL], cnt[PPL], bits[PPL], dropped[PPL];
     int2 pik[PPL];
@@ -990,6 +1141,13 @@ __device__ __forceinline__ void nbr_chunk(const Dev& D, uint32_t c0, uint32_t nc
       const uint32_t rlo = (uint32_t)((int32_t)D.cellStart[rowc + (uint32_t)xa] + sh);
       const uint32_t rhi = (uint32_t)((int32_t)D.cellStart[rowc + (uint32_t)xb + 1u] + sh);
       for (uint32_t q0 = rlo; q0 < rhi; q0 += 64u) {
+        if constexpr (!STRIP && BND && !WIN && PPL == 1) {
+          if (binCount && __all(!live[0] || cnt[0] >= MAXNBR)) {
+            dropped[0] += rest_count(D, P, cy, xlo, xhi, gy, q0, c0, live[0], pk[0], pik[0], bx0, bx1, by0, by1, lane);
+            done = true;
+            break;
+          }
+        }
         uint32_t open = 0;   // lists still filling, over the wave
         if (stopFull) {
 #pragma unroll
@@ -1210,6 +1368,7 @@ __device__ __forceinline__ bool nbr_chunk_staged(const Dev& D, uint16_t* lst, ui
   if (cb2 + (rhi[2] - rlo[2]) > CAND_OFF_MAX) return false;
   const uint32_t cbase[3] = {0u, cb1, cb2};
   const bool stopFull = STRIP && D.sd[s].flagged == 0u;   // as in nbr_chunk
+  const bool binCount = !STRIP && D.dbgOn && D.sd[s].flagged == 0u;   // as in nbr_chunk
   // The candidate loops below are branch-free per candidate: a lane counts
   // its in-range candidates in nin (its list keeps the first MAXNBR, so cnt =
   // min(nin, MAXNBR) and the truncation count is nin - cnt), and every lane
@@ -1227,19 +1386,39 @@ __device__ __forceinline__ bool nbr_chunk_staged(const Dev& D, uint16_t* lst, ui
   for (uint32_t r = 0; r < 3; ++r) {
     if (done) break;
     const uint32_t rowc = cellBase + (uint32_t)(cy + (int32_t)r - 1) * gridW;
+    // each block's candidates are loaded while the block before it is tested
+    int2 ncp = make_int2(0, 0);
+    uint32_t nci = 0;
+    int32_t ncc = 0;
+    if (rlo[r] + lane < rhi[r]) {
+      ncp = D.pred[rlo[r] + lane];
+      nci = D.info[rlo[r] + lane];
+      if constexpr (WIN) ncc = (int32_t)(D.cellY[rlo[r] + lane] - rowc);
+    }
     for (uint32_t q0 = rlo[r]; q0 < rhi[r]; q0 += 64u) {
       if (stopFull && __all(!live || cnt >= MAXNBR)) {
         done = true;
         break;
       }
+      if constexpr (!STRIP && BND && !WIN) {
+        if (binCount && __all(!live || cnt >= MAXNBR)) {
+          // every list is full: the rest of each lane's in-range count at once
+          nin += rest_count(D, P, cy, xlo, xhi, cy + (int32_t)r - 1, q0, c0, live, p, pi, bx0, bx1, by0, by1, lane);
+          done = true;
+          break;
+        }
+      }
       const uint32_t qn = (uint32_t)mini((int32_t)(rhi[r] - q0), 64);
-      int2 cp = make_int2(0, 0);
-      uint32_t ci = 0;
-      int32_t cc = 0;
-      if (lane < qn) {
-        cp = D.pred[q0 + lane];
-        ci = D.info[q0 + lane];
-        cc = (int32_t)(D.cellY[q0 + lane] - rowc);
+      const int2 cp = ncp;
+      const uint32_t ci = nci;
+      const int32_t cc = ncc;
+      ncp = make_int2(0, 0);
+      nci = 0;
+      ncc = 0;
+      if (q0 + 64u + lane < rhi[r]) {
+        ncp = D.pred[q0 + 64u + lane];
+        nci = D.info[q0 + 64u + lane];
+        if constexpr (WIN) ncc = (int32_t)(D.cellY[q0 + 64u + lane] - rowc);
       }
       bool want = lane < qn && !(ci & INF_DEAD);   // nobody sees a corpse
       if constexpr (BND) want = want && cp.x >= bx0 && cp.x <= bx1 && cp.y >= by0 && cp.y <= by1;
@@ -1374,17 +1553,15 @@ __device__ __forceinline__ bool nbr_chunk_staged(const Dev& D, uint16_t* lst, ui
 // workgroup per tile, bin order within a bin left to the LDS atomics) and
 // writes them to hperm.  A chunk of 64 neighbouring particles then has a
 // tight reach box, so whole blocks of far candidates drop out of its ballot.
-__device__ __forceinline__ uint32_t morton_bin(const SceneP& P, uint32_t c0, int2 pr) {
-  const uint32_t lc0 = c0 - P.cellBase;
-  const int32_t x0 = (int32_t)((lc0 % P.gridW) << P.cellShift), y0 = (int32_t)((lc0 / P.gridW) << P.cellShift);
-  const int32_t bs = maxi((int32_t)P.cellShift - 3, 0);
-  const uint32_t bx = (uint32_t)clampi((pr.x - x0) >> bs, 0, 7), by = (uint32_t)clampi((pr.y - y0) >> bs, 0, 7);
-  return (bx & 1u) | ((by & 1u) << 1) | ((bx & 2u) << 1) | ((by & 2u) << 2) | ((bx & 4u) << 2) | ((by & 4u) << 3);
-}
-
+// The same pass records, per substep, each bin of such a cell: its range in
+// hperm, its live (not DEAD) slots and the box of their predicted positions
+// (binTab / binBox at the cell's cellBin slot), for bins_count.
 __global__ __launch_bounds__(256) void k_heavy_order(Dev D) {
-  __shared__ uint32_t bins[64];
+  __shared__ uint32_t bins[64], bstart[64], blive[64];
+  __shared__ int32_t bx0[64], by0[64], bx1[64], by1[64];
+  __shared__ uint32_t slotS;
   const uint32_t n = (uint32_t)__builtin_amdgcn_readfirstlane((int)D.listCount[6]);
+  const uint32_t t = threadIdx.x;
   for (uint32_t w = blockIdx.x; w < n; w += gridDim.x) {
     const uint2 item = D.dlist[6][w];
     const uint32_t chunk = (uint32_t)__builtin_amdgcn_readfirstlane((int)item.y);
@@ -1398,23 +1575,50 @@ __global__ __launch_bounds__(256) void k_heavy_order(Dev D) {
     const uint32_t hi = (uint32_t)((int32_t)D.cellStart[c0 + 1] + P.posShift);
     if (hi - lo <= TILE_CHUNK) continue;
     __syncthreads();   // the previous tile's bins are no longer read
-    if (threadIdx.x < 64u) bins[threadIdx.x] = 0u;
+    if (t < 64u) {
+      bins[t] = 0u;
+      blive[t] = 0u;
+      bx0[t] = by0[t] = I32_MAX;
+      bx1[t] = by1[t] = I32_MIN;
+    }
     __syncthreads();
-    for (uint32_t i = lo + threadIdx.x; i < hi; i += 256u) atomicAdd(&bins[morton_bin(P, c0, D.pred[i])], 1u);
+    for (uint32_t i = lo + t; i < hi; i += 256u) {
+      const int2 pr = D.pred[i];
+      const uint32_t b = morton_bin(P, c0, pr);
+      atomicAdd(&bins[b], 1u);
+      if (!(D.info[i] & INF_DEAD)) {
+        atomicAdd(&blive[b], 1u);
+        atomicMin(&bx0[b], pr.x);
+        atomicMax(&bx1[b], pr.x);
+        atomicMin(&by0[b], pr.y);
+        atomicMax(&by1[b], pr.y);
+      }
+    }
     __syncthreads();
-    if (threadIdx.x < 64u) {   // exclusive scan of the 64 bin counts, one wave
-      const uint32_t v = bins[threadIdx.x];
+    if (t < 64u) {   // exclusive scan of the 64 bin counts, one wave
+      const uint32_t v = bins[t];
       uint32_t incl = v;
 #pragma unroll
       for (uint32_t off = 1; off < 64u; off <<= 1) {
         const uint32_t u = (uint32_t)__shfl_up((int)incl, off);
-        if (threadIdx.x >= off) incl += u;
+        if (t >= off) incl += u;
       }
-      bins[threadIdx.x] = incl - v;
+      bins[t] = incl - v;
+      bstart[t] = incl - v;
     }
+    if (t == 0u) slotS = atomicAdd(&D.listCount[8], 1u);
     __syncthreads();
-    for (uint32_t i = lo + threadIdx.x; i < hi; i += 256u)
+    for (uint32_t i = lo + t; i < hi; i += 256u)
       D.hperm[lo + atomicAdd(&bins[morton_bin(P, c0, D.pred[i])], 1u)] = i;
+    __syncthreads();   // bins[] now holds each bin's end
+    const uint32_t slot = slotS;
+    if (slot < D.binCap) {
+      if (t < 64u) {
+        D.binTab[(size_t)slot * 64u + t] = make_uint4(bstart[t], bins[t], blive[t], 0u);
+        D.binBox[(size_t)slot * 64u + t] = make_int4(bx0[t], by0[t], bx1[t], by1[t]);
+      }
+      if (t == 0u) D.cellBin[c0] = slot;
+    }
   }
 }
 
@@ -2580,6 +2784,11 @@ int sn_create(const sn_opts* opts, const int32_t* luts, const int32_t* mats, sn_
     D.drop = dalloc<uint32_t>(e, NC);
     D.crank = dalloc<uint32_t>(e, NC);
     D.hperm = dalloc<uint32_t>(e, NC);
+    // a binned cell holds more than TILE_CHUNK slots, so at most NC / 65 are
+    // binned in one substep
+    D.binCap = NC / 64u + 1u;
+    D.binTab = dalloc<uint4>(e, (size_t)D.binCap * 64u);
+    D.binBox = dalloc<int4>(e, (size_t)D.binCap * 64u);
     // every aligned 64-slot block of the 256-slot blocks; empty boxes until
     // first written (a sweep reads only blocks holding live candidates)
     {
@@ -2629,11 +2838,12 @@ int sn_create(const sn_opts* opts, const int32_t* luts, const int32_t* mats, sn_
     // per-substep work lists: at most one item per cell (tiles <= cells)
     D.cellScene = dalloc<uint32_t>(e, CC);
     D.tileAll = dalloc<uint2>(e, CC);
+    D.cellBin = dalloc<uint32_t>(e, CC);
     // a list holds at most one item per cell (tile) plus one per full chunk
     // at most one item per cell or tile plus one per full 32 slots (every list's
     // chunks are of at least 32)
     for (uint32_t c = 0; c < 7; ++c) D.dlist[c] = dalloc<uint2>(e, (size_t)CC + NC / 32 + 64);
-    D.listCount = dalloc<uint32_t>(e, 8);
+    D.listCount = dalloc<uint32_t>(e, 16);
     // persistent grids: as many workgroups as can be resident, a multiple of 8 (XCDs)
     {
       hipDeviceProp_t prop;
