@@ -1326,7 +1326,7 @@ __device__ __forceinline__ void nbr_chunk(const Dev& D, uint32_t c0, uint32_t nc
 constexpr uint32_t CAND_OFF_MAX = 1u << 15;
 
 template <bool BND, bool WIN, bool STRIP>
-__device__ __forceinline__ bool nbr_chunk_staged(const Dev& D, uint16_t* lst, uint32_t c0,
+__device__ __forceinline__ bool nbr_chunk_staged(const Dev& D, uint16_t* lst, uint4* stg, uint32_t c0,
                                                  uint32_t ncell, uint32_t s, uint32_t plo, uint32_t phi,
                                                  uint32_t lane, int32_t W0, Dbg& g, const uint32_t* perm) {
   const SceneP& P = D.sp[s];
@@ -1446,6 +1446,42 @@ __device__ __forceinline__ bool nbr_chunk_staged(const Dev& D, uint16_t* lst, ui
         // iteration on the pile's longest wave.
         const uint32_t upx = (uint32_t)__shfl_up((int)cux, 1), upy = (uint32_t)__shfl_up((int)cuy, 1);
         grouped = __builtin_popcountll(__ballot(lane > 0u && lane < qn && cux == upx && cuy == upy)) >= DUP_MIN;
+      }
+      if constexpr (BND && !WIN) {
+        if (!grouped && !flagged) {
+          // The block's todo candidates in order, staged in this wave's LDS
+          // row as {x, y, list entry, slot} and read back by every lane: one
+          // LDS broadcast per candidate instead of a scalar pick and lane
+          // reads.  Each lane still takes them in canonical order.
+          if ((todo >> lane) & 1ull)
+            stg[__builtin_amdgcn_mbcnt_hi((uint32_t)(todo >> 32), __builtin_amdgcn_mbcnt_lo((uint32_t)todo, 0u))] =
+                make_uint4(cux, cuy, ent, q0 + lane);
+          __builtin_amdgcn_fence(__ATOMIC_RELEASE, "wavefront");
+          __builtin_amdgcn_fence(__ATOMIC_ACQUIRE, "wavefront");
+          const uint32_t n = (uint32_t)__builtin_popcountll(todo);
+          if (allFull) {
+            // only the truncation count changes (debug counters)
+#pragma unroll 4
+            for (uint32_t j = 0; j < n; ++j) {
+              const uint4 c = stg[j];
+              const uint32_t ax = (max(pux, c.x) - min(pux, c.x) + 8u) >> 4;
+              const uint32_t ay = (max(puy, c.y) - min(puy, c.y) + 8u) >> 4;
+              nin += (uint32_t)live & (uint32_t)(c.w != p) & (uint32_t)(__umul24(ax, ax) + __umul24(ay, ay) < (uint32_t)H2S);
+            }
+          } else {
+#pragma unroll 4
+            for (uint32_t j = 0; j < n; ++j) {
+              const uint4 c = stg[j];
+              const uint32_t ax = (max(pux, c.x) - min(pux, c.x) + 8u) >> 4;
+              const uint32_t ay = (max(puy, c.y) - min(puy, c.y) + 8u) >> 4;
+              lst[min(nin, MAXNBR) * 64u + lane] = (uint16_t)c.z;
+              nin += (uint32_t)live & (uint32_t)(c.w != p) & (uint32_t)(__umul24(ax, ax) + __umul24(ay, ay) < (uint32_t)H2S);
+            }
+          }
+          __builtin_amdgcn_fence(__ATOMIC_RELEASE, "wavefront");   // reads done before the next block's stores
+          cnt = min(nin, MAXNBR);
+          continue;
+        }
       }
       if (grouped) {
         // Candidate j is taken together with every later todo candidate of
@@ -1623,8 +1659,8 @@ __global__ __launch_bounds__(256) void k_heavy_order(Dev D) {
 }
 
 template <bool STRIP>
-__device__ __forceinline__ void nbr_item(const Dev& D, uint2 item, uint16_t* lst, uint32_t lane, int32_t W0,
-                                         bool heavy) {
+__device__ __forceinline__ void nbr_item(const Dev& D, uint2 item, uint16_t* lst, uint4* stg, uint32_t lane,
+                                         int32_t W0, bool heavy) {
   const uint32_t tix = __builtin_amdgcn_readfirstlane(item.x);
   const uint32_t chunk = __builtin_amdgcn_readfirstlane(item.y);
   const uint2 tile = D.tileAll[tix];
@@ -1646,10 +1682,10 @@ __device__ __forceinline__ void nbr_item(const Dev& D, uint2 item, uint16_t* lst
     // one-cell tiles of cells no wider than 4 wu: every particle's own
     // stencil is the whole candidate union (no window) and |d| < 2^19
     if (ncell == 1u && D.sp[s].cellShift <= 18u) {
-      if (!(one && nbr_chunk_staged<true, false, STRIP>(D, lst, c0, ncell, s, plo, phi, lane, W0, g, perm)))
+      if (!(one && nbr_chunk_staged<true, false, STRIP>(D, lst, stg, c0, ncell, s, plo, phi, lane, W0, g, perm)))
         nbr_chunk<true, false, NBR_PPL, STRIP>(D, c0, ncell, s, plo, phi, lane, W0, g, perm);
     } else {
-      if (!(one && nbr_chunk_staged<true, true, STRIP>(D, lst, c0, ncell, s, plo, phi, lane, W0, g, perm)))
+      if (!(one && nbr_chunk_staged<true, true, STRIP>(D, lst, stg, c0, ncell, s, plo, phi, lane, W0, g, perm)))
         nbr_chunk<true, true, NBR_PPL, STRIP>(D, c0, ncell, s, plo, phi, lane, W0, g, perm);
     }
   } else {
@@ -1662,10 +1698,12 @@ __device__ __forceinline__ void nbr_item(const Dev& D, uint2 item, uint16_t* lst
 // sweep, with the per-particle tails); the debug build's kernel carries
 // none of that code, so its register budget is its own.
 template <bool STRIP>
-__global__ __launch_bounds__(256, 6) void k_nbr_tile(Dev D) {
+__global__ __launch_bounds__(256, 5) void k_nbr_tile(Dev D) {
   __shared__ uint16_t lists[4][(MAXNBR + 1) * 64];   // a spare row per wave: see nbr_chunk_staged
+  __shared__ uint4 stage[4][64];                     // a block's candidates per wave: see nbr_chunk_staged
   const uint32_t lane = threadIdx.x & 63u;
   uint16_t* lst = lists[threadIdx.x >> 6];
+  uint4* stg = stage[threadIdx.x >> 6];
   const int32_t W0 = D.lut4[0].x;
   // heavy tiles first, each chunk taken by the next free wave anywhere
   const uint32_t nheavy = (uint32_t)__builtin_amdgcn_readfirstlane((int)D.listCount[6]);
@@ -1674,11 +1712,11 @@ __global__ __launch_bounds__(256, 6) void k_nbr_tile(Dev D) {
     if (lane == 0u) w = atomicAdd(&D.listCount[7], 1u);
     w = (uint32_t)__builtin_amdgcn_readfirstlane((int)w);
     if (w >= nheavy) break;
-    nbr_item<STRIP>(D, D.dlist[6][w], lst, lane, W0, true);
+    nbr_item<STRIP>(D, D.dlist[6][w], lst, stg, lane, W0, true);
   }
   const uint32_t nwork = (uint32_t)__builtin_amdgcn_readfirstlane((int)D.listCount[4]);
   const ListRange R = xcd_range(nwork, 4u, threadIdx.x >> 6);
-  for (uint32_t w = R.first; w < R.hi; w += R.stride) nbr_item<STRIP>(D, D.dlist[4][w], lst, lane, W0, false);
+  for (uint32_t w = R.first; w < R.hi; w += R.stride) nbr_item<STRIP>(D, D.dlist[4][w], lst, stg, lane, W0, false);
 }
 
 // ================================================================ solveA (k>0)
