@@ -884,9 +884,9 @@ __device__ __forceinline__ uint32_t r2q_abs(uint32_t a, uint32_t b) {
 // the chunk's reach box, not their own slot pk, and in range by the sweep's
 // one-cell test (biased positions, |d| < 2^19 inside the reach box of a
 // one-cell chunk).  Uniform arguments; every lane active.
-__device__ __forceinline__ uint32_t count_exact(const Dev& D, const uint32_t* perm, uint32_t a, uint32_t b, bool on,
-                                                uint32_t pk, uint32_t pux, uint32_t puy, int32_t bx0, int32_t bx1,
-                                                int32_t by0, int32_t by1, uint32_t lane) {
+__device__ __forceinline__ uint32_t count_exact(const Dev& D, uint4* stg, const uint32_t* perm, uint32_t a, uint32_t b,
+                                                bool on, uint32_t pk, uint32_t pux, uint32_t puy, int32_t bx0,
+                                                int32_t bx1, int32_t by0, int32_t by1, uint32_t lane) {
   uint32_t n = 0;
   for (uint32_t i0 = a; i0 < b; i0 += 64u) {
     const uint32_t i = i0 + lane;
@@ -898,25 +898,31 @@ __device__ __forceinline__ uint32_t count_exact(const Dev& D, const uint32_t* pe
       ci = D.info[q];
     }
     const bool want = !(ci & INF_DEAD) && cp.x >= bx0 && cp.x <= bx1 && cp.y >= by0 && cp.y <= by1;
-    uint64_t todo = __ballot(want);
-    const uint32_t cux = (uint32_t)cp.x ^ 0x80000000u, cuy = (uint32_t)cp.y ^ 0x80000000u;
-    while (todo) {
-      const uint32_t j = (uint32_t)__builtin_ctzll(todo);
-      todo &= todo - 1ull;
-      const uint32_t jx = (uint32_t)__builtin_amdgcn_readlane((int)cux, j);
-      const uint32_t jy = (uint32_t)__builtin_amdgcn_readlane((int)cuy, j);
-      const uint32_t qj = (uint32_t)__builtin_amdgcn_readlane((int)q, j);
-      const uint32_t ax = (max(pux, jx) - min(pux, jx) + 8u) >> 4;
-      const uint32_t ay = (max(puy, jy) - min(puy, jy) + 8u) >> 4;
-      n += (uint32_t)on & (uint32_t)(qj != pk) & (uint32_t)(__umul24(ax, ax) + __umul24(ay, ay) < (uint32_t)H2S);
+    const uint64_t todo = __ballot(want);
+    if (todo == 0ull) continue;
+    // the block's candidates in reach, staged in the wave's LDS row and read
+    // back by every lane (as in nbr_chunk_staged)
+    if (want)
+      stg[__builtin_amdgcn_mbcnt_hi((uint32_t)(todo >> 32), __builtin_amdgcn_mbcnt_lo((uint32_t)todo, 0u))] =
+          make_uint4((uint32_t)cp.x ^ 0x80000000u, (uint32_t)cp.y ^ 0x80000000u, q, 0u);
+    __builtin_amdgcn_fence(__ATOMIC_RELEASE, "wavefront");
+    __builtin_amdgcn_fence(__ATOMIC_ACQUIRE, "wavefront");
+    const uint32_t m = (uint32_t)__builtin_popcountll(todo);
+#pragma unroll 4
+    for (uint32_t j = 0; j < m; ++j) {
+      const uint4 c = stg[j];
+      const uint32_t ax = (max(pux, c.x) - min(pux, c.x) + 8u) >> 4;
+      const uint32_t ay = (max(puy, c.y) - min(puy, c.y) + 8u) >> 4;
+      n += (uint32_t)on & (uint32_t)(c.z != pk) & (uint32_t)(__umul24(ax, ax) + __umul24(ay, ay) < (uint32_t)H2S);
     }
+    __builtin_amdgcn_fence(__ATOMIC_RELEASE, "wavefront");   // reads done before the next block's stores
   }
   return n;
 }
 
 // A lane's in-range count over the whole of binned cell c (bin table at
 // `slot`); c0 is the chunk's own cell and mybin the lane's bin in it.
-__device__ __forceinline__ uint32_t bins_count(const Dev& D, uint32_t slot, uint32_t lo, uint32_t c, uint32_t c0,
+__device__ __forceinline__ uint32_t bins_count(const Dev& D, uint4* stg, uint32_t slot, uint32_t lo, uint32_t c, uint32_t c0,
                                                uint32_t mybin, bool live, uint32_t pk, int2 pi, int32_t bx0,
                                                int32_t bx1, int32_t by0, int32_t by1, uint32_t lane) {
   const uint32_t pux = (uint32_t)pi.x ^ 0x80000000u, puy = (uint32_t)pi.y ^ 0x80000000u;
@@ -941,7 +947,7 @@ __device__ __forceinline__ uint32_t bins_count(const Dev& D, uint32_t slot, uint
     if (__ballot(part)) {
       const uint32_t first = (uint32_t)__builtin_amdgcn_readfirstlane((int)tb.x);
       const uint32_t end = (uint32_t)__builtin_amdgcn_readfirstlane((int)tb.y);
-      n += count_exact(D, D.hperm + lo, first, end, part, pk, pux, puy, bx0, bx1, by0, by1, lane);
+      n += count_exact(D, stg, D.hperm + lo, first, end, part, pk, pux, puy, bx0, bx1, by0, by1, lane);
     }
   }
   return n;
@@ -954,7 +960,7 @@ __device__ __forceinline__ uint32_t bins_count(const Dev& D, uint32_t slot, uint
 // range is counted from its bins, less the slots of the cell outside the
 // range counted exactly when those are fewer.
 constexpr uint32_t BIN_MIN = 256;
-__device__ __forceinline__ uint32_t range_count(const Dev& D, const SceneP& P, int32_t gy, int32_t xlo, int32_t xhi,
+__device__ __forceinline__ uint32_t range_count(const Dev& D, uint4* stg, const SceneP& P, int32_t gy, int32_t xlo, int32_t xhi,
                                                 uint32_t a, uint32_t b, uint32_t c0, bool live, uint32_t pk,
                                                 int2 pi, int32_t bx0, int32_t bx1, int32_t by0, int32_t by1,
                                                 uint32_t lane) {
@@ -970,12 +976,12 @@ __device__ __forceinline__ uint32_t range_count(const Dev& D, const SceneP& P, i
     if (u >= v) continue;
     const uint32_t slot = (uint32_t)__builtin_amdgcn_readfirstlane((int)D.cellBin[c]);
     if (slot == BIN_NONE || v - u <= BIN_MIN || (u - lo) + (hi - v) >= v - u) {
-      n += count_exact(D, nullptr, u, v, live, pk, pux, puy, bx0, bx1, by0, by1, lane);
+      n += count_exact(D, stg, nullptr, u, v, live, pk, pux, puy, bx0, bx1, by0, by1, lane);
       continue;
     }
-    n += bins_count(D, slot, lo, c, c0, morton_bin(P, c0, pi), live, pk, pi, bx0, bx1, by0, by1, lane);
-    n -= count_exact(D, nullptr, lo, u, live, pk, pux, puy, bx0, bx1, by0, by1, lane);
-    n -= count_exact(D, nullptr, v, hi, live, pk, pux, puy, bx0, bx1, by0, by1, lane);
+    n += bins_count(D, stg, slot, lo, c, c0, morton_bin(P, c0, pi), live, pk, pi, bx0, bx1, by0, by1, lane);
+    n -= count_exact(D, stg, nullptr, lo, u, live, pk, pux, puy, bx0, bx1, by0, by1, lane);
+    n -= count_exact(D, stg, nullptr, v, hi, live, pk, pux, puy, bx0, bx1, by0, by1, lane);
   }
   return n;
 }
@@ -983,13 +989,13 @@ __device__ __forceinline__ uint32_t range_count(const Dev& D, const SceneP& P, i
 // The in-range count of the candidates a one-cell sweep has not yet visited:
 // row gy0 from slot q0 on, then every later row of the stencil (rows below
 // gy0 + 1 .. cy + 1, cells xlo .. xhi).
-__device__ __forceinline__ uint32_t rest_count(const Dev& D, const SceneP& P, int32_t cy, int32_t xlo, int32_t xhi,
+__device__ __forceinline__ uint32_t rest_count(const Dev& D, uint4* stg, const SceneP& P, int32_t cy, int32_t xlo, int32_t xhi,
                                                int32_t gy0, uint32_t q0, uint32_t c0, bool live, uint32_t pk,
                                                int2 pi, int32_t bx0, int32_t bx1, int32_t by0, int32_t by1,
                                                uint32_t lane) {
-  uint32_t n = range_count(D, P, gy0, xlo, xhi, q0, 0xffffffffu, c0, live, pk, pi, bx0, bx1, by0, by1, lane);
+  uint32_t n = range_count(D, stg, P, gy0, xlo, xhi, q0, 0xffffffffu, c0, live, pk, pi, bx0, bx1, by0, by1, lane);
   for (int32_t gy = gy0 + 1; gy <= mini(cy + 1, (int32_t)P.gridH - 1); ++gy)
-    n += range_count(D, P, gy, xlo, xhi, 0u, 0xffffffffu, c0, live, pk, pi, bx0, bx1, by0, by1, lane);
+    n += range_count(D, stg, P, gy, xlo, xhi, 0u, 0xffffffffu, c0, live, pk, pi, bx0, bx1, by0, by1, lane);
   return n;
 }
 
@@ -1078,7 +1084,7 @@ __device__ __forceinline__ void nbr_tail(const Dev& D, const SceneP& P, int32_t 
 }
 
 template <bool BND, bool WIN, int PPL, bool STRIP>
-__device__ __forceinline__ void nbr_chunk(const Dev& D, uint32_t c0, uint32_t ncell, uint32_t s,
+__device__ __forceinline__ void nbr_chunk(const Dev& D, uint4* stg, uint32_t c0, uint32_t ncell, uint32_t s,
                                           uint32_t plo, uint32_t phi, uint32_t lane, int32_t W0, Dbg& g,
                                           const uint32_t* perm) {
   const SceneP& P = D.sp[s];
@@ -1143,7 +1149,7 @@ __device__ __forceinline__ void nbr_chunk(const Dev& D, uint32_t c0, uint32_t nc
       for (uint32_t q0 = rlo; q0 < rhi; q0 += 64u) {
         if constexpr (!STRIP && BND && !WIN && PPL == 1) {
           if (binCount && __all(!live[0] || cnt[0] >= MAXNBR)) {
-            dropped[0] += rest_count(D, P, cy, xlo, xhi, gy, q0, c0, live[0], pk[0], pik[0], bx0, bx1, by0, by1, lane);
+            dropped[0] += rest_count(D, stg, P, cy, xlo, xhi, gy, q0, c0, live[0], pk[0], pik[0], bx0, bx1, by0, by1, lane);
             done = true;
             break;
           }
@@ -1403,7 +1409,7 @@ __device__ __forceinline__ bool nbr_chunk_staged(const Dev& D, uint16_t* lst, ui
       if constexpr (!STRIP && BND && !WIN) {
         if (binCount && __all(!live || cnt >= MAXNBR)) {
           // every list is full: the rest of each lane's in-range count at once
-          nin += rest_count(D, P, cy, xlo, xhi, cy + (int32_t)r - 1, q0, c0, live, p, pi, bx0, bx1, by0, by1, lane);
+          nin += rest_count(D, stg, P, cy, xlo, xhi, cy + (int32_t)r - 1, q0, c0, live, p, pi, bx0, bx1, by0, by1, lane);
           done = true;
           break;
         }
@@ -1683,13 +1689,13 @@ __device__ __forceinline__ void nbr_item(const Dev& D, uint2 item, uint16_t* lst
     // stencil is the whole candidate union (no window) and |d| < 2^19
     if (ncell == 1u && D.sp[s].cellShift <= 18u) {
       if (!(one && nbr_chunk_staged<true, false, STRIP>(D, lst, stg, c0, ncell, s, plo, phi, lane, W0, g, perm)))
-        nbr_chunk<true, false, NBR_PPL, STRIP>(D, c0, ncell, s, plo, phi, lane, W0, g, perm);
+        nbr_chunk<true, false, NBR_PPL, STRIP>(D, stg, c0, ncell, s, plo, phi, lane, W0, g, perm);
     } else {
       if (!(one && nbr_chunk_staged<true, true, STRIP>(D, lst, stg, c0, ncell, s, plo, phi, lane, W0, g, perm)))
-        nbr_chunk<true, true, NBR_PPL, STRIP>(D, c0, ncell, s, plo, phi, lane, W0, g, perm);
+        nbr_chunk<true, true, NBR_PPL, STRIP>(D, stg, c0, ncell, s, plo, phi, lane, W0, g, perm);
     }
   } else {
-    nbr_chunk<false, true, NBR_PPL, STRIP>(D, c0, ncell, s, plo, phi, lane, W0, g, perm);
+    nbr_chunk<false, true, NBR_PPL, STRIP>(D, stg, c0, ncell, s, plo, phi, lane, W0, g, perm);
   }
   flush_dbg(D, s, g, 0);
 }

@@ -363,6 +363,7 @@ __device__ __forceinline__ int32_t wave_max3(int32_t v) {
 __global__ __launch_bounds__(256) void k3_nbr_tile(Dev3 D) {
   __shared__ int32_t lutW[LUT_N3], lutG[LUT_N3];
   __shared__ int32_t rhoW[NMAT3];
+  __shared__ int4 stage[4][64];   // a block's candidates in reach, per wave
   for (uint32_t i = threadIdx.x; i < LUT_N3; i += 256u) {
     lutW[i] = D.luts[OFF_W + i];
     lutG[i] = D.luts[OFF_G + i];
@@ -370,6 +371,7 @@ __global__ __launch_bounds__(256) void k3_nbr_tile(Dev3 D) {
   if (threadIdx.x < NMAT3) rhoW[threadIdx.x] = mat_rhoW(D, threadIdx.x);
   __syncthreads();
   const uint32_t lane = threadIdx.x & 63u;
+  int4* stg = stage[threadIdx.x >> 6];
   const uint32_t nwork = (uint32_t)__builtin_amdgcn_readfirstlane((int)D.listCount[0]);
   const ListRange3 R = xcd_range3(nwork, 4u, threadIdx.x >> 6);
   for (uint32_t w = R.first; w < R.hi; w += R.stride) {
@@ -441,15 +443,21 @@ __global__ __launch_bounds__(256) void k3_nbr_tile(Dev3 D) {
           __builtin_amdgcn_s_waitcnt(0x0F70);  // vmcnt(0)
           const bool want = lane < qn && cp.x >= bx0 && cp.x <= bx1 && cp.y >= by0 && cp.y <= by1 &&
                             cp.z >= bz0 && cp.z <= bz1;
-          uint64_t todo = __ballot(want);
-          while (todo) {
-            const uint32_t j = (uint32_t)__builtin_ctzll(todo);
-            todo &= todo - 1ull;
-            const int32_t px = __builtin_amdgcn_readlane(cp.x, j);
-            const int32_t py = __builtin_amdgcn_readlane(cp.y, j);
-            const int32_t pz = __builtin_amdgcn_readlane(cp.z, j);
-            const uint32_t mj = (uint32_t)__builtin_amdgcn_readlane((int)cm, j);
-            const uint32_t q = q0 + j;
+          const uint64_t todo = __ballot(want);
+          if (todo == 0ull) continue;
+          // the candidates in reach, in order, staged in the wave's LDS row as
+          // {x, y, z, material | block lane << 8} and read back by every lane
+          if (want)
+            stg[__builtin_amdgcn_mbcnt_hi((uint32_t)(todo >> 32), __builtin_amdgcn_mbcnt_lo((uint32_t)todo, 0u))] =
+                make_int4(cp.x, cp.y, cp.z, (int32_t)(cm | (lane << 8)));
+          __builtin_amdgcn_fence(__ATOMIC_RELEASE, "wavefront");
+          __builtin_amdgcn_fence(__ATOMIC_ACQUIRE, "wavefront");
+          const uint32_t m = (uint32_t)__builtin_popcountll(todo);
+          for (uint32_t j = 0; j < m; ++j) {
+            const int4 c = stg[j];
+            const int32_t px = c.x, py = c.y, pz = c.z;
+            const uint32_t mj = (uint32_t)c.w & 0xffu;
+            const uint32_t q = q0 + ((uint32_t)c.w >> 8);
             if (!valid || q == p) continue;
             // Every lane's particle lies in this cell and the candidate inside
             // the reach box, so |d| < 2H - 1 on each axis: the quantised squared
@@ -474,6 +482,7 @@ __global__ __launch_bounds__(256) void k3_nbr_tile(Dev3 D) {
             A.sz = wadd(A.sz, gz2);
             A.S = wadd(wadd(wadd(A.S, s_term(gx, g)), s_term(gy2, g)), s_term(gz2, g));
           }
+          __builtin_amdgcn_fence(__ATOMIC_RELEASE, "wavefront");   // reads done before the next block's stores
         }
       }
     }
