@@ -270,9 +270,10 @@ template <bool BND>
 __device__ __forceinline__ int32_t lut_idx(int2 d, Dbg& g) {
   if constexpr (BND) {
     const uint32_t ax = ((uint32_t)abs(d.x) + 8u) >> 4, ay = ((uint32_t)abs(d.y) + 8u) >> 4;
+    // clamped at HS + 1, an axis past HS alone gives r2 >= (HS + 1)^2 >= H2S
     const uint32_t axc = min(ax, (uint32_t)HS + 1u), ayc = min(ay, (uint32_t)HS + 1u);
     const uint32_t r2 = __umul24(axc, axc) + __umul24(ayc, ayc);
-    return (ax > (uint32_t)HS || ay > (uint32_t)HS || r2 >= (uint32_t)H2S) ? -1 : (int32_t)(r2 >> 15);
+    return r2 >= (uint32_t)H2S ? -1 : (int32_t)(r2 >> 15);
   } else {
     return lut_index(d, g);
   }
@@ -1815,6 +1816,7 @@ __device__ __forceinline__ ColItem col_item(uint2 item) {
 constexpr uint32_t SB_E = MAXNBR / 8;
 static_assert(COLOUR_THREADS / CELL_CHUNK >= 8u && SB_E * 8u >= MAXNBR, "k_colour gives every particle at least 8 lanes");
 static_assert(CELL_CHUNK <= COL_COUNT, "an item fits the count field");
+static_assert(2ll * LAM_MAX <= (1ll << 23), "a zero-S solveB coefficient li + lj stays within 2^23");
 // S0: the world's S table is all zero (the frozen image's is), so the LDS
 // table holds G alone (12.8 KB instead of 25.6 KB: more workgroups per CU
 // to hide the gathers).  The list words are loaded without waiting for the
@@ -1851,6 +1853,17 @@ __device__ __forceinline__ int2 solveB_part(const Dev& D, uint32_t p, uint32_t s
     const int32_t ix = lut_idx<BND>(d, g);
     if (ix < 0) continue;
     const int2 T = S0 ? make_int2(lutT[ix], 0) : reinterpret_cast<const int2*>(lutT)[ix];
+    if constexpr (BND && S0) {
+      // with a zero S table: |co| <= 2 LAM_MAX, so the sum cannot saturate,
+      // and |co g| / 2^12 < 2^23 * 2^18.33 / 2^12 < 2^31, so neither can the
+      // term (no event is lost: fp_add_sat and q_sat_clamp would count none)
+      const int32_t co = li + ((e[u] & NBR_BND) ? li : lj[u]);
+      const double gs = (double)T.x * 0x1p-26, cf = (double)co * 0x1p-12;
+      const double gx = q_rne(gs * (double)d.x), gy = q_rne(gs * (double)d.y);
+      dpx = wadd(dpx, (int32_t)__builtin_fmin(__builtin_fmax(q_rne(cf * gx), (double)-DP_TERM), (double)DP_TERM));
+      dpy = wadd(dpy, (int32_t)__builtin_fmin(__builtin_fmax(q_rne(cf * gy), (double)-DP_TERM), (double)DP_TERM));
+      continue;
+    }
     const int32_t co = fp_add_sat(fp_add_sat(li, (e[u] & NBR_BND) ? li : lj[u], g), T.y, g);
     if constexpr (BND) {
       // binary64 forms (bounds as in solveA_term): exact gradients with
