@@ -649,11 +649,6 @@ __device__ __forceinline__ void solveA_finish(const Dev& D, uint32_t p, SolveAAc
 constexpr uint32_t NBR_UNROLL = 4;
 constexpr uint32_t XNC_UNROLL = 2;
 static_assert(MAXNBR % NBR_UNROLL == 0 && MAXNBR % XNC_UNROLL == 0, "list slots k + u stay below MAXNBR");
-__device__ __forceinline__ uint32_t nbr_at(const Dev& D, const uint32_t* nb, uint32_t k, uint32_t cnt,
-                                           uint32_t self) {
-  uint32_t e = nb[(size_t)k * D.NS];
-  return k < cnt ? e : self;
-}
 
 // ============================================================ per-substep work
 // After the scan: work lists of this substep.  [0..3]: every occupied cell
@@ -1729,18 +1724,26 @@ __global__ __launch_bounds__(256, 5) void k_nbr_tile(Dev D) {
 // ================================================================ solveA (k>0)
 
 template <bool BND>
-__device__ __forceinline__ void solveA_at(const Dev& D, uint32_t p, const int2* lutWG, Dbg& g) {
-  const int2 pi = D.pred[p];
-  const uint32_t cnt = D.nbrN[p] & 0x1fffffffu;
+__device__ __forceinline__ void solveA_at(const Dev& D, uint32_t p, int2 pi, uint32_t cnt, const int2* lutWG,
+                                          Dbg& g) {
   const uint32_t* nb = D.nbr + p;
   SolveAAcc acc{D.lut4[0].x, 0, 0, 0};
+  // the list words of each group are loaded while the group before it is
+  // gathered and summed (list slots below MAXNBR are always inside the array)
+  uint32_t qn[NBR_UNROLL];
+#pragma unroll
+  for (uint32_t u = 0; u < NBR_UNROLL; ++u) qn[u] = nb[(size_t)u * D.NS];
   for (uint32_t k = 0; k < cnt; k += NBR_UNROLL) {
     uint32_t q[NBR_UNROLL];
     int2 d[NBR_UNROLL];
     int32_t idx[NBR_UNROLL];
     int4 L[NBR_UNROLL];
 #pragma unroll
-    for (uint32_t u = 0; u < NBR_UNROLL; ++u) q[u] = nbr_at(D, nb, k + u, cnt, p);
+    for (uint32_t u = 0; u < NBR_UNROLL; ++u) q[u] = k + u < cnt ? qn[u] : p;
+    if (k + NBR_UNROLL < MAXNBR) {
+#pragma unroll
+      for (uint32_t u = 0; u < NBR_UNROLL; ++u) qn[u] = nb[(size_t)(k + NBR_UNROLL + u) * D.NS];
+    }
 #pragma unroll
     for (uint32_t u = 0; u < NBR_UNROLL; ++u) {
       d[u] = pdelta<BND>(pi, D.pred[q[u] & ~NBR_BND], g);
@@ -1769,12 +1772,20 @@ __global__ __launch_bounds__(256) void k_solveA(Dev D) {
   }
   __syncthreads();
   const uint32_t p = slot_block(D) * 256u + threadIdx.x;
-  const uint32_t s = p < D.N ? D.sidOf[p] : INACTIVE;
+  // the particle's own words are loaded with its scene index (one round trip)
+  uint32_t s = INACTIVE, inf = 0, nn = 0;
+  int2 pi = make_int2(0, 0);
+  if (p < D.N) {
+    s = D.sidOf[p];
+    inf = D.info[p];
+    nn = D.nbrN[p];
+    pi = D.pred[p];
+  }
   Dbg g;
   const bool bnd = wave_bounded(D, s);
-  if (s != INACTIVE && (D.info[p] & INF_FLUID)) {
-    if (bnd) solveA_at<true>(D, p, lutWG, g);
-    else solveA_at<false>(D, p, lutWG, g);
+  if (s != INACTIVE && (inf & INF_FLUID)) {
+    if (bnd) solveA_at<true>(D, p, pi, nn & 0x1fffffffu, lutWG, g);
+    else solveA_at<false>(D, p, pi, nn & 0x1fffffffu, lutWG, g);
   }
   flush_dbg(D, s, g, 0);
 }
@@ -1969,6 +1980,11 @@ __device__ __forceinline__ void xnc_fluid(const Dev& D, uint32_t p, const int2* 
   uint32_t cnt = D.nbrN[p] & 0x1fffffffu;
   const uint32_t* nb = D.nbr + p;
   int32_t ax = 0, ay = 0, sx = 0, sy = 0, q = 0, c = 0;
+  // the list words of each group are loaded while the group before it is
+  // gathered and summed (list slots below MAXNBR are always inside the array)
+  uint32_t en[XNC_UNROLL];
+#pragma unroll
+  for (uint32_t u = 0; u < XNC_UNROLL; ++u) en[u] = nb[(size_t)u * D.NS];
   for (uint32_t k = 0; k < cnt; k += XNC_UNROLL) {
     // padding entries read as boundary entries, which this loop skips
     uint32_t e[XNC_UNROLL];
@@ -1976,7 +1992,11 @@ __device__ __forceinline__ void xnc_fluid(const Dev& D, uint32_t p, const int2* 
     int32_t idx[XNC_UNROLL], rhoj[XNC_UNROLL], rjs[XNC_UNROLL];
     int4 Ls[XNC_UNROLL], bjs[XNC_UNROLL];
 #pragma unroll
-    for (uint32_t u = 0; u < XNC_UNROLL; ++u) e[u] = nbr_at(D, nb, k + u, cnt, p | NBR_BND);
+    for (uint32_t u = 0; u < XNC_UNROLL; ++u) e[u] = k + u < cnt ? en[u] : (p | NBR_BND);
+    if (k + XNC_UNROLL < MAXNBR) {
+#pragma unroll
+      for (uint32_t u = 0; u < XNC_UNROLL; ++u) en[u] = nb[(size_t)(k + XNC_UNROLL + u) * D.NS];
+    }
 #pragma unroll
     for (uint32_t u = 0; u < XNC_UNROLL; ++u) {
       // a skipped (boundary or padding) entry is measured against the
@@ -2100,12 +2120,20 @@ __device__ __forceinline__ int2 st_sum(const Dev& D, uint32_t p, const int32_t* 
   uint32_t cnt = D.nbrN[p] & 0x1fffffffu;
   const uint32_t* nb = D.nbr + p;
   int32_t ax = 0, ay = 0;
+  // list words one group ahead, as in xnc_fluid
+  uint32_t en[XNC_UNROLL];
+#pragma unroll
+  for (uint32_t u = 0; u < XNC_UNROLL; ++u) en[u] = nb[(size_t)u * D.NS];
   for (uint32_t k = 0; k < cnt; k += XNC_UNROLL) {
     uint32_t e[XNC_UNROLL];
     int2 d[XNC_UNROLL], njs[XNC_UNROLL];
     int32_t idx[XNC_UNROLL], rhoj[XNC_UNROLL], cts[XNC_UNROLL], ats[XNC_UNROLL], matj[XNC_UNROLL];
 #pragma unroll
-    for (uint32_t u = 0; u < XNC_UNROLL; ++u) e[u] = nbr_at(D, nb, k + u, cnt, p);
+    for (uint32_t u = 0; u < XNC_UNROLL; ++u) e[u] = k + u < cnt ? en[u] : p;
+    if (k + XNC_UNROLL < MAXNBR) {
+#pragma unroll
+      for (uint32_t u = 0; u < XNC_UNROLL; ++u) en[u] = nb[(size_t)(k + XNC_UNROLL + u) * D.NS];
+    }
 #pragma unroll
     for (uint32_t u = 0; u < XNC_UNROLL; ++u) {
       uint32_t qj = e[u] & ~NBR_BND;
