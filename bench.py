@@ -427,7 +427,14 @@ def run_batched(args):
         "debug_counters": not args.no_debug,
         "autotune": tune,
     }
-    print(json.dumps(_line(agg, "particle-substeps/s", K, W, dt * 1e3 / K, cfg)), flush=True)
+    # the whole substep against the HBM peak, on the same per-datum model as the
+    # 1M line (no single kernel dominates a world of 512 small scenes)
+    ab = algorithmic_bytes(F, N, C, iters)["total"]
+    ach = ab / (dt / K) / 1e9
+    roof = {"bound": "hbm", "achieved": round(ach, 2), "peak": HBM_PEAK_GBS, "unit": "GB/s",
+            "frac": round(ach / HBM_PEAK_GBS, 6), "traffic": None, "kernel": "whole substep (every kernel of the world)",
+            "algorithmic_bytes_per_substep": ab}
+    print(json.dumps(_line(agg, "particle-substeps/s", K, W, dt * 1e3 / K, cfg, roofline=roof)), flush=True)
     eng.destroy()
 
 
