@@ -141,6 +141,9 @@ def main():
     ap.add_argument("--rates", default="10,50,200,500", help="serve: arrival rates in jobs/s")
     ap.add_argument("--jobs", type=int, default=16, help="serve: jobs per arrival rate")
     ap.add_argument("--max-steps", type=int, default=5000, help="serve: upper bound of a job's substeps")
+    ap.add_argument("--serve-alternate", action="store_true",
+                    help="serve: one round on the device at a time, the worlds taking turns (default: each world "
+                         "keeps its own round in flight, so 2D and 3D rounds overlap)")
     ap.add_argument("--host", default="python", choices=("python", "js"),
                     help="serve: the host that runs the serving loop (js = node js/serve_bench.js over the addon)")
     ap.add_argument("--full-substeps", type=int, default=500,
@@ -463,7 +466,7 @@ def run_serve(args):
                 print(f"# rate {rate}/s: {len(finished)}/{len(jobs)} jobs done, {time.perf_counter() - t0:.1f}s",
                       file=sys.stderr, flush=True)
 
-        st = srv.run(jobs, on_finish=progress)
+        st = srv.run(jobs, on_finish=progress, overlap=not args.serve_alternate)
         summ = serve.summarize(jobs, st["wall_s"])
         summ["rate_jobs_per_s"] = rate
         summ["world_busy_s"] = {k: round(v, 3) for k, v in st["busy_s_by_world"].items()}

@@ -156,8 +156,8 @@ function poissonJobs(count, rate, seed, o) {
 }
 
 // ---- worlds ---------------------------------------------------------------
-const DEFAULT_2D = [[20000, 2048, 12], [70000, 8192, 8], [270000, 8192, 4]];
-const DEFAULT_3D = [[20000, 8192, 6], [70000, 16384, 4], [270000, 65536, 2]];
+const DEFAULT_2D = [[20000, 2048, 24], [70000, 8192, 16], [270000, 8192, 8]];
+const DEFAULT_3D = [[20000, 8192, 12], [70000, 16384, 8], [270000, 65536, 4]];
 let LUTS3 = null;
 
 function footprint(job) {
@@ -234,7 +234,7 @@ const sleepMs = (ms) => { if (ms > 0) Atomics.wait(SLEEP, 0, 0, ms); };
 
 class Server {
   constructor(o) {
-    o = Object.assign({ classes2d: DEFAULT_2D, classes3d: DEFAULT_3D, iters2d: 4, iters3d: 3, chunk: 64,
+    o = Object.assign({ classes2d: DEFAULT_2D, classes3d: DEFAULT_3D, iters2d: 4, iters3d: 3, chunk: 64, overlap: true,
                         streamCap: 8192, debug: false }, o || {});
     this.o = o;
     this.worlds = {};
@@ -298,7 +298,20 @@ class Server {
         w.inflight = k;
       }
     };
-    for (;;) {
+    // overlap (the default): each world keeps a round in flight on its own
+    // stream, so 2D and 3D rounds run side by side on the device
+    while (this.o.overlap) {
+      const t = now() - t0;
+      for (const k of kinds) while (pending[k].length && (!realtime || pending[k][0].arrival <= t)) queue[k].push(pending[k].shift());
+      for (const k of kinds) if (this.worlds[k].inflight && !this.worlds[k].query()) finishRound(this.worlds[k]);
+      for (const k of kinds) if (!this.worlds[k].inflight) admitAndLaunch(k);
+      if (kinds.some((k) => this.worlds[k].inflight)) { sleepMs(0.2); continue; }
+      if (kinds.some((k) => queue[k].length)) throw new Error('queued jobs fit no free partition of an idle world');
+      if (!kinds.some((k) => pending[k].length)) break;
+      const next = Math.min(...kinds.filter((k) => pending[k].length).map((k) => pending[k][0].arrival));
+      sleepMs(Math.min(5, (next - (now() - t0)) * 1e3));
+    }
+    for (;!this.o.overlap;) {
       const t = now() - t0;
       for (const k of kinds) while (pending[k].length && (!realtime || pending[k][0].arrival <= t)) queue[k].push(pending[k].shift());
       if (current !== null) {
