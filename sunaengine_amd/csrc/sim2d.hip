@@ -181,6 +181,7 @@ struct Dev {
   uint32_t* info;
   int32_t* rho;
   int32_t* lam;
+  int4* pl4;       // {pred.x, pred.y, lam, 0} per slot: the colour pass's one gather per neighbour
   int2* dp;
   int2* vtmp;
   int32_t* rinv;
@@ -458,7 +459,9 @@ __device__ __forceinline__ void canon_place(const Dev& D, uint32_t s, uint32_t m
   D.ptY[q] = pt;
   D.idY[q] = myid;
   D.stY[q] = D.st[me];
-  D.pred[q] = D.pred0[me];
+  const int2 pr = D.pred0[me];
+  D.pred[q] = pr;
+  D.pl4[q] = make_int4(pr.x, pr.y, 0, 0);
   D.cellY[q] = c;
   uint32_t fl = (uint32_t)pt.y;
   bool fluid = myid < P.nFluid;
@@ -632,12 +635,15 @@ __device__ __forceinline__ void solveA_term(SolveAAcc& a, int2 d, int4 L, Dbg& g
     a.S = wadd(wadd(a.S, clampi(fp_mul_s(gx, gx, 11, g), 0, S_TERM)), clampi(fp_mul_s(gy, gy, 11, g), 0, S_TERM));
   }
 }
-__device__ __forceinline__ void solveA_finish(const Dev& D, uint32_t p, SolveAAcc& a, int32_t eps, Dbg& g) {
+__device__ __forceinline__ void solveA_finish(const Dev& D, uint32_t p, int2 pi, SolveAAcc& a, int32_t eps,
+                                              Dbg& g) {
   a.S = wadd(wadd(a.S, clampi(fp_mul_s(a.sgx, a.sgx, 11, g), 0, S_TERM)),
              clampi(fp_mul_s(a.sgy, a.sgy, 11, g), 0, S_TERM));
   int32_t Cq = clampi(fp_shr_rne(wsub(a.rho, RHO0), 4), 0, C_MAX);
   D.rho[p] = a.rho;
-  D.lam[p] = clampi(fp_neg_sat(fp_divshift(Cq, wadd(a.S, eps), 25, g), g), -LAM_MAX, LAM_MAX);
+  const int32_t lam = clampi(fp_neg_sat(fp_divshift(Cq, wadd(a.S, eps), 25, g), g), -LAM_MAX, LAM_MAX);
+  D.lam[p] = lam;
+  D.pl4[p] = make_int4(pi.x, pi.y, lam, 0);   // pi: the particle's current pred
 }
 
 // ============================================================ neighbour loops
@@ -1309,7 +1315,7 @@ __device__ __forceinline__ void nbr_chunk(const Dev& D, uint4* stg, uint32_t c0,
                                  ((bits[k] & INF_CONGEAL) ? NBR_CONGEAL_BIT : 0u))
                               : 0u;
       D.drop[pk[k]] = D.dbgOn ? dropped[k] : 0u;
-      if (inf[k] & INF_FLUID) solveA_finish(D, pk[k], acc[k], MAT(D, (uint32_t)D.ptY[pk[k]].x)[M_EPS], g);
+      if (inf[k] & INF_FLUID) solveA_finish(D, pk[k], pik[k], acc[k], MAT(D, (uint32_t)D.ptY[pk[k]].x)[M_EPS], g);
     }
   }
 }
@@ -1578,7 +1584,7 @@ __device__ __forceinline__ bool nbr_chunk_staged(const Dev& D, uint16_t* lst, ui
     D.nbrN[p] = live ? (cnt | ((bits & INF_SEAL) ? NBR_SEAL_BIT : 0u) | ((bits & INF_CONGEAL) ? NBR_CONGEAL_BIT : 0u))
                      : 0u;
     D.drop[p] = D.dbgOn ? dropped : 0u;
-    if (inf & INF_FLUID) solveA_finish(D, p, acc, MAT(D, (uint32_t)D.ptY[p].x)[M_EPS], g);
+    if (inf & INF_FLUID) solveA_finish(D, p, pi, acc, MAT(D, (uint32_t)D.ptY[p].x)[M_EPS], g);
   }
   return true;
 }
@@ -1755,7 +1761,7 @@ __device__ __forceinline__ void solveA_at(const Dev& D, uint32_t p, int2 pi, uin
     for (uint32_t u = 0; u < NBR_UNROLL; ++u)
       if (idx[u] >= 0) solveA_term<BND>(acc, d[u], L[u], g);
   }
-  solveA_finish(D, p, acc, MAT(D, (uint32_t)D.ptY[p].x)[M_EPS], g);
+  solveA_finish(D, p, pi, acc, MAT(D, (uint32_t)D.ptY[p].x)[M_EPS], g);
 }
 
 // True on every lane when the wave's active lanes all belong to bounded scenes.
@@ -1844,17 +1850,18 @@ __device__ __forceinline__ int2 solveB_part(const Dev& D, uint32_t p, uint32_t s
     const uint32_t k = min(sub + u * L, MAXNBR - 1u);
     e[u] = nb[(size_t)k * D.NS];
   }
-  const int2 pi = D.pred[p];
-  const int32_t li = D.lam[p];
+  const int4 own = D.pl4[p];
+  const int2 pi = make_int2(own.x, own.y);
+  const int32_t li = own.z;
   const uint32_t cnt = D.nbrN[p] & 0x1fffffffu;
   int2 pj[SB_E];
   int32_t lj[SB_E];
 #pragma unroll
   for (uint32_t u = 0; u < SB_E; ++u) {
     if (sub + u * L >= cnt) e[u] = p;
-    const uint32_t q = e[u] & ~NBR_BND;
-    pj[u] = D.pred[q];
-    lj[u] = D.lam[q];
+    const int4 r = D.pl4[e[u] & ~NBR_BND];   // pred and lambda in one gather
+    pj[u] = make_int2(r.x, r.y);
+    lj[u] = r.z;
   }
   int32_t dpx = 0, dpy = 0;
 #pragma unroll
@@ -1905,6 +1912,7 @@ __device__ __forceinline__ void apply_at(const Dev& D, const SceneP& P, uint32_t
     const int32_t ddy = clampi(fp_shr_rne(dp.y, ACC), dmN, dm);
     pr = wall_clamp(P, make_int2(fp_add_sat(pr.x, ddx, g), fp_add_sat(pr.y, ddy, g)));
     D.pred[p] = pr;
+    *reinterpret_cast<int2*>(&D.pl4[p]) = pr;   // the record's pred half
   }
   if (last) {
     const int4 a = D.psY[p];
@@ -2857,6 +2865,7 @@ int sn_create(const sn_opts* opts, const int32_t* luts, const int32_t* mats, sn_
     D.info = dalloc<uint32_t>(e, NC);
     D.rho = dalloc<int32_t>(e, NC);
     D.lam = dalloc<int32_t>(e, NC);
+    D.pl4 = dalloc<int4>(e, NC);
     D.dp = dalloc<int2>(e, NC);
     D.vtmp = dalloc<int2>(e, NC);
     D.rinv = dalloc<int32_t>(e, NC);
