@@ -407,8 +407,11 @@ __global__ __launch_bounds__(256) void k_finalize(Dev D) {
     const int32_t vmaxN = fp_neg_sat(P.vmax, g);
     const int32_t vx = clampi(fp_sub_sat(pr.x, a.x, g), vmaxN, P.vmax);
     const int32_t vy = clampi(fp_sub_sat(pr.y, a.y, g), vmaxN, P.vmax);
+    const int32_t rho = D.rho[p];
+    const int32_t ri = clampi(fp_divshift(RHO0, maxi(rho, RHO_FLOOR_RINV), 16, g), 0, RINV_MAX);
     D.vtmp[p] = make_int2(vx, vy);
-    D.rinv[p] = clampi(fp_divshift(RHO0, maxi(D.rho[p], RHO_FLOOR_RINV), 16, g), 0, RINV_MAX);
+    D.rinv[p] = ri;
+    D.vr4[p] = make_int4(vx, vy, rho, ri);   // xnc's gather record
   }
   flush_dbg(D, s, g, 0);
 }

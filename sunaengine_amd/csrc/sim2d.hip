@@ -182,6 +182,7 @@ struct Dev {
   int32_t* rho;
   int32_t* lam;
   int4* pl4;       // {pred.x, pred.y, lam, 0} per slot: the colour pass's one gather per neighbour
+  int4* vr4;       // {vtmp.x, vtmp.y, rho, rinv} per fluid slot after finalize: xnc's one gather for the three
   int2* dp;
   int2* vtmp;
   int32_t* rinv;
@@ -1919,8 +1920,11 @@ __device__ __forceinline__ void apply_at(const Dev& D, const SceneP& P, uint32_t
     const int32_t vmaxN = fp_neg_sat(P.vmax, g);
     const int32_t vx = clampi(fp_sub_sat(pr.x, a.x, g), vmaxN, P.vmax);
     const int32_t vy = clampi(fp_sub_sat(pr.y, a.y, g), vmaxN, P.vmax);
+    const int32_t rho = D.rho[p];
+    const int32_t ri = clampi(fp_divshift(RHO0, maxi(rho, RHO_FLOOR_RINV), 16, g), 0, RINV_MAX);
     D.vtmp[p] = make_int2(vx, vy);
-    D.rinv[p] = clampi(fp_divshift(RHO0, maxi(D.rho[p], RHO_FLOOR_RINV), 16, g), 0, RINV_MAX);
+    D.rinv[p] = ri;
+    D.vr4[p] = make_int4(vx, vy, rho, ri);
   }
 }
 
@@ -2016,9 +2020,10 @@ __device__ __forceinline__ void xnc_fluid(const Dev& D, uint32_t p, const int2* 
       const int2 wg = lutWG[maxi(idx[u], 0)];
       Ls[u] = make_int4(wg.x, wg.y, 0, 0);   // xsph, normals and conduct use W and G only
       bjs[u] = D.ptY[qj];
-      rhoj[u] = D.rho[qj];
-      vjs[u] = D.vtmp[qj];
-      rjs[u] = D.rinv[qj];
+      const int4 vr = D.vr4[qj];   // velocity, rho and rinv in one gather
+      vjs[u] = make_int2(vr.x, vr.y);
+      rhoj[u] = vr.z;
+      rjs[u] = vr.w;
     }
 #pragma unroll
     for (uint32_t u = 0; u < XNC_UNROLL; ++u) {
@@ -2866,6 +2871,7 @@ int sn_create(const sn_opts* opts, const int32_t* luts, const int32_t* mats, sn_
     D.rho = dalloc<int32_t>(e, NC);
     D.lam = dalloc<int32_t>(e, NC);
     D.pl4 = dalloc<int4>(e, NC);
+    D.vr4 = dalloc<int4>(e, NC);
     D.dp = dalloc<int2>(e, NC);
     D.vtmp = dalloc<int2>(e, NC);
     D.rinv = dalloc<int32_t>(e, NC);
