@@ -183,6 +183,7 @@ struct Dev {
   int32_t* lam;
   int4* pl4;       // {pred.x, pred.y, lam, 0} per slot: the colour pass's one gather per neighbour
   int4* vr4;       // {vtmp.x, vtmp.y, rho, rinv} per fluid slot after finalize: xnc's one gather for the three
+  int4* nr4;       // {nrm.x, nrm.y, rho, material} per fluid slot after xnc: surface tension's one gather
   int2* dp;
   int2* vtmp;
   int32_t* rinv;
@@ -2092,7 +2093,9 @@ __device__ __forceinline__ void xnc_fluid(const Dev& D, uint32_t p, const int2* 
     vy = fp_add_sat(vy, fp_shr_rne(ay, ACC), g);
   }
   D.ps[p] = make_int4(pi.x, pi.y, vx, vy);
-  D.nrm[p] = make_int2(fp_mul_s(sx, H, 16, g), fp_mul_s(sy, H, 16, g));
+  const int2 nrm = make_int2(fp_mul_s(sx, H, 16, g), fp_mul_s(sy, H, 16, g));
+  D.nrm[p] = nrm;
+  D.nr4[p] = make_int4(nrm.x, nrm.y, rhoi, (int32_t)mi);   // surface tension's gather record
   D.heat[p] = q;
   D.corr[p] = c;
 }
@@ -2156,9 +2159,10 @@ __device__ __forceinline__ int2 st_sum(const Dev& D, uint32_t p, const int32_t* 
       const int2 ac = lutAC[ii];
       ats[u] = ac.x;
       cts[u] = ac.y;
-      rhoj[u] = D.rho[qj];
-      matj[u] = D.ptY[qj].x;
-      njs[u] = D.nrm[qj];
+      const int4 nr = D.nr4[qj];   // normal, rho and material in one gather (fluid neighbours)
+      rhoj[u] = nr.z;
+      matj[u] = nr.w;
+      njs[u] = make_int2(nr.x, nr.y);
     }
 #pragma unroll
     for (uint32_t u = 0; u < XNC_UNROLL; ++u) {
@@ -2872,6 +2876,7 @@ int sn_create(const sn_opts* opts, const int32_t* luts, const int32_t* mats, sn_
     D.lam = dalloc<int32_t>(e, NC);
     D.pl4 = dalloc<int4>(e, NC);
     D.vr4 = dalloc<int4>(e, NC);
+    D.nr4 = dalloc<int4>(e, NC);
     D.dp = dalloc<int2>(e, NC);
     D.vtmp = dalloc<int2>(e, NC);
     D.rinv = dalloc<int32_t>(e, NC);
